@@ -1,0 +1,140 @@
+"""Column batch container and its binary file form.
+
+A Case holds one `runes` call's inputs (socrates_runes.F90:47-168) as numpy
+arrays in the interface's full-field layout: Fortran (n_profile, n_layer),
+i.e. numpy shape (n_layer, n_profile), profile fastest, layer 1 at the top;
+level fields have n_layer+1 rows (level 0 = top of atmosphere); per-band
+surface albedos have shape (n_band, n_profile).
+
+The binary form is read by the reference harness in oracle/harness and by
+the tests; all reals are float64, integers int32, native byte order.
+"""
+import dataclasses
+
+import numpy as np
+
+MAGIC = 1397703491
+VERSION = 1
+
+_HDR = ("n_profile", "n_layer", "n_band", "i_source", "l_cloud", "i_cloud_repr",
+        "i_overlap", "i_inhom", "i_mcica_sampling", "l_rayleigh", "i_st_water",
+        "i_st_ice", "n_subcol_gen", "i_solver_kind")
+_SCALARS = ("co2_mix_ratio", "cloud_vertical_decorr", "cloud_horizontal_rsd",
+            "conv_vertical_decorr")
+_LAYER = ("p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3")
+_LEVEL = ("p_level", "t_level")
+_COLUMN = ("t_ground", "cos_zen", "solar_irrad")
+_BAND = ("albedo_diff", "albedo_dir")
+_CLOUD = ("cloud_frac", "liq_mmr", "liq_dim")
+_CONV = ("conv_frac", "liq_conv_mmr", "liq_conv_dim")
+
+
+@dataclasses.dataclass
+class Case:
+    n_profile: int
+    n_layer: int
+    n_band: int
+    i_source: int = 2
+    l_cloud: int = 0
+    i_cloud_repr: int = 0
+    i_overlap: int = 0
+    i_inhom: int = 0
+    i_mcica_sampling: int = 0
+    l_rayleigh: int = 0
+    i_st_water: int = 5
+    i_st_ice: int = 8
+    n_subcol_gen: int = 0
+    i_solver_kind: int = 0
+    co2_mix_ratio: float = 0.0
+    cloud_vertical_decorr: float = 0.0
+    cloud_horizontal_rsd: float = 0.0
+    conv_vertical_decorr: float = 0.0
+    p_layer: np.ndarray = None
+    t_layer: np.ndarray = None
+    mass: np.ndarray = None
+    density: np.ndarray = None
+    layer_heat_capacity: np.ndarray = None
+    h2o: np.ndarray = None
+    o3: np.ndarray = None
+    p_level: np.ndarray = None
+    t_level: np.ndarray = None
+    t_ground: np.ndarray = None
+    cos_zen: np.ndarray = None
+    solar_irrad: np.ndarray = None
+    albedo_diff: np.ndarray = None
+    albedo_dir: np.ndarray = None
+    cloud_frac: np.ndarray = None
+    liq_mmr: np.ndarray = None
+    liq_dim: np.ndarray = None
+    conv_frac: np.ndarray = None
+    liq_conv_mmr: np.ndarray = None
+    liq_conv_dim: np.ndarray = None
+    rand_seed: np.ndarray = None
+
+    def fill_defaults(self):
+        L, P = self.n_layer, self.n_profile
+        for k in _CLOUD + _CONV:
+            if getattr(self, k) is None:
+                setattr(self, k, np.zeros((L, P)) if k != "liq_dim" and k != "liq_conv_dim"
+                        else np.full((L, P), 1.0e-5))
+        if self.rand_seed is None:
+            self.rand_seed = np.arange(P, dtype=np.int32)
+        return self
+
+    def subset(self, cols):
+        """The same case restricted to the given column indices."""
+        cols = np.asarray(cols)
+        out = dataclasses.replace(self, n_profile=int(cols.size))
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if isinstance(v, np.ndarray):
+                setattr(out, f.name, np.ascontiguousarray(v[..., cols]))
+        return out
+
+    def write(self, path):
+        self.fill_defaults()
+        with open(path, "wb") as f:
+            np.array([MAGIC, VERSION] + [getattr(self, k) for k in _HDR],
+                     dtype=np.int32).tofile(f)
+            np.array([getattr(self, k) for k in _SCALARS], dtype=np.float64).tofile(f)
+            for k in _LAYER + _LEVEL + _COLUMN + _BAND + _CLOUD + _CONV:
+                np.ascontiguousarray(getattr(self, k), dtype=np.float64).tofile(f)
+            np.ascontiguousarray(self.rand_seed, dtype=np.int32).tofile(f)
+
+    @classmethod
+    def read(cls, path):
+        raw = open(path, "rb").read()
+        hdr = np.frombuffer(raw, dtype=np.int32, count=2 + len(_HDR))
+        if hdr[0] != MAGIC:
+            raise ValueError("not a case file: %s" % path)
+        kw = dict(zip(_HDR, (int(x) for x in hdr[2:])))
+        c = cls(**kw)
+        off = 4 * (2 + len(_HDR))
+        sc = np.frombuffer(raw, dtype=np.float64, count=len(_SCALARS), offset=off)
+        off += 8 * len(_SCALARS)
+        for k, v in zip(_SCALARS, sc):
+            setattr(c, k, float(v))
+        L, P, B = c.n_layer, c.n_profile, c.n_band
+        shapes = {**{k: (L, P) for k in _LAYER + _CLOUD + _CONV},
+                  **{k: (L + 1, P) for k in _LEVEL}, **{k: (P,) for k in _COLUMN},
+                  **{k: (B, P) for k in _BAND}}
+        for k in _LAYER + _LEVEL + _COLUMN + _BAND + _CLOUD + _CONV:
+            n = int(np.prod(shapes[k]))
+            setattr(c, k, np.frombuffer(raw, dtype=np.float64, count=n,
+                                        offset=off).reshape(shapes[k]).copy())
+            off += 8 * n
+        c.rand_seed = np.frombuffer(raw, dtype=np.int32, count=P, offset=off).copy()
+        return c
+
+
+def read_output(path, n_profile, n_layer):
+    """Outputs of the reference harness: flux_up, flux_down, flux_direct
+    (n_layer+1, n_profile) and heating_rate (n_layer, n_profile) [K/s]."""
+    a = np.fromfile(path, dtype=np.float64)
+    P, L = n_profile, n_layer
+    n1 = P * (L + 1)
+    up = a[:n1].reshape(L + 1, P)
+    down = a[n1:2 * n1].reshape(L + 1, P)
+    direct = a[2 * n1:3 * n1].reshape(L + 1, P)
+    hr = a[3 * n1:3 * n1 + P * L].reshape(L, P)
+    return dict(flux_up=up, flux_down=down, flux_direct=direct, heating_rate=hr)
