@@ -1,0 +1,90 @@
+"""ctypes front end of the C restatement (oracle/soc_oracle.c).
+
+Test infrastructure only: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libsoc_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class _Case(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "n_profile", "n_layer", "n_band", "i_source", "l_cloud", "i_cloud_repr",
+        "i_overlap", "i_inhom", "i_mcica_sampling", "l_rayleigh", "i_st_water",
+        "n_subcol_gen")] + [(n, ctypes.c_double) for n in (
+        "co2_mix_ratio", "cloud_vertical_decorr", "cloud_horizontal_rsd",
+        "conv_vertical_decorr")] + [(n, _D) for n in (
+        "p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
+        "p_level", "t_level", "t_ground", "cos_zen", "solar_irrad", "albedo_diff",
+        "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr",
+        "liq_conv_dim")] + [("rand_seed", _I)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.soc_oracle_runes.restype = ctypes.c_int
+    return _lib
+
+
+def _ptr(a, keep, dtype=np.float64):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    keep.append(a)
+    return a.ctypes.data_as(_I if dtype == np.int32 else _D)
+
+
+def runes(case, spectral_file, mcica_file=None):
+    """Fluxes [level][profile] and heating rate [layer][profile] (K/s)."""
+    case.fill_defaults()
+    keep = []
+    cs = _Case()
+    for f, _ in _Case._fields_:
+        v = getattr(case, f)
+        if f == "rand_seed":
+            setattr(cs, f, _ptr(v, keep, np.int32))
+        elif isinstance(v, np.ndarray):
+            setattr(cs, f, _ptr(v, keep))
+        else:
+            setattr(cs, f, v)
+    L, P = case.n_layer, case.n_profile
+    up = np.zeros((L + 1, P)); down = np.zeros((L + 1, P))
+    direct = np.zeros((L + 1, P)); hr = np.zeros((L, P))
+    rc = lib().soc_oracle_runes(spectral_file.encode(),
+                                (mcica_file or "").encode(), ctypes.byref(cs),
+                                _ptr(up, keep), _ptr(down, keep), _ptr(direct, keep),
+                                _ptr(hr, keep))
+    if rc != 0:
+        raise RuntimeError("soc_oracle_runes failed: %d" % rc)
+    return dict(flux_up=up, flux_down=down, flux_direct=direct, heating_rate=hr)
+
+
+def cloud_gen(rand_seed, rlc_cf, rlc_cw, sigma_qcw, avg_cf, p_layer, xcw, n_subcol=64,
+              n_req=64, ipph=1, ioverlap=3):
+    L, P = avg_cf.shape
+    keep = []
+    c_sub = np.zeros((n_subcol, L, P))
+    ncld = np.zeros(P, dtype=np.int32)
+    n1, n2 = xcw.shape
+    lib().soc_oracle_cloud_gen(
+        P, L, n_subcol, n_req, ipph, ioverlap, _ptr(rand_seed, keep, np.int32),
+        _ptr(rlc_cf, keep), _ptr(rlc_cw, keep), _ptr(sigma_qcw, keep), _ptr(avg_cf, keep),
+        _ptr(p_layer, keep), n1, n2, _ptr(np.asfortranarray(xcw).ravel(order="F"), keep),
+        _ptr(c_sub, keep), _ptr(ncld, keep, np.int32))
+    return c_sub, ncld

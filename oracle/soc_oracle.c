@@ -1,0 +1,936 @@
+/* CPU restatement of the SOCRATES two-stream flux path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the parity tests and the CPU
+ * baseline of the benchmark. Nothing in the product links or calls it.
+ * Build: oracle/Makefile (gcc -O2 -fno-fast-math -ffp-contract=off).
+ *
+ * Each routine below restates one reference routine (cited file:line, paths
+ * relative to the reference tree) keeping its operation order, so that the
+ * results agree with the compiled reference to rounding.
+ */
+#include "soc_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NB 16      /* bands */
+#define NG 8       /* gases per file */
+#define NK 32      /* k-terms per gas and band */
+#define NDROP 16   /* droplet types */
+#define NPARM 20   /* parameters per droplet fit */
+
+/* option codes, rad_pcf.F90 */
+enum { IP_SOLAR = 1, IP_INFRA_RED = 2 };
+enum { IP_H2O = 1, IP_CO2 = 2, IP_O3 = 3 };
+enum { IP_SCALE_TERM = 2, IP_SCALE_POWER_LAW = 1 };
+enum { IP_DROP_PADE_2 = 5, IP_SLINGO_SCHRECKER = 1 };
+
+static const double ELSASSER = 1.66;  /* diffusivity_factor.F90 */
+static const double DIFF_MINOR = 1.66;
+
+/* ------------------------------------------------------------------------ */
+/* Spectral data: the subset of StrSpecData (def_spectrum.F90:33-472) that   */
+/* the two-stream flux path reads.                                           */
+typedef struct {
+  int n_band, n_absorb;
+  int type_absorb[NG];
+  double solar_flux_band[NB];
+  double rayleigh_coeff[NB];
+  int n_band_absorb[NB];
+  int index_absorb[NB][NG]; /* 1-based gas index */
+  int i_band_k[NB][NG], i_scale_k[NB][NG], i_scale_fnc[NB][NG];
+  double p_ref[NB][NG], t_ref[NB][NG];
+  double k[NB][NG][NK], w[NB][NG][NK], scale[NB][NG][NK][2];
+  int n_deg_fit;
+  double t_ref_planck;
+  double thermal_coeff[NB][8];
+  int l_drop_type[NDROP], i_drop_parm[NDROP], n_drop_parm[NDROP];
+  double drop_min[NDROP], drop_max[NDROP];
+  double drop_parm[NDROP][NB][NPARM];
+} spec_t;
+
+typedef struct {
+  int n_subcol_gen, n_req_single, n_req_optimal, ipph;
+  int reorder_single[512], reorder_optimal[512];
+  int sw_subcol_k[NB][NK], lw_subcol_k[NB][NK];
+  int n1, n2;
+  double *xcw; /* n1 x n2, column-major */
+} mcica_t;
+
+static int block_header(const char *ln, int *type, int *sub, int *ver) {
+  /* '(a15, i4, 12x, i4, 12x, i4)' read_spectrum.F90:218 */
+  char buf[8];
+  if (strncmp(ln, "*BLOCK", 6) != 0 || strlen(ln) < 51) return 0;
+  memcpy(buf, ln + 15, 4); buf[4] = 0; *type = atoi(buf);
+  memcpy(buf, ln + 31, 4); buf[4] = 0; *sub = atoi(buf);
+  memcpy(buf, ln + 47, 4); buf[4] = 0; *ver = atoi(buf);
+  return 1;
+}
+
+static double fld(const char *ln, int c0, int w) {
+  char buf[64];
+  int n = (int)strlen(ln);
+  if (c0 >= n) return 0.0;
+  if (c0 + w > n) w = n - c0;
+  memcpy(buf, ln + c0, (size_t)w); buf[w] = 0;
+  return strtod(buf, NULL);
+}
+
+static char *next_line(FILE *f, char *ln, int n) {
+  char *r = fgets(ln, n, f);
+  if (r) { size_t m = strlen(ln); while (m && (ln[m-1] == '\n' || ln[m-1] == '\r')) ln[--m] = 0; }
+  return r;
+}
+
+/* Text readers for the blocks used by the synthetic files: a restatement of
+ * read_spectrum.F90 block readers 0_0_1 (557-621), 2_0_0 (764-789),
+ * 3_0_0 (872-900), 4_0_0 (951-992), 5_0_1 (1218-1641, power-law terms),
+ * 6_0_0 (1746-1777), 10_0_2 (2245-2331). Other blocks are skipped. */
+static int read_spectrum(const char *path, spec_t *s) {
+  FILE *f = fopen(path, "r");
+  char ln[512];
+  int t, st, v;
+  if (!f) return -1;
+  memset(s, 0, sizeof(*s));
+  while (next_line(f, ln, sizeof ln)) {
+    if (!block_header(ln, &t, &st, &v)) continue;
+    if (t == 0) {
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln); s->n_band = (int)fld(ln, 27, 5);
+      next_line(f, ln, sizeof ln); s->n_absorb = (int)fld(ln, 36, 5);
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      for (int i = 0; i < s->n_absorb; i++) {
+        next_line(f, ln, sizeof ln);
+        s->type_absorb[i] = (int)fld(ln, 12, 5);
+      }
+    } else if (t == 2) {
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++) {
+        next_line(f, ln, sizeof ln); s->solar_flux_band[b] = fld(ln, 12, 16);
+      }
+    } else if (t == 3) {
+      for (int j = 0; j < 3; j++) next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++) {
+        next_line(f, ln, sizeof ln); s->rayleigh_coeff[b] = fld(ln, 12, 16);
+      }
+    } else if (t == 4) {
+      for (int j = 0; j < 5; j++) next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++) {
+        next_line(f, ln, sizeof ln);
+        s->n_band_absorb[b] = (int)fld(ln, 12, 5);
+        int got = 0, first = 1;
+        while (got < s->n_band_absorb[b]) {
+          next_line(f, ln, sizeof ln);
+          int c0 = first ? 5 : 0;
+          first = 0;
+          for (int q = 0; q < 4 && got < s->n_band_absorb[b]; q++)
+            s->index_absorb[b][got++] = (int)fld(ln, c0 + 5 * q + 2, 3);
+        }
+      }
+    } else if (t == 5) {
+      while (next_line(f, ln, sizeof ln))
+        if (strncmp(ln, "Band        Gas, Number of k-terms", 34) == 0) break;
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++)
+        for (int j = 0; j < s->n_band_absorb[b]; j++) {
+          next_line(f, ln, sizeof ln);
+          int band = (int)fld(ln, 0, 5) - 1, gas = (int)fld(ln, 12, 5) - 1;
+          int nt = (int)fld(ln, 24, 5);
+          s->i_band_k[band][gas] = nt;
+          s->i_scale_k[band][gas] = (int)fld(ln, 36, 5);
+          s->i_scale_fnc[band][gas] = (int)fld(ln, 48, 5);
+          next_line(f, ln, sizeof ln);
+          s->p_ref[band][gas] = fld(ln, 6, 16);
+          s->t_ref[band][gas] = fld(ln, 28, 16);
+          for (int k = 0; k < nt; k++) {
+            next_line(f, ln, sizeof ln);
+            s->k[band][gas][k] = fld(ln, 3, 16);
+            s->w[band][gas][k] = fld(ln, 22, 16);
+            s->scale[band][gas][k][0] = fld(ln, 41, 16);
+            s->scale[band][gas][k][1] = fld(ln, 60, 16);
+          }
+        }
+    } else if (t == 6) {
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln);
+      s->n_deg_fit = (int)fld(ln, 23, 5);
+      s->t_ref_planck = fld(ln, 54, 16);
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++) {
+        int got = 0;
+        double vals[16];
+        while (got < s->n_deg_fit + 2) {
+          next_line(f, ln, sizeof ln);
+          char *p = ln;
+          char *e;
+          if (got == 0) { strtol(p, &e, 10); p = e; vals[got++] = 0; }
+          for (;;) {
+            double x = strtod(p, &e);
+            if (e == p) break;
+            vals[got++] = x;
+            p = e;
+          }
+        }
+        for (int k = 0; k <= s->n_deg_fit; k++) s->thermal_coeff[b][k] = vals[k + 1];
+      }
+    } else if (t == 10) {
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln); int idrop = (int)fld(ln, 27, 5);
+      next_line(f, ln, sizeof ln);
+      int iparm = (int)fld(ln, 34, 5), npar = (int)fld(ln, 66, 5);
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln);
+      s->drop_min[idrop] = fld(ln, 39, 12);
+      s->drop_max[idrop] = fld(ln, 55, 12);
+      s->l_drop_type[idrop] = 1;
+      s->i_drop_parm[idrop] = iparm;
+      s->n_drop_parm[idrop] = npar;
+      for (int b = 0; b < s->n_band; b++) {
+        next_line(f, ln, sizeof ln);
+        int got = 0;
+        while (got < npar) {
+          next_line(f, ln, sizeof ln);
+          char *p = ln, *e;
+          for (;;) {
+            double x = strtod(p, &e);
+            if (e == p) break;
+            s->drop_parm[idrop][b][got++] = x;
+            p = e;
+          }
+        }
+      }
+    }
+  }
+  fclose(f);
+  return s->n_band > 0 ? 0 : -2;
+}
+
+/* read_mcica_data, def_mcica.F90:61-190 */
+static int read_mcica(const char *path, mcica_t *m) {
+  FILE *f = fopen(path, "r");
+  char ln[512];
+  if (!f) return -1;
+  memset(m, 0, sizeof(*m));
+  for (int b = 0; b < NB; b++)
+    for (int k = 0; k < NK; k++) { m->sw_subcol_k[b][k] = 1; m->lw_subcol_k[b][k] = 1; }
+  while (next_line(f, ln, sizeof ln))
+    if (strncmp(ln, "*DATA", 5) == 0) break;
+  while (next_line(f, ln, sizeof ln)) {
+    if (strncmp(ln, "*END", 4) == 0) break;
+    if (!strcmp(ln, "tot_subcol_gen") || !strcmp(ln, "n_subcol_gen")) {
+      if (fscanf(f, "%d", &m->n_subcol_gen) != 1) return -3;
+    } else if (!strcmp(ln, "subcol_need_single") || !strcmp(ln, "n_subcol_req_single")) {
+      if (fscanf(f, "%d", &m->n_req_single) != 1) return -3;
+    } else if (!strcmp(ln, "subcol_need_optimal") || !strcmp(ln, "n_subcol_req_optimal")) {
+      if (fscanf(f, "%d", &m->n_req_optimal) != 1) return -3;
+    } else if (!strcmp(ln, "ipph")) {
+      if (fscanf(f, "%d", &m->ipph) != 1) return -3;
+    } else if (!strcmp(ln, "lw_subcol_reorder_single")) {
+      for (int i = 0; i < m->n_req_single; i++)
+        if (fscanf(f, "%d", &m->reorder_single[i]) != 1) return -3;
+    } else if (!strcmp(ln, "lw_subcol_reorder_optimal")) {
+      for (int i = 0; i < m->n_req_optimal; i++)
+        if (fscanf(f, "%d", &m->reorder_optimal[i]) != 1) return -3;
+    } else if (!strcmp(ln, "sw_subcol_k") || !strcmp(ln, "lw_subcol_k")) {
+      int sw = ln[0] == 's';
+      for (;;) {
+        int b, k, n;
+        next_line(f, ln, sizeof ln);
+        if (sscanf(ln, "%d %d %d", &b, &k, &n) != 3) return -3;
+        if (b == -99) break;
+        if (sw) m->sw_subcol_k[b - 1][k - 1] = n; else m->lw_subcol_k[b - 1][k - 1] = n;
+      }
+    } else if (!strcmp(ln, "n1")) {
+      if (fscanf(f, "%d", &m->n1) != 1) return -3;
+    } else if (!strcmp(ln, "n2")) {
+      if (fscanf(f, "%d", &m->n2) != 1) return -3;
+    } else if (!strcmp(ln, "xcw")) {
+      m->xcw = (double *)malloc(sizeof(double) * (size_t)m->n1 * (size_t)m->n2);
+      for (int i = 0; i < m->n1 * m->n2; i++)
+        if (fscanf(f, "%lf", &m->xcw[i]) != 1) return -3;
+    }
+    /* list-directed READs leave the rest of the record: skip to a new line */
+    if (ln[0] && strcmp(ln, "*END")) {
+      int ch = fgetc(f);
+      if (ch != '\n' && ch != EOF) ungetc(ch, f);
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* planck_flux_band, planck_flux_band_mod.F90:18-49 (polynomial form)        */
+static double planck_flux_band(const spec_t *s, int b, double temperature) {
+  double t_ratio = temperature / s->t_ref_planck;
+  double v = s->thermal_coeff[b][s->n_deg_fit];
+  for (int j = s->n_deg_fit - 1; j >= 0; j--) v = v * t_ratio + s->thermal_coeff[b][j];
+  return v;
+}
+
+/* two_coeff_basic (two_coeff_basic.F90:108-161), lambda (two_coeff.F90:
+ * 158-163), solar_coefficient_basic (solar_coefficient_basic.F90:124-219),
+ * trans_source_coeff (trans_source_coeff.F90:142-263), for one layer. */
+typedef struct { double trans, reflect, trans_0, sc1, sc2; } coeff_t;
+
+static void two_coeff_one(int isolir, double tau, double omega, double asymmetry,
+                          double sec_0, coeff_t *o) {
+  const double eps = DBL_EPSILON;
+  double sum, diff, lambda;
+  if (isolir == IP_SOLAR) {
+    /* ip_pifm80 */
+    sum = 2.0 - 1.5 * omega * asymmetry - 0.5 * omega;
+    diff = 2.0 * (1.0 - omega);
+  } else {
+    /* ip_elsasser */
+    sum = ELSASSER - 1.5 * omega * asymmetry;
+    diff = ELSASSER * (1.0 - omega);
+  }
+  double temp = sum * diff;
+  lambda = sqrt(temp);
+  double gamma_up = 0.0, gamma_down = 0.0;
+  if (isolir == IP_SOLAR) {
+    double tol_perturb = 3.2e+01 * eps;
+    if (fabs(lambda - sec_0) < tol_perturb) {
+      sum = (1.0 + tol_perturb) * sum;
+      diff = (1.0 + tol_perturb) * diff;
+      lambda = (1.0 + tol_perturb) * lambda;
+    }
+    double ksi_0 = 1.5 * asymmetry / sec_0;
+    double factor = 0.5 * omega * sec_0 / ((lambda - sec_0) * (lambda + sec_0));
+    gamma_up = factor * (sum - sec_0 - ksi_0 * (diff - sec_0));
+    gamma_down = factor * (sum + sec_0 + ksi_0 * (diff + sec_0));
+  }
+  double sq_eps_r = sqrt(eps);
+  double tol = sqrt(sq_eps_r);
+  double exponential = exp(-lambda * tau);
+  double exponential2 = exponential * exponential;
+  double gamma = (sum - lambda) / (sum + lambda);
+  double gamma2 = gamma * gamma;
+  double tmp_inv = 1.0 / (1.0 - exponential2 * gamma2);
+  o->trans = exponential * (1.0 - gamma2) * tmp_inv;
+  o->reflect = gamma * (1.0 - exponential2) * tmp_inv;
+  if (isolir == IP_SOLAR) {
+    o->trans_0 = exp(-tau * sec_0);
+    o->sc1 = (gamma_up - o->reflect * (1.0 + gamma_down)) - gamma_up * o->trans * o->trans_0;
+    o->sc2 = o->trans_0 * (1.0 + gamma_down - gamma_up * o->reflect) - (1.0 + gamma_down) * o->trans;
+  } else {
+    o->trans_0 = 0.0;
+    o->sc1 = (1.0 - o->trans + o->reflect + sq_eps_r) / (sq_eps_r + tau * sum);
+    double sc2;
+    if (tau > tol)
+      sc2 = -2.0 * (1.0 - o->trans - o->reflect + sq_eps_r) / (diff * tau + sq_eps_r);
+    else
+      sc2 = -2.0 + diff * tau;
+    o->sc2 = -(1.0 + o->reflect + o->trans + sc2) / (sum * tau + sq_eps_r);
+  }
+}
+
+void soc_oracle_two_coeff(int n, int isolir, const double *tau, const double *omega,
+                          const double *asymmetry, const double *sec_0, double *trans,
+                          double *reflect, double *trans_0, double *sc1, double *sc2) {
+  for (int i = 0; i < n; i++) {
+    coeff_t o;
+    two_coeff_one(isolir, tau[i], omega[i], asymmetry[i], sec_0 ? sec_0[i] : 1.0, &o);
+    trans[i] = o.trans; reflect[i] = o.reflect; trans_0[i] = o.trans_0;
+    sc1[i] = o.sc1; sc2[i] = o.sc2;
+  }
+}
+
+/* solver_homogen_direct.F90:93-127 for one column; flux_total 1-based
+ * index k is stored at flux_total[k-1]. */
+void soc_oracle_solver_homogen_direct(int n_layer, const double *trans,
+                                      const double *reflect, const double *s_down,
+                                      const double *s_up, double diffuse_albedo,
+                                      double flux_inc_down, double source_ground,
+                                      double *flux_total) {
+  double *beta = (double *)malloc(sizeof(double) * 3 * (size_t)(n_layer + 1));
+  double *gam = beta + (n_layer + 1), *h = gam + (n_layer + 1);
+  double alpha = diffuse_albedo, s_up_prime = source_ground;
+  for (int i = n_layer; i >= 1; i--) {
+    beta[i] = 1.0 / (1.0 - alpha * reflect[i - 1]);
+    gam[i] = alpha * trans[i - 1];
+    h[i] = s_up_prime + alpha * s_down[i - 1];
+    alpha = reflect[i - 1] + beta[i] * gam[i] * trans[i - 1];
+    s_up_prime = s_up[i - 1] + beta[i] * trans[i - 1] * h[i];
+  }
+  flux_total[1] = flux_inc_down;
+  flux_total[0] = alpha * flux_total[1] + s_up_prime;
+  for (int i = 1; i <= n_layer; i++) {
+    flux_total[2 * i] = beta[i] * (h[i] + gam[i] * flux_total[2 * i - 1]);
+    flux_total[2 * i + 1] = s_down[i - 1] + trans[i - 1] * flux_total[2 * i - 1]
+                            + reflect[i - 1] * flux_total[2 * i];
+  }
+  free(beta);
+}
+
+/* ------------------------------------------------------------------------ */
+/* cloud_gen, socrates_cloud_gen.F90:54-478 (ioverlap 1-3 and 0)            */
+#define LCG_M 86436
+#define LCG_A 1093
+#define LCG_C 18257
+
+static inline int lcg(int x) {
+  const double rm = 1.0 / (double)LCG_M;
+  int temp = x * LCG_A + LCG_C;
+  return temp - (int)((double)temp * rm) * LCG_M;
+}
+
+static double zcw_lookup(double y, double sigma, int n1, int n2, const double *xcw) {
+  double rind1 = y * (double)(n1 - 1) + 1.0;
+  int ind1 = (int)rind1;
+  if (ind1 > n1 - 1) ind1 = n1 - 1;
+  if (ind1 < 1) ind1 = 1;
+  rind1 = rind1 - (double)ind1;
+  rind1 = rind1 < 1.0 ? rind1 : 1.0;
+  rind1 = rind1 > 0.0 ? rind1 : 0.0;
+  double rind2 = 40.0 * sigma - 3.0;
+  int ind2 = (int)rind2;
+  if (ind2 > n2 - 1) ind2 = n2 - 1;
+  if (ind2 < 1) ind2 = 1;
+  rind2 = rind2 - (double)ind2;
+  rind2 = rind2 < 1.0 ? rind2 : 1.0;
+  rind2 = rind2 > 0.0 ? rind2 : 0.0;
+#define XCW(a, b) xcw[((a) - 1) + (size_t)((b) - 1) * (size_t)n1]
+  return (1.0 - rind1) * (1.0 - rind2) * XCW(ind1, ind2)
+       + (1.0 - rind1) * rind2 * XCW(ind1, ind2 + 1)
+       + rind1 * (1.0 - rind2) * XCW(ind1 + 1, ind2)
+       + rind1 * rind2 * XCW(ind1 + 1, ind2 + 1);
+#undef XCW
+}
+
+int soc_oracle_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ipph,
+                         int ioverlap, const int *rand_seed, const double *rlc_cf,
+                         const double *rlc_cw, const double *sigma_qcw,
+                         const double *avg_cf, const double *p_layer, int n1, int n2,
+                         const double *xcw, double *c_sub, int *n_subcol_cld) {
+  const int P = n_profile, L = n_layer;
+  const int rand_seed_max = (2147483647 - LCG_C) / LCG_A;
+  const double rm = 1.0 / (double)LCG_M;
+  const double cut = DBL_EPSILON;
+#define AT(a, i, l) (a)[(size_t)((i) - 1) * P + (l)]
+#define CSUB(s, i, l) c_sub[((size_t)((s) - 1) * L + (size_t)((i) - 1)) * P + (l)]
+  memset(c_sub, 0, sizeof(double) * (size_t)n_subcol * L * P);
+  /* cloud top / base over the whole batch (cloud_gen:224-245) */
+  int k_top = 0, k_base = 0;
+  for (int k = 1; k <= L && !k_top; k++)
+    for (int l = 0; l < P; l++) if (AT(avg_cf, k, l) > cut) { k_top = k; break; }
+  if (k_top == 0) {
+    for (int l = 0; l < P; l++) n_subcol_cld[l] = 0;
+    return 0;
+  }
+  for (int k = L; k >= 1 && !k_base; k--)
+    for (int l = 0; l < P; l++) if (AT(avg_cf, k, l) >= cut) { k_base = k; break; }
+  double *alpha = (double *)calloc((size_t)(L + 1), sizeof(double));
+  double *rcorr = (double *)calloc((size_t)(L + 1), sizeof(double));
+  int *seed_x = (int *)malloc(sizeof(int) * (size_t)n_subcol);
+  int *seed_y = (int *)malloc(sizeof(int) * (size_t)n_subcol);
+  for (int l = 0; l < P; l++) {
+    int rs = abs(rand_seed[l]) % rand_seed_max;
+    for (int i = 0; i < n_subcol; i++) rs = lcg(rs);
+    for (int i = 0; i < n_subcol; i++) { rs = lcg(rs); seed_x[i] = rs; }
+    for (int i = 0; i < n_subcol; i++) { rs = lcg(rs); seed_y[i] = rs; }
+    /* overlap factors (cloud_gen:339-376); cloud_top = 1 here */
+    for (int k = k_top - 1; k <= k_base - 1; k++) {
+      if (ioverlap == 1 || k < 1) { alpha[k] = 0.0; rcorr[k] = 0.0; continue; }
+      if (ioverlap == 2 || AT(avg_cf, k, l) > 0.0) {
+        alpha[k] = AT(rlc_cf, k, l) > 0.0
+          ? exp((AT(p_layer, k, l) - AT(p_layer, k + 1, l)) / AT(rlc_cf, k, l)) : 0.0;
+        rcorr[k] = AT(rlc_cw, k, l) > 0.0
+          ? exp((AT(p_layer, k, l) - AT(p_layer, k + 1, l)) / AT(rlc_cw, k, l)) : 0.0;
+      } else {
+        alpha[k] = 0.0; rcorr[k] = 0.0;
+      }
+    }
+    int i_loc = 1;
+    double x = 0.0, y = 0.0;
+    for (int i = 0; i < n_subcol; i++) {
+      int int_x = seed_x[i], int_y = seed_y[i];
+      int l_cld = 0;
+      for (int k = k_top; k <= k_base; k++) {
+        double cf = AT(avg_cf, k, l);
+        if (!(cf > 0.0)) continue;
+        if (ioverlap == 0) {
+          int_x = lcg(int_x); x = (double)int_x * rm;
+          int_y = lcg(int_y); y = (double)int_y * rm;
+          if (k == 1) {
+            int_x = lcg(int_x); x = (double)int_x * rm;
+            int_y = lcg(int_y); y = (double)int_y * rm;
+          } else if (x < (1.0 - AT(avg_cf, k - 1, l))) {
+            int_x = lcg(int_x); double x1 = (double)int_x * rm;
+            int_y = lcg(int_y); double y1 = (double)int_y * rm;
+            x = x1 * (1.0 - AT(avg_cf, k - 1, l));
+            y = y1;
+          }
+        } else {
+          int_x = lcg(int_x); double x1 = (double)int_x * rm;
+          int_y = lcg(int_y); double y1 = (double)int_y * rm;
+          if (x1 >= alpha[k - 1]) { int_x = lcg(int_x); x = (double)int_x * rm; }
+          if (y1 >= rcorr[k - 1]) { int_y = lcg(int_y); y = (double)int_y * rm; }
+        }
+        if (x < cf) {
+          double zcw;
+          if (ipph == 0) zcw = 1.0;
+          else zcw = zcw_lookup(y, AT(sigma_qcw, k, l), n1, n2, xcw) * 1.0; /* ls_ratio = 1 */
+          l_cld = 1;
+          CSUB(i_loc, k, l) = zcw;
+        }
+      }
+      if (l_cld) i_loc++;
+    }
+    n_subcol_cld[l] = i_loc - 1;
+    /* fill of set_cld_mcica (socrates_set_cld_mcica.F90:202-220) */
+    int n_fill = n_req < n_subcol ? n_req : n_subcol;
+    int nc = n_subcol_cld[l];
+    if (nc < n_fill && nc > 0)
+      for (int j = nc + 1; j <= n_fill; j++)
+        for (int k = 1; k <= L; k++) CSUB(j, k, l) = CSUB(j - nc, k, l);
+  }
+  free(alpha); free(rcorr); free(seed_x); free(seed_y);
+#undef AT
+#undef CSUB
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* runes(): socrates_runes.F90:47-1079 restricted to the flux path.          */
+
+typedef struct {
+  int L;
+  double *trans, *reflect, *trans_0, *sc1, *sc2, *s_up, *s_down, *flux_direct,
+      *flux_total;
+} work_t;
+
+/* column_solver for the homogeneous direct solver (column_solver.F90:160-243)
+ * with solar_source (solar_source.F90:104-209, l_scale_solar) or ir_source
+ * (ir_source.F90:93-120). Coefficients are in w; fluxes out in w. */
+static void solve_column(int isolir, work_t *w, const double *adjust, const double *dplanck,
+                         const double *dplanck2, double flux_inc_direct, double flux_inc_down,
+                         double alb_diff, double alb_dir, double d_planck_flux_surface) {
+  const int L = w->L;
+  double source_ground;
+  if (isolir == IP_SOLAR) {
+    w->flux_direct[0] = flux_inc_direct;
+    for (int i = 1; i <= L; i++) {
+      w->flux_direct[i] = w->flux_direct[i - 1] * w->trans_0[i - 1] * adjust[i - 1];
+      w->s_up[i - 1] = w->sc1[i - 1] * w->flux_direct[i - 1];
+      w->s_down[i - 1] = (w->sc2[i - 1] - w->trans_0[i - 1]) * w->flux_direct[i - 1]
+                         + w->flux_direct[i];
+    }
+    source_ground = (alb_dir - alb_diff) * w->flux_direct[L];
+  } else {
+    for (int i = 0; i < L; i++) {
+      w->s_up[i] = w->sc1[i] * dplanck[i] + w->sc2[i] * dplanck2[i];
+      w->s_down[i] = -w->sc1[i] * dplanck[i] + w->sc2[i] * dplanck2[i];
+    }
+    source_ground = (1.0 - alb_diff) * d_planck_flux_surface;
+  }
+  soc_oracle_solver_homogen_direct(L, w->trans, w->reflect, w->s_down, w->s_up, alb_diff,
+                                   flux_inc_down, source_ground, w->flux_total);
+}
+
+/* Padé-2 droplet optics, opt_prop_pade_2_mod.F90:100-143 */
+static void pade_2(const double *cp, double mmr, double re, double *k_ext_tot,
+                   double *k_ext_scat, double *phf, double *fwd) {
+  *k_ext_tot = mmr * (cp[0] + re * (cp[1] + re * cp[2]))
+             / (1.0 + re * (cp[3] + re * (cp[4] + re * cp[5])));
+  *k_ext_scat = *k_ext_tot * (1.0 - (cp[6] + re * (cp[7] + re * cp[8]))
+                               / (1.0 + re * (cp[9] + re * cp[10])));
+  double asym = (cp[11] + re * (cp[12] + re * cp[13])) / (1.0 + re * (cp[14] + re * cp[15]));
+  *phf = *k_ext_scat * asym;
+  *fwd = *phf * asym;
+}
+
+int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const soc_case *c,
+                     double *flux_up, double *flux_down, double *flux_direct,
+                     double *heating_rate) {
+  static spec_t sp;
+  static mcica_t mc;
+  const int P = c->n_profile, L = c->n_layer;
+  const int isolir = c->i_source;
+  const double eps = DBL_EPSILON, tiny = DBL_MIN, huge = DBL_MAX;
+  int rc = read_spectrum(spectral_file, &sp);
+  if (rc) return rc;
+  const int NBAND = sp.n_band;
+  int mcica = c->l_cloud && c->i_inhom == 2;
+  if (c->l_cloud && !mcica) return -10; /* triple-column restated separately */
+  if (mcica) {
+    rc = read_mcica(mcica_file, &mc);
+    if (rc) return rc;
+  }
+#define LAY(a, i, l) (a)[(size_t)(i) * P + (l)]     /* i: 0-based layer */
+  /* --- set_cld / set_cld_dim / set_cld_mcica inputs (McICA) ----------- */
+  double *w_cloud = NULL, *cond_mmr = NULL, *re = NULL, *c_sub = NULL, *frac_cloudy = NULL;
+  int *n_subcol_cld = NULL, n_req = 1, n_gen = 1, *subcol_reorder = NULL;
+  int first_subcol_k[NB][NK + 1];
+  if (mcica) {
+    const double min_cloud_fraction = 0.001;
+    w_cloud = (double *)calloc((size_t)L * P, sizeof(double));
+    cond_mmr = (double *)calloc((size_t)L * P, sizeof(double));
+    re = (double *)calloc((size_t)L * P, sizeof(double));
+    double *rsd = (double *)calloc((size_t)L * P, sizeof(double));
+    double *dp = (double *)calloc((size_t)L * P, sizeof(double));
+    double *dpc = (double *)calloc((size_t)L * P, sizeof(double));
+    const int it = c->i_st_water;
+    for (int i = 0; i < L; i++)
+      for (int l = 0; l < P; l++) {
+        double frac = fmax(LAY(c->cloud_frac, i, l), 0.0);
+        double mmr = fmax(LAY(c->liq_mmr, i, l), 0.0);
+        /* in-cloud mixing ratio (socrates_set_cld.F90:817-826) */
+        double m_in = frac > eps ? mmr / frac : 0.0;
+        /* w_cloud and fraction normalisation (socrates_set_cld.F90:828-852) */
+        double wc = frac, fr1 = frac;
+        if (wc > min_cloud_fraction) fr1 = fr1 / wc;
+        else { wc = 0.0; fr1 = 0.0; }
+        if (wc > 1.0 + min_cloud_fraction) return -11;
+        else if (wc > 1.0) wc = 1.0;
+        LAY(w_cloud, i, l) = wc;
+        LAY(cond_mmr, i, l) = m_in;
+        /* effective radius clamp (socrates_set_cld_dim.F90:214-220) */
+        double r = fmax(LAY(c->liq_dim, i, l), 0.0);
+        LAY(re, i, l) = fmin(fmax(r, sp.drop_min[it]), sp.drop_max[it]);
+        /* cond_rsd (socrates_set_cld_mcica.F90:168-180): liquid, then the ice
+         * component whose mixing ratio is zero; both carry the horizontal rsd */
+        double wl = fmax(eps, m_in * fr1);
+        double wi = fmax(eps, 0.0 * fr1);
+        double sum_r = 0.0 + c->cloud_horizontal_rsd * wl;
+        sum_r = sum_r + c->cloud_horizontal_rsd * wi;
+        double sum_w = 0.0 + wl;
+        sum_w = sum_w + wi;
+        LAY(rsd, i, l) = sum_r / sum_w;
+        LAY(dp, i, l) = c->cloud_vertical_decorr;
+        LAY(dpc, i, l) = c->cloud_vertical_decorr * 0.5;
+      }
+    n_gen = c->n_subcol_gen > 0 ? c->n_subcol_gen : mc.n_subcol_gen;
+    n_req = mc.n_req_optimal;
+    /* sampling tables (socrates_set_cld_mcica.F90:103-151) */
+    int (*subk)[NK] = isolir == IP_SOLAR ? mc.sw_subcol_k : mc.lw_subcol_k;
+    first_subcol_k[0][0] = 1;
+    for (int b = 0; b < NBAND; b++) {
+      int maj = sp.index_absorb[b][0] - 1;
+      int nk = sp.i_band_k[b][maj];
+      for (int k = 0; k < nk; k++) first_subcol_k[b][k + 1] = first_subcol_k[b][k] + subk[b][k];
+      if (b < NBAND - 1) first_subcol_k[b + 1][0] = first_subcol_k[b][nk];
+    }
+    subcol_reorder = (int *)malloc(sizeof(int) * (size_t)(n_req + 1));
+    for (int l = 1; l <= n_req; l++) {
+      int v = isolir == IP_SOLAR ? l % n_gen : mc.reorder_optimal[l - 1] % n_gen;
+      subcol_reorder[l] = v == 0 ? n_gen : v;
+    }
+    c_sub = (double *)calloc((size_t)n_gen * L * P, sizeof(double));
+    n_subcol_cld = (int *)calloc((size_t)P, sizeof(int));
+    soc_oracle_cloud_gen(P, L, n_gen, n_req, mc.ipph, c->i_overlap, c->rand_seed, dp, dpc,
+                         rsd, w_cloud, c->p_layer, mc.n1, mc.n2, mc.xcw, c_sub, n_subcol_cld);
+    frac_cloudy = (double *)malloc(sizeof(double) * (size_t)P);
+    for (int l = 0; l < P; l++) frac_cloudy[l] = (double)n_subcol_cld[l] / (double)n_gen;
+    free(rsd); free(dp); free(dpc);
+  }
+
+  /* --- per-column radiative transfer ------------------------------------ */
+  work_t w, wc;
+  double *buf = (double *)malloc(sizeof(double) * (size_t)(30 * (L + 2) + 16 * (2 * L + 2)));
+  double *q = buf;
+#define TAKE(n) (q += (n), q - (n))
+  w.L = wc.L = L;
+  w.trans = TAKE(L); w.reflect = TAKE(L); w.trans_0 = TAKE(L); w.sc1 = TAKE(L);
+  w.sc2 = TAKE(L); w.s_up = TAKE(L); w.s_down = TAKE(L); w.flux_direct = TAKE(L + 1);
+  w.flux_total = TAKE(2 * L + 2);
+  wc.trans = TAKE(L); wc.reflect = TAKE(L); wc.trans_0 = TAKE(L); wc.sc1 = TAKE(L);
+  wc.sc2 = TAKE(L); wc.s_up = TAKE(L); wc.s_down = TAKE(L); wc.flux_direct = TAKE(L + 1);
+  wc.flux_total = TAKE(2 * L + 2);
+  double *k_grey_tot = TAKE(L), *k_ext_scat = TAKE(L), *keqv = TAKE(L), *kgrey = TAKE(L);
+  double *adjust = TAKE(L), *kmin = TAKE(L), *tau_clr = TAKE(L), *om_clr = TAKE(L);
+  double *planck = TAKE(L + 1), *dpl = TAKE(L), *dpl2 = TAKE(L), *flux_gas = TAKE(L + 1);
+  double *kc_tot = TAKE(L), *kc_scat = TAKE(L), *kc_phf = TAKE(L), *kc_fwd = TAKE(L);
+  double *sum_flux = TAKE(2 * L + 3), *sum_k_flux = TAKE(2 * L + 3);
+  double *ft = TAKE(2 * L + 2), *tg = TAKE(L), *src = TAKE(L);
+  double *acc_total = TAKE(2 * L + 2), *acc_direct = TAKE(L + 1);
+  double *clr_total = TAKE(2 * L + 2), *clr_direct = TAKE(L + 1);
+  double *up = TAKE(L + 1), *down = TAKE(L + 1), *dir = TAKE(L + 1);
+  double *kabs = (double *)malloc(sizeof(double) * (size_t)NG * NK * L);
+#define KABS(j, k, i) kabs[((size_t)(j) * NK + (k)) * L + (i)]
+  const double temp_max = log(1.0 / eps);
+
+  for (int l = 0; l < P; l++) {
+    double zen_0 = 1.0, sol = 0.0;
+    if (isolir == IP_SOLAR) {
+      /* set_bound (socrates_set_bound.F90:611-618) */
+      if (c->cos_zen[l] > 0.0) { sol = c->solar_irrad[l]; zen_0 = 1.0 / c->cos_zen[l]; }
+      else { sol = 0.0; zen_0 = 1.0; }
+    }
+    int first = 1;
+    for (int b = 0; b < NBAND; b++) {
+      const double alb_diff = c->albedo_diff[(size_t)b * P + l];
+      const double alb_dir = c->albedo_dir[(size_t)b * P + l];
+      /* gas absorption per absorber and k-term (radiance_calc.F90:1457-1491,
+       * scale_absorb.F90:137-152) */
+      const int ngas = sp.n_band_absorb[b];
+      for (int j = 0; j < ngas; j++) {
+        int g = sp.index_absorb[b][j] - 1;
+        int type = sp.type_absorb[g];
+        for (int k = 0; k < sp.i_band_k[b][g]; k++) {
+          double p_inv = 1.0 / (sp.p_ref[b][g] + 0.0), t_inv = 1.0 / sp.t_ref[b][g];
+          for (int i = 0; i < L; i++) {
+            double pwk = pow((LAY(c->p_layer, i, l) + 0.0) * p_inv, sp.scale[b][g][k][0]);
+            double twk = pow(LAY(c->t_layer, i, l) * t_inv, sp.scale[b][g][k][1]);
+            double frac = pwk * twk;
+            double mr = type == IP_H2O ? fmax(LAY(c->h2o, i, l), 0.0)
+                      : type == IP_O3 ? fmax(LAY(c->o3, i, l), 0.0) : c->co2_mix_ratio;
+            KABS(j, k, i) = sp.k[b][g][k] * fmax(0.0, frac * mr);
+          }
+        }
+      }
+      /* grey optics: Rayleigh (grey_opt_prop.F90:476-541, 816-835) */
+      for (int i = 0; i < L; i++) {
+        k_ext_scat[i] = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
+        k_grey_tot[i] = 0.0 + k_ext_scat[i];
+      }
+      /* Planck terms (diff_planck_source_mod.F90:100-134) */
+      double flux_ground = 0.0;
+      if (isolir == IP_INFRA_RED) {
+        planck[0] = planck_flux_band(&sp, b, LAY(c->t_level, 0, l));
+        for (int i = 1; i <= L; i++) {
+          planck[i] = planck_flux_band(&sp, b, LAY(c->t_level, i, l));
+          dpl[i - 1] = planck[i] - planck[i - 1];
+        }
+        for (int i = 1; i <= L; i++) {
+          double bt = planck_flux_band(&sp, b, LAY(c->t_layer, i - 1, l));
+          dpl2[i - 1] = 2.0 * (planck[i] + planck[i - 1] - 2.0 * bt);
+        }
+        flux_ground = planck_flux_band(&sp, b, c->t_ground[l]);
+      }
+      /* equivalent extinction of minor gases (solve_band_k_eqv_scl.F90:
+       * 572-1047) */
+      for (int i = 0; i < L; i++) { keqv[i] = 0.0; kgrey[i] = 0.0; adjust[i] = 1.0; }
+      for (int j = 1; j < ngas; j++) {
+        int g = sp.index_absorb[b][j] - 1;
+        int nk = sp.i_band_k[b][g];
+        for (int i = 0; i < L; i++) kmin[i] = huge;
+        if (isolir == IP_SOLAR) {
+          double sflux = 0.0;
+          flux_gas[0] = 1.0;
+          for (int i = 1; i <= L; i++) { flux_gas[i] = 0.0; sum_k_flux[i] = 0.0; }
+          for (int iex = 0; iex < nk; iex++) {
+            double wk = sp.w[b][g][iex];
+            double fdt = wk, fdt_prev;
+            for (int i = 1; i <= L; i++) {
+              double temp = -KABS(j, iex, i - 1) * LAY(c->mass, i - 1, l) * zen_0;
+              fdt_prev = fdt;
+              fdt = fdt_prev * exp(temp);
+              flux_gas[i] = flux_gas[i] + fdt;
+            }
+            for (int i = 1; i <= L; i++) {
+              sum_k_flux[i] = sum_k_flux[i] + KABS(j, iex, i - 1) * fdt;
+              kmin[i - 1] = fmin(kmin[i - 1], KABS(j, iex, i - 1));
+            }
+            sflux = sflux + fdt;
+          }
+          for (int i = 1; i <= L; i++) {
+            if (sflux > 0.0) keqv[i - 1] = keqv[i - 1] + sum_k_flux[i] / sflux;
+            else keqv[i - 1] = keqv[i - 1] + kmin[i - 1];
+            if (flux_gas[i - 1] > 0.0)
+              adjust[i - 1] = adjust[i - 1] * flux_gas[i] / flux_gas[i - 1];
+          }
+        } else {
+          for (int i = 1; i <= 2 * L + 2; i++) { sum_flux[i] = 0.0; sum_k_flux[i] = 0.0; }
+          for (int iex = 0; iex < nk; iex++) {
+            double wk = sp.w[b][g][iex];
+            double flux_inc_down = -planck[0];
+            double dps = flux_ground - planck[L];
+            for (int i = 0; i < L; i++) {
+              tg[i] = KABS(j, iex, i) * LAY(c->mass, i, l);
+              kmin[i] = fmin(kmin[i], KABS(j, iex, i));
+            }
+            /* monochromatic_gas_flux.F90:115-196 (infra-red) */
+            double eps_r = eps, sqe = sqrt(eps_r);
+            for (int i = 0; i < L; i++) {
+              double tr = exp(-DIFF_MINOR * tg[i]);
+              src[i] = (1.0 - tr + sqe) * dpl[i] / (DIFF_MINOR * tg[i] + sqe);
+              tg[i] = tr; /* reuse: transmission */
+            }
+            /* ft: 1-based index k at ft[k-1] */
+            ft[1] = flux_inc_down;
+            for (int i = 1; i <= L; i++) ft[2 * i + 1] = tg[i - 1] * ft[2 * i - 1] + (-src[i - 1]);
+            ft[2 * L] = dps + alb_diff * ft[2 * L + 1];
+            for (int i = L; i >= 1; i--) ft[2 * i - 2] = tg[i - 1] * ft[2 * i] + src[i - 1];
+            for (int i = 2; i <= 2 * L + 1; i++) {
+              sum_k_flux[i] = sum_k_flux[i] + KABS(j, iex, i / 2 - 1) * wk * fabs(ft[i - 1]);
+              sum_flux[i] = sum_flux[i] + wk * fabs(ft[i - 1]);
+            }
+          }
+          for (int i = 1; i <= L; i++) {
+            double lkf = sum_k_flux[2 * i] + sum_k_flux[2 * i + 1];
+            double lf = sum_flux[2 * i] + sum_flux[2 * i + 1];
+            if (lf > 0.0) keqv[i - 1] = keqv[i - 1] + lkf / lf;
+            else keqv[i - 1] = keqv[i - 1] + kmin[i - 1];
+          }
+        }
+      }
+      if (isolir == IP_SOLAR)
+        for (int i = 0; i < L; i++)
+          adjust[i] = adjust[i] * exp(fmin(keqv[i] * LAY(c->mass, i, l) * zen_0, temp_max));
+      for (int i = 0; i < L; i++) k_grey_tot[i] = k_grey_tot[i] + keqv[i] + kgrey[i];
+
+      /* cloud component optics (opt_prop_water_cloud.F90:212-300 with the
+       * Padé-2 branch opt_prop_pade_2_mod.F90:100-143) */
+      if (mcica) {
+        const int it = c->i_st_water;
+        for (int i = 0; i < L; i++) {
+          if (LAY(w_cloud, i, l) > 0.0)
+            pade_2(sp.drop_parm[it][b], LAY(cond_mmr, i, l), LAY(re, i, l), &kc_tot[i],
+                   &kc_scat[i], &kc_phf[i], &kc_fwd[i]);
+          else kc_tot[i] = kc_scat[i] = kc_phf[i] = kc_fwd[i] = 0.0;
+        }
+      }
+
+      /* major k-term loop (solve_band_k_eqv_scl.F90:1051-1337) */
+      const int gmaj = sp.index_absorb[b][0] - 1;
+      const int nkmaj = sp.i_band_k[b][gmaj];
+      for (int iex = 0; iex < nkmaj; iex++) {
+        const double esft_weight = sp.w[b][gmaj][iex];
+        double flux_inc_down, flux_inc_direct = 0.0, dps = 0.0;
+        if (isolir == IP_SOLAR) {
+          double sib = sol * sp.solar_flux_band[b];
+          flux_inc_down = sib / zen_0;
+          flux_inc_direct = sib / zen_0;
+        } else {
+          flux_inc_down = -planck[0];
+          dps = flux_ground - planck[L];
+        }
+        /* single scattering + delta-rescaling, clear (single_scattering.F90:
+         * 114-128, rescale_tau_omega.F90:73-80 with zero forward scatter) */
+        for (int i = 0; i < L; i++) {
+          double k_total = k_grey_tot[i] + KABS(0, iex, i);
+          double tau = k_total * LAY(c->mass, i, l);
+          double om = k_ext_scat[i] / (k_total + tiny);
+          om = fmin(om, 1.0 - 3.2e+01 * eps);
+          tau = tau * (1.0 - om * 0.0);
+          om = om * (1.0 - 0.0) / (1.0 - om * 0.0);
+          tau_clr[i] = tau; om_clr[i] = om;
+        }
+        /* clear coefficients (two_stream / mcica_column.F90:266-298) */
+        for (int i = 0; i < L; i++) {
+          coeff_t o;
+          two_coeff_one(isolir, tau_clr[i], om_clr[i], 0.0, zen_0, &o);
+          w.trans[i] = o.trans; w.reflect[i] = o.reflect; w.trans_0[i] = o.trans_0;
+          w.sc1[i] = o.sc1; w.sc2[i] = o.sc2;
+        }
+        solve_column(isolir, &w, adjust, dpl, dpl2, flux_inc_direct, flux_inc_down, alb_diff,
+                     alb_dir, dps);
+        double *F = w.flux_total, *FD = w.flux_direct;
+        if (mcica) {
+          /* mcica_sample.F90:453-930 */
+          memcpy(clr_total, w.flux_total, sizeof(double) * (2 * L + 2));
+          memcpy(clr_direct, w.flux_direct, sizeof(double) * (L + 1));
+          int m0 = first_subcol_k[b][iex], m1 = first_subcol_k[b][iex + 1];
+          int nsk = m1 - m0;
+          double subcol_k_inv = 1.0 / (double)nsk;
+          for (int m = m0; m < m1; m++) {
+            int isub = m % n_req;
+            if (isub == 0) isub = n_req;
+            isub = subcol_reorder[isub];
+            for (int i = 0; i < L; i++) {
+              wc.trans[i] = w.trans[i]; wc.reflect[i] = w.reflect[i];
+              wc.trans_0[i] = w.trans_0[i]; wc.sc1[i] = w.sc1[i]; wc.sc2[i] = w.sc2[i];
+              double cs = c_sub[((size_t)(isub - 1) * L + i) * P + l];
+              double frac1 = LAY(w_cloud, i, l) > 0.0 ? 1.0 : 0.0;
+              /* frac_cloud(l,i,1) = frac/w_cloud = 1 where cloudy */
+              if (LAY(w_cloud, i, l) > 0.0) {
+                double fr = fmax(LAY(c->cloud_frac, i, l), 0.0);
+                frac1 = fr / LAY(w_cloud, i, l);
+              }
+              if (!(cs * frac1 > 0.0)) continue;
+              /* cloudy single-scattering (mcica_sample.F90:614-700) */
+              double kgt = k_grey_tot[i] + kc_tot[i] * cs;
+              kgt = kgt + 0.0 * cs;
+              double ksc = k_ext_scat[i] + kc_scat[i] * cs;
+              ksc = ksc + 0.0 * cs;
+              double fwd = 0.0 + kc_fwd[i] * cs;
+              fwd = fwd + 0.0 * cs;
+              double phf = 0.0 + kc_phf[i] * cs;
+              phf = phf + 0.0 * cs;
+              phf = (phf - fwd) / fmax(ksc - fwd, eps);
+              fwd = fwd / fmax(ksc, eps);
+              double k_total = kgt + KABS(0, iex, i);
+              double tau = k_total * LAY(c->mass, i, l);
+              double om = ksc / (k_total + tiny);
+              om = fmin(om, 1.0 - 3.2e+01 * eps);
+              tau = tau * (1.0 - om * fwd);
+              om = om * (1.0 - fwd) / (1.0 - om * fwd);
+              coeff_t o;
+              two_coeff_one(isolir, tau, om, phf, zen_0, &o);
+              /* mcica_column.F90:360-520: zero then frac-weighted sum */
+              wc.trans[i] = 0.0 + frac1 * o.trans;
+              wc.reflect[i] = 0.0 + frac1 * o.reflect;
+              wc.sc1[i] = 0.0 + frac1 * o.sc1;
+              wc.sc2[i] = 0.0 + frac1 * o.sc2;
+              if (isolir == IP_SOLAR) wc.trans_0[i] = 0.0 + frac1 * o.trans_0;
+            }
+            solve_column(isolir, &wc, adjust, dpl, dpl2, flux_inc_direct, flux_inc_down,
+                         alb_diff, alb_dir, dps);
+            if (m == m0) {
+              memcpy(acc_total, wc.flux_total, sizeof(double) * (2 * L + 2));
+              memcpy(acc_direct, wc.flux_direct, sizeof(double) * (L + 1));
+            } else {
+              for (int k = 0; k < 2 * L + 2; k++) acc_total[k] = acc_total[k] + wc.flux_total[k];
+              for (int k = 0; k <= L; k++) acc_direct[k] = acc_direct[k] + wc.flux_direct[k];
+            }
+          }
+          double fc = frac_cloudy[l];
+          for (int k = 0; k <= L; k++) {
+            acc_direct[k] = acc_direct[k] * subcol_k_inv;
+            acc_direct[k] = fc * acc_direct[k] + ((1.0 - fc) * clr_direct[k]);
+          }
+          for (int k = 0; k < 2 * L + 2; k++) {
+            acc_total[k] = acc_total[k] * subcol_k_inv;
+            acc_total[k] = (fc * acc_total[k]) + ((1.0 - fc) * clr_total[k]);
+          }
+          F = acc_total; FD = acc_direct;
+        }
+        /* augment_channel (augment_channel_mod.F90:159-233, 496-509) */
+        double wincr = 1.0 * esft_weight;
+        for (int i = 0; i <= L; i++) {
+          if (first) {
+            up[i] = wincr * F[2 * i];
+            down[i] = wincr * F[2 * i + 1];
+            if (isolir == IP_SOLAR) dir[i] = wincr * FD[i];
+          } else {
+            up[i] = up[i] + wincr * F[2 * i];
+            down[i] = down[i] + wincr * F[2 * i + 1];
+            if (isolir == IP_SOLAR) dir[i] = dir[i] + wincr * FD[i];
+          }
+        }
+        first = 0;
+      }
+      /* adjust_ir_radiance (adjust_ir_radiance.F90:186-200) */
+      if (isolir == IP_INFRA_RED)
+        for (int i = 0; i <= L; i++) {
+          up[i] = up[i] + planck[i] * 1.0;
+          down[i] = down[i] + planck[i] * 1.0;
+        }
+    }
+    /* set_diag (socrates_set_diag.F90:140-210) */
+    for (int i = 0; i <= L; i++) {
+      flux_up[(size_t)i * P + l] = up[i];
+      flux_down[(size_t)i * P + l] = down[i];
+      if (flux_direct) flux_direct[(size_t)i * P + l] = isolir == IP_SOLAR ? dir[i] : 0.0;
+    }
+    for (int i = 1; i <= L; i++) {
+      double fdiv = down[i - 1] - down[i] + up[i] - up[i - 1];
+      heating_rate[(size_t)(i - 1) * P + l] = fdiv / LAY(c->layer_heat_capacity, i - 1, l);
+    }
+  }
+  free(kabs); free(buf);
+  if (mcica) {
+    free(w_cloud); free(cond_mmr); free(re); free(c_sub); free(n_subcol_cld);
+    free(frac_cloudy); free(subcol_reorder); free(mc.xcw); mc.xcw = NULL;
+  }
+#undef LAY
+#undef TAKE
+#undef KABS
+  return 0;
+}
