@@ -1,0 +1,155 @@
+/* socgpu: MI355X (gfx950) two-stream radiative flux path for SOCRATES.
+ *
+ * C ABI: plain pointers and sizes only. Every entry point returns 0 on
+ * success or a negative SOCGPU_E* code; socgpu_last_error() gives the message
+ * (per host thread). Nothing here aborts the process: the Fortran shim
+ * (socrates_amd/fortran/socrates_runes_gpu.f90) turns a non-zero return into
+ * ereport(..., i_err_fatal), which is the reference's STOP-on-fatal-error
+ * behaviour (ereport_mod.F90:27-40).
+ *
+ * Layouts follow the reference runes interface for full fields
+ * (socrates_runes.F90:47-168 with l_invert = l_profile_last = .false.):
+ * Fortran (n_profile, n_layer) arrays, i.e. C [layer][profile], layer 1 at the
+ * top of the atmosphere; level fields (n_profile, 0:n_layer); per-band surface
+ * albedos (n_profile, n_band). Reals are float64.
+ */
+#ifndef SOCGPU_H
+#define SOCGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define SOCGPU_OK 0
+#define SOCGPU_EINVAL (-1)     /* bad argument */
+#define SOCGPU_ENOTFOUND (-2)  /* unknown spectrum / McICA name */
+#define SOCGPU_EUNSUPPORTED (-3) /* option or file feature outside the path */
+#define SOCGPU_EIO (-4)        /* file could not be read / parsed */
+#define SOCGPU_EDEVICE (-5)    /* HIP runtime error */
+#define SOCGPU_ENOMEM (-6)     /* device memory / admission limit */
+
+/* precision forms */
+#define SOCGPU_FP64 0          /* plain FP64: matches the reference to 1e-10 */
+#define SOCGPU_MIXED 1         /* FP32 bulk with FP64 guarded branches */
+
+/* option codes, identical to the reference's rad_pcf.F90 values */
+#define SOCGPU_SOURCE_SOLAR 1        /* ip_solar / ip_source_illuminate */
+#define SOCGPU_SOURCE_THERMAL 2      /* ip_infra_red / ip_source_thermal */
+#define SOCGPU_CLOUD_OFF 0           /* ip_cloud_off */
+#define SOCGPU_CLOUD_HOMOGEN 1       /* ip_cloud_homogen */
+#define SOCGPU_CLOUD_CONV_STRAT 3    /* ip_cloud_conv_strat */
+#define SOCGPU_OVERLAP_MAX_RAND 0    /* ip_max_rand */
+#define SOCGPU_OVERLAP_RANDOM 1      /* ip_rand */
+#define SOCGPU_OVERLAP_EXP_RAND 3    /* ip_exponential_rand */
+#define SOCGPU_INHOM_HOMOGENEOUS 0   /* ip_homogeneous */
+#define SOCGPU_INHOM_MCICA 2         /* ip_mcica */
+#define SOCGPU_MCICA_SINGLE 1        /* ip_mcica_single_sampling */
+#define SOCGPU_MCICA_OPTIMAL 2       /* ip_mcica_optimal_sampling */
+
+/* Inputs of one radiation call: the subset of runes()'s optional arguments
+ * on the two-stream flux path (socrates_runes.F90:47-168). A NULL pointer is
+ * an absent optional argument. */
+typedef struct SocgpuRunesArgs {
+  int n_profile;              /* columns */
+  int n_layer;                /* layers */
+  int i_source;               /* SOCGPU_SOURCE_* */
+  int l_rayleigh;             /* Rayleigh scattering (solar) */
+  int i_cloud_representation; /* SOCGPU_CLOUD_* */
+  int i_overlap;              /* SOCGPU_OVERLAP_* */
+  int i_inhom;                /* SOCGPU_INHOM_* */
+  int i_mcica_sampling;       /* SOCGPU_MCICA_* */
+  int i_st_water;             /* droplet type of stratiform liquid */
+  int i_cnv_water;            /* droplet type of convective liquid */
+  int n_subcol_gen;           /* McICA sub-columns generated (<= 64) */
+  int l_inputs_on_device;     /* pointers below are device pointers */
+  double co2_mix_ratio;       /* well-mixed CO2 mass mixing ratio */
+  double cloud_vertical_decorr; /* Pa */
+  double conv_vertical_decorr;  /* Pa */
+  double cloud_horizontal_rsd;
+  const double *p_layer, *t_layer, *mass, *density, *layer_heat_capacity;
+  const double *h2o, *o3;                  /* [layer][profile] mmr */
+  const double *p_level, *t_level;         /* [level][profile] */
+  const double *t_ground, *cos_zenith_angle, *solar_irrad; /* [profile] */
+  const double *albedo_diff, *albedo_dir;  /* [band][profile] */
+  const double *cloud_frac, *liq_mmr, *liq_dim;           /* [layer][profile] */
+  const double *conv_frac, *liq_conv_mmr, *liq_conv_dim;  /* [layer][profile] */
+  const int *rand_seed;                    /* [profile] McICA seeds */
+} SocgpuRunesArgs;
+
+/* Outputs (StrDiag, socrates_def_diag.F90:31-76): caller-owned, NULL = not
+ * requested. Fluxes [level][profile] in W m-2, heating rate [layer][profile]
+ * in K s-1 (set_diag, socrates_set_diag.F90:140-210). */
+typedef struct SocgpuDiag {
+  double *flux_up, *flux_down, *flux_direct, *heating_rate;
+  int l_outputs_on_device;
+} SocgpuDiag;
+
+/* Select the device and create the library's stream. Replaces nothing in the
+ * reference (it has no device); called once before the other entry points. */
+int socgpu_init(int device);
+
+/* Read a spectral file and keep its tables resident on the device under
+ * `name`. Replaces set_spectrum (socrates_set_spectrum.F90:29-149), which
+ * reads into the module-saved spectrum_array (:20-23). */
+int socgpu_spectrum_register(const char *name, const char *path);
+
+/* Read McICA data and associate it with the SW/LW spectrum names (either may
+ * be NULL). Replaces set_mcica (socrates_set_spectrum.F90:551-694) and
+ * read_mcica_data (def_mcica.F90:61-190). */
+int socgpu_mcica_register(const char *path, const char *sw_name,
+                          const char *lw_name);
+
+/* One radiation call on the library stream. Replaces runes
+ * (socrates_runes.F90:47-1079): set_control defaults (socrates_set_control.F90
+ * :236-398), set_atm/bound/cld/cld_dim/cld_mcica, radiance_calc on the
+ * two-stream path, set_diag. precision: SOCGPU_FP64 or SOCGPU_MIXED.
+ * Synchronous: returns when the outputs are written. */
+int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *args,
+                 SocgpuDiag *diag, int precision);
+
+/* Kernel-level entry (no reference counterpart as a single routine): the
+ * two_coeff + trans_source_coeff + source + solver_homogen_direct chain of
+ * two_stream.F90:160-205 for prescribed layer optical properties, one solve
+ * per (profile, term). tau/omega/asymmetry are [term][layer][profile];
+ * sec_0 / flux_inc / albedos per profile; planck_* for the infra-red
+ * ([level][profile], ground [profile]). Outputs flux_up/flux_down
+ * [term][level][profile] (differential for the infra-red, as the solver
+ * returns them), flux_direct likewise (solar). Host pointers. */
+int socgpu_two_stream(int n_profile, int n_layer, int n_term, int isolir,
+                      const double *tau, const double *omega,
+                      const double *asymmetry, const double *sec_0,
+                      const double *flux_inc_down, const double *albedo_diff,
+                      const double *albedo_dir, const double *planck_level,
+                      const double *planck_layer, const double *planck_ground,
+                      double *flux_up, double *flux_down, double *flux_direct,
+                      int precision);
+
+/* McICA sub-column generator on the device (cloud_gen,
+ * socrates_cloud_gen.F90:54-478, plus the fill of set_cld_mcica.F90:202-220)
+ * for tests: inputs [layer][profile]; c_sub out [sub][layer][profile],
+ * n_subcol_cld [profile]. Host pointers. */
+int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req,
+                     int ipph, int ioverlap, const int *rand_seed,
+                     const double *rlc_cf, const double *rlc_cw,
+                     const double *sigma_qcw, const double *avg_cf,
+                     const double *p_layer, int n1, int n2, const double *xcw,
+                     double *c_sub, int *n_subcol_cld);
+
+/* Message of the last failure on this host thread. */
+const char *socgpu_last_error(void);
+
+/* Timing of the last socgpu_runes call on the device (ms), measured with
+ * HIP events on the library stream around the whole call and around the
+ * solve kernel alone. */
+int socgpu_last_timing(double *total_ms, double *solve_ms);
+
+/* Release device memory and the stream. */
+int socgpu_finalize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -138,3 +138,23 @@ def read_output(path, n_profile, n_layer):
     direct = a[2 * n1:3 * n1].reshape(L + 1, P)
     hr = a[3 * n1:3 * n1 + P * L].reshape(L, P)
     return dict(flux_up=up, flux_down=down, flux_direct=direct, heating_rate=hr)
+
+
+def case_to_arrays(case):
+    """Flat dict of a Case (for npz storage)."""
+    case.fill_defaults()
+    out = {}
+    for f in dataclasses.fields(case):
+        v = getattr(case, f.name)
+        out["case_" + f.name] = np.asarray(v)
+    return out
+
+
+def case_from_arrays(d):
+    kw = {}
+    for f in dataclasses.fields(Case):
+        v = d["case_" + f.name]
+        if v.ndim == 0:
+            v = v.item()
+        kw[f.name] = v
+    return Case(**kw)

@@ -1,0 +1,100 @@
+// Device-resident tables of one spectral file (plus its McICA sampling
+// tables) and the per-call column records, shared by host and device code.
+//
+// The spectral content is the subset of the reference's StrSpecData
+// (def_spectrum.F90:33-472) that the two-stream flux path reads: per band the
+// major-gas k-terms (ESFT terms), minor-gas k-terms for equivalent
+// extinction, the solar flux fraction, the Rayleigh coefficient, the Planck
+// polynomial and the droplet parametrisations.
+#pragma once
+
+#include <stdint.h>
+
+#define SG_MAXB 16          // bands per file
+#define SG_MAXG 96          // major k-terms (g-points) per file
+#define SG_MAXMINOR_GAS 6   // minor gases per band
+#define SG_MAXMINOR 256     // minor k-terms per file
+#define SG_MAXSUB 192       // McICA sub-column items per file (sum of subcol_k)
+#define SG_NPL 8            // Planck polynomial coefficients
+#define SG_NDROP 8          // droplet types kept
+#define SG_NDPARM 16        // droplet fit parameters
+#define SG_MAXL 160         // layers per column
+#define SG_MAXSUBCOL 64     // generated McICA sub-columns per column
+
+// gas slots of the column record
+#define SG_GAS_H2O 0
+#define SG_GAS_CO2 1
+#define SG_GAS_O3 2
+#define SG_GAS_ZERO 3       // absorber the call supplies no amount for
+
+struct SgTerm {             // one k-term (ESFT term)
+  double k, w;              // absorption coefficient (m2/kg), weight
+  double a, b;              // power-law exponents (pressure, temperature)
+  double lnpref, lntref;    // log of the reference pressure and temperature
+  int band, gas;            // band, gas slot
+  int n_sub, sub0;          // McICA: sampled sub-columns, first sub item
+};
+
+struct SgBand {
+  int n_major, g0;                     // major terms [g0, g0+n_major)
+  int n_minor_gas, minor0;             // minor terms start at minor0
+  int minor_nk[SG_MAXMINOR_GAS];       // terms per minor gas
+  double solar_flux;                   // normalised solar flux in the band
+  double rayleigh;                     // Rayleigh mass scattering coefficient
+  double planck[SG_NPL];               // thermal_coeff(0:n_deg)
+};
+
+struct SgDrop {
+  int present, scheme, n_parm, pad;
+  double dim_min, dim_max;
+  double parm[SG_MAXB][SG_NDPARM];
+};
+
+struct SgSpec {
+  int n_band, n_g, n_minor, n_deg;
+  double t_ref_planck;
+  SgBand band[SG_MAXB];
+  SgTerm g[SG_MAXG];
+  SgTerm minor[SG_MAXMINOR];
+  SgDrop drop[SG_NDROP];
+  // McICA sampling (set_cld_mcica, socrates_set_cld_mcica.F90:103-151) for
+  // [0] single and [1] optimal sampling
+  int has_mcica, n_gen, ipph, n1, n2;
+  int n_req[2];
+  int n_sub_items[2];
+  int sub_g[2][SG_MAXSUB];      // g-point of each sub item
+  int sub_isub[2][SG_MAXSUB];   // generated sub-column (1-based) it samples
+  double sub_inv_s[2][SG_MAXSUB];
+  int g_n_sub[2][SG_MAXG];      // sub items per g-point
+  int g_sub0[2][SG_MAXG];       // first sub item of a g-point
+};
+
+// per-layer column record, [col][layer][SG_NQ]
+#define SG_Q_LNP 0
+#define SG_Q_LNT 1
+#define SG_Q_MASS 2
+#define SG_Q_MMR 3     // 4 slots: h2o, co2, o3, zero
+#define SG_Q_WCLD 7    // cloud fraction w_cloud (0 if below threshold)
+#define SG_Q_LWC 8     // in-cloud liquid mass mixing ratio
+#define SG_Q_RE 9      // effective radius clamped to the fit range
+#define SG_Q_FRAC1 10  // frac_cloud of the homogeneous type (1 in cloud)
+#define SG_Q_P 11      // layer pressure
+#define SG_Q_T 12      // layer temperature
+#define SG_Q_RSD 13    // condensate relative standard deviation (McICA)
+#define SG_NQ 14
+
+// per-column scalars, [col][SG_NS]
+#define SG_S_SEC0 0    // zen_0 (secant of the solar zenith angle, or 1)
+#define SG_S_SOL 1     // solar irradiance (0 when the sun is down)
+#define SG_S_TGROUND 2
+#define SG_S_FC 3      // frac_cloudy
+#define SG_S_NCLD 4    // number of cloudy sub-columns (as a real)
+#define SG_NS 5
+
+// per (col, band, layer) record from the preparation kernel, [col][band][layer][4]
+//  LW: k_grey_tot, planck diff, planck diff_2, planck at the bottom level
+//  SW: k_grey_tot, adjust_solar_ke, 0, 0
+// per (col, band) scalars [col][band][4]
+//  LW: planck at level 0, planck at the ground, albedo diff, albedo dir
+//  SW: band solar irradiance, 0, albedo diff, albedo dir
+#define SG_NBQ 4

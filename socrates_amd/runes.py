@@ -1,0 +1,243 @@
+"""Python mirror of the SOCRATES `runes` interface over the socgpu C ABI.
+
+Names and argument meanings follow the reference interface:
+  set_spectrum   -> socrates_set_spectrum::set_spectrum (socrates_set_spectrum.F90:29-149)
+  set_mcica      -> socrates_set_spectrum::set_mcica (socrates_set_spectrum.F90:551-694)
+  runes          -> socrates_runes::runes (socrates_runes.F90:47-1079), the
+                    arguments on the two-stream flux path
+Fatal errors of the reference (ereport with i_err_fatal, which STOPs the
+process, ereport_mod.F90:27-40) raise SocgpuError here instead.
+
+Arrays may be numpy arrays (host memory) or torch tensors on the GPU (device
+memory, no copies); layouts are the reference's full-field layouts:
+(n_layer, n_profile) for layer fields, (n_layer+1, n_profile) for levels,
+(n_band, n_profile) for albedos. The HIP library must be built
+(socrates_amd/build.py); nothing here falls back to a CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .case_io import Case
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsocgpu.so")
+DATA = os.path.join(HERE, "data")
+
+FP64 = 0
+MIXED = 1
+_PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+
+ip_source_illuminate = 1
+ip_source_thermal = 2
+
+
+class SocgpuError(RuntimeError):
+    pass
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class RunesArgs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "n_profile", "n_layer", "i_source", "l_rayleigh", "i_cloud_representation",
+        "i_overlap", "i_inhom", "i_mcica_sampling", "i_st_water", "i_cnv_water",
+        "n_subcol_gen", "l_inputs_on_device")] + [(n, ctypes.c_double) for n in (
+        "co2_mix_ratio", "cloud_vertical_decorr", "conv_vertical_decorr",
+        "cloud_horizontal_rsd")] + [(n, _D) for n in (
+        "p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
+        "p_level", "t_level", "t_ground", "cos_zenith_angle", "solar_irrad",
+        "albedo_diff", "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac",
+        "liq_conv_mmr", "liq_conv_dim")] + [("rand_seed", _I)]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("flux_up", _D), ("flux_down", _D), ("flux_direct", _D),
+                ("heating_rate", _D), ("l_outputs_on_device", ctypes.c_int)]
+
+
+_lib = None
+_initialised = False
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SocgpuError("socgpu library not built: %s (run socrates_amd/build.py)" % LIB_PATH)
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.socgpu_last_error.restype = ctypes.c_char_p
+        for fn in ("socgpu_init", "socgpu_spectrum_register", "socgpu_mcica_register",
+                   "socgpu_runes", "socgpu_cloud_gen", "socgpu_two_stream",
+                   "socgpu_last_timing", "socgpu_finalize"):
+            getattr(_lib, fn).restype = ctypes.c_int
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SocgpuError("socgpu error %d: %s" % (rc, lib().socgpu_last_error().decode()))
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        _check(lib().socgpu_init(int(device)))
+        _initialised = True
+
+
+def set_spectrum(spectrum_name, spectral_file):
+    init()
+    _check(lib().socgpu_spectrum_register(spectrum_name.encode(), spectral_file.encode()))
+
+
+def set_mcica(mcica_data_file, sw_spectrum_name=None, lw_spectrum_name=None):
+    init()
+    _check(lib().socgpu_mcica_register(
+        mcica_data_file.encode(),
+        sw_spectrum_name.encode() if sw_spectrum_name else None,
+        lw_spectrum_name.encode() if lw_spectrum_name else None))
+
+
+def last_timing():
+    t, s = ctypes.c_double(), ctypes.c_double()
+    lib().socgpu_last_timing(ctypes.byref(t), ctypes.byref(s))
+    return t.value, s.value
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x, keep, dtype=np.float64):
+    if x is None:
+        return None
+    if _is_torch(x):
+        keep.append(x)
+        return ctypes.cast(ctypes.c_void_p(x.data_ptr()), _I if dtype == np.int32 else _D)
+    a = np.ascontiguousarray(x, dtype=dtype)
+    keep.append(a)
+    return a.ctypes.data_as(_I if dtype == np.int32 else _D)
+
+
+def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
+          density=None, layer_heat_capacity=None, h2o=None, o3=None, co2_mix_ratio=0.0,
+          p_level=None, t_level=None, t_ground=None, cos_zenith_angle=None,
+          solar_irrad=None, albedo_diff=None, albedo_dir=None, l_rayleigh=False,
+          cloud_frac=None, liq_mmr=None, liq_dim=None, conv_frac=None, liq_conv_mmr=None,
+          liq_conv_dim=None, cloud_vertical_decorr=0.0, conv_vertical_decorr=0.0,
+          cloud_horizontal_rsd=0.0, i_cloud_representation=0, i_overlap=0, i_inhom=0,
+          i_mcica_sampling=0, i_st_water=5, i_cnv_water=5, n_subcol_gen=0,
+          rand_seed=None, precision="fp64", out=None):
+    """One radiation call. Returns dict(flux_up, flux_down, flux_direct,
+    heating_rate) in the input memory space (numpy or torch)."""
+    init()
+    inputs = dict(p_layer=p_layer, t_layer=t_layer, mass=mass, density=density,
+                  layer_heat_capacity=layer_heat_capacity, h2o=h2o, o3=o3, p_level=p_level,
+                  t_level=t_level, t_ground=t_ground, cos_zenith_angle=cos_zenith_angle,
+                  solar_irrad=solar_irrad, albedo_diff=albedo_diff, albedo_dir=albedo_dir,
+                  cloud_frac=cloud_frac, liq_mmr=liq_mmr, liq_dim=liq_dim, conv_frac=conv_frac,
+                  liq_conv_mmr=liq_conv_mmr, liq_conv_dim=liq_conv_dim)
+    on_dev = _is_torch(p_layer)
+    keep = []
+    a = RunesArgs()
+    a.n_profile, a.n_layer, a.i_source = int(n_profile), int(n_layer), int(i_source)
+    a.l_rayleigh = int(bool(l_rayleigh))
+    a.i_cloud_representation, a.i_overlap = int(i_cloud_representation), int(i_overlap)
+    a.i_inhom, a.i_mcica_sampling = int(i_inhom), int(i_mcica_sampling)
+    a.i_st_water, a.i_cnv_water, a.n_subcol_gen = int(i_st_water), int(i_cnv_water), int(n_subcol_gen)
+    a.l_inputs_on_device = int(on_dev)
+    a.co2_mix_ratio = float(co2_mix_ratio)
+    a.cloud_vertical_decorr, a.conv_vertical_decorr = float(cloud_vertical_decorr), float(conv_vertical_decorr)
+    a.cloud_horizontal_rsd = float(cloud_horizontal_rsd)
+    for k, v in inputs.items():
+        if v is not None and _is_torch(v) != on_dev:
+            raise SocgpuError("all inputs must be in the same memory space (%s)" % k)
+        setattr(a, k, _ptr(v, keep))
+    a.rand_seed = _ptr(rand_seed, keep, np.int32)
+    L, P = int(n_layer), int(n_profile)
+    if out is None:
+        if on_dev:
+            import torch
+            dev = p_layer.device
+            out = dict(flux_up=torch.empty((L + 1, P), dtype=torch.float64, device=dev),
+                       flux_down=torch.empty((L + 1, P), dtype=torch.float64, device=dev),
+                       flux_direct=torch.empty((L + 1, P), dtype=torch.float64, device=dev),
+                       heating_rate=torch.empty((L, P), dtype=torch.float64, device=dev))
+        else:
+            out = dict(flux_up=np.zeros((L + 1, P)), flux_down=np.zeros((L + 1, P)),
+                       flux_direct=np.zeros((L + 1, P)), heating_rate=np.zeros((L, P)))
+    d = Diag()
+    d.flux_up = _ptr(out["flux_up"], keep)
+    d.flux_down = _ptr(out["flux_down"], keep)
+    d.flux_direct = _ptr(out["flux_direct"], keep)
+    d.heating_rate = _ptr(out["heating_rate"], keep)
+    d.l_outputs_on_device = int(on_dev)
+    if not on_dev:
+        for k in out:
+            if not out[k].flags["C_CONTIGUOUS"]:
+                raise SocgpuError("output %s must be contiguous" % k)
+    _check(lib().socgpu_runes(spectrum_name.encode(), ctypes.byref(a), ctypes.byref(d),
+                              _PREC[precision]))
+    return out
+
+
+def runes_case(case: Case, spectrum_name, precision="fp64", device_arrays=None, out=None):
+    """runes() on a Case (socrates_amd.case_io). device_arrays: optional dict
+    of torch tensors replacing the case's numpy arrays."""
+    src = dict(device_arrays) if device_arrays else {}
+
+    def g(name, attr=None):
+        return src.get(name, getattr(case, attr or name))
+
+    cloudy = bool(case.l_cloud)
+    return runes(
+        case.n_profile, case.n_layer, spectrum_name, case.i_source,
+        p_layer=g("p_layer"), t_layer=g("t_layer"), mass=g("mass"), density=g("density"),
+        layer_heat_capacity=g("layer_heat_capacity"), h2o=g("h2o"), o3=g("o3"),
+        co2_mix_ratio=case.co2_mix_ratio, p_level=g("p_level"), t_level=g("t_level"),
+        t_ground=g("t_ground"), cos_zenith_angle=g("cos_zenith_angle", "cos_zen"),
+        solar_irrad=g("solar_irrad"), albedo_diff=g("albedo_diff"), albedo_dir=g("albedo_dir"),
+        l_rayleigh=case.l_rayleigh,
+        cloud_frac=g("cloud_frac") if cloudy else None,
+        liq_mmr=g("liq_mmr") if cloudy else None, liq_dim=g("liq_dim") if cloudy else None,
+        conv_frac=g("conv_frac") if cloudy and case.i_solver_kind == 1 else None,
+        liq_conv_mmr=g("liq_conv_mmr") if cloudy and case.i_solver_kind == 1 else None,
+        liq_conv_dim=g("liq_conv_dim") if cloudy and case.i_solver_kind == 1 else None,
+        cloud_vertical_decorr=case.cloud_vertical_decorr,
+        conv_vertical_decorr=case.conv_vertical_decorr,
+        cloud_horizontal_rsd=case.cloud_horizontal_rsd,
+        i_cloud_representation=case.i_cloud_repr if cloudy else 0, i_overlap=case.i_overlap,
+        i_inhom=case.i_inhom, i_mcica_sampling=case.i_mcica_sampling,
+        i_st_water=case.i_st_water, i_cnv_water=case.i_st_water,
+        n_subcol_gen=case.n_subcol_gen, rand_seed=g("rand_seed") if cloudy else None,
+        precision=precision, out=out)
+
+
+def register_default_spectra():
+    """Register the synthetic LW/SW spectral files and McICA data shipped in
+    socrates_amd/data as 'lw' and 'sw'."""
+    set_spectrum("sw", os.path.join(DATA, "sp_sw_synth"))
+    set_spectrum("lw", os.path.join(DATA, "sp_lw_synth"))
+    set_mcica(os.path.join(DATA, "mcica_synth"), sw_spectrum_name="sw", lw_spectrum_name="lw")
+
+
+def cloud_gen(rand_seed, rlc_cf, rlc_cw, sigma_qcw, avg_cf, p_layer, xcw, n_subcol=64,
+              n_req=64, ipph=1, ioverlap=3):
+    """Device McICA generator on host arrays; returns (c_sub[sub,layer,prof], n_subcol_cld)."""
+    init()
+    L, P = avg_cf.shape
+    keep = []
+    c_sub = np.zeros((n_subcol, L, P))
+    ncld = np.zeros(P, dtype=np.int32)
+    n1, n2 = xcw.shape
+    _check(lib().socgpu_cloud_gen(
+        P, L, n_subcol, n_req, ipph, ioverlap, _ptr(rand_seed, keep, np.int32),
+        _ptr(rlc_cf, keep), _ptr(rlc_cw, keep), _ptr(sigma_qcw, keep), _ptr(avg_cf, keep),
+        _ptr(p_layer, keep), n1, n2, _ptr(np.asfortranarray(xcw).ravel(order="F"), keep),
+        _ptr(c_sub, keep), _ptr(ncld, keep, np.int32)))
+    return c_sub, ncld
