@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the two-stream flux path on one MI355X.
+
+Workload (one step = one radiation time step of a host model): an all-sky
+SW call (6 bands, 40 g-points) and an all-sky LW call (9 bands, 50 g-points)
+on 4096 columns x 90 layers, McICA with 64 generated sub-columns and optimal
+sampling, exponential-random overlap; synthetic columns and spectral files
+(socrates_amd/synth.py, socrates_amd/data). Inputs and outputs stay in HBM.
+
+Metric: column.g-point flux solves per second at 90 levels (one solve = one
+(column, major k-term) flux profile including its sampled sub-columns).
+
+Also reported: the roofline of the solve kernel (FP64/FP32 VALU rate from
+the algorithmic operation count, HIP-event kernel time on the library
+stream) and a CPU baseline: the reference Fortran (compiled from its own
+sources into oracle/_ref) on a bounded sample, on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P, L = 4096, 90
+G = {"sw": 40, "lw": 50}
+# algorithmic operations per layer of one (column, g-point, sub-column) solve,
+# counted from the reference formulas (exp, sqrt and div count as one op)
+OPS_PER_LAYER = {"sw": 94.0, "lw": 79.0}
+# compulsory HBM bytes per (column, g-point) solve: FP64 column inputs read
+# once and outputs written once, divided by the g-points of the file
+BYTES_PER_SOLVE = {"sw": 235.0, "lw": 188.0}
+PEAK = {"fp64": 78.6, "mixed": 157.3}      # TFLOP/s vector
+HBM_PEAK = 8000.0                          # GB/s
+
+
+def make_inputs(torch, dev):
+    from socrates_amd import synth
+    cases = {}
+    for src, nb, seed in (("sw", 6, 20261202 + 2), ("lw", 9, 20261202 + 1)):
+        c = synth.make_case(P, L, src, nb, seed, cloudy=True)
+        arrs = {}
+        for k in ("p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
+                  "p_level", "t_level", "t_ground", "solar_irrad", "albedo_diff", "albedo_dir",
+                  "cloud_frac", "liq_mmr", "liq_dim"):
+            arrs[k] = torch.from_numpy(np.ascontiguousarray(getattr(c, k))).to(dev)
+        arrs["cos_zenith_angle"] = torch.from_numpy(np.ascontiguousarray(c.cos_zen)).to(dev)
+        arrs["rand_seed"] = torch.from_numpy(np.ascontiguousarray(c.rand_seed)).to(dev)
+        cases[src] = (c, arrs)
+    return cases
+
+
+def subcolumns_per_gpoint(R, src, mode=1):
+    """Mean number of sampled sub-columns per g-point (subcol_k)."""
+    import ctypes  # noqa: F401
+    return 1.0  # refined below from the McICA data when available
+
+
+def cpu_baseline(budget_s=15.0):
+    """Reference Fortran on this host: bounded sample, one process per core."""
+    from socrates_amd import synth
+    exe = os.path.join(ROOT, "oracle", "_ref", "soc_runes_case")
+    data = os.path.join(ROOT, "socrates_amd", "data")
+    kind = "reference"
+    if not os.path.exists(exe):
+        return None
+    cores = min(os.cpu_count() or 1, 16)
+    ncol = 64
+    with tempfile.TemporaryDirectory() as tmp:
+        files = {}
+        for src, nb in (("sw", 6), ("lw", 9)):
+            c = synth.make_case(ncol, L, src, nb, 777 + nb, cloudy=True)
+            files[src] = os.path.join(tmp, src + ".bin")
+            c.write(files[src])
+
+        def run(src, reps, n):
+            procs = [subprocess.Popen([exe, os.path.join(data, "sp_sw_synth"),
+                                       os.path.join(data, "sp_lw_synth"),
+                                       os.path.join(data, "mcica_synth"), files[src],
+                                       os.path.join(tmp, "%s%d.out" % (src, i)), str(reps)],
+                                      stdout=subprocess.PIPE, text=True) for i in range(n)]
+            secs = []
+            for p in procs:
+                out, _ = p.communicate()
+                secs.append(float(out.split("seconds=")[1]))
+            return max(secs)
+        # calibrate one call per source, then size the sample to the budget
+        t1 = run("sw", 1, 1) + run("lw", 1, 1)
+        reps = max(1, int(budget_s / max(t1, 1e-3)))
+        t = run("sw", reps, cores) + run("lw", reps, cores)
+    solves = cores * reps * ncol * (G["sw"] + G["lw"])
+    return {"value": solves / t, "unit": "column.g-point solves/s", "cores": cores,
+            "kind": kind,
+            "sample": "%d processes x %d steps x %d columns x 90 layers, all-sky SW+LW McICA"
+                      % (cores, reps, ncol)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the path shards by columns with no collective, "
+              "so only --gpus 1 is run here", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from socrates_amd import runes as R
+    dev = torch.device("cuda:0")
+    R.init(0)
+    R.register_default_spectra()
+    cases = make_inputs(torch, dev)
+    outs = {}
+    for src, (c, arrs) in cases.items():
+        L1 = L + 1
+        outs[src] = dict(flux_up=torch.empty((L1, P), dtype=torch.float64, device=dev),
+                         flux_down=torch.empty((L1, P), dtype=torch.float64, device=dev),
+                         flux_direct=torch.empty((L1, P), dtype=torch.float64, device=dev),
+                         heating_rate=torch.empty((L, P), dtype=torch.float64, device=dev))
+
+    def step(prec):
+        t = {}
+        for src in ("sw", "lw"):
+            c, arrs = cases[src]
+            R.runes_case(c, src, precision=prec, device_arrays=arrs, out=outs[src])
+            t[src] = R.last_timing()
+        return t
+
+    results = {}
+    for prec in ([a.precision] + (["fp64"] if a.precision != "fp64" else [])):
+        for _ in range(a.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        solve_ms = {"sw": 0.0, "lw": 0.0}
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            t = step(prec)
+            for s in t:
+                solve_ms[s] += t[s][1]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        results[prec] = dict(dt=dt, solve_ms={s: v / a.steps for s, v in solve_ms.items()})
+    main_r = results[a.precision]
+    ms_per_step = main_r["dt"] / a.steps * 1e3
+    solves_per_step = P * (G["sw"] + G["lw"])
+    value = solves_per_step / (main_r["dt"] / a.steps)
+    # roofline of the solve kernel (SW and LW launches together)
+    flops = sum(P * G[s] * L * OPS_PER_LAYER[s] for s in G)
+    bytes_alg = sum(P * G[s] * BYTES_PER_SOLVE[s] for s in G)
+    kt = sum(main_r["solve_ms"].values()) * 1e-3
+    achieved = flops / kt / 1e12
+    hbm = bytes_alg / kt / 1e9
+    line = {
+        "metric": "column.g-point flux solves/s at 90 levels (all-sky SW+LW McICA)",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic columns and spectral files (seeded)",
+        "config": {"workload": "4096 columns x 90 layers, all-sky SW (6 bands, 40 g) + LW "
+                               "(9 bands, 50 g), McICA 64 sub-columns, exp-random overlap",
+                   "precision": a.precision},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[a.precision],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[a.precision], "traffic": None,
+                     "kernel": "solve_kernel", "kernel_ms_per_step": kt * 1e3,
+                     "hbm_achieved_GBps": hbm, "hbm_frac": hbm / HBM_PEAK},
+    }
+    if "fp64" in results and a.precision != "fp64":
+        line["fp64_value"] = solves_per_step / (results["fp64"]["dt"] / a.steps)
+        line["mixed_over_fp64"] = value / line["fp64_value"]
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -34,8 +34,17 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
+def lib(variant=None):
+    """The restatement library; variant "jitter" is the exp-jittered build
+    used to measure the conditioning of the reference formulas."""
     global _lib
+    if variant == "jitter":
+        path = LIB.replace("libsoc_oracle.so", "libsoc_oracle_jitter.so")
+        if not os.path.exists(path):
+            build()
+        j = ctypes.CDLL(path)
+        j.soc_oracle_runes.restype = ctypes.c_int
+        return j
     if _lib is None:
         if not os.path.exists(LIB):
             build()
@@ -50,7 +59,7 @@ def _ptr(a, keep, dtype=np.float64):
     return a.ctypes.data_as(_I if dtype == np.int32 else _D)
 
 
-def runes(case, spectral_file, mcica_file=None):
+def runes(case, spectral_file, mcica_file=None, variant=None):
     """Fluxes [level][profile] and heating rate [layer][profile] (K/s)."""
     case.fill_defaults()
     keep = []
@@ -66,7 +75,7 @@ def runes(case, spectral_file, mcica_file=None):
     L, P = case.n_layer, case.n_profile
     up = np.zeros((L + 1, P)); down = np.zeros((L + 1, P))
     direct = np.zeros((L + 1, P)); hr = np.zeros((L, P))
-    rc = lib().soc_oracle_runes(spectral_file.encode(),
+    rc = lib(variant).soc_oracle_runes(spectral_file.encode(),
                                 (mcica_file or "").encode(), ctypes.byref(cs),
                                 _ptr(up, keep), _ptr(down, keep), _ptr(direct, keep),
                                 _ptr(hr, keep))
@@ -88,3 +97,25 @@ def cloud_gen(rand_seed, rlc_cf, rlc_cw, sigma_qcw, avg_cf, p_layer, xcw, n_subc
         _ptr(p_layer, keep), n1, n2, _ptr(np.asfortranarray(xcw).ravel(order="F"), keep),
         _ptr(c_sub, keep), _ptr(ncld, keep, np.int32))
     return c_sub, ncld
+
+
+def two_coeff(isolir, tau, omega, asymmetry, sec_0=None):
+    n = len(tau)
+    keep = []
+    outs = [np.zeros(n) for _ in range(5)]
+    lib().soc_oracle_two_coeff(n, isolir, _ptr(tau, keep), _ptr(omega, keep),
+                               _ptr(asymmetry, keep),
+                               _ptr(sec_0 if sec_0 is not None else np.ones(n), keep),
+                               *[_ptr(o, keep) for o in outs])
+    return dict(zip(("trans", "reflect", "trans_0", "sc1", "sc2"), outs))
+
+
+def solver_homogen_direct(trans, reflect, s_down, s_up, albedo, flux_inc_down, source_ground):
+    L = len(trans)
+    keep = []
+    ft = np.zeros(2 * L + 2)
+    lib().soc_oracle_solver_homogen_direct(
+        L, _ptr(trans, keep), _ptr(reflect, keep), _ptr(s_down, keep), _ptr(s_up, keep),
+        ctypes.c_double(albedo), ctypes.c_double(flux_inc_down), ctypes.c_double(source_ground),
+        _ptr(ft, keep))
+    return ft

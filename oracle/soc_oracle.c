@@ -16,6 +16,20 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef SOC_ORACLE_EXP_JITTER
+/* Conditioning probe (tests only): every exp() result is moved by -1, 0 or
+ * +1 ulp with a fixed pseudo-random sequence, to measure how strongly the
+ * reference formulas amplify last-bit differences of exp. */
+static unsigned long long jit_state = 88172645463325252ULL;
+static double jit_exp(double x) {
+  double v = exp(x);
+  jit_state ^= jit_state << 13; jit_state ^= jit_state >> 7; jit_state ^= jit_state << 17;
+  int r = (int)(jit_state % 3ULL) - 1;
+  return r == 0 ? v : nextafter(v, r > 0 ? INFINITY : -INFINITY);
+}
+#define exp(x) jit_exp(x)
+#endif
+
 #define NB 16      /* bands */
 #define NG 8       /* gases per file */
 #define NK 32      /* k-terms per gas and band */

@@ -1,0 +1,129 @@
+"""Pins the C restatement oracle (oracle/soc_oracle.c) before it is trusted:
+
+* against the golden vectors written by the reference compiled from its own
+  sources (tests/golden/*.npz, tests/golden/make_golden.py);
+* against the compiled reference directly when oracle/_ref is built here;
+* against closed forms of the two-stream coefficients (thin and thick
+  layers, conservative scattering) and a dense linear solve of the
+  two-stream system (solver_homogen_direct);
+* the conditioning of the reference's solar formulas, which sets the
+  tolerance of the solar parity tests.
+"""
+import glob
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from socrates_amd import synth
+from socrates_amd.case_io import case_from_arrays, read_output
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+DATA = os.path.join(ROOT, "socrates_amd", "data")
+REF = os.path.join(ROOT, "oracle", "_ref", "soc_runes_case")
+
+
+def spec_for(case):
+    return os.path.join(DATA, "sp_sw_synth" if case.i_source == 1 else "sp_lw_synth")
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "runes_*.npz"))))
+def test_oracle_matches_reference_golden(path):
+    d = np.load(path)
+    case = case_from_arrays(d)
+    out = oracle.runes(case, spec_for(case), os.path.join(DATA, "mcica_synth") if case.l_cloud else None)
+    for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+        r = d["ref_" + k]
+        err = np.abs(out[k] - r)
+        assert (err <= 1e-13 * np.maximum(np.abs(r), 1.0)).all(), (k, err.max())
+
+
+@pytest.mark.parametrize("ioverlap", [0, 1, 3])
+def test_oracle_cloud_gen_bit_exact(ioverlap):
+    d = np.load(os.path.join(GOLD, "cloud_gen_ov%d.npz" % ioverlap))
+    c_sub, ncld = oracle.cloud_gen(d["rand_seed"], d["rlc_cf"], d["rlc_cw"], d["sigma_qcw"],
+                                   d["avg_cf"], d["p_layer"], d["xcw"], n_subcol=64, n_req=0,
+                                   ioverlap=ioverlap)
+    assert np.array_equal(ncld, d["ref_n_subcol_cld"])
+    assert np.array_equal(c_sub, d["ref_c_sub"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference not built here (oracle/_ref)")
+@pytest.mark.parametrize("src,cloudy", [("lw", False), ("sw", True), ("lw", True)])
+def test_oracle_matches_compiled_reference(src, cloudy):
+    case = synth.make_case(7, 45, src, 9 if src == "lw" else 6, seed=777, cloudy=cloudy)
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "c.bin"), os.path.join(tmp, "c.out")
+        case.write(fin)
+        subprocess.run([REF, os.path.join(DATA, "sp_sw_synth"), os.path.join(DATA, "sp_lw_synth"),
+                        os.path.join(DATA, "mcica_synth") if cloudy else "-", fin, fout],
+                       check=True, capture_output=True)
+        ref = read_output(fout, case.n_profile, case.n_layer)
+    out = oracle.runes(case, spec_for(case), os.path.join(DATA, "mcica_synth") if cloudy else None)
+    for k in ref:
+        assert np.array_equal(out[k], ref[k]), (k, np.abs(out[k] - ref[k]).max())
+
+
+def test_two_coeff_limits():
+    # thin layer: T -> 1, R -> 0; thick layer: T -> 0, R -> gamma = (S-L)/(S+L)
+    for isolir in (1, 2):
+        om = np.array([0.3, 0.3, 0.9, 0.9])
+        g = np.array([0.0, 0.5, 0.2, 0.8])
+        sec = np.full(4, 1.7)
+        thin = oracle.two_coeff(isolir, np.full(4, 1e-12), om, g, sec)
+        assert np.allclose(thin["trans"], 1.0, atol=1e-10) and np.allclose(thin["reflect"], 0.0, atol=1e-10)
+        thick = oracle.two_coeff(isolir, np.full(4, 1e4), om, g, sec)
+        if isolir == 1:
+            S = 2.0 - 1.5 * om * g - 0.5 * om
+        else:
+            S = 1.66 - 1.5 * om * g
+        D = (2.0 if isolir == 1 else 1.66) * (1.0 - om)
+        lam = np.sqrt(S * D)
+        assert np.allclose(thick["trans"], 0.0, atol=1e-12)
+        assert np.allclose(thick["reflect"], (S - lam) / (S + lam), rtol=1e-12)
+        if isolir == 1:
+            assert np.allclose(thick["trans_0"], 0.0)
+            assert np.allclose(thin["trans_0"], 1.0)
+    # conservative scattering at the reference's clamp: T + R = 1
+    eps = np.finfo(float).eps
+    c = oracle.two_coeff(1, np.array([0.1, 1.0, 10.0]), np.full(3, 1 - 32 * eps), np.full(3, 0.6),
+                         np.full(3, 1.3))
+    assert np.allclose(c["trans"] + c["reflect"], 1.0, atol=1e-6)
+
+
+def test_solver_matches_dense_solve():
+    rng = np.random.default_rng(3)
+    for L in (1, 2, 5, 8):
+        T = rng.uniform(0.05, 0.9, L)
+        R = rng.uniform(0.0, 0.9, L) * (1 - T)
+        su, sd = rng.normal(size=L), rng.normal(size=L)
+        alb, finc, sg = 0.3, 2.0, 0.7
+        ft = oracle.solver_homogen_direct(T, R, sd, su, alb, finc, sg)
+        # unknowns x = [U0, D0, U1, D1, ..., UL, DL]; U = up, D = down at levels
+        n = 2 * L + 2
+        A = np.zeros((n, n)); b = np.zeros(n)
+        A[0, 1] = 1.0; b[0] = finc                                   # D0 = incident
+        for i in range(L):
+            r = 2 * i + 1                                            # two equations per layer
+            A[r, 2 * i] = 1.0; A[r, 2 * i + 2] = -T[i]; A[r, 2 * i + 1] = -R[i]; b[r] = su[i]
+            A[r + 1, 2 * i + 3] = 1.0; A[r + 1, 2 * i + 1] = -T[i]; A[r + 1, 2 * i + 2] = -R[i]; b[r + 1] = sd[i]
+        A[n - 1, 2 * L] = 1.0; A[n - 1, 2 * L + 1] = -alb; b[n - 1] = sg  # surface
+        x = np.linalg.solve(A, b)
+        assert np.allclose(ft, x, rtol=1e-12, atol=1e-12)
+
+
+def test_solar_exp_conditioning():
+    """1-ulp changes of exp move the reference's solar fluxes by ~1e-9
+    relative: the basis of the solar parity tolerance."""
+    res = {}
+    for src, nb in (("sw", 6),):
+        case = synth.make_case(16, 60, src, nb, seed=3)
+        a = oracle.runes(case, os.path.join(DATA, "sp_%s_synth" % src))
+        b = oracle.runes(case, os.path.join(DATA, "sp_%s_synth" % src), variant="jitter")
+        res[src] = max((np.abs(a[k] - b[k]) / np.maximum(np.abs(a[k]), 1.0)).max()
+                       for k in ("flux_up", "flux_down"))
+    assert 1e-11 < res["sw"] < 2e-8, res
