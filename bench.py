@@ -64,6 +64,13 @@ def subcolumns_per_gpoint(R, src, mode=1):
     return 1.0  # refined below from the McICA data when available
 
 
+def _raise_stack_limit():
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    want = resource.RLIM_INFINITY if hard == resource.RLIM_INFINITY else hard
+    resource.setrlimit(resource.RLIMIT_STACK, (want, hard))
+
+
 def cpu_baseline(budget_s=15.0):
     """Reference Fortran on this host: bounded sample, one process per core."""
     from socrates_amd import synth
@@ -82,14 +89,21 @@ def cpu_baseline(budget_s=15.0):
             c.write(files[src])
 
         def run(src, reps, n):
+            # the reference keeps its per-call work arrays on the stack: a
+            # 64-column call needs more than the usual 8 MiB
             procs = [subprocess.Popen([exe, os.path.join(data, "sp_sw_synth"),
                                        os.path.join(data, "sp_lw_synth"),
                                        os.path.join(data, "mcica_synth"), files[src],
                                        os.path.join(tmp, "%s%d.out" % (src, i)), str(reps)],
-                                      stdout=subprocess.PIPE, text=True) for i in range(n)]
+                                      stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                      text=True, preexec_fn=_raise_stack_limit)
+                     for i in range(n)]
             secs = []
             for p in procs:
                 out, _ = p.communicate()
+                if p.returncode != 0 or "seconds=" not in out:
+                    raise RuntimeError("reference harness failed (rc %d): %s"
+                                       % (p.returncode, out[-300:]))
                 secs.append(float(out.split("seconds=")[1]))
             return max(secs)
         # calibrate one call per source, then size the sample to the budget
@@ -179,7 +193,10 @@ def main():
         line["fp64_value"] = solves_per_step / (results["fp64"]["dt"] / a.steps)
         line["mixed_over_fp64"] = value / line["fp64_value"]
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as exc:  # reported, not fatal: the GPU figure stands on its own
+            line["cpu_baseline"] = {"value": None, "error": str(exc)[:300]}
     print(json.dumps(line))
 
 

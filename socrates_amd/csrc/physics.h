@@ -121,3 +121,176 @@ SG_HD int lcg_step(int x) {
 }
 
 }  // namespace sg
+
+#ifdef __HIP_DEVICE_COMPILE__
+#define SG_EXPF(x) __expf(x)
+#else
+#define SG_EXPF(x) expf(x)
+#endif
+
+namespace sg {
+
+struct CoefF {
+  float T, R, T0, s1, s2;
+};
+
+// -expm1(-x) in FP32: series below 0.2 (terms to x^8, relative error < 1e-10
+// before rounding), 1 - exp(-x) above, where the subtraction loses < 2 bits.
+SG_HD float one_minus_exp(float x) {
+  if (x < 0.2f) {
+    return x * (1.0f - x * (0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f -
+                x * (0.0013888889f - x * (1.984127e-4f - x * 2.4801587e-5f)))))));
+  }
+  return 1.0f - SG_EXPF(-x);
+}
+
+// Mixed-precision form of two_coeff (see two_coeff above for the reference
+// routines): the same two-stream solution evaluated in FP32 through forms
+// free of cancellation. With e = exp(-lambda tau), em = 1 - e,
+// gc = 1 - gamma = 2 lambda / (S + lambda), d1 = 1 - e gamma = em + gc e:
+//   T = e gc (1 + gamma) / (d1 (1 + e gamma)),  R = gamma em (1 + e) / (d1 (1 + e gamma)),
+//   1 - T + R = em (1 + gamma) / d1,            1 - T - R = em gc / (1 + e gamma),
+// and gamma = S (S - D) / (S + lambda)^2 with S - D taken from omega.
+// omc = 1 - omega, computed by the caller from the absorption, keeps the
+// near-conservative limit exact (diff = 2 omc or 1.66 omc). The
+// regularisation constants keep their FP64 values (sqrt(eps) = 2^-26,
+// tol = 2^-13).
+//
+// Solar sources without the resonance. The reference forms
+//   s1 = f [A (1 - T T0) - B R] - R,   s2 = f [B (T0 - T) - A R T0] + T0 - T,
+// f = omega mu / (2 (lambda - mu)(lambda + mu)), A, B = S -+ mu -+ ksi0 (D -+ mu),
+// ksi0 = 1.5 g / mu, mu = sec0, have brackets that vanish at mu = lambda
+// (there A = gamma B, since lambda^2 = S D). Dividing each bracket by
+// (mu - lambda) as a divided difference in mu gives, for every mu,
+//   s1 = -omega mu Q / (2 (lambda + mu)) - R,
+//   Q = (c - 1)(1 - T T0) + A_l T tphi - (1 - c) R,
+//   s2 = -omega mu M / (2 (lambda + mu)) + T0 - T,
+//   M = (1 - c)(T0 - T) - B_l tphi - (c - 1) R T0 + A_l R tphi,
+// with c = 1.5 g D / (mu lambda), A_l = S - lambda - 1.5 g (D / lambda - 1),
+// B_l = S + lambda + 1.5 g (D / lambda + 1) (A, B at mu = lambda) and
+// tphi = (T0 - e) / (lambda - mu) = tau e (1 - exp(-x)) / x, x = (mu - lambda) tau.
+// This is the same function as the reference's (which perturbs lambda by
+// 32 eps at exact resonance and loses digits near it), evaluated without a
+// 1 / (lambda - mu) amplification, so FP32 needs no FP64 fallback.
+SG_HD void two_coeff_f32(bool solar, float tau, float om, float omc, float g, float sec0,
+                         CoefF &c) {
+  float S, D, smd;  // smd = S - D, formed from omega without cancellation
+  if (solar) {
+    S = 2.0f - 1.5f * om * g - 0.5f * om;
+    D = 2.0f * omc;
+    smd = 1.5f * om * (1.0f - g);
+  } else {
+    S = 1.66f - 1.5f * om * g;
+    D = 1.66f * omc;
+    smd = om * (1.66f - 1.5f * g);
+  }
+  float lambda = sqrtf(S * D);
+  const float sqe = 1.4901161193847656e-08f;
+  const float tol = 1.220703125e-04f;
+  float x = lambda * tau;
+  float em = one_minus_exp(x);
+  float e = x >= 0.2f ? SG_EXPF(-x) : 1.0f - em;
+  float inv_sl = 1.0f / (S + lambda);
+  float gc = 2.0f * lambda * inv_sl;
+  float gam = S * smd * inv_sl * inv_sl;             // (S - lambda) / (S + lambda)
+  float d1 = em + gc * e;
+  float ope = 1.0f + e * gam;
+  float rden = 1.0f / (d1 * ope);
+  c.T = e * gc * (1.0f + gam) * rden;
+  c.R = gam * em * (1.0f + e) * rden;
+  if (solar) {
+    const float mu = sec0;
+    float xs = tau * mu;
+    float em0 = one_minus_exp(xs);
+    c.T0 = xs >= 0.2f ? SG_EXPF(-xs) : 1.0f - em0;
+    float omt = em * (1.0f + gam * gam * e) * rden;   // 1 - T
+    float omtt0 = omt + c.T * em0;                    // 1 - T T0
+    float t0mt = omt - em0;                           // T0 - T
+    float r = D / lambda;
+    float cc = 1.5f * g * r / mu;
+    float al = S - lambda + 1.5f * g * (1.0f - r);
+    float bl = S + lambda + 1.5f * g * (r + 1.0f);
+    float xd = (mu - lambda) * tau;
+    float tphi;
+    if (fabsf(xd) < 0.2f) {
+      // (1 - exp(-x)) / x, Taylor terms to x^7
+      float e1 = 1.0f - xd * (0.5f - xd * (0.16666667f - xd * (0.041666668f - xd * (0.008333334f -
+                 xd * (0.0013888889f - xd * (1.984127e-4f - xd * 2.4801587e-5f))))));
+      tphi = tau * e * e1;
+    } else {
+      tphi = (c.T0 - e) / (lambda - mu);
+    }
+    float fac = -0.5f * om * mu / (lambda + mu);
+    float q = (cc - 1.0f) * omtt0 + al * c.T * tphi - (1.0f - cc) * c.R;
+    float m = (1.0f - cc) * t0mt - bl * tphi - (cc - 1.0f) * c.R * c.T0 + al * c.R * tphi;
+    c.s1 = fac * q - c.R;
+    c.s2 = fac * m + t0mt;
+  } else {
+    c.T0 = 0.0f;
+    float omtr = em * (1.0f + gam) / d1;              // 1 - T + R
+    c.s1 = (omtr + sqe) / (sqe + tau * S);
+    float u = em * gc / ope;                          // 1 - T - R
+    // w = D tau - u, exact to O(x^3) terms for thin layers
+    float w;
+    if (x < 0.2f) {
+      float f = em / fmaxf(x, 1e-30f);                // (1 - e) / x
+      float g1 = 0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f - x * 0.0013888889f)));
+      w = D * tau * x * (2.0f * S * g1 - (S - lambda) * f) / (2.0f * S - (S - lambda) * x * f);
+      if (x <= 0.0f) w = 0.0f;
+    } else {
+      w = D * tau - u;
+    }
+    // s2 = -(1 + R + T + q) / (S tau + sqe), q the reference's two branches;
+    // tau > tol: 1 + R + T + q = 2 - u + q = (2 (D tau - u) - u dts) / dts, dts = D tau + sqe
+    // (the sqe terms of the reference's 2 (u + sqe) / dts cancel against 2 dts / dts);
+    // tau <= tol: 1 + R + T - 2 + D tau = D tau - u. Formed in FP64 from the
+    // accurate pieces.
+    double num;
+    if (tau > tol) {
+      double dts = (double)D * tau + 1.4901161193847656e-08;
+      num = (2.0 * (double)w - (double)u * dts) / dts;
+    } else {
+      num = (double)w;
+    }
+    c.s2 = (float)(-num / ((double)S * tau + 1.4901161193847656e-08));
+  }
+}
+
+}  // namespace sg
+
+namespace sg {
+
+// FP32 k-term (the mixed form's k-term exponent sum and exponential)
+SG_HD float k_term_f32(const SgTerm &t, float lnp, float lnt, float mmr) {
+  float s = SG_EXPF((float)t.a * (lnp - (float)t.lnpref) + (float)t.b * (lnt - (float)t.lntref));
+  return (float)t.k * fmaxf(0.0f, s * mmr);
+}
+
+// FP32 Pade-2 droplet optics with the absorption returned directly
+// (kext * X with X the single-scattering co-albedo fit), so that 1 - omega
+// stays accurate for weakly absorbing droplets.
+SG_HD void pade2_f32(const double *cpd, float mmr, float re, float &kext, float &kabs,
+                     float &phf, float &fwd) {
+  float cp[16];
+  for (int i = 0; i < 16; i++) cp[i] = (float)cpd[i];
+  kext = mmr * (cp[0] + re * (cp[1] + re * cp[2])) / (1.0f + re * (cp[3] + re * (cp[4] + re * cp[5])));
+  float X = (cp[6] + re * (cp[7] + re * cp[8])) / (1.0f + re * (cp[9] + re * cp[10]));
+  kabs = kext * X;
+  float kscat = kext * (1.0f - X);
+  float g = (cp[11] + re * (cp[12] + re * cp[13])) / (1.0f + re * (cp[14] + re * cp[15]));
+  phf = kscat * g;
+  fwd = phf * g;
+}
+
+SG_HD void slingo_f32(const double *cpd, float mmr, float re, float &kext, float &kabs,
+                      float &phf, float &fwd) {
+  kext = mmr * ((float)cpd[0] + (float)cpd[1] / re);
+  float X = (float)cpd[2] + (float)cpd[3] * re;
+  kabs = kext * X;
+  float kscat = kext * (1.0f - X);
+  float g = (float)cpd[4] + (float)cpd[5] * re;
+  phf = kscat * g;
+  fwd = kscat * g * g;
+}
+
+}  // namespace sg

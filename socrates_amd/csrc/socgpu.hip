@@ -57,6 +57,72 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// Butterfly reduce-scatter of 16 values per lane over the 64 lanes of a
+// wave: afterwards lane 4j (j = 0..15) holds the wave total of value j in
+// v[0] (lanes 4j+1..4j+3 hold the same total). 17 shuffles instead of the 96
+// of sixteen separate all-reductions.
+__device__ __forceinline__ void reduce16(double (&v)[16], int lane) {
+  {
+    const bool hi = lane & 32;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      double keep = hi ? v[k + 8] : v[k], give = hi ? v[k] : v[k + 8];
+      v[k] = keep + __shfl_xor(give, 32, kWave);
+    }
+  }
+  {
+    const bool hi = lane & 16;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      double keep = hi ? v[k + 4] : v[k], give = hi ? v[k] : v[k + 4];
+      v[k] = keep + __shfl_xor(give, 16, kWave);
+    }
+  }
+  {
+    const bool hi = lane & 8;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      double keep = hi ? v[k + 2] : v[k], give = hi ? v[k] : v[k + 2];
+      v[k] = keep + __shfl_xor(give, 8, kWave);
+    }
+  }
+  {
+    const bool hi = lane & 4;
+    double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
+    v[0] = keep + __shfl_xor(give, 4, kWave);
+  }
+  v[0] += __shfl_xor(v[0], 2, kWave);
+  v[0] += __shfl_xor(v[0], 1, kWave);
+}
+
+// Same for 8 values per lane: lane 8j (j = 0..7) holds the total of value j.
+__device__ __forceinline__ void reduce8(double (&v)[8], int lane) {
+  {
+    const bool hi = lane & 32;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      double keep = hi ? v[k + 4] : v[k], give = hi ? v[k] : v[k + 4];
+      v[k] = keep + __shfl_xor(give, 32, kWave);
+    }
+  }
+  {
+    const bool hi = lane & 16;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      double keep = hi ? v[k + 2] : v[k], give = hi ? v[k] : v[k + 2];
+      v[k] = keep + __shfl_xor(give, 16, kWave);
+    }
+  }
+  {
+    const bool hi = lane & 8;
+    double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
+    v[0] = keep + __shfl_xor(give, 8, kWave);
+  }
+  v[0] += __shfl_xor(v[0], 4, kWave);
+  v[0] += __shfl_xor(v[0], 2, kWave);
+  v[0] += __shfl_xor(v[0], 1, kWave);
+}
+
 struct CallDims {
   int P, L, NB, isolir;
   int mcica, mode;   // McICA on, sampling table (0 single, 1 optimal)
@@ -277,160 +343,163 @@ __global__ __launch_bounds__(64) void cloudgen_kernel(CallDims d, const double *
   }
 }
 
-// Per (column, band): grey extinction (Rayleigh, grey_opt_prop.F90:476-541,
-// 816-835), equivalent extinction of the minor gases
-// (solve_band_k_eqv_scl.F90:572-1047; monochromatic_gas_flux.F90:115-196 in
-// the infra-red), the direct-beam adjustment (ibid. :883-891) and the
-// Planck source terms (diff_planck_source_mod.F90:100-134).
-__global__ __launch_bounds__(64) void prep_kernel(CallDims d, const SgSpec *__restrict__ spec,
-                                                  int l_rayleigh, const double *__restrict__ colrec,
-                                                  const double *__restrict__ tlev,
-                                                  const double *__restrict__ colscal,
-                                                  double *__restrict__ bandrec,
-                                                  double *__restrict__ bandscal) {
+// Minor-gas k-terms, one lane per (column, minor term): the per-term part
+// of the equivalent extinction of solve_band_k_eqv_scl.F90:572-1047.
+//  infra-red: the non-scattering flux of the term alone
+//    (monochromatic_gas_flux.F90:115-196, diffusivity 1.66) and per layer its
+//    |flux|-weighted contributions, out[t][layer] = (sum over the layer's two
+//    incident fluxes of k w |F|, of w |F|);
+//  solar: the direct-beam transmission of the term, out[t][layer] =
+//    (w * transmission to the bottom of the layer, k * w * transmission to
+//    the surface).
+// minor[col][t][layer][2]
+__global__ __launch_bounds__(64) void minor_kernel(CallDims d, const SgSpec *__restrict__ spec,
+                                                   const double *__restrict__ colrec,
+                                                   const double *__restrict__ tlev,
+                                                   const double *__restrict__ colscal,
+                                                   const double *__restrict__ bandscal,
+                                                   double *__restrict__ minor) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int NB = d.NB, L = d.L;
-  if (idx >= d.P * NB) return;
-  const int col = idx / NB, b = idx % NB;
+  const int NT = spec->n_minor, L = d.L;
+  if (idx >= d.P * NT) return;
+  const int col = idx / NT, ti = idx % NT;
+  const SgTerm &t = spec->minor[ti];
+  const SgBand &B = spec->band[t.band];
+  const double *rec = colrec + (size_t)col * L * SG_NQ;
+  double *o = minor + ((size_t)col * NT + ti) * L * 2;
+  if (d.isolir == SOCGPU_SOURCE_SOLAR) {
+    const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
+    double kk[SG_MAXL];
+    double fdt = t.w;
+    for (int i = 0; i < L; i++) {
+      const double *r = rec + i * SG_NQ;
+      double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
+      kk[i] = k;
+      fdt = fdt * exp(-k * r[SG_Q_MASS] * sec0);
+      o[2 * i] = fdt;
+    }
+    for (int i = 0; i < L; i++) o[2 * i + 1] = kk[i] * fdt;
+  } else {
+    const double *tl = tlev + (size_t)col * (L + 1);
+    const double *bs = bandscal + ((size_t)col * d.NB + t.band) * SG_NBQ;
+    const double alb = bs[2];
+    const double sqe = 1.4901161193847656e-08;
+    double tr[SG_MAXL], su[SG_MAXL], kk[SG_MAXL], fdl[SG_MAXL + 1];
+    double bprev = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
+    double fdn = -bprev;
+    fdl[0] = fdn;
+    double bl = bprev;
+    for (int i = 0; i < L; i++) {
+      const double *r = rec + i * SG_NQ;
+      double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
+      double tau = k * r[SG_Q_MASS];
+      bl = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
+      double trans = exp(-kDiffMinor * tau);
+      double s = (1.0 - trans + sqe) * (bl - bprev) / (kDiffMinor * tau + sqe);
+      bprev = bl;
+      kk[i] = k; tr[i] = trans; su[i] = s;
+      fdn = trans * fdn - s;
+      fdl[i + 1] = fdn;
+    }
+    double bg = planck_band(B, spec->n_deg, spec->t_ref_planck,
+                            colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
+    double fup = (bg - bl) + alb * fdn;
+    for (int i = L - 1; i >= 0; i--) {
+      double a = t.w * fabs(fdl[i]), b = t.w * fabs(fup);
+      o[2 * i] = kk[i] * a + kk[i] * b;
+      o[2 * i + 1] = a + b;
+      fup = tr[i] * fup + su[i];
+    }
+  }
+}
+
+// Per (column, band, layer): the grey extinction (Rayleigh, grey_opt_prop.F90:
+// 476-541, 816-835) plus the equivalent extinction of the minor gases from
+// the per-term sums (solve_band_k_eqv_scl.F90:826-846, 995-1019), the solar
+// direct-beam adjustment (ibid. :848-891), and the Planck source terms
+// (diff_planck_source_mod.F90:100-134). bandrec[col][band][layer][4].
+__global__ __launch_bounds__(256) void prep_kernel(CallDims d, const SgSpec *__restrict__ spec,
+                                                   int l_rayleigh, const double *__restrict__ colrec,
+                                                   const double *__restrict__ tlev,
+                                                   const double *__restrict__ colscal,
+                                                   const double *__restrict__ minor,
+                                                   double *__restrict__ bandrec,
+                                                   double *__restrict__ bandscal) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NB = d.NB, L = d.L, NT = spec->n_minor;
+  if (idx >= d.P * NB * L) return;
+  const int i = idx % L, b = (idx / L) % NB, col = idx / (L * NB);
   const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
-  double *br = bandrec + ((size_t)col * NB + b) * L * SG_NBQ;
+  const double *r = rec + i * SG_NQ;
+  double *br = bandrec + (((size_t)col * NB + b) * L + i) * SG_NBQ;
   double *bs = bandscal + ((size_t)col * NB + b) * SG_NBQ;
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
-  const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
   const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
-  const double alb_diff = bs[2];
-  // keqv accumulates in br[.][0], adjust in br[.][1]
-  for (int i = 0; i < L; i++) { br[i * SG_NBQ + 0] = 0.0; br[i * SG_NBQ + 1] = 1.0; }
-  double b0 = 0.0, bg = 0.0;
-  if (!solar) {
-    // Planck at levels, differences and quadratic terms
-    const double *tl = tlev + (size_t)col * (L + 1);
-    double bprev = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
-    b0 = bprev;
-    for (int i = 0; i < L; i++) {
-      double bl = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
-      double bt = planck_band(B, spec->n_deg, spec->t_ref_planck, rec[i * SG_NQ + SG_Q_T]);
-      br[i * SG_NBQ + 2] = bl - bprev;
-      br[i * SG_NBQ + 3] = bl;
-      // diff_2 kept in the adjust slot for the infra-red
-      br[i * SG_NBQ + 1] = 2.0 * (bl + bprev - 2.0 * bt);
-      bprev = bl;
-    }
-    bg = planck_band(B, spec->n_deg, spec->t_ref_planck, colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
-    bs[0] = b0;
-    bs[1] = bg;
-  } else {
-    bs[0] = colscal[(size_t)col * SG_NS + SG_S_SOL] * B.solar_flux;
-    bs[1] = 0.0;
-  }
-  // minor gases
+  const double *mc = minor + (size_t)col * NT * L * 2;
+  double keq = 0.0, adjust = 1.0;
   int t0 = B.minor0;
   for (int j = 0; j < B.n_minor_gas; j++) {
     const int nk = B.minor_nk[j];
     if (solar) {
-      // direct-beam weighting: sum over terms of k * (surface transmission)
-      double sflux = 0.0;
-      // flux_gas(i) and sum_k_flux(i) live in br slots 2 and 3 while summing
-      for (int i = 0; i < L; i++) { br[i * SG_NBQ + 2] = 0.0; br[i * SG_NBQ + 3] = 0.0; }
-      double kmin_dummy = 0.0;
-      (void)kmin_dummy;
+      double num = 0.0, sflux = 0.0, fg = 0.0, fgp = 0.0;
       for (int q = 0; q < nk; q++) {
-        const SgTerm &t = spec->minor[t0 + q];
-        double fdt = t.w;
-        for (int i = 0; i < L; i++) {
-          const double *r = rec + i * SG_NQ;
-          double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-          fdt = fdt * exp(-k * r[SG_Q_MASS] * sec0);
-          br[i * SG_NBQ + 2] += fdt;
-        }
-        for (int i = 0; i < L; i++) {
-          const double *r = rec + i * SG_NQ;
-          double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-          br[i * SG_NBQ + 3] += k * fdt;
-        }
-        sflux += fdt;
+        const double *o = mc + (size_t)(t0 + q) * L * 2;
+        num += o[2 * i + 1];
+        sflux += o[2 * (L - 1)];
+        fg += o[2 * i];
+        fgp += i > 0 ? o[2 * (i - 1)] : 0.0;
       }
-      double fg_prev = 1.0;
-      for (int i = 0; i < L; i++) {
-        double keq;
-        if (sflux > 0.0) keq = br[i * SG_NBQ + 3] / sflux;
-        else {
-          const double *r = rec + i * SG_NQ;
-          keq = DBL_MAX;
-          for (int q = 0; q < nk; q++)
-            keq = fmin(keq, k_term(spec->minor[t0 + q], r[SG_Q_LNP], r[SG_Q_LNT],
-                                   r[SG_Q_MMR + spec->minor[t0 + q].gas]));
-        }
-        br[i * SG_NBQ + 0] += keq;
-        double fg = br[i * SG_NBQ + 2];
-        if (fg_prev > 0.0) br[i * SG_NBQ + 1] = br[i * SG_NBQ + 1] * fg / fg_prev;
-        fg_prev = fg;
+      if (i == 0) fgp = 1.0;
+      double kq;
+      if (sflux > 0.0) kq = num / sflux;
+      else {
+        kq = DBL_MAX;
+        for (int q = 0; q < nk; q++)
+          kq = fmin(kq, k_term(spec->minor[t0 + q], r[SG_Q_LNP], r[SG_Q_LNT],
+                               r[SG_Q_MMR + spec->minor[t0 + q].gas]));
       }
+      keq += kq;
+      if (fgp > 0.0) adjust = adjust * fg / fgp;
     } else {
-      // |flux|-weighted extinction from non-scattering solves of each term
-      double sf[2 * SG_MAXL + 4], skf[2 * SG_MAXL + 4];
-      for (int i = 0; i < 2 * L + 4; i++) { sf[i] = 0.0; skf[i] = 0.0; }
-      const double sqe = 1.4901161193847656e-08;
+      double num = 0.0, den = 0.0;
       for (int q = 0; q < nk; q++) {
-        const SgTerm &t = spec->minor[t0 + q];
-        // downward pass: flux index 2i+2 (1-based) = down at level i
-        double fdn = -b0;
-        double fdn_lev[SG_MAXL + 1];
-        double tr_l[SG_MAXL], su_l[SG_MAXL];
-        fdn_lev[0] = fdn;
-        for (int i = 0; i < L; i++) {
-          const double *r = rec + i * SG_NQ;
-          double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-          double tau = k * r[SG_Q_MASS];
-          double tr = exp(-kDiffMinor * tau);
-          double su = (1.0 - tr + sqe) * br[i * SG_NBQ + 2] / (kDiffMinor * tau + sqe);
-          tr_l[i] = tr; su_l[i] = su;
-          fdn = tr * fdn - su;
-          fdn_lev[i + 1] = fdn;
-        }
-        // upward pass from the surface
-        double fup = (bg - br[(L - 1) * SG_NBQ + 3]) + alb_diff * fdn;
-        // accumulate |F| weights: index 2i+1 (up at level i) pairs with layer i,
-        // index 2i (down at level i-1) with layer i
-        for (int i = L; i >= 1; i--) {
-          const double *r = rec + (i - 1) * SG_NQ;
-          double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-          skf[2 * i + 1] += k * t.w * fabs(fup);
-          sf[2 * i + 1] += t.w * fabs(fup);
-          skf[2 * i] += k * t.w * fabs(fdn_lev[i - 1]);
-          sf[2 * i] += t.w * fabs(fdn_lev[i - 1]);
-          fup = tr_l[i - 1] * fup + su_l[i - 1];
-        }
+        const double *o = mc + ((size_t)(t0 + q) * L + i) * 2;
+        num += o[0];
+        den += o[1];
       }
-      for (int i = 1; i <= L; i++) {
-        double lkf = skf[2 * i] + skf[2 * i + 1];
-        double lf = sf[2 * i] + sf[2 * i + 1];
-        double keq;
-        if (lf > 0.0) keq = lkf / lf;
-        else {
-          const double *r = rec + (i - 1) * SG_NQ;
-          keq = DBL_MAX;
-          for (int q = 0; q < nk; q++)
-            keq = fmin(keq, k_term(spec->minor[t0 + q], r[SG_Q_LNP], r[SG_Q_LNT],
-                                   r[SG_Q_MMR + spec->minor[t0 + q].gas]));
-        }
-        br[(i - 1) * SG_NBQ + 0] += keq;
+      double kq;
+      if (den > 0.0) kq = num / den;
+      else {
+        kq = DBL_MAX;
+        for (int q = 0; q < nk; q++)
+          kq = fmin(kq, k_term(spec->minor[t0 + q], r[SG_Q_LNP], r[SG_Q_LNT],
+                               r[SG_Q_MMR + spec->minor[t0 + q].gas]));
       }
+      keq += kq;
     }
     t0 += nk;
   }
-  // final: k_grey_tot = Rayleigh + k_eqv; solar adjustment for the
-  // increased grey extinction (solve_band_k_eqv_scl.F90:883-891)
-  const double temp_max = 36.043653389117154;  // log(1/DBL_EPSILON)
-  for (int i = 0; i < L; i++) {
-    double keq = br[i * SG_NBQ + 0];
-    br[i * SG_NBQ + 0] = (0.0 + ray) + keq;
-    if (solar) {
-      double m = rec[i * SG_NQ + SG_Q_MASS];
-      br[i * SG_NBQ + 1] = br[i * SG_NBQ + 1] * exp(fmin(keq * m * sec0, temp_max));
-      br[i * SG_NBQ + 2] = 0.0;
-      br[i * SG_NBQ + 3] = 0.0;
+  const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
+  br[0] = (0.0 + ray) + keq;
+  if (solar) {
+    const double temp_max = 36.043653389117154;  // log(1/DBL_EPSILON)
+    br[1] = adjust * exp(fmin(keq * r[SG_Q_MASS] * sec0, temp_max));
+    br[2] = 0.0;
+    br[3] = 0.0;
+    if (i == 0) { bs[0] = colscal[(size_t)col * SG_NS + SG_S_SOL] * B.solar_flux; bs[1] = 0.0; }
+  } else {
+    const double *tl = tlev + (size_t)col * (L + 1);
+    double b0 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i]);
+    double b1 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
+    double bt = planck_band(B, spec->n_deg, spec->t_ref_planck, r[SG_Q_T]);
+    br[1] = 2.0 * (b1 + b0 - 2.0 * bt);  // diff_2
+    br[2] = b1 - b0;                     // diff
+    br[3] = b1;                          // planck at the bottom of the layer
+    if (i == 0) {
+      bs[0] = b0;
+      bs[1] = planck_band(B, spec->n_deg, spec->t_ref_planck,
+                          colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
     }
   }
 }
@@ -479,6 +548,54 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
   }
 }
 
+// Mixed form of layer_coef: gas k-term, cloud optics, single scattering and
+// two-stream coefficients in FP32 (two_coeff_f32), carrying the co-albedo
+// 1 - omega from the absorption so the near-conservative limit keeps its
+// accuracy; the reference's clamp omega <= 1 - 32 eps(FP64) is kept as
+// 1 - omega >= 32 eps(FP64). No FP64 fallback: two_coeff_f32 has no
+// resonance singularity.
+__device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm &t, const double *r,
+                                                double kgt, double csub, Coef &o) {
+  const float kgas = k_term_f32(t, (float)r[SG_Q_LNP], (float)r[SG_Q_LNT], (float)r[SG_Q_MMR + t.gas]);
+  const float frac1 = (float)r[SG_Q_FRAC1];
+  const float kag = (float)(kgt - c.ksc_clear);  // grey absorption (equivalent extinction)
+  const float mass = (float)r[SG_Q_MASS];
+  const float omc_min = (float)(32.0 * kEps);
+  float tau, om, omc, g;
+  bool cloudy = csub * frac1 > 0.0;
+  if (cloudy) {
+    float kext, kabs, phf, fwd;
+    const float cs = (float)csub;
+    if (c.scheme == 5) pade2_f32(c.cp, (float)r[SG_Q_LWC], (float)r[SG_Q_RE], kext, kabs, phf, fwd);
+    else slingo_f32(c.cp, (float)r[SG_Q_LWC], (float)r[SG_Q_RE], kext, kabs, phf, fwd);
+    float ks = (float)c.ksc_clear + (kext - kabs) * cs;
+    float fw = fwd * cs;
+    float pf = phf * cs;
+    pf = (pf - fw) / fmaxf(ks - fw, (float)kEps);
+    fw = fw / fmaxf(ks, (float)kEps);
+    float ktot = (float)c.ksc_clear + kag + kgas + kext * cs;
+    float kab = kag + kgas + kabs * cs;
+    tau = ktot * mass;
+    omc = fmaxf(kab / ktot, omc_min);
+    om = 1.0f - omc;
+    float sc = 1.0f - om * fw;
+    tau = tau * sc;
+    omc = omc / sc;
+    om = 1.0f - omc;
+    g = pf;
+  } else {
+    float ktot = (float)c.ksc_clear + kag + kgas;
+    tau = ktot * mass;
+    omc = ktot > 0.0f ? fmaxf((kag + kgas) / ktot, omc_min) : 1.0f;
+    om = 1.0f - omc;
+    g = 0.0f;
+  }
+  CoefF f;
+  two_coeff_f32(c.solar, tau, om, omc, g, (float)c.sec0, f);
+  if (cloudy) { f.T *= frac1; f.R *= frac1; f.T0 *= frac1; f.s1 *= frac1; f.s2 *= frac1; }
+  o.T = f.T; o.R = f.R; o.T0 = f.T0; o.s1 = f.s1; o.s2 = f.s2;
+}
+
 // The column solve, one wave per column (or per chunk of 64 items of a
 // column), one lane per item: a clear-sky k-term (weight w_k (1 - frac_cloudy))
 // or a McICA sub-column sample (weight w_k frac_cloudy / subcol_k)
@@ -488,7 +605,8 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
 // downward diffuse flux it emits) and back-substituted bottom-up; the
 // per-layer elimination state lives in lane-private memory. Output:
 // partial[col][chunk][3][L+1] (up, down, direct) of this wave's weighted sum.
-__global__ __launch_bounds__(256) void solve_kernel(CallDims d, const SgSpec *__restrict__ spec,
+template <bool MIXED>
+__global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec *__restrict__ spec,
                                                     int l_rayleigh,
                                                     const double *__restrict__ colrec,
                                                     const double *__restrict__ colscal,
@@ -541,17 +659,26 @@ __global__ __launch_bounds__(256) void solve_kernel(CallDims d, const SgSpec *__
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down, flux_inc_direct
   double fdir = solar ? finc : 0.0;
   double rho = 0.0, dn = finc;
+  double vdir[8];
   for (int i = 0; i < L; i++) {
     if (solar) {
-      double v = wave_sum(active ? wgt * fdir : 0.0);
-      if (lane == 0) out[2 * (L + 1) + i] = v;
+      // direct flux at the top of layer i, summed 8 levels at a time
+      const int m = i & 7;
+      double x = active ? wgt * fdir : 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) if (k == m) vdir[k] = x;
+      if (m == 7) {
+        reduce8(vdir, lane);
+        if ((lane & 7) == 0) out[2 * (L + 1) + i - 7 + (lane >> 3)] = vdir[0];
+      }
     }
     if (active) {
       const double *r = rec + i * SG_NQ;
       const double *q = br + i * SG_NBQ;
       double cv = isub > 0 ? csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
       Coef o;
-      layer_coef(c, t, r, q[0], cv, o);
+      if (MIXED) layer_coef_mixed(c, t, r, q[0], cv, o);
+      else layer_coef(c, t, r, q[0], cv, o);
       double su, sd;
       if (solar) {
         double fnew = fdir * o.T0 * q[1];
@@ -571,6 +698,21 @@ __global__ __launch_bounds__(256) void solve_kernel(CallDims d, const SgSpec *__
       rho = rho_n;
     }
   }
+  if (solar) {
+    // remaining direct-flux levels (the partial block and the surface)
+    const int base = L & ~7;
+    const int m = L & 7;
+    double x = active ? wgt * fdir : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (k == m) vdir[k] = x;
+      else if (k > m) vdir[k] = 0.0;
+    }
+    reduce8(vdir, lane);
+    if ((lane & 7) == 0 && base + (lane >> 3) <= L) out[2 * (L + 1) + base + (lane >> 3)] = vdir[0];
+  } else if (lane < 64) {
+    for (int i = lane; i <= L; i += kWave) out[2 * (L + 1) + i] = 0.0;
+  }
   double fu = 0.0, fd = 0.0;
   if (active) {
     double sg = solar ? (alb_dir - alb_diff) * fdir
@@ -578,24 +720,25 @@ __global__ __launch_bounds__(256) void solve_kernel(CallDims d, const SgSpec *__
     fu = (alb_diff * dn + sg) / (1.0 - alb_diff * rho);
     fd = rho * fu + dn;
   }
-  {
-    double vu = wave_sum(wgt * fu), vd = wave_sum(wgt * fd);
-    if (lane == 0) { out[L] = vu; out[(L + 1) + L] = vd; }
-    if (solar) {
-      double v = wave_sum(active ? wgt * fdir : 0.0);
-      if (lane == 0) out[2 * (L + 1) + L] = v;
+  // back-substitution, levels L..0, 4 levels (x up/down) per reduction
+  for (int top = L; top >= 0; top -= 4) {
+    double v[8];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int lev = top - m;
+      if (lev >= 0 && lev < L && active) {
+        fu = st[lev][0] * fu + st[lev][1];
+        fd = st[lev][2] * fu + st[lev][3];
+      }
+      v[2 * m] = lev >= 0 ? wgt * fu : 0.0;
+      v[2 * m + 1] = lev >= 0 ? wgt * fd : 0.0;
+    }
+    reduce8(v, lane);
+    if ((lane & 7) == 0) {
+      const int j = lane >> 3, lev = top - (j >> 1);
+      if (lev >= 0) out[(j & 1) * (L + 1) + lev] = v[0];
     }
   }
-  for (int i = L - 1; i >= 0; i--) {
-    if (active) {
-      fu = st[i][0] * fu + st[i][1];
-      fd = st[i][2] * fu + st[i][3];
-    }
-    double vu = wave_sum(wgt * fu), vd = wave_sum(wgt * fd);
-    if (lane == 0) { out[i] = vu; out[(L + 1) + i] = vd; }
-  }
-  if (!solar && lane == 0)
-    for (int i = 0; i <= L; i++) out[2 * (L + 1) + i] = 0.0;
 }
 
 // Sum of the per-wave partials, the Planck flux added back for the
@@ -672,7 +815,7 @@ struct Context {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::map<std::string, std::unique_ptr<SpecEntry>> spectra;
-  DevBuf colrec, tlev, colscal, bandrec, bandscal, csub, partial, krange;
+  DevBuf colrec, tlev, colscal, bandrec, bandscal, csub, partial, krange, minor;
   DevBuf in_stage, out_stage;
   double last_total_ms = 0.0, last_solve_ms = 0.0;
   std::mutex mu;
@@ -834,7 +977,6 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   }
   const int items = S.n_g + (d.mcica ? S.n_sub_items[d.mode] : 0);
   d.nwpc = (items + kWave - 1) / kWave;
-  (void)precision;
 
   SocgpuRunesArgs dv;
   if (int rc = stage_inputs(a, &dv, NB)) return rc;
@@ -842,6 +984,7 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
       g_ctx.colscal.ensure((size_t)P * SG_NS * 8) || g_ctx.bandrec.ensure((size_t)P * NB * L * SG_NBQ * 8) ||
       g_ctx.bandscal.ensure((size_t)P * NB * SG_NBQ * 8) ||
       g_ctx.partial.ensure((size_t)P * d.nwpc * 3 * (L + 1) * 8) || g_ctx.krange.ensure(64) ||
+      g_ctx.minor.ensure((size_t)P * (S.n_minor > 0 ? S.n_minor : 1) * L * 2 * 8) ||
       (d.mcica && g_ctx.csub.ensure((size_t)P * L * SG_MAXSUBCOL * 8)) ||
       g_ctx.out_stage.ensure((size_t)P * (L + 1) * 3 * 8 + (size_t)P * L * 8))
     return fail(SOCGPU_ENOMEM, "device buffer allocation failed");
@@ -867,16 +1010,28 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
                                      a->cloud_vertical_decorr * 0.5, g_ctx.krange.as<int>(), e.xcw, S.n1, S.n2,
                                      g_ctx.csub.as<double>(), g_ctx.colscal.as<double>());
   }
-  prep_kernel<<<(P * NB + 63) / 64, 64, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
-                                                g_ctx.tlev.as<double>(), g_ctx.colscal.as<double>(),
-                                                g_ctx.bandrec.as<double>(), g_ctx.bandscal.as<double>());
+  if (S.n_minor > 0)
+    minor_kernel<<<(P * S.n_minor + 63) / 64, 64, 0, s>>>(d, e.dev, g_ctx.colrec.as<double>(), g_ctx.tlev.as<double>(),
+                                                          g_ctx.colscal.as<double>(), g_ctx.bandscal.as<double>(),
+                                                          g_ctx.minor.as<double>());
+  prep_kernel<<<(P * NB * L + T - 1) / T, T, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
+                                                     g_ctx.tlev.as<double>(), g_ctx.colscal.as<double>(),
+                                                     g_ctx.minor.as<double>(), g_ctx.bandrec.as<double>(),
+                                                     g_ctx.bandscal.as<double>());
   HIPCHK(hipEventRecord(g_ctx.ev1, s));
   int waves = P * d.nwpc;
-  solve_kernel<<<(waves + 3) / 4, 256, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
-                                              g_ctx.colscal.as<double>(), g_ctx.bandrec.as<double>(),
-                                              g_ctx.bandscal.as<double>(),
-                                              d.mcica ? g_ctx.csub.as<double>() : nullptr,
-                                              g_ctx.partial.as<double>());
+  if (precision == SOCGPU_MIXED)
+    solve_kernel<true><<<(waves + 3) / 4, 256, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
+                                                       g_ctx.colscal.as<double>(), g_ctx.bandrec.as<double>(),
+                                                       g_ctx.bandscal.as<double>(),
+                                                       d.mcica ? g_ctx.csub.as<double>() : nullptr,
+                                                       g_ctx.partial.as<double>());
+  else
+    solve_kernel<false><<<(waves + 3) / 4, 256, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
+                                                        g_ctx.colscal.as<double>(), g_ctx.bandrec.as<double>(),
+                                                        g_ctx.bandscal.as<double>(),
+                                                        d.mcica ? g_ctx.csub.as<double>() : nullptr,
+                                                        g_ctx.partial.as<double>());
   HIPCHK(hipEventRecord(g_ctx.ev2, s));
   finish_kernel<<<(nlev + T - 1) / T, T, 0, s>>>(d, g_ctx.partial.as<double>(), g_ctx.bandrec.as<double>(),
                                                 g_ctx.bandscal.as<double>(), oup, odn, odir);
@@ -972,7 +1127,7 @@ int socgpu_finalize(void) {
   }
   g_ctx.spectra.clear();
   for (DevBuf *b : {&g_ctx.colrec, &g_ctx.tlev, &g_ctx.colscal, &g_ctx.bandrec, &g_ctx.bandscal, &g_ctx.csub,
-                    &g_ctx.partial, &g_ctx.krange, &g_ctx.in_stage, &g_ctx.out_stage}) {
+                    &g_ctx.partial, &g_ctx.krange, &g_ctx.minor, &g_ctx.in_stage, &g_ctx.out_stage}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->n = 0;
