@@ -33,7 +33,9 @@ extern "C" {
 
 /* precision forms */
 #define SOCGPU_FP64 0          /* plain FP64: matches the reference to 1e-10 */
-#define SOCGPU_MIXED 1         /* FP32 bulk with FP64 guarded branches */
+#define SOCGPU_MIXED 1         /* FP32 optical properties and two-stream
+                                  coefficients, FP64 elimination and sums:
+                                  within 0.05 W m-2 and 0.01 K/day of FP64 */
 
 /* option codes, identical to the reference's rad_pcf.F90 values */
 #define SOCGPU_SOURCE_SOLAR 1        /* ip_solar / ip_source_illuminate */
@@ -69,11 +71,14 @@ typedef struct SocgpuRunesArgs {
   double cloud_vertical_decorr; /* Pa */
   double conv_vertical_decorr;  /* Pa */
   double cloud_horizontal_rsd;
+  int l_grey_albedo;          /* non-zero: grey_albedo for every band, both
+                                 beams (socrates_set_bound.F90:138-153) */
+  double grey_albedo;
   const double *p_layer, *t_layer, *mass, *density, *layer_heat_capacity;
   const double *h2o, *o3;                  /* [layer][profile] mmr */
   const double *p_level, *t_level;         /* [level][profile] */
   const double *t_ground, *cos_zenith_angle, *solar_irrad; /* [profile] */
-  const double *albedo_diff, *albedo_dir;  /* [band][profile] */
+  const double *albedo_diff, *albedo_dir;  /* [band][profile]; NULL = 0 */
   const double *cloud_frac, *liq_mmr, *liq_dim;           /* [layer][profile] */
   const double *conv_frac, *liq_conv_mmr, *liq_conv_dim;  /* [layer][profile] */
   const int *rand_seed;                    /* [profile] McICA seeds */
@@ -95,6 +100,15 @@ int socgpu_init(int device);
  * `name`. Replaces set_spectrum (socrates_set_spectrum.F90:29-149), which
  * reads into the module-saved spectrum_array (:20-23). */
 int socgpu_spectrum_register(const char *name, const char *path);
+
+/* Shape of a registered spectrum: band count and the identifiers of its
+ * gaseous absorbers (type_absorb of the file's block 0, rad_pcf.F90 ip_h2o = 1,
+ * ip_co2 = 2, ip_o3 = 3, ...; at most max_absorb written). Lets a binding
+ * expand a grey albedo or check which gas amounts the file uses; the
+ * reference keeps the same facts in StrSpecData%basic / %gas
+ * (def_spectrum.F90:153-237). */
+int socgpu_spectrum_query(const char *name, int *n_band, int *n_absorb,
+                          int *type_absorb, int max_absorb);
 
 /* Read McICA data and associate it with the SW/LW spectrum names (either may
  * be NULL). Replaces set_mcica (socrates_set_spectrum.F90:551-694) and

@@ -132,6 +132,8 @@ namespace sg {
 
 struct CoefF {
   float T, R, T0, s1, s2;
+  float u;    // 1 - T - R (layer diffuse absorption), to relative accuracy
+  float em0;  // 1 - T0 (direct-beam extinction), to relative accuracy
 };
 
 // -expm1(-x) in FP32: series below 0.2 (terms to x^8, relative error < 1e-10
@@ -198,10 +200,13 @@ SG_HD void two_coeff_f32(bool solar, float tau, float om, float omc, float g, fl
   float rden = 1.0f / (d1 * ope);
   c.T = e * gc * (1.0f + gam) * rden;
   c.R = gam * em * (1.0f + e) * rden;
+  const float u = em * gc / ope;                      // 1 - T - R
+  c.u = u;
   if (solar) {
     const float mu = sec0;
     float xs = tau * mu;
     float em0 = one_minus_exp(xs);
+    c.em0 = em0;
     c.T0 = xs >= 0.2f ? SG_EXPF(-xs) : 1.0f - em0;
     float omt = em * (1.0f + gam * gam * e) * rden;   // 1 - T
     float omtt0 = omt + c.T * em0;                    // 1 - T T0
@@ -227,9 +232,9 @@ SG_HD void two_coeff_f32(bool solar, float tau, float om, float omc, float g, fl
     c.s2 = fac * m + t0mt;
   } else {
     c.T0 = 0.0f;
+    c.em0 = 1.0f;
     float omtr = em * (1.0f + gam) / d1;              // 1 - T + R
     c.s1 = (omtr + sqe) / (sqe + tau * S);
-    float u = em * gc / ope;                          // 1 - T - R
     // w = D tau - u, exact to O(x^3) terms for thin layers
     float w;
     if (x < 0.2f) {

@@ -50,8 +50,12 @@ struct SgDrop {
   double parm[SG_MAXB][SG_NDPARM];
 };
 
+#define SG_MAXABS 32
+
 struct SgSpec {
   int n_band, n_g, n_minor, n_deg;
+  int n_absorb;                  // gaseous absorbers of the file and their
+  int type_absorb[SG_MAXABS];    // identifiers (rad_pcf.F90 ip_h2o = 1, ...)
   double t_ref_planck;
   SgBand band[SG_MAXB];
   SgTerm g[SG_MAXG];

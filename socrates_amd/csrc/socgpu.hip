@@ -202,8 +202,14 @@ __global__ void pack_kernel(CallDims d, const SgSpec *__restrict__ spec, SocgpuR
     s[SG_S_NCLD] = 0.0;
     for (int b = 0; b < d.NB; b++) {
       double *bs = bandscal + ((size_t)col * d.NB + b) * SG_NBQ;
-      bs[2] = a.albedo_diff ? a.albedo_diff[(size_t)b * P + col] : 0.0;
-      bs[3] = a.albedo_dir ? a.albedo_dir[(size_t)b * P + col] : 0.0;
+      // set_bound (socrates_set_bound.F90:138-233): grey albedo, else per band, else 0
+      if (a.l_grey_albedo) {
+        bs[2] = a.grey_albedo;
+        bs[3] = a.grey_albedo;
+      } else {
+        bs[2] = a.albedo_diff ? a.albedo_diff[(size_t)b * P + col] : 0.0;
+        bs[3] = a.albedo_dir ? a.albedo_dir[(size_t)b * P + col] : 0.0;
+      }
     }
   }
 }
@@ -592,8 +598,16 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   }
   CoefF f;
   two_coeff_f32(c.solar, tau, om, omc, g, (float)c.sec0, f);
-  if (cloudy) { f.T *= frac1; f.R *= frac1; f.T0 *= frac1; f.s1 *= frac1; f.s2 *= frac1; }
-  o.T = f.T; o.R = f.R; o.T0 = f.T0; o.s1 = f.s1; o.s2 = f.s2;
+  // T and T0 are taken to FP64 through their complements, so the layer's
+  // absorption (1 - T - R, 1 - T0) keeps FP32 relative accuracy in the FP64
+  // elimination: a thin layer's flux divergence, and so its heating rate,
+  // is then accurate where T itself is 1 to within FP32 rounding.
+  o.R = f.R;
+  o.T = (1.0 - (double)f.R) - (double)f.u;
+  o.T0 = 1.0 - (double)f.em0;
+  o.s1 = f.s1;
+  o.s2 = f.s2;
+  if (cloudy) { o.T *= frac1; o.R *= frac1; o.T0 *= frac1; o.s1 *= frac1; o.s2 *= frac1; }
 }
 
 // The column solve, one wave per column (or per chunk of 64 items of a
@@ -867,6 +881,20 @@ int socgpu_spectrum_register(const char *name, const char *path) {
     if (it->second->xcw) (void)hipFree(it->second->xcw);
   }
   g_ctx.spectra[name] = std::move(e);
+  return 0;
+}
+
+int socgpu_spectrum_query(const char *name, int *n_band, int *n_absorb, int *type_absorb,
+                          int max_absorb) {
+  if (!name) return fail(SOCGPU_EINVAL, "spectrum name is required");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  auto it = g_ctx.spectra.find(name);
+  if (it == g_ctx.spectra.end()) return fail(SOCGPU_ENOTFOUND, "Spectrum name not found: %s", name);
+  const SgSpec &S = it->second->host;
+  if (n_band) *n_band = S.n_band;
+  if (n_absorb) *n_absorb = S.n_absorb;
+  if (type_absorb)
+    for (int i = 0; i < S.n_absorb && i < max_absorb; i++) type_absorb[i] = S.type_absorb[i];
   return 0;
 }
 

@@ -115,6 +115,9 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
       }
       L.next();
       for (int i = 0; i < nabs; i++) type_absorb.push_back(inum(L.next(), 12, 5));
+      if (nabs > SG_MAXABS) { *err = "more than 32 gaseous absorbers"; return -3; }
+      s->n_absorb = nabs;
+      for (int i = 0; i < nabs; i++) s->type_absorb[i] = type_absorb[i];
       if (s->n_band <= 0 || s->n_band > SG_MAXB) { *err = "band count outside 1..16"; return -1; }
     } else if (type == 2) {
       if (ver != 0) { *err = "solar block version not supported"; return -3; }

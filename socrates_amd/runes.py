@@ -47,7 +47,8 @@ class RunesArgs(ctypes.Structure):
         "i_overlap", "i_inhom", "i_mcica_sampling", "i_st_water", "i_cnv_water",
         "n_subcol_gen", "l_inputs_on_device")] + [(n, ctypes.c_double) for n in (
         "co2_mix_ratio", "cloud_vertical_decorr", "conv_vertical_decorr",
-        "cloud_horizontal_rsd")] + [(n, _D) for n in (
+        "cloud_horizontal_rsd")] + [("l_grey_albedo", ctypes.c_int),
+                                    ("grey_albedo", ctypes.c_double)] + [(n, _D) for n in (
         "p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
         "p_level", "t_level", "t_ground", "cos_zenith_angle", "solar_irrad",
         "albedo_diff", "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac",
@@ -127,7 +128,8 @@ def _ptr(x, keep, dtype=np.float64):
 def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
           density=None, layer_heat_capacity=None, h2o=None, o3=None, co2_mix_ratio=0.0,
           p_level=None, t_level=None, t_ground=None, cos_zenith_angle=None,
-          solar_irrad=None, albedo_diff=None, albedo_dir=None, l_rayleigh=False,
+          solar_irrad=None, albedo_diff=None, albedo_dir=None, l_grey_albedo=False,
+          grey_albedo=0.0, l_rayleigh=False,
           cloud_frac=None, liq_mmr=None, liq_dim=None, conv_frac=None, liq_conv_mmr=None,
           liq_conv_dim=None, cloud_vertical_decorr=0.0, conv_vertical_decorr=0.0,
           cloud_horizontal_rsd=0.0, i_cloud_representation=0, i_overlap=0, i_inhom=0,
@@ -154,6 +156,7 @@ def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
     a.co2_mix_ratio = float(co2_mix_ratio)
     a.cloud_vertical_decorr, a.conv_vertical_decorr = float(cloud_vertical_decorr), float(conv_vertical_decorr)
     a.cloud_horizontal_rsd = float(cloud_horizontal_rsd)
+    a.l_grey_albedo, a.grey_albedo = int(bool(l_grey_albedo)), float(grey_albedo)
     for k, v in inputs.items():
         if v is not None and _is_torch(v) != on_dev:
             raise SocgpuError("all inputs must be in the same memory space (%s)" % k)

@@ -1,0 +1,61 @@
+"""The Fortran binding (socrates_amd/fortran/socrates_runes_gpu.f90): a host
+program written against the reference's module and keyword names
+(set_spectrum, set_mcica, runes, StrDiag) runs on the device and matches the
+oracle to the FP64 tolerances of tests/test_gpu_parity.py; its mixed form
+stays within the mixed bounds. Also: the grey-albedo option of set_bound."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from socrates_amd import runes as R
+from socrates_amd import synth
+from socrates_amd.case_io import read_output
+from test_gpu_parity import MCICA, NBAND, SPEC, check_fp64, check_mixed
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "socrates_amd", "lib", "runes_gpu_case")
+
+
+def test_fortran_binding_built():
+    assert os.path.exists(EXE), "build the Fortran binding (python socrates_amd/build.py)"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("src,cloudy", [("lw", True), ("sw", True), ("sw", False)])
+def test_fortran_binding_matches_oracle(src, cloudy):
+    case = synth.make_case(12, 50, src, NBAND[src], seed=900 + cloudy, cloudy=cloudy)
+    ref = oracle.runes(case, SPEC[src], MCICA if cloudy else None)
+    with tempfile.TemporaryDirectory() as tmp:
+        cf = os.path.join(tmp, "case.bin")
+        case.write(cf)
+        for prec in ("fp64", "mixed"):
+            of = os.path.join(tmp, "out_%s.bin" % prec)
+            r = subprocess.run([EXE, SPEC["sw"], SPEC["lw"], MCICA if cloudy else "-", cf, of, prec],
+                               capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stdout + r.stderr
+            out = read_output(of, case.n_profile, case.n_layer)
+            if prec == "fp64":
+                check_fp64(out, ref, "fortran %s" % src, src, case.layer_heat_capacity)
+            else:
+                check_mixed(out, ref, "fortran mixed %s" % src)
+
+
+@pytest.mark.gpu
+def test_grey_albedo_is_every_band():
+    R.register_default_spectra()
+    case = synth.make_case(6, 40, "sw", NBAND["sw"], seed=77, cloudy=False)
+    grey = 0.23
+    case.albedo_diff[:] = grey
+    case.albedo_dir[:] = grey
+    ref = oracle.runes(case, SPEC["sw"], None)
+    out = R.runes(case.n_profile, case.n_layer, "sw", 1, case.p_layer, case.t_layer, case.mass,
+                  density=case.density, layer_heat_capacity=case.layer_heat_capacity,
+                  h2o=case.h2o, o3=case.o3, co2_mix_ratio=case.co2_mix_ratio,
+                  p_level=case.p_level, t_level=case.t_level, t_ground=case.t_ground,
+                  cos_zenith_angle=case.cos_zen, solar_irrad=case.solar_irrad,
+                  l_grey_albedo=True, grey_albedo=grey, l_rayleigh=case.l_rayleigh)
+    check_fp64(out, ref, "grey albedo", "sw", case.layer_heat_capacity)
