@@ -124,21 +124,33 @@ int socgpu_mcica_register(const char *path, const char *sw_name,
 int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *args,
                  SocgpuDiag *diag, int precision);
 
-/* Kernel-level entry (no reference counterpart as a single routine): the
- * two_coeff + trans_source_coeff + source + solver_homogen_direct chain of
- * two_stream.F90:160-205 for prescribed layer optical properties, one solve
- * per (profile, term). tau/omega/asymmetry are [term][layer][profile];
- * sec_0 / flux_inc / albedos per profile; planck_* for the infra-red
- * ([level][profile], ground [profile]). Outputs flux_up/flux_down
- * [term][level][profile] (differential for the infra-red, as the solver
- * returns them), flux_direct likewise (solar). Host pointers. */
+/* Kernel-level entry: the two-stream solve on prescribed layer optical
+ * properties, n_term independent solves per profile (for example the
+ * g-points of a band). Replaces two_stream (two_stream.F90:160-205):
+ * two_coeff, ir_source (ir_source.F90:93-120) or solar_source
+ * (solar_source.F90:104-162), the surface source (column_solver.F90:
+ * 160-185) and solver_homogen_direct, with the same argument meanings:
+ * isolir SOCGPU_SOURCE_*; l_scale_solar multiplies the direct beam by
+ * adjust_solar_ke; l_ir_source_quad selects the quadratic infra-red source;
+ * infra-red fluxes are differential (Planck flux removed), as the solver
+ * returns them. Layouts: tau, omega, asymmetry [term][layer][profile];
+ * diff_planck, diff_planck_2, adjust_solar_ke [layer][profile]; sec_0,
+ * flux_inc_down, flux_inc_direct, albedos, d_planck_flux_surface [profile];
+ * outputs flux_direct (solar, may be NULL), flux_up, flux_down
+ * [term][level][profile]. Pointers are device pointers when l_on_device.
+ * Synchronous. */
 int socgpu_two_stream(int n_profile, int n_layer, int n_term, int isolir,
+                      int l_scale_solar, int l_ir_source_quad,
                       const double *tau, const double *omega,
                       const double *asymmetry, const double *sec_0,
-                      const double *flux_inc_down, const double *albedo_diff,
-                      const double *albedo_dir, const double *planck_level,
-                      const double *planck_layer, const double *planck_ground,
-                      double *flux_up, double *flux_down, double *flux_direct,
+                      const double *flux_inc_down,
+                      const double *flux_inc_direct,
+                      const double *diffuse_albedo,
+                      const double *direct_albedo, const double *diff_planck,
+                      const double *diff_planck_2,
+                      const double *d_planck_flux_surface,
+                      const double *adjust_solar_ke, double *flux_direct,
+                      double *flux_up, double *flux_down, int l_on_device,
                       int precision);
 
 /* McICA sub-column generator on the device (cloud_gen,
@@ -155,9 +167,9 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req,
 /* Message of the last failure on this host thread. */
 const char *socgpu_last_error(void);
 
-/* Timing of the last socgpu_runes call on the device (ms), measured with
- * HIP events on the library stream around the whole call and around the
- * solve kernel alone. */
+/* Timing of the last socgpu_runes or socgpu_two_stream call on the device
+ * (ms), measured with HIP events on the library stream around the whole
+ * call and around the solve kernel alone. */
 int socgpu_last_timing(double *total_ms, double *solve_ms);
 
 /* Release device memory and the stream. */

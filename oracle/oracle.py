@@ -119,3 +119,21 @@ def solver_homogen_direct(trans, reflect, s_down, s_up, albedo, flux_inc_down, s
         ctypes.c_double(albedo), ctypes.c_double(flux_inc_down), ctypes.c_double(source_ground),
         _ptr(ft, keep))
     return ft
+
+
+def two_stream(tsc):
+    """soc_oracle_two_stream on a kernel-level case (dict from
+    socrates_amd.synth.make_two_stream_case, one term: arrays [layer][profile]).
+    Returns (flux_direct [level][profile], flux_up, flux_down [level][profile])."""
+    keep = []
+    P, L = tsc["n_profile"], tsc["n_layer"]
+    fdir = np.zeros((L + 1, P))
+    ft = np.zeros((2 * L + 2, P))
+    lib().soc_oracle_two_stream(
+        P, L, tsc["isolir"], tsc["l_scale_solar"], tsc["l_ir_source_quad"],
+        *[_ptr(tsc[k], keep) for k in ("tau", "omega", "asymmetry", "sec_0", "flux_inc_down",
+                                       "flux_inc_direct", "diffuse_albedo", "direct_albedo",
+                                       "diff_planck", "diff_planck_2", "d_planck_flux_surface",
+                                       "adjust_solar_ke")],
+        _ptr(fdir, keep), _ptr(ft, keep))
+    return fdir, ft[0::2], ft[1::2]

@@ -382,6 +382,63 @@ void soc_oracle_solver_homogen_direct(int n_layer, const double *trans,
   free(beta);
 }
 
+/* two_stream.F90:160-205 with ir_source.F90:93-120, solar_source.F90:
+ * 104-162 and column_solver.F90:160-185 (no spherical geometry, no
+ * orography, direct tau unscaled: the default control). */
+void soc_oracle_two_stream(int n_profile, int n_layer, int isolir, int l_scale_solar,
+                           int l_ir_source_quad, const double *tau, const double *omega,
+                           const double *asymmetry, const double *sec_0,
+                           const double *flux_inc_down, const double *flux_inc_direct,
+                           const double *diffuse_albedo, const double *direct_albedo,
+                           const double *diff_planck, const double *diff_planck_2,
+                           const double *d_planck_flux_surface,
+                           const double *adjust_solar_ke, double *flux_direct,
+                           double *flux_total) {
+  const int P = n_profile, L = n_layer;
+  double *tr = (double *)malloc(sizeof(double) * 4 * (size_t)L);
+  double *rf = tr + L, *sd = rf + L, *su = sd + L;
+  double *ft = (double *)malloc(sizeof(double) * (size_t)(2 * L + 2));
+  for (int l = 0; l < P; l++) {
+    double fd = isolir == IP_SOLAR ? flux_inc_direct[l] : 0.0;
+    if (isolir == IP_SOLAR) flux_direct[l] = fd;
+    for (int i = 0; i < L; i++) {
+      const size_t k = (size_t)i * P + l;
+      coeff_t o;
+      two_coeff_one(isolir, tau[k], omega[k], asymmetry[k], isolir == IP_SOLAR ? sec_0[l] : 1.0, &o);
+      tr[i] = o.trans;
+      rf[i] = o.reflect;
+      if (isolir == IP_SOLAR) {
+        if (l_scale_solar) {
+          double fnew = fd * o.trans_0 * adjust_solar_ke[k];
+          su[i] = o.sc1 * fd;
+          sd[i] = (o.sc2 - o.trans_0) * fd + fnew;
+          fd = fnew;
+        } else {
+          double fnew = fd * o.trans_0;
+          su[i] = o.sc1 * fd;
+          sd[i] = o.sc2 * fd;
+          fd = fnew;
+        }
+        flux_direct[(size_t)(i + 1) * P + l] = fd;
+      } else if (l_ir_source_quad) {
+        su[i] = o.sc1 * diff_planck[k] + o.sc2 * diff_planck_2[k];
+        sd[i] = -o.sc1 * diff_planck[k] + o.sc2 * diff_planck_2[k];
+      } else {
+        su[i] = o.sc1 * diff_planck[k];
+        sd[i] = -su[i];
+      }
+    }
+    double source_ground = isolir == IP_SOLAR
+        ? (direct_albedo[l] - diffuse_albedo[l]) * fd
+        : (1.0 - diffuse_albedo[l]) * d_planck_flux_surface[l];
+    soc_oracle_solver_homogen_direct(L, tr, rf, sd, su, diffuse_albedo[l], flux_inc_down[l],
+                                     source_ground, ft);
+    for (int k = 0; k < 2 * L + 2; k++) flux_total[(size_t)k * P + l] = ft[k];
+  }
+  free(ft);
+  free(tr);
+}
+
 /* ------------------------------------------------------------------------ */
 /* cloud_gen, socrates_cloud_gen.F90:54-478 (ioverlap 1-3 and 0)            */
 #define LCG_M 86436

@@ -87,6 +87,23 @@ void soc_oracle_solver_homogen_direct(int n_layer, const double *trans,
                                       double flux_inc_down,
                                       double source_ground, double *flux_total);
 
+/* two_stream (two_stream.F90:160-205) on prescribed optical properties:
+ * two_coeff for every layer, ir_source (ir_source.F90:93-120) or
+ * solar_source (solar_source.F90:104-162, column_solver.F90:160-185), the
+ * surface source, and solver_homogen_direct. Arrays [layer][profile] or
+ * [profile]; adjust_solar_ke is read only when l_scale_solar. Outputs:
+ * flux_direct [level][profile] (solar), flux_total [k][profile] for the
+ * 1-based interleaved index k = 1..2L+2 stored at k-1 (odd up, even down). */
+void soc_oracle_two_stream(int n_profile, int n_layer, int isolir, int l_scale_solar,
+                           int l_ir_source_quad, const double *tau, const double *omega,
+                           const double *asymmetry, const double *sec_0,
+                           const double *flux_inc_down, const double *flux_inc_direct,
+                           const double *diffuse_albedo, const double *direct_albedo,
+                           const double *diff_planck, const double *diff_planck_2,
+                           const double *d_planck_flux_surface,
+                           const double *adjust_solar_ke, double *flux_direct,
+                           double *flux_total);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -800,6 +801,94 @@ __global__ void heating_kernel(int P, int L, const double *__restrict__ up,
   hr[idx] = heat_cap ? fdiv / heat_cap[idx] : fdiv;
 }
 
+// Kernel-level two-stream solve on prescribed optical properties, one lane
+// per (term, profile) (profile fastest, so every per-layer read and
+// per-level write is coalesced): two_coeff, the infra-red (ir_source.F90:
+// 93-120) or solar (solar_source.F90:104-162) sources, the surface source
+// (column_solver.F90:160-185) and the column solve (solver_homogen_direct.F90
+// :93-127 as a top-down elimination and bottom-up back-substitution).
+// Arrays [term][layer][profile] (optical properties, outputs [term][level]
+// [profile]) and [layer][profile] / [profile] for the rest.
+struct TwoStreamArgs {
+  int P, L, G, isolir, l_scale_solar, l_quad;
+  const double *tau, *omega, *asym, *sec0, *finc_down, *finc_dir, *alb_diff, *alb_dir;
+  const double *dpl, *dpl2, *dsurf, *adjust;
+  double *fdir, *fup, *fdown;
+};
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void two_stream_kernel(TwoStreamArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P * a.G) return;
+  const int P = a.P, L = a.L;
+  const int l = idx % P, g = idx / P;
+  const bool solar = a.isolir == SOCGPU_SOURCE_SOLAR;
+  const double sec0 = solar ? a.sec0[l] : 1.0;
+  const size_t t0 = (size_t)g * L * P + l;
+  const size_t o0 = (size_t)g * (L + 1) * P + l;
+  double st[SG_MAXL][4];
+  double fd = solar ? a.finc_dir[l] : 0.0;
+  if (solar && a.fdir) a.fdir[o0] = fd;
+  double rho = 0.0, dn = a.finc_down[l];
+  for (int i = 0; i < L; i++) {
+    const size_t k = t0 + (size_t)i * P;
+    const size_t kl = (size_t)i * P + l;
+    const double tau = a.tau[k], om = a.omega[k], asym = a.asym[k];
+    Coef o;
+    if (MIXED) {
+      CoefF f;
+      two_coeff_f32(solar, (float)tau, (float)om, (float)fmax(1.0 - om, 32.0 * kEps), (float)asym,
+                    (float)sec0, f);
+      o.R = f.R;
+      o.T = (1.0 - (double)f.R) - (double)f.u;
+      o.T0 = 1.0 - (double)f.em0;
+      o.s1 = f.s1;
+      o.s2 = f.s2;
+    } else {
+      two_coeff(solar, tau, om, asym, sec0, o);
+    }
+    double su, sd;
+    if (solar) {
+      if (a.l_scale_solar) {
+        double fnew = fd * o.T0 * a.adjust[kl];
+        su = o.s1 * fd;
+        sd = (o.s2 - o.T0) * fd + fnew;
+        fd = fnew;
+      } else {
+        su = o.s1 * fd;
+        sd = o.s2 * fd;
+        fd = fd * o.T0;
+      }
+      if (a.fdir) a.fdir[o0 + (size_t)(i + 1) * P] = fd;
+    } else if (a.l_quad) {
+      su = o.s1 * a.dpl[kl] + o.s2 * a.dpl2[kl];
+      sd = -o.s1 * a.dpl[kl] + o.s2 * a.dpl2[kl];
+    } else {
+      su = o.s1 * a.dpl[kl];
+      sd = -su;
+    }
+    double bp = 1.0 / (1.0 - o.R * rho);
+    double aa = o.T * bp;
+    double bb = (o.R * dn + su) * bp;
+    st[i][0] = aa; st[i][1] = bb; st[i][2] = rho; st[i][3] = dn;
+    double rho_n = o.R + o.T * rho * aa;
+    dn = o.T * (rho * bb + dn) + sd;
+    rho = rho_n;
+  }
+  const double ad = a.alb_diff[l];
+  const double sg = solar ? (a.alb_dir[l] - ad) * fd : (1.0 - ad) * a.dsurf[l];
+  double fu = (ad * dn + sg) / (1.0 - ad * rho);
+  double fdn = rho * fu + dn;
+  a.fup[o0 + (size_t)L * P] = fu;
+  a.fdown[o0 + (size_t)L * P] = fdn;
+  for (int i = L - 1; i >= 0; i--) {
+    fu = st[i][0] * fu + st[i][1];
+    fdn = st[i][2] * fu + st[i][3];
+    a.fup[o0 + (size_t)i * P] = fu;
+    a.fdown[o0 + (size_t)i * P] = fdn;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 struct DevBuf {
   void *p = nullptr;
@@ -1089,10 +1178,80 @@ int socgpu_last_timing(double *total_ms, double *solve_ms) {
   return 0;
 }
 
-int socgpu_two_stream(int, int, int, int, const double *, const double *, const double *, const double *,
-                      const double *, const double *, const double *, const double *, const double *,
-                      const double *, double *, double *, double *, int) {
-  return fail(SOCGPU_EUNSUPPORTED, "socgpu_two_stream is not built yet");
+int socgpu_two_stream(int n_profile, int n_layer, int n_term, int isolir, int l_scale_solar,
+                      int l_ir_source_quad, const double *tau, const double *omega,
+                      const double *asymmetry, const double *sec_0, const double *flux_inc_down,
+                      const double *flux_inc_direct, const double *diffuse_albedo,
+                      const double *direct_albedo, const double *diff_planck,
+                      const double *diff_planck_2, const double *d_planck_flux_surface,
+                      const double *adjust_solar_ke, double *flux_direct, double *flux_up,
+                      double *flux_down, int l_on_device, int precision) {
+  if (int rc = ensure_init()) return rc;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  const int P = n_profile, L = n_layer, G = n_term;
+  if (P <= 0 || L <= 0 || G <= 0) return fail(SOCGPU_EINVAL, "n_profile, n_layer and n_term must be positive");
+  if (L > SG_MAXL) return fail(SOCGPU_EUNSUPPORTED, "n_layer %d above %d", L, SG_MAXL);
+  if ((size_t)P * G > (size_t)INT32_MAX) return fail(SOCGPU_EINVAL, "n_profile * n_term too large");
+  if (precision != SOCGPU_FP64 && precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision %d", precision);
+  const bool solar = isolir == SOCGPU_SOURCE_SOLAR;
+  if (!solar && isolir != SOCGPU_SOURCE_THERMAL) return fail(SOCGPU_EINVAL, "Radiative source option unrecognised.");
+  if (!tau || !omega || !asymmetry || !flux_inc_down || !diffuse_albedo || !flux_up || !flux_down)
+    return fail(SOCGPU_EINVAL, "tau, omega, asymmetry, flux_inc_down, diffuse_albedo and the outputs are required");
+  if (solar && (!sec_0 || !flux_inc_direct || !direct_albedo || (l_scale_solar && !adjust_solar_ke)))
+    return fail(SOCGPU_EINVAL, "solar solve needs sec_0, flux_inc_direct, direct_albedo (and adjust_solar_ke when scaled)");
+  if (!solar && (!diff_planck || !d_planck_flux_surface || (l_ir_source_quad && !diff_planck_2)))
+    return fail(SOCGPU_EINVAL, "infra-red solve needs diff_planck, d_planck_flux_surface (and diff_planck_2 when quadratic)");
+  const size_t nopt = (size_t)G * L * P, nlay = (size_t)L * P, nout = (size_t)G * (L + 1) * P;
+  TwoStreamArgs a;
+  a.P = P; a.L = L; a.G = G; a.isolir = isolir; a.l_scale_solar = l_scale_solar != 0;
+  a.l_quad = l_ir_source_quad != 0;
+  hipStream_t s = g_ctx.stream;
+  if (l_on_device) {
+    a.tau = tau; a.omega = omega; a.asym = asymmetry; a.sec0 = sec_0; a.finc_down = flux_inc_down;
+    a.finc_dir = flux_inc_direct; a.alb_diff = diffuse_albedo; a.alb_dir = direct_albedo;
+    a.dpl = diff_planck; a.dpl2 = diff_planck_2; a.dsurf = d_planck_flux_surface; a.adjust = adjust_solar_ke;
+    a.fdir = flux_direct; a.fup = flux_up; a.fdown = flux_down;
+  } else {
+    struct F { const double **dst; const double *src; size_t n; };
+    F f[] = {{&a.tau, tau, nopt}, {&a.omega, omega, nopt}, {&a.asym, asymmetry, nopt},
+             {&a.sec0, sec_0, (size_t)P}, {&a.finc_down, flux_inc_down, (size_t)P},
+             {&a.finc_dir, flux_inc_direct, (size_t)P}, {&a.alb_diff, diffuse_albedo, (size_t)P},
+             {&a.alb_dir, direct_albedo, (size_t)P}, {&a.dpl, diff_planck, nlay},
+             {&a.dpl2, diff_planck_2, nlay}, {&a.dsurf, d_planck_flux_surface, (size_t)P},
+             {&a.adjust, adjust_solar_ke, nlay}};
+    size_t total = 0;
+    for (auto &x : f) if (x.src) total += x.n;
+    if (g_ctx.in_stage.ensure(total * 8 + 64) || g_ctx.out_stage.ensure(3 * nout * 8))
+      return fail(SOCGPU_ENOMEM, "staging allocation failed");
+    double *base = g_ctx.in_stage.as<double>();
+    size_t off = 0;
+    for (auto &x : f) {
+      if (!x.src) { *x.dst = nullptr; continue; }
+      HIPCHK(hipMemcpyAsync(base + off, x.src, x.n * 8, hipMemcpyHostToDevice, s));
+      *x.dst = base + off;
+      off += x.n;
+    }
+    double *ob = g_ctx.out_stage.as<double>();
+    a.fup = ob; a.fdown = ob + nout; a.fdir = flux_direct ? ob + 2 * nout : nullptr;
+  }
+  if (!solar) a.fdir = nullptr;
+  const int threads = 256, blocks = (int)(((size_t)P * G + threads - 1) / threads);
+  HIPCHK(hipEventRecord(g_ctx.ev1, s));
+  if (precision == SOCGPU_MIXED) two_stream_kernel<true><<<blocks, threads, 0, s>>>(a);
+  else two_stream_kernel<false><<<blocks, threads, 0, s>>>(a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(g_ctx.ev2, s));
+  if (!l_on_device) {
+    HIPCHK(hipMemcpyAsync(flux_up, a.fup, nout * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(flux_down, a.fdown, nout * 8, hipMemcpyDeviceToHost, s));
+    if (a.fdir) HIPCHK(hipMemcpyAsync(flux_direct, a.fdir, nout * 8, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, g_ctx.ev1, g_ctx.ev2));
+  g_ctx.last_total_ms = ms;
+  g_ctx.last_solve_ms = ms;
+  return 0;
 }
 
 int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ipph, int ioverlap,

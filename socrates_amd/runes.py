@@ -244,3 +244,54 @@ def cloud_gen(rand_seed, rlc_cf, rlc_cw, sigma_qcw, avg_cf, p_layer, xcw, n_subc
         _ptr(p_layer, keep), n1, n2, _ptr(np.asfortranarray(xcw).ravel(order="F"), keep),
         _ptr(c_sub, keep), _ptr(ncld, keep, np.int32)))
     return c_sub, ncld
+
+
+_TS_INPUTS = ("tau", "omega", "asymmetry", "sec_0", "flux_inc_down", "flux_inc_direct",
+              "diffuse_albedo", "direct_albedo", "diff_planck", "diff_planck_2",
+              "d_planck_flux_surface", "adjust_solar_ke")
+
+
+def two_stream(n_profile, n_layer, n_term, isolir, tau, omega, asymmetry, sec_0=None,
+               flux_inc_down=None, flux_inc_direct=None, diffuse_albedo=None,
+               direct_albedo=None, diff_planck=None, diff_planck_2=None,
+               d_planck_flux_surface=None, adjust_solar_ke=None, l_scale_solar=True,
+               l_ir_source_quad=True, precision="fp64", out=None):
+    """Kernel-level two-stream solve (socgpu_two_stream; the reference's
+    two_stream, two_stream.F90:160-205) on prescribed optical properties.
+    Arrays [term][layer][profile] for tau/omega/asymmetry, [layer][profile]
+    or [profile] for the rest, numpy or torch device tensors (all alike).
+    Returns dict(flux_direct, flux_up, flux_down), each [term][level][profile]
+    (infra-red fluxes differential)."""
+    init()
+    G, L, P = int(n_term), int(n_layer), int(n_profile)
+    vals = dict(tau=tau, omega=omega, asymmetry=asymmetry, sec_0=sec_0,
+                flux_inc_down=flux_inc_down, flux_inc_direct=flux_inc_direct,
+                diffuse_albedo=diffuse_albedo, direct_albedo=direct_albedo,
+                diff_planck=diff_planck, diff_planck_2=diff_planck_2,
+                d_planck_flux_surface=d_planck_flux_surface, adjust_solar_ke=adjust_solar_ke)
+    on_dev = _is_torch(tau)
+    for k, v in vals.items():
+        if v is not None and _is_torch(v) != on_dev:
+            raise SocgpuError("all inputs must be in the same memory space (%s)" % k)
+    if out is None:
+        if on_dev:
+            import torch
+            mk = lambda: torch.empty((G, L + 1, P), dtype=torch.float64, device=tau.device)  # noqa: E731
+        else:
+            mk = lambda: np.zeros((G, L + 1, P))  # noqa: E731
+        out = dict(flux_direct=mk(), flux_up=mk(), flux_down=mk())
+    keep = []
+    _check(lib().socgpu_two_stream(
+        P, L, G, int(isolir), int(bool(l_scale_solar)), int(bool(l_ir_source_quad)),
+        *[_ptr(vals[k], keep) for k in _TS_INPUTS],
+        _ptr(out["flux_direct"], keep), _ptr(out["flux_up"], keep), _ptr(out["flux_down"], keep),
+        int(on_dev), _PREC[precision]))
+    return out
+
+
+def two_stream_case(tsc, precision="fp64", out=None):
+    """two_stream() on a case dict of socrates_amd.synth.make_two_stream_case."""
+    return two_stream(tsc["n_profile"], tsc["n_layer"], tsc["n_term"], tsc["isolir"],
+                      **{k: tsc[k] for k in _TS_INPUTS},
+                      l_scale_solar=tsc["l_scale_solar"],
+                      l_ir_source_quad=tsc["l_ir_source_quad"], precision=precision, out=out)

@@ -162,3 +162,66 @@ def make_case(n_profile, n_layer, source, n_band, seed, cloudy=False,
             c.n_subcol_gen = 0
             c.i_solver_kind = 1
     return c
+
+
+SIGMA_SB = 5.670374419e-8
+
+
+def make_two_stream_case(n_profile, n_layer, isolir, seed, n_term=1, l_scale_solar=True,
+                         l_ir_source_quad=True, omc_range=(-8.0, -1.0),
+                         tau_range=(-6.0, 3.0)):
+    """Prescribed optical properties for the kernel-level entry (the
+    two-stream solve without gas or cloud optics): per (term, layer,
+    profile) tau ~ logU(10^tau_range), omega = 1 - 10^U(omc_range),
+    g ~ U(0, 0.95); per profile sec(theta_0) from cos ~ U(0.05, 1),
+    albedos ~ U(0.05, 0.8), incident beam 1361 cos(theta_0) W m-2;
+    infra-red: band Planck fluxes B = sigma T^4 / 4 at the levels, layers and
+    surface of a profiles() column (zero incident radiation, so the
+    differential incident flux is -B(TOA)).
+    Arrays are [term][layer][profile] for the optical properties and
+    [layer][profile] / [profile] for the rest."""
+    rng = np.random.default_rng(seed)
+    P, L, G = n_profile, n_layer, n_term
+    tau = 10.0 ** rng.uniform(tau_range[0], tau_range[1], (G, L, P))
+    omega = 1.0 - 10.0 ** rng.uniform(omc_range[0], omc_range[1], (G, L, P))
+    asym = rng.uniform(0.0, 0.95, (G, L, P))
+    cosz = rng.uniform(0.05, 1.0, P)
+    st = profiles(P, L, seed + 1)
+    b_lev = 0.25 * SIGMA_SB * st["t_level"] ** 4
+    b_lay = 0.25 * SIGMA_SB * st["t_layer"] ** 4
+    b_gnd = 0.25 * SIGMA_SB * (st["t_ground"] + rng.normal(0.0, 2.0, P)) ** 4
+    solar = isolir == IP_SOLAR
+    return dict(
+        n_profile=P, n_layer=L, n_term=G, isolir=isolir, l_scale_solar=int(l_scale_solar),
+        l_ir_source_quad=int(l_ir_source_quad), tau=tau, omega=omega, asymmetry=asym,
+        sec_0=1.0 / cosz,
+        flux_inc_direct=(1361.0 * cosz) if solar else np.zeros(P),
+        flux_inc_down=(1361.0 * cosz) if solar else -b_lev[0].copy(),
+        diffuse_albedo=rng.uniform(0.05, 0.8, P), direct_albedo=rng.uniform(0.05, 0.8, P),
+        diff_planck=b_lev[1:] - b_lev[:-1],
+        diff_planck_2=2.0 * (b_lev[1:] + b_lev[:-1] - 2.0 * b_lay),
+        d_planck_flux_surface=b_gnd - b_lev[-1],
+        adjust_solar_ke=np.exp(rng.uniform(-0.02, 0.02, (L, P))) if solar else np.ones((L, P)))
+
+
+def two_stream_term(tsc, g):
+    """The single-term view of a kernel-level case (arrays [layer][profile])."""
+    out = dict(tsc)
+    for k in ("tau", "omega", "asymmetry"):
+        out[k] = np.ascontiguousarray(tsc[k][g])
+    out["n_term"] = 1
+    return out
+
+
+def write_two_stream_case(tsc, path, g=0):
+    """Binary input of the reference harness oracle/harness/soc_two_stream_case.f90
+    (one term); arrays written in Fortran (profile, layer) order, which is the
+    C [layer][profile] memory order of these arrays."""
+    t = two_stream_term(tsc, g)
+    with open(path, "wb") as f:
+        np.array([t["n_profile"], t["n_layer"], t["isolir"], t["l_scale_solar"],
+                  t["l_ir_source_quad"]], dtype=np.int32).tofile(f)
+        for k in ("tau", "omega", "asymmetry", "sec_0", "flux_inc_down", "flux_inc_direct",
+                  "diffuse_albedo", "direct_albedo", "diff_planck", "diff_planck_2",
+                  "d_planck_flux_surface", "adjust_solar_ke"):
+            np.ascontiguousarray(t[k], dtype=np.float64).tofile(f)

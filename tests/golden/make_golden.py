@@ -7,6 +7,8 @@ harnesses in oracle/harness) and stored with its inputs:
   runes_<name>.npz      inputs of one runes() call + reference fluxes and
                         heating rates (K/s)
   cloud_gen_ov<k>.npz   inputs of cloud_gen + reference c_sub / n_subcol_cld
+  two_stream_<v>.npz    prescribed optical properties of a kernel-level
+                        two_stream call + reference direct and total fluxes
   meta.json             compiler, flags and seeds
 Usage: python3 tests/golden/make_golden.py
 """
@@ -76,16 +78,44 @@ def run_cloud_gen(ioverlap, tmp):
                         p_layer=st["p_layer"], xcw=xcw, ref_c_sub=c_sub, ref_n_subcol_cld=ncld)
 
 
+TWO_STREAM = {
+    # name: (n_profile, n_layer, isolir, l_scale_solar, l_ir_source_quad, seed, omc_range)
+    "sw_scaled": (8, 40, 1, 1, 1, 20261208, (-8.0, -1.0)),
+    "sw_plain": (8, 40, 1, 0, 1, 20261209, (-3.0, 0.0)),
+    "lw_quad": (8, 40, 2, 1, 1, 20261210, (-3.0, 0.0)),
+    "lw_linear": (8, 40, 2, 1, 0, 20261211, (-3.0, 0.0)),
+}
+
+
+def run_two_stream(name, P, L, isolir, lss, lq, seed, omc, tmp):
+    tsc = synth.make_two_stream_case(P, L, isolir, seed, l_scale_solar=lss, l_ir_source_quad=lq,
+                                     omc_range=omc)
+    fin, fout = os.path.join(tmp, "ts.bin"), os.path.join(tmp, "ts.out")
+    synth.write_two_stream_case(tsc, fin)
+    subprocess.run([os.path.join(REF, "soc_two_stream_case"), fin, fout], check=True)
+    a = np.fromfile(fout)
+    fdir = a[:P * (L + 1)].reshape(L + 1, P)
+    ftot = a[P * (L + 1):].reshape(2 * L + 2, P)
+    t = synth.two_stream_term(tsc, 0)
+    np.savez_compressed(os.path.join(HERE, "two_stream_%s.npz" % name),
+                        **{k: np.asarray(v) for k, v in t.items()},
+                        ref_flux_direct=fdir, ref_flux_total=ftot)
+
+
 def main():
     with tempfile.TemporaryDirectory() as tmp:
         for name, (P, L, src, cloudy, seed) in CASES.items():
             run_case(name, P, L, src, cloudy, seed, tmp)
         for ov in (0, 1, 3):
             run_cloud_gen(ov, tmp)
+        for name, args in TWO_STREAM.items():
+            run_two_stream(name, *args, tmp)
     ver = subprocess.run(["/opt/rocm/bin/amdflang", "--version"], capture_output=True,
                          text=True).stdout.splitlines()[0]
     json.dump({"compiler": ver, "flags": "-O2 -fPIC -ffp-contract=off",
                "cases": {k: list(v) for k, v in CASES.items()},
+               "two_stream_cases": {k: [list(x) if isinstance(x, tuple) else x for x in v]
+                                    for k, v in TWO_STREAM.items()},
                "spectral_files": ["socrates_amd/data/sp_lw_synth", "socrates_amd/data/sp_sw_synth",
                                   "socrates_amd/data/mcica_synth"]},
               open(os.path.join(HERE, "meta.json"), "w"), indent=1)

@@ -1,0 +1,136 @@
+"""CPU checks of the device physics (socrates_amd/csrc/physics.h, built for
+the host by tests/host/physics_host.cpp):
+
+* the FP64 two-stream coefficients equal the oracle's bit for bit (both
+  follow the reference's operation order, no contraction on the host);
+* the mixed-precision coefficients (two_coeff_f32) are within FP32
+  accuracy of the reference formulas evaluated exactly, including at the
+  solar resonance lambda = sec(theta_0) that the reference itself guards only
+  within 32 eps;
+* on the kernel-level stress case (tau down to 1e-6, 1 - omega down to 1e-8)
+  the mixed two-stream solve is within 0.05 W m-2 of the exact solution,
+  where the reference's own FP64 infra-red formulas are not (their quadratic
+  source coefficient cancels: errors of watts).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from mpmath import exp, mp, mpf, sqrt
+
+from oracle import oracle
+from socrates_amd import synth
+from exact_two_stream import two_stream_column
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def host(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("host") / "physics_host.so")
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", os.path.join(HERE, "host", "physics_host.cpp"),
+                    "-o", out], check=True)
+    return ctypes.CDLL(out)
+
+
+def _p(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_D)
+
+
+def _sample(rng, n, solar):
+    tau = 10.0 ** rng.uniform(-6, 3, n)
+    omc = 10.0 ** rng.uniform(-8, 0, n)
+    g = rng.uniform(0.0, 0.95, n) if solar else np.zeros(n)
+    sec0 = 1.0 / rng.uniform(0.05, 1.0, n)
+    return tau, 1.0 - omc, g, sec0
+
+
+def _exact_coef(solar, tau, om, g, mu):
+    mp.dps = 40
+    t, w, gg, mu = mpf(tau), mpf(om), mpf(g), mpf(mu)
+    if solar:
+        S, D = 2 - mpf(1.5) * w * gg - w / 2, 2 * (1 - w)
+    else:
+        S, D = mpf(1.66) - mpf(1.5) * w * gg, mpf(1.66) * (1 - w)
+    lam = sqrt(S * D)
+    e = exp(-lam * t)
+    gam = (S - lam) / (S + lam)
+    den = 1 - e * e * gam * gam
+    T, R = e * (1 - gam * gam) / den, gam * (1 - e * e) / den
+    if not solar:
+        return [float(T), float(R)]
+    ksi = mpf(1.5) * gg / mu
+    f = w * mu / 2 / ((lam - mu) * (lam + mu))
+    gup, gdn = f * (S - mu - ksi * (D - mu)), f * (S + mu + ksi * (D + mu))
+    T0 = exp(-t * mu)
+    s1 = (gup - R * (1 + gdn)) - gup * T * T0
+    s2 = T0 * (1 + gdn - gup * R) - (1 + gdn) * T
+    return [float(T), float(R), float(T0), float(s1), float(s2)]
+
+
+@pytest.mark.parametrize("solar", [1, 0])
+def test_fp64_coefficients_equal_oracle(host, solar):
+    rng = np.random.default_rng(3 + solar)
+    n = 4000
+    tau, om, g, sec0 = _sample(rng, n, solar)
+    out = np.zeros((n, 5))
+    host.host_two_coeff(n, solar, _p(tau), _p(om), _p(g), _p(sec0), out.ctypes.data_as(_D))
+    ref = oracle.two_coeff(1 if solar else 2, tau, om, g, sec0)
+    cols = ("trans", "reflect", "trans_0", "sc1", "sc2") if solar else ("trans", "reflect", "sc1", "sc2")
+    idx = (0, 1, 2, 3, 4) if solar else (0, 1, 3, 4)
+    for c, k in zip(cols, idx):
+        assert np.array_equal(out[:, k], ref[c]), c
+
+
+def test_mixed_solar_coefficients_near_exact(host):
+    """FP32 solar coefficients within 2e-6 (absolute, per unit incident beam)
+    of the exact reference formulas, with a third of the samples within
+    1e-4 relative of the resonance."""
+    rng = np.random.default_rng(11)
+    n = 1500
+    tau, om, g, sec0 = _sample(rng, n, True)
+    S = 2 - 1.5 * om * g - 0.5 * om
+    lam = np.sqrt(S * 2 * (1 - om))
+    near = (np.arange(n) % 3 == 0) & (lam >= 1.0)
+    sec0 = np.where(near, lam * (1 + rng.uniform(-1e-4, 1e-4, n)), sec0)
+    out = np.zeros((n, 7))
+    host.host_two_coeff_f32(n, 1, _p(tau), _p(om), _p(g), _p(sec0), out.ctypes.data_as(_D))
+    worst = np.zeros(5)
+    for i in range(n):
+        ex = _exact_coef(True, tau[i], om[i], g[i], sec0[i])
+        worst = np.maximum(worst, np.abs(out[i, :5] - ex))
+    assert (worst < 2e-6).all(), worst
+    # complements carried to FP64 keep relative accuracy
+    ex_u = np.array([1 - sum(_exact_coef(True, tau[i], om[i], g[i], sec0[i])[:2]) for i in range(200)])
+    u = out[:200, 5]
+    sel = np.abs(ex_u) > 1e-30
+    assert (np.abs(u[sel] - ex_u[sel]) <= 1e-5 * np.abs(ex_u[sel]) + 1e-12).all()
+
+
+@pytest.mark.parametrize("isolir", [1, 2])
+def test_mixed_two_stream_beats_bound_against_exact(host, isolir):
+    tsc = synth.make_two_stream_case(10, 40, isolir, seed=21 + isolir)
+    L = tsc["n_layer"]
+    err = {0: 0.0, 1: 0.0}
+    for l in range(tsc["n_profile"]):
+        ex_up, ex_dn = two_stream_column(tsc, l)
+        col = lambda k: np.ascontiguousarray(tsc[k][:, l])  # noqa: E731
+        for mixed in (0, 1):
+            up, dn = np.zeros(L + 1), np.zeros(L + 1)
+            host.host_two_stream(
+                L, int(isolir == 1), 1, 1, mixed, _p(tsc["tau"][0, :, l]), _p(tsc["omega"][0, :, l]),
+                _p(tsc["asymmetry"][0, :, l]), ctypes.c_double(tsc["sec_0"][l]),
+                ctypes.c_double(tsc["flux_inc_down"][l]), ctypes.c_double(tsc["flux_inc_direct"][l]),
+                ctypes.c_double(tsc["diffuse_albedo"][l]), ctypes.c_double(tsc["direct_albedo"][l]),
+                _p(col("diff_planck")), _p(col("diff_planck_2")),
+                ctypes.c_double(tsc["d_planck_flux_surface"][l]), _p(col("adjust_solar_ke")),
+                up.ctypes.data_as(_D), dn.ctypes.data_as(_D))
+            err[mixed] = max(err[mixed], np.abs(up - ex_up).max(), np.abs(dn - ex_dn).max())
+    assert err[1] <= 0.05, err
+    if isolir == 2:
+        # the reference's FP64 infra-red quadratic source loses watts on
+        # these inputs; the mixed form does not
+        assert err[0] > 100 * err[1], err

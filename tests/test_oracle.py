@@ -52,6 +52,38 @@ def test_oracle_cloud_gen_bit_exact(ioverlap):
     assert np.array_equal(c_sub, d["ref_c_sub"])
 
 
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "two_stream_*.npz"))))
+def test_oracle_two_stream_matches_reference_golden(path):
+    """Kernel-level restatement (two_coeff, sources, solver) equals the
+    reference's two_stream bit for bit, stress inputs included."""
+    d = dict(np.load(path))
+    t = {k: (v.item() if v.ndim == 0 else v) for k, v in d.items() if not k.startswith("ref_")}
+    fdir, up, dn = oracle.two_stream(t)
+    assert np.array_equal(up, d["ref_flux_total"][0::2])
+    assert np.array_equal(dn, d["ref_flux_total"][1::2])
+    if t["isolir"] == 1:
+        assert np.array_equal(fdir, d["ref_flux_direct"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference not built here (oracle/_ref)")
+@pytest.mark.parametrize("isolir,lss,lq", [(1, 1, 1), (1, 0, 1), (2, 1, 1), (2, 1, 0)])
+def test_oracle_two_stream_matches_compiled_reference(isolir, lss, lq):
+    tsc = synth.make_two_stream_case(9, 33, isolir, seed=808 + isolir + lss + lq,
+                                     l_scale_solar=lss, l_ir_source_quad=lq)
+    exe = os.path.join(ROOT, "oracle", "_ref", "soc_two_stream_case")
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "t.bin"), os.path.join(tmp, "t.out")
+        synth.write_two_stream_case(tsc, fin)
+        subprocess.run([exe, fin, fout], check=True, capture_output=True)
+        a = np.fromfile(fout)
+    P, L = 9, 33
+    fdir, up, dn = oracle.two_stream(synth.two_stream_term(tsc, 0))
+    ft = a[P * (L + 1):].reshape(2 * L + 2, P)
+    assert np.array_equal(up, ft[0::2]) and np.array_equal(dn, ft[1::2])
+    if isolir == 1:
+        assert np.array_equal(fdir, a[:P * (L + 1)].reshape(L + 1, P))
+
+
 @pytest.mark.skipif(not os.path.exists(REF), reason="reference not built here (oracle/_ref)")
 @pytest.mark.parametrize("src,cloudy", [("lw", False), ("sw", True), ("lw", True)])
 def test_oracle_matches_compiled_reference(src, cloudy):

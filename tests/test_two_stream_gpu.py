@@ -1,0 +1,103 @@
+"""The kernel-level entry socgpu_two_stream (the reference's two_stream on
+prescribed optical properties) on the device, through the C ABI.
+
+FP64 form against the oracle (soc_oracle_two_stream, itself equal bit for
+bit to the compiled reference's two_stream, tests/test_oracle.py):
+* well-conditioned inputs (1 - omega >= 1e-3): within 1e-10 relative, floor
+  max(1e-9, 1e-11 x column scale) W m-2;
+* stress inputs (tau down to 1e-6, 1 - omega down to 1e-8): the reference's
+  formulas amplify last-bit differences (infra-red quadratic source), so the
+  tolerance adds 8x the spread the oracle itself shows when its exp is
+  perturbed by one ulp (measured per element, three perturbed runs).
+Mixed form: within 0.05 W m-2 of the exact solution of the reference's
+formulas (tests/exact_two_stream.py) on the stress inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from socrates_amd import runes as R
+from socrates_amd import synth
+from exact_two_stream import two_stream_column
+
+pytestmark = pytest.mark.gpu
+
+_IN = ("tau", "omega", "asymmetry", "sec_0", "flux_inc_down", "flux_inc_direct", "diffuse_albedo",
+       "direct_albedo", "diff_planck", "diff_planck_2", "d_planck_flux_surface", "adjust_solar_ke")
+
+
+def _oracle_terms(tsc, variant=None):
+    lib = oracle.lib(variant)
+    P, L, G = tsc["n_profile"], tsc["n_layer"], tsc["n_term"]
+    up, dn, dr = (np.zeros((G, L + 1, P)) for _ in range(3))
+    for g in range(G):
+        t = synth.two_stream_term(tsc, g)
+        keep = []
+        fdir = np.zeros((L + 1, P))
+        ft = np.zeros((2 * L + 2, P))
+        lib.soc_oracle_two_stream(P, L, t["isolir"], t["l_scale_solar"], t["l_ir_source_quad"],
+                                  *[oracle._ptr(t[k], keep) for k in _IN],
+                                  oracle._ptr(fdir, keep), oracle._ptr(ft, keep))
+        up[g], dn[g], dr[g] = ft[0::2], ft[1::2], fdir
+    return up, dn, dr
+
+
+def _check(gpu, ref, spread, label):
+    for k, r in zip(("flux_up", "flux_down"), ref[:2]):
+        g = np.asarray(gpu[k])
+        scale = np.abs(r).max(axis=1, keepdims=True)
+        tol = np.maximum(np.maximum(1e-10 * np.abs(r), 1e-9), 1e-11 * scale)
+        if spread is not None:
+            tol = tol + 8.0 * spread[k]
+        bad = np.abs(g - r) > tol
+        assert not bad.any(), "%s %s: max err %.3e at %s" % (label, k, np.abs(g - r).max(),
+                                                             np.argwhere(bad)[0])
+
+
+@pytest.mark.parametrize("isolir", [1, 2])
+@pytest.mark.parametrize("flags", [(1, 1), (0, 0)])
+def test_two_stream_fp64_well_conditioned(isolir, flags):
+    tsc = synth.make_two_stream_case(64, 60, isolir, seed=40 + isolir + 2 * flags[0], n_term=8,
+                                     l_scale_solar=flags[0], l_ir_source_quad=flags[1],
+                                     omc_range=(-3.0, 0.0))
+    ref = _oracle_terms(tsc)
+    gpu = R.two_stream_case(tsc, "fp64")
+    _check(gpu, ref, None, "fp64 isolir=%d flags=%s" % (isolir, flags))
+    if isolir == 1:
+        assert np.abs(np.asarray(gpu["flux_direct"]) - ref[2]).max() <= 1e-10 * np.abs(ref[2]).max()
+
+
+@pytest.mark.parametrize("isolir", [1, 2])
+def test_two_stream_fp64_stress(isolir):
+    tsc = synth.make_two_stream_case(64, 60, isolir, seed=50 + isolir, n_term=4)
+    ref = _oracle_terms(tsc)
+    spread = {"flux_up": np.zeros_like(ref[0]), "flux_down": np.zeros_like(ref[1])}
+    for _ in range(3):
+        j = _oracle_terms(tsc, "jitter")
+        spread["flux_up"] = np.maximum(spread["flux_up"], np.abs(j[0] - ref[0]))
+        spread["flux_down"] = np.maximum(spread["flux_down"], np.abs(j[1] - ref[1]))
+    gpu = R.two_stream_case(tsc, "fp64")
+    _check(gpu, ref, spread, "fp64 stress isolir=%d" % isolir)
+
+
+@pytest.mark.parametrize("isolir", [1, 2])
+def test_two_stream_mixed_against_exact(isolir):
+    tsc = synth.make_two_stream_case(10, 40, isolir, seed=60 + isolir, n_term=2)
+    gpu = R.two_stream_case(tsc, "mixed")
+    worst = 0.0
+    for g in range(tsc["n_term"]):
+        for l in range(tsc["n_profile"]):
+            ex_up, ex_dn = two_stream_column(tsc, l, g)
+            worst = max(worst, np.abs(np.asarray(gpu["flux_up"])[g, :, l] - ex_up).max(),
+                        np.abs(np.asarray(gpu["flux_down"])[g, :, l] - ex_dn).max())
+    assert worst <= 0.05, worst
+
+
+def test_two_stream_rejects_bad_arguments():
+    tsc = synth.make_two_stream_case(4, 10, 2, seed=1)
+    with pytest.raises(R.SocgpuError):
+        R.two_stream(4, 10, 1, 3, tsc["tau"], tsc["omega"], tsc["asymmetry"],
+                     flux_inc_down=tsc["flux_inc_down"], diffuse_albedo=tsc["diffuse_albedo"])
+    with pytest.raises(R.SocgpuError):
+        R.two_stream(4, 10, 1, 2, tsc["tau"], tsc["omega"], tsc["asymmetry"],
+                     flux_inc_down=tsc["flux_inc_down"], diffuse_albedo=tsc["diffuse_albedo"])
