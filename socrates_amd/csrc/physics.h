@@ -299,3 +299,28 @@ SG_HD void slingo_f32(const double *cpd, float mmr, float re, float &kext, float
 }
 
 }  // namespace sg
+
+namespace sg {
+
+// FP64 coefficients from the mixed form's FP32 ones. The smaller of T and R
+// is taken as computed (FP32 relative accuracy) and the larger closes the
+// balance T + R + (1 - T - R) = 1 in FP64 with the absorption u = 1 - T - R,
+// which is itself accurate to FP32 relative precision; T0 = 1 - (1 - T0)
+// likewise. The layer's absorption and the small one of T and R then keep
+// their relative accuracy in the FP64 elimination: thin layers (T ~ 1) keep
+// their flux divergence, and so their heating rates, and thick
+// near-conservative layers (R ~ 1) keep their transmission.
+SG_HD void coef_from_f32(const CoefF &f, Coef &o) {
+  if (f.T < f.R) {
+    o.T = f.T;
+    o.R = (1.0 - (double)f.T) - (double)f.u;
+  } else {
+    o.R = f.R;
+    o.T = (1.0 - (double)f.R) - (double)f.u;
+  }
+  o.T0 = 1.0 - (double)f.em0;
+  o.s1 = f.s1;
+  o.s2 = f.s2;
+}
+
+}  // namespace sg

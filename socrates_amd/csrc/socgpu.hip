@@ -599,15 +599,7 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   }
   CoefF f;
   two_coeff_f32(c.solar, tau, om, omc, g, (float)c.sec0, f);
-  // T and T0 are taken to FP64 through their complements, so the layer's
-  // absorption (1 - T - R, 1 - T0) keeps FP32 relative accuracy in the FP64
-  // elimination: a thin layer's flux divergence, and so its heating rate,
-  // is then accurate where T itself is 1 to within FP32 rounding.
-  o.R = f.R;
-  o.T = (1.0 - (double)f.R) - (double)f.u;
-  o.T0 = 1.0 - (double)f.em0;
-  o.s1 = f.s1;
-  o.s2 = f.s2;
+  coef_from_f32(f, o);
   if (cloudy) { o.T *= frac1; o.R *= frac1; o.T0 *= frac1; o.s1 *= frac1; o.s2 *= frac1; }
 }
 
@@ -839,11 +831,7 @@ __global__ __launch_bounds__(256) void two_stream_kernel(TwoStreamArgs a) {
       CoefF f;
       two_coeff_f32(solar, (float)tau, (float)om, (float)fmax(1.0 - om, 32.0 * kEps), (float)asym,
                     (float)sec0, f);
-      o.R = f.R;
-      o.T = (1.0 - (double)f.R) - (double)f.u;
-      o.T0 = 1.0 - (double)f.em0;
-      o.s1 = f.s1;
-      o.s2 = f.s2;
+      coef_from_f32(f, o);
     } else {
       two_coeff(solar, tau, om, asym, sec0, o);
     }

@@ -52,11 +52,7 @@ void host_two_stream(int L, int solar, int lss, int lq, int mixed, const double 
       CoefF f;
       two_coeff_f32(solar != 0, (float)tau[i], (float)om[i], (float)fmax(1.0 - om[i], 32.0 * kEps),
                     (float)g[i], (float)sec0, f);
-      o.R = f.R;
-      o.T = (1.0 - (double)f.R) - (double)f.u;
-      o.T0 = 1.0 - (double)f.em0;
-      o.s1 = f.s1;
-      o.s2 = f.s2;
+      coef_from_f32(f, o);
     } else {
       two_coeff(solar != 0, tau[i], om[i], g[i], sec0, o);
     }

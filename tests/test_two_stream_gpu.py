@@ -3,10 +3,12 @@ prescribed optical properties) on the device, through the C ABI.
 
 FP64 form against the oracle (soc_oracle_two_stream, itself equal bit for
 bit to the compiled reference's two_stream, tests/test_oracle.py):
-* well-conditioned inputs (1 - omega >= 1e-3): within 1e-10 relative, floor
+* solar, 1 - omega >= 1e-3: within 1e-10 relative, floor
   max(1e-9, 1e-11 x column scale) W m-2;
-* stress inputs (tau down to 1e-6, 1 - omega down to 1e-8): the reference's
-  formulas amplify last-bit differences (infra-red quadratic source), so the
+* infra-red at any omega, and stress inputs (tau down to 1e-6, 1 - omega
+  down to 1e-8): the reference's quadratic source coefficient
+  -(1 + R + T + q) / (Sigma tau + sqrt(eps)) cancels for tau near its 2^-13
+  branch point, so last-bit differences grow to 1e-3 W m-2 and beyond; the
   tolerance adds 8x the spread the oracle itself shows when its exp is
   perturbed by one ulp (measured per element, three perturbed runs).
 Mixed form: within 0.05 W m-2 of the exact solution of the reference's
@@ -42,6 +44,15 @@ def _oracle_terms(tsc, variant=None):
     return up, dn, dr
 
 
+def _spread(tsc, ref, runs=3):
+    spread = {"flux_up": np.zeros_like(ref[0]), "flux_down": np.zeros_like(ref[1])}
+    for _ in range(runs):
+        j = _oracle_terms(tsc, "jitter")
+        spread["flux_up"] = np.maximum(spread["flux_up"], np.abs(j[0] - ref[0]))
+        spread["flux_down"] = np.maximum(spread["flux_down"], np.abs(j[1] - ref[1]))
+    return spread
+
+
 def _check(gpu, ref, spread, label):
     for k, r in zip(("flux_up", "flux_down"), ref[:2]):
         g = np.asarray(gpu[k])
@@ -62,7 +73,8 @@ def test_two_stream_fp64_well_conditioned(isolir, flags):
                                      omc_range=(-3.0, 0.0))
     ref = _oracle_terms(tsc)
     gpu = R.two_stream_case(tsc, "fp64")
-    _check(gpu, ref, None, "fp64 isolir=%d flags=%s" % (isolir, flags))
+    _check(gpu, ref, _spread(tsc, ref) if isolir == 2 else None,
+           "fp64 isolir=%d flags=%s" % (isolir, flags))
     if isolir == 1:
         assert np.abs(np.asarray(gpu["flux_direct"]) - ref[2]).max() <= 1e-10 * np.abs(ref[2]).max()
 
@@ -71,13 +83,8 @@ def test_two_stream_fp64_well_conditioned(isolir, flags):
 def test_two_stream_fp64_stress(isolir):
     tsc = synth.make_two_stream_case(64, 60, isolir, seed=50 + isolir, n_term=4)
     ref = _oracle_terms(tsc)
-    spread = {"flux_up": np.zeros_like(ref[0]), "flux_down": np.zeros_like(ref[1])}
-    for _ in range(3):
-        j = _oracle_terms(tsc, "jitter")
-        spread["flux_up"] = np.maximum(spread["flux_up"], np.abs(j[0] - ref[0]))
-        spread["flux_down"] = np.maximum(spread["flux_down"], np.abs(j[1] - ref[1]))
     gpu = R.two_stream_case(tsc, "fp64")
-    _check(gpu, ref, spread, "fp64 stress isolir=%d" % isolir)
+    _check(gpu, ref, _spread(tsc, ref), "fp64 stress isolir=%d" % isolir)
 
 
 @pytest.mark.parametrize("isolir", [1, 2])
