@@ -36,6 +36,19 @@ def host(tmp_path_factory):
     return ctypes.CDLL(out)
 
 
+@pytest.fixture(scope="module")
+def host_fma(tmp_path_factory):
+    """The same physics built as the device compiler builds it: fused
+    multiply-add allowed except where a pragma turns it off."""
+    clang = "/opt/rocm/llvm/bin/clang++"
+    if not os.path.exists(clang):
+        pytest.skip("ROCm clang not present")
+    out = str(tmp_path_factory.mktemp("host_fma") / "physics_host_fma.so")
+    subprocess.run([clang, "-O2", "-mfma", "-ffp-contract=fast-honor-pragmas", "-shared", "-fPIC",
+                    os.path.join(HERE, "host", "physics_host.cpp"), "-o", out], check=True)
+    return ctypes.CDLL(out)
+
+
 def _p(a):
     return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_D)
 
@@ -136,3 +149,27 @@ def test_mixed_two_stream_beats_bound_against_exact(host, isolir):
         # the reference's FP64 infra-red quadratic source loses watts on
         # these inputs; the mixed form does not
         assert err[0] > 100 * err[1], err
+
+
+def test_fp64_solve_unaffected_by_contraction(host_fma):
+    """Infra-red quadratic sources cancel for tau near 2^-13: a contracted
+    product inside two_coeff alone moves fluxes by ~1e-4 W m-2. two_coeff
+    turns contraction off, so an FMA build stays at the oracle's value."""
+    tsc = synth.make_two_stream_case(16, 60, 2, seed=44, n_term=4, omc_range=(-3.0, 0.0))
+    L = tsc["n_layer"]
+    worst = 0.0
+    for g in range(tsc["n_term"]):
+        t = synth.two_stream_term(tsc, g)
+        _, up, dn = oracle.two_stream(t)
+        for l in range(tsc["n_profile"]):
+            u, d = np.zeros(L + 1), np.zeros(L + 1)
+            col = lambda k: np.ascontiguousarray(t[k][:, l])  # noqa: E731
+            host_fma.host_two_stream(
+                L, 0, 1, 1, 0, _p(col("tau")), _p(col("omega")), _p(col("asymmetry")),
+                ctypes.c_double(1.0), ctypes.c_double(t["flux_inc_down"][l]), ctypes.c_double(0.0),
+                ctypes.c_double(t["diffuse_albedo"][l]), ctypes.c_double(t["direct_albedo"][l]),
+                _p(col("diff_planck")), _p(col("diff_planck_2")),
+                ctypes.c_double(t["d_planck_flux_surface"][l]), _p(col("adjust_solar_ke")),
+                u.ctypes.data_as(_D), d.ctypes.data_as(_D))
+            worst = max(worst, np.abs(u - up[:, l]).max(), np.abs(d - dn[:, l]).max())
+    assert worst < 1e-10, worst
