@@ -164,6 +164,23 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req,
                      const double *p_layer, int n1, int n2, const double *xcw,
                      double *c_sub, int *n_subcol_cld);
 
+/* Launch shape of the column-solve kernel, per precision form (no reference
+ * counterpart; the autotuner in socrates_amd/tune.py picks it):
+ *   recompute        0: keep each layer's elimination state in lane-private
+ *                       memory between the two sweeps (64 B per lane and
+ *                       layer of memory traffic); 1: keep a checkpoint every
+ *                       4 layers and recompute each segment in registers on
+ *                       the way up (twice the arithmetic, a sixth of the
+ *                       traffic). Results are bit-identical either way.
+ *   waves_per_block  1, 2 or 4 column chunks (64 items each) per block. */
+typedef struct SocgpuTile {
+  int recompute;
+  int waves_per_block;
+} SocgpuTile;
+
+int socgpu_set_tile(int precision, const SocgpuTile *tile);
+int socgpu_get_tile(int precision, SocgpuTile *tile);
+
 /* Message of the last failure on this host thread. */
 const char *socgpu_last_error(void);
 

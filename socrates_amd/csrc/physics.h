@@ -129,8 +129,12 @@ SG_HD int lcg_step(int x) {
 
 #ifdef __HIP_DEVICE_COMPILE__
 #define SG_EXPF(x) __expf(x)
+#define SG_FDIV(a, b) __fdividef((a), (b))
+#define SG_SQRTF(x) __builtin_amdgcn_sqrtf(x)
 #else
 #define SG_EXPF(x) expf(x)
+#define SG_FDIV(a, b) ((a) / (b))
+#define SG_SQRTF(x) sqrtf(x)
 #endif
 
 namespace sg {
@@ -141,14 +145,14 @@ struct CoefF {
   float em0;  // 1 - T0 (direct-beam extinction), to relative accuracy
 };
 
-// -expm1(-x) in FP32: series below 0.2 (terms to x^8, relative error < 1e-10
-// before rounding), 1 - exp(-x) above, where the subtraction loses < 2 bits.
-SG_HD float one_minus_exp(float x) {
-  if (x < 0.2f) {
-    return x * (1.0f - x * (0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f -
-                x * (0.0013888889f - x * (1.984127e-4f - x * 2.4801587e-5f)))))));
-  }
-  return 1.0f - SG_EXPF(-x);
+// 1 - exp(-x) = -expm1(-x) in FP32 given e = exp(-x): Taylor series below
+// x = 0.2 (terms to x^7, relative error < 1e-9 before rounding), 1 - e above,
+// where the subtraction loses under 2 bits. Both sides are evaluated and one
+// selected, so lanes of a wave do not diverge.
+SG_HD float one_minus_exp(float x, float e) {
+  float p = x * (1.0f - x * (0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f -
+            x * (0.0013888889f - x * 1.984127e-4f))))));
+  return x < 0.2f ? p : 1.0f - e;
 }
 
 // Mixed-precision form of two_coeff (see two_coeff above for the reference
@@ -191,46 +195,42 @@ SG_HD void two_coeff_f32(bool solar, float tau, float om, float omc, float g, fl
     D = 1.66f * omc;
     smd = om * (1.66f - 1.5f * g);
   }
-  float lambda = sqrtf(S * D);
+  float lambda = SG_SQRTF(S * D);
   const float sqe = 1.4901161193847656e-08f;
   const float tol = 1.220703125e-04f;
   float x = lambda * tau;
-  float em = one_minus_exp(x);
-  float e = x >= 0.2f ? SG_EXPF(-x) : 1.0f - em;
-  float inv_sl = 1.0f / (S + lambda);
+  float e = SG_EXPF(-x);
+  float em = one_minus_exp(x, e);
+  float inv_sl = SG_FDIV(1.0f, S + lambda);
   float gc = 2.0f * lambda * inv_sl;
   float gam = S * smd * inv_sl * inv_sl;             // (S - lambda) / (S + lambda)
   float d1 = em + gc * e;
   float ope = 1.0f + e * gam;
-  float rden = 1.0f / (d1 * ope);
+  float rden = SG_FDIV(1.0f, d1 * ope);
   c.T = e * gc * (1.0f + gam) * rden;
   c.R = gam * em * (1.0f + e) * rden;
-  const float u = em * gc / ope;                      // 1 - T - R
+  const float u = em * gc * SG_FDIV(1.0f, ope);      // 1 - T - R
   c.u = u;
   if (solar) {
     const float mu = sec0;
     float xs = tau * mu;
-    float em0 = one_minus_exp(xs);
+    c.T0 = SG_EXPF(-xs);
+    float em0 = one_minus_exp(xs, c.T0);
     c.em0 = em0;
-    c.T0 = xs >= 0.2f ? SG_EXPF(-xs) : 1.0f - em0;
     float omt = em * (1.0f + gam * gam * e) * rden;   // 1 - T
     float omtt0 = omt + c.T * em0;                    // 1 - T T0
     float t0mt = omt - em0;                           // T0 - T
-    float r = D / lambda;
-    float cc = 1.5f * g * r / mu;
+    float r = SG_FDIV(D, lambda);
+    float cc = SG_FDIV(1.5f * g * r, mu);
     float al = S - lambda + 1.5f * g * (1.0f - r);
     float bl = S + lambda + 1.5f * g * (r + 1.0f);
     float xd = (mu - lambda) * tau;
-    float tphi;
-    if (fabsf(xd) < 0.2f) {
-      // (1 - exp(-x)) / x, Taylor terms to x^7
-      float e1 = 1.0f - xd * (0.5f - xd * (0.16666667f - xd * (0.041666668f - xd * (0.008333334f -
-                 xd * (0.0013888889f - xd * (1.984127e-4f - xd * 2.4801587e-5f))))));
-      tphi = tau * e * e1;
-    } else {
-      tphi = (c.T0 - e) / (lambda - mu);
-    }
-    float fac = -0.5f * om * mu / (lambda + mu);
+    // (1 - exp(-x)) / x, Taylor terms to x^7, near the resonance or for thin
+    // layers; the direct quotient elsewhere
+    float e1 = 1.0f - xd * (0.5f - xd * (0.16666667f - xd * (0.041666668f - xd * (0.008333334f -
+               xd * (0.0013888889f - xd * (1.984127e-4f - xd * 2.4801587e-5f))))));
+    float tphi = fabsf(xd) < 0.2f ? tau * e * e1 : SG_FDIV(c.T0 - e, lambda - mu);
+    float fac = SG_FDIV(-0.5f * om * mu, lambda + mu);
     float q = (cc - 1.0f) * omtt0 + al * c.T * tphi - (1.0f - cc) * c.R;
     float m = (1.0f - cc) * t0mt - bl * tphi - (cc - 1.0f) * c.R * c.T0 + al * c.R * tphi;
     c.s1 = fac * q - c.R;
@@ -238,31 +238,24 @@ SG_HD void two_coeff_f32(bool solar, float tau, float om, float omc, float g, fl
   } else {
     c.T0 = 0.0f;
     c.em0 = 1.0f;
-    float omtr = em * (1.0f + gam) / d1;              // 1 - T + R
-    c.s1 = (omtr + sqe) / (sqe + tau * S);
+    float omtr = em * (1.0f + gam) * SG_FDIV(1.0f, d1);   // 1 - T + R
+    float den = sqe + tau * S;
+    c.s1 = SG_FDIV(omtr + sqe, den);
     // w = D tau - u, exact to O(x^3) terms for thin layers
-    float w;
-    if (x < 0.2f) {
-      float f = em / fmaxf(x, 1e-30f);                // (1 - e) / x
-      float g1 = 0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f - x * 0.0013888889f)));
-      w = D * tau * x * (2.0f * S * g1 - (S - lambda) * f) / (2.0f * S - (S - lambda) * x * f);
-      if (x <= 0.0f) w = 0.0f;
-    } else {
-      w = D * tau - u;
-    }
+    float f = SG_FDIV(em, fmaxf(x, 1e-30f));             // (1 - e) / x
+    float g1 = 0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f - x * 0.0013888889f)));
+    float ws = D * tau * x * SG_FDIV(2.0f * S * g1 - (S - lambda) * f, 2.0f * S - (S - lambda) * x * f);
+    float w = x < 0.2f ? (x > 0.0f ? ws : 0.0f) : D * tau - u;
     // s2 = -(1 + R + T + q) / (S tau + sqe), q the reference's two branches;
     // tau > tol: 1 + R + T + q = 2 - u + q = (2 (D tau - u) - u dts) / dts, dts = D tau + sqe
     // (the sqe terms of the reference's 2 (u + sqe) / dts cancel against 2 dts / dts);
-    // tau <= tol: 1 + R + T - 2 + D tau = D tau - u. Formed in FP64 from the
-    // accurate pieces.
-    double num;
-    if (tau > tol) {
-      double dts = (double)D * tau + 1.4901161193847656e-08;
-      num = (2.0 * (double)w - (double)u * dts) / dts;
-    } else {
-      num = (double)w;
-    }
-    c.s2 = (float)(-num / ((double)S * tau + 1.4901161193847656e-08));
+    // tau <= tol: 1 + R + T - 2 + D tau = D tau - u. The numerator's
+    // difference is formed in FP64 from the FP32 pieces (exact products),
+    // then rounded once.
+    double dts = (double)D * (double)tau + 1.4901161193847656e-08;
+    float nf = (float)(2.0 * (double)w - (double)u * dts);
+    float qn = tau > tol ? SG_FDIV(nf, (float)dts) : w;
+    c.s2 = -SG_FDIV(qn, den);
   }
 }
 

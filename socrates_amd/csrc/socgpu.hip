@@ -220,11 +220,23 @@ __global__ void pack_kernel(CallDims d, const SgSpec *__restrict__ spec, SocgpuR
 // layer with any w >= epsilon. krange[0] = min, krange[1] = max (1-based).
 __global__ void cldrange_kernel(CallDims d, const double *__restrict__ colrec, int *krange) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= d.P * d.L) return;
-  int col = idx / d.L, i = idx % d.L;
-  double w = colrec[((size_t)col * d.L + i) * SG_NQ + SG_Q_WCLD];
-  if (w > kEps) atomicMin(&krange[0], i + 1);
-  if (w >= kEps) atomicMax(&krange[1], i + 1);
+  int kmin = INT32_MAX, kmax = 0;
+  if (idx < d.P * d.L) {
+    int col = idx / d.L, i = idx % d.L;
+    double w = colrec[((size_t)col * d.L + i) * SG_NQ + SG_Q_WCLD];
+    if (w > kEps) kmin = i + 1;
+    if (w >= kEps) kmax = i + 1;
+  }
+  // one atomic per wave: the whole call shares one cloud top and base
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    kmin = min(kmin, __shfl_xor(kmin, o));
+    kmax = max(kmax, __shfl_xor(kmax, o));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (kmin != INT32_MAX) atomicMin(&krange[0], kmin);
+    if (kmax != 0) atomicMax(&krange[1], kmax);
+  }
 }
 
 __global__ void fill_int_kernel(int *p, int n, int v) {
@@ -350,72 +362,125 @@ __global__ __launch_bounds__(64) void cloudgen_kernel(CallDims d, const double *
   }
 }
 
-// Minor-gas k-terms, one lane per (column, minor term): the per-term part
-// of the equivalent extinction of solve_band_k_eqv_scl.F90:572-1047.
-//  infra-red: the non-scattering flux of the term alone
-//    (monochromatic_gas_flux.F90:115-196, diffusivity 1.66) and per layer its
-//    |flux|-weighted contributions, out[t][layer] = (sum over the layer's two
-//    incident fluxes of k w |F|, of w |F|);
-//  solar: the direct-beam transmission of the term, out[t][layer] =
-//    (w * transmission to the bottom of the layer, k * w * transmission to
-//    the surface).
-// minor[col][t][layer][2]
-__global__ __launch_bounds__(64) void minor_kernel(CallDims d, const SgSpec *__restrict__ spec,
-                                                   const double *__restrict__ colrec,
-                                                   const double *__restrict__ tlev,
-                                                   const double *__restrict__ colscal,
-                                                   const double *__restrict__ bandscal,
-                                                   double *__restrict__ minor) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// Wave-wide inclusive scans over the 64 lanes (lane order = scan order).
+__device__ __forceinline__ double scan_mul(double x, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    double y = __shfl_up(x, off, kWave);
+    if (lane >= off) x *= y;
+  }
+  return x;
+}
+
+// Composition of affine maps F -> A F + C in lane order: lane j ends with
+// the map of lanes 0..j applied in turn.
+__device__ __forceinline__ void scan_affine(double &A, double &C, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    double A0 = __shfl_up(A, off, kWave), C0 = __shfl_up(C, off, kWave);
+    if (lane >= off) {
+      C = A * C0 + C;
+      A = A * A0;
+    }
+  }
+}
+
+// Minor-gas weights for the equivalent extinction, one wave per (column,
+// minor k-term), lanes over layers (64 at a time with a carry). Solar: the
+// direct-beam transmission of the term down to each layer
+// (solve_band_k_eqv_scl.F90:654-739). Infra-red: the non-scattering fluxes
+// of monochromatic_gas_flux (monochromatic_gas_flux.F90:115-196,
+// diffusivity 1.66) as two affine recurrences, solved by wave scans.
+// minor[col][term][layer][2]: solar (fdt, k fdt(surface)), infra-red
+// (k |F| sum, |F| sum) with the term's weight. The scans associate the
+// products differently from the reference's left-to-right sweep (relative
+// differences of order 1e-16 in the weights).
+constexpr int kMinorWaves = 4;
+
+__global__ __launch_bounds__(64 * kMinorWaves) void minor_kernel(CallDims d, const SgSpec *__restrict__ spec,
+                                                                const double *__restrict__ colrec,
+                                                                const double *__restrict__ tlev,
+                                                                const double *__restrict__ colscal,
+                                                                const double *__restrict__ bandscal,
+                                                                double *__restrict__ minor) {
+  __shared__ double sh[kMinorWaves][4][SG_MAXL];
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int wave = blockIdx.x * kMinorWaves + wv;
   const int NT = spec->n_minor, L = d.L;
-  if (idx >= d.P * NT) return;
-  const int col = idx / NT, ti = idx % NT;
+  if (wave >= d.P * NT) return;
+  const int col = wave / NT, ti = wave % NT;
   const SgTerm &t = spec->minor[ti];
   const SgBand &B = spec->band[t.band];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   double *o = minor + ((size_t)col * NT + ti) * L * 2;
+  double *kk = sh[wv][0];
+  const int nch = (L + kWave - 1) / kWave;
   if (d.isolir == SOCGPU_SOURCE_SOLAR) {
     const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
-    double kk[SG_MAXL];
     double fdt = t.w;
-    for (int i = 0; i < L; i++) {
-      const double *r = rec + i * SG_NQ;
-      double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-      kk[i] = k;
-      fdt = fdt * exp(-k * r[SG_Q_MASS] * sec0);
-      o[2 * i] = fdt;
+    for (int ch = 0; ch < nch; ch++) {
+      const int i = ch * kWave + lane;
+      double x = 1.0;
+      if (i < L) {
+        const double *r = rec + i * SG_NQ;
+        double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
+        kk[i] = k;
+        x = exp(-k * r[SG_Q_MASS] * sec0);
+      }
+      double f = fdt * scan_mul(x, lane);
+      if (i < L) o[2 * i] = f;
+      fdt = __shfl(f, kWave - 1, kWave);
     }
-    for (int i = 0; i < L; i++) o[2 * i + 1] = kk[i] * fdt;
+    __syncwarp();
+    for (int i = lane; i < L; i += kWave) o[2 * i + 1] = kk[i] * fdt;
   } else {
+    double *tr = sh[wv][1], *su = sh[wv][2], *fdl = sh[wv][3];
     const double *tl = tlev + (size_t)col * (L + 1);
-    const double *bs = bandscal + ((size_t)col * d.NB + t.band) * SG_NBQ;
-    const double alb = bs[2];
+    const double alb = bandscal[((size_t)col * d.NB + t.band) * SG_NBQ + 2];
     const double sqe = 1.4901161193847656e-08;
-    double tr[SG_MAXL], su[SG_MAXL], kk[SG_MAXL], fdl[SG_MAXL + 1];
-    double bprev = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
-    double fdn = -bprev;
-    fdl[0] = fdn;
-    double bl = bprev;
-    for (int i = 0; i < L; i++) {
-      const double *r = rec + i * SG_NQ;
-      double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-      double tau = k * r[SG_Q_MASS];
-      bl = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
-      double trans = exp(-kDiffMinor * tau);
-      double s = (1.0 - trans + sqe) * (bl - bprev) / (kDiffMinor * tau + sqe);
-      bprev = bl;
-      kk[i] = k; tr[i] = trans; su[i] = s;
-      fdn = trans * fdn - s;
-      fdl[i + 1] = fdn;
+    const double b_top = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
+    double F = -b_top;  // differential downward flux at the top of the chunk
+    for (int ch = 0; ch < nch; ch++) {
+      const int i = ch * kWave + lane;
+      double A = 1.0, C = 0.0;
+      if (i < L) {
+        const double *r = rec + i * SG_NQ;
+        double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
+        double tau = k * r[SG_Q_MASS];
+        double b0 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i]);
+        double b1 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
+        double trans = exp(-kDiffMinor * tau);
+        double s = (1.0 - trans + sqe) * (b1 - b0) / (kDiffMinor * tau + sqe);
+        kk[i] = k; tr[i] = trans; su[i] = s;
+        A = trans;
+        C = -s;
+      }
+      scan_affine(A, C, lane);
+      double below = A * F + C;                       // F at the bottom of layer i
+      double above = __shfl_up(below, 1, kWave);      // F at its top
+      if (lane == 0) above = F;
+      if (i < L) fdl[i] = above;
+      F = __shfl(below, kWave - 1, kWave);
     }
-    double bg = planck_band(B, spec->n_deg, spec->t_ref_planck,
-                            colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
-    double fup = (bg - bl) + alb * fdn;
-    for (int i = L - 1; i >= 0; i--) {
-      double a = t.w * fabs(fdl[i]), b = t.w * fabs(fup);
-      o[2 * i] = kk[i] * a + kk[i] * b;
-      o[2 * i + 1] = a + b;
-      fup = tr[i] * fup + su[i];
+    const double b_bot = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[L]);
+    const double bg = planck_band(B, spec->n_deg, spec->t_ref_planck,
+                                  colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
+    double U = (bg - b_bot) + alb * F;                // upward flux at the surface
+    __syncwarp();
+    for (int ch = 0; ch < nch; ch++) {
+      const int i = L - 1 - (ch * kWave + lane);      // bottom-up
+      double A = 1.0, C = 0.0;
+      if (i >= 0) { A = tr[i]; C = su[i]; }
+      scan_affine(A, C, lane);
+      double above = A * U + C;                       // F up at the top of layer i
+      double below = __shfl_up(above, 1, kWave);      // at its bottom
+      if (lane == 0) below = U;
+      if (i >= 0) {
+        double a = t.w * fabs(fdl[i]), bb = t.w * fabs(below);
+        o[2 * i] = kk[i] * a + kk[i] * bb;
+        o[2 * i + 1] = a + bb;
+      }
+      U = __shfl(above, kWave - 1, kWave);
     }
   }
 }
@@ -431,7 +496,8 @@ __global__ __launch_bounds__(256) void prep_kernel(CallDims d, const SgSpec *__r
                                                    const double *__restrict__ colscal,
                                                    const double *__restrict__ minor,
                                                    double *__restrict__ bandrec,
-                                                   double *__restrict__ bandscal) {
+                                                   double *__restrict__ bandscal,
+                                                   double *__restrict__ cldopt) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int NB = d.NB, L = d.L, NT = spec->n_minor;
   if (idx >= d.P * NB * L) return;
@@ -439,6 +505,19 @@ __global__ __launch_bounds__(256) void prep_kernel(CallDims d, const SgSpec *__r
   const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   const double *r = rec + i * SG_NQ;
+  if (d.mcica) {
+    // droplet optics of the layer's in-cloud water at unit condensate
+    // scaling (they depend on the band, not on the k-term; the solve scales
+    // them by each sub-column's c_sub)
+    double *co = cldopt + (((size_t)col * NB + b) * L + i) * 4;
+    double kext = 0.0, kscat = 0.0, phf = 0.0, fwd = 0.0;
+    if (r[SG_Q_FRAC1] > 0.0) {
+      const SgDrop &dr = spec->drop[d.drop];
+      if (dr.scheme == 5) pade2(dr.parm[b], r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
+      else slingo(dr.parm[b], r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
+    }
+    co[0] = kext; co[1] = kscat; co[2] = phf; co[3] = fwd;
+  }
   double *br = bandrec + (((size_t)col * NB + b) * L + i) * SG_NBQ;
   double *bs = bandscal + ((size_t)col * NB + b) * SG_NBQ;
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
@@ -522,18 +601,16 @@ __global__ __launch_bounds__(256) void prep_kernel(CallDims d, const SgSpec *__r
 struct LaneCtx {
   bool solar;
   double sec0, ksc_clear;
-  int scheme;
-  const double *cp;
 };
 
+// co: the layer's droplet optics at unit condensate scaling (kext, kscat,
+// first moment, forward scattering), from prep_kernel.
 __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, const double *r,
-                                           double kgt, double csub, Coef &o) {
+                                           double kgt, double csub, const double *co, Coef &o) {
   double kgas = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
   const double frac1 = r[SG_Q_FRAC1];
   if (csub * frac1 > 0.0) {
-    double kext, kscat, phf, fwd;
-    if (c.scheme == 5) pade2(c.cp, r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
-    else slingo(c.cp, r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
+    const double kext = co[0], kscat = co[1], phf = co[2], fwd = co[3];
     double kg = kgt + kext * csub;
     double ks = c.ksc_clear + kscat * csub;
     double fw = fwd * csub;
@@ -562,7 +639,7 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
 // 1 - omega >= 32 eps(FP64). No FP64 fallback: two_coeff_f32 has no
 // resonance singularity.
 __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm &t, const double *r,
-                                                double kgt, double csub, Coef &o) {
+                                                double kgt, double csub, const double *co, Coef &o) {
   const float kgas = k_term_f32(t, (float)r[SG_Q_LNP], (float)r[SG_Q_LNT], (float)r[SG_Q_MMR + t.gas]);
   const float frac1 = (float)r[SG_Q_FRAC1];
   const float kag = (float)(kgt - c.ksc_clear);  // grey absorption (equivalent extinction)
@@ -571,29 +648,30 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   float tau, om, omc, g;
   bool cloudy = csub * frac1 > 0.0;
   if (cloudy) {
-    float kext, kabs, phf, fwd;
+    // absorption kext - kscat formed in FP64 (relative accuracy ~1e-10 even
+    // for weakly absorbing droplets), then rounded
+    const float kext = (float)co[0], kabs = (float)(co[0] - co[1]);
+    const float phf = (float)co[2], fwd = (float)co[3];
     const float cs = (float)csub;
-    if (c.scheme == 5) pade2_f32(c.cp, (float)r[SG_Q_LWC], (float)r[SG_Q_RE], kext, kabs, phf, fwd);
-    else slingo_f32(c.cp, (float)r[SG_Q_LWC], (float)r[SG_Q_RE], kext, kabs, phf, fwd);
     float ks = (float)c.ksc_clear + (kext - kabs) * cs;
     float fw = fwd * cs;
     float pf = phf * cs;
-    pf = (pf - fw) / fmaxf(ks - fw, (float)kEps);
-    fw = fw / fmaxf(ks, (float)kEps);
+    pf = SG_FDIV(pf - fw, fmaxf(ks - fw, (float)kEps));
+    fw = SG_FDIV(fw, fmaxf(ks, (float)kEps));
     float ktot = (float)c.ksc_clear + kag + kgas + kext * cs;
     float kab = kag + kgas + kabs * cs;
     tau = ktot * mass;
-    omc = fmaxf(kab / ktot, omc_min);
+    omc = fmaxf(SG_FDIV(kab, ktot), omc_min);
     om = 1.0f - omc;
     float sc = 1.0f - om * fw;
     tau = tau * sc;
-    omc = omc / sc;
+    omc = SG_FDIV(omc, sc);
     om = 1.0f - omc;
     g = pf;
   } else {
     float ktot = (float)c.ksc_clear + kag + kgas;
     tau = ktot * mass;
-    omc = ktot > 0.0f ? fmaxf((kag + kgas) / ktot, omc_min) : 1.0f;
+    omc = ktot > 0.0f ? fmaxf(SG_FDIV(kag + kgas, ktot), omc_min) : 1.0f;
     om = 1.0f - omc;
     g = 0.0f;
   }
@@ -603,16 +681,67 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   if (cloudy) { o.T *= frac1; o.R *= frac1; o.T0 *= frac1; o.s1 *= frac1; o.s2 *= frac1; }
 }
 
+// One layer of the top-down elimination for one lane: the layer's
+// coefficients (FP64 layer_coef or the mixed layer_coef_mixed), its sources
+// (solar_source.F90:104-162 with the k_eqv direct-beam adjustment, or
+// ir_source.F90:93-120), and the update of the reflectance of the atmosphere
+// above (rho) and the diffuse flux it sends down (dn). Returns the
+// back-substitution pair fu(i) = a fu(i+1) + bb.
+template <bool MIXED>
+__device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, const double *r,
+                                           const double *q, double cv, const double *co,
+                                           double &fdir, double &rho, double &dn, double &a,
+                                           double &bb) {
+  Coef o;
+  if (MIXED) layer_coef_mixed(c, t, r, q[0], cv, co, o);
+  else layer_coef(c, t, r, q[0], cv, co, o);
+  double su, sd;
+  if (c.solar) {
+    double fnew = fdir * o.T0 * q[1];
+    su = o.s1 * fdir;
+    sd = (o.s2 - o.T0) * fdir + fnew;
+    fdir = fnew;
+  } else {
+    su = o.s1 * q[2] + o.s2 * q[1];
+    sd = -o.s1 * q[2] + o.s2 * q[1];
+  }
+  double bp;
+  if (MIXED) {
+    // reciprocal from the hardware estimate and two Newton steps (a few ulp)
+    const double x = 1.0 - o.R * rho;
+    bp = __builtin_amdgcn_rcp(x);
+    bp = fma(bp, fma(-x, bp, 1.0), bp);
+    bp = fma(bp, fma(-x, bp, 1.0), bp);
+  } else {
+    bp = 1.0 / (1.0 - o.R * rho);
+  }
+  a = o.T * bp;
+  bb = (o.R * dn + su) * bp;
+  double rho_n = o.R + o.T * rho * a;
+  dn = o.T * (rho * bb + dn) + sd;
+  rho = rho_n;
+}
+
 // The column solve, one wave per column (or per chunk of 64 items of a
 // column), one lane per item: a clear-sky k-term (weight w_k (1 - frac_cloudy))
 // or a McICA sub-column sample (weight w_k frac_cloudy / subcol_k)
 // (mcica_sample.F90:453-930; monochromatic_radiance.F90:429-691;
 // mcica_column.F90:255-622; solver_homogen_direct.F90:93-127). The two-stream
 // system is eliminated top-down (reflectance of the atmosphere above and the
-// downward diffuse flux it emits) and back-substituted bottom-up; the
-// per-layer elimination state lives in lane-private memory. Output:
+// downward diffuse flux it emits) and back-substituted bottom-up. Output:
 // partial[col][chunk][3][L+1] (up, down, direct) of this wave's weighted sum.
-template <bool MIXED>
+//
+// RECOMP = false keeps every layer's elimination state (a, bb, rho, dn) in
+// lane-private memory between the sweeps: 64 B per lane and layer of
+// scratch traffic. RECOMP = true keeps only (fdir, rho, dn) at the top of
+// every kSeg-th layer and, going up, recomputes each segment's layers
+// (coefficients and elimination, bit-identical to the first sweep) into
+// registers before back-substituting it: a second pass of arithmetic in
+// place of the memory traffic, which pays where the coefficients are cheap
+// (the mixed form's FP32 arithmetic).
+constexpr int kSeg = 4;
+
+template <bool MIXED, bool RECOMP>
 __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec *__restrict__ spec,
                                                     int l_rayleigh,
                                                     const double *__restrict__ colrec,
@@ -620,6 +749,7 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
                                                     const double *__restrict__ bandrec,
                                                     const double *__restrict__ bandscal,
                                                     const double *__restrict__ c_sub,
+                                                    const double *__restrict__ cldopt,
                                                     double *__restrict__ partial) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x % kWave;
@@ -657,12 +787,12 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
   c.solar = solar;
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
-  c.scheme = spec->drop[d.drop].scheme;
-  c.cp = spec->drop[d.drop].parm[b];
+  const double *cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
   const double *csub = c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
   const double alb_diff = bs[2], alb_dir = bs[3];
 
-  double st[SG_MAXL][4];
+  constexpr int kNst = RECOMP ? (SG_MAXL + kSeg - 1) / kSeg : SG_MAXL;
+  double st[kNst][RECOMP ? 3 : 4];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down, flux_inc_direct
   double fdir = solar ? finc : 0.0;
   double rho = 0.0, dn = finc;
@@ -680,29 +810,18 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
       }
     }
     if (active) {
-      const double *r = rec + i * SG_NQ;
-      const double *q = br + i * SG_NBQ;
-      double cv = isub > 0 ? csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
-      Coef o;
-      if (MIXED) layer_coef_mixed(c, t, r, q[0], cv, o);
-      else layer_coef(c, t, r, q[0], cv, o);
-      double su, sd;
-      if (solar) {
-        double fnew = fdir * o.T0 * q[1];
-        su = o.s1 * fdir;
-        sd = (o.s2 - o.T0) * fdir + fnew;
-        fdir = fnew;
-      } else {
-        su = o.s1 * q[2] + o.s2 * q[1];
-        sd = -o.s1 * q[2] + o.s2 * q[1];
+      if (RECOMP) {
+        if (i % kSeg == 0) {
+          st[i / kSeg][0] = fdir; st[i / kSeg][1] = rho; st[i / kSeg][2] = dn;
+        }
       }
-      double bp = 1.0 / (1.0 - o.R * rho);
-      double a = o.T * bp;
-      double bb = (o.R * dn + su) * bp;
-      st[i][0] = a; st[i][1] = bb; st[i][2] = rho; st[i][3] = dn;
-      double rho_n = o.R + o.T * rho * a;
-      dn = o.T * (rho * bb + dn) + sd;
-      rho = rho_n;
+      double cv = isub > 0 ? csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
+      double a, bb;
+      const double rho0 = rho, dn0 = dn;
+      layer_step<MIXED>(c, t, rec + i * SG_NQ, br + i * SG_NBQ, cv, cop + i * 4, fdir, rho, dn, a, bb);
+      if (!RECOMP) {
+        st[i][0] = a; st[i][1] = bb; st[i][2] = rho0; st[i][3] = dn0;
+      }
     }
   }
   if (solar) {
@@ -727,23 +846,72 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
     fu = (alb_diff * dn + sg) / (1.0 - alb_diff * rho);
     fd = rho * fu + dn;
   }
-  // back-substitution, levels L..0, 4 levels (x up/down) per reduction
-  for (int top = L; top >= 0; top -= 4) {
-    double v[8];
+  if (!RECOMP) {
+    // back-substitution, levels L..0, 4 levels (x up/down) per reduction
+    for (int top = L; top >= 0; top -= 4) {
+      double v[8];
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const int lev = top - m;
-      if (lev >= 0 && lev < L && active) {
-        fu = st[lev][0] * fu + st[lev][1];
-        fd = st[lev][2] * fu + st[lev][3];
+      for (int m = 0; m < 4; m++) {
+        const int lev = top - m;
+        if (lev >= 0 && lev < L && active) {
+          fu = st[lev][0] * fu + st[lev][1];
+          fd = st[lev][2] * fu + st[lev][3];
+        }
+        v[2 * m] = lev >= 0 ? wgt * fu : 0.0;
+        v[2 * m + 1] = lev >= 0 ? wgt * fd : 0.0;
       }
-      v[2 * m] = lev >= 0 ? wgt * fu : 0.0;
-      v[2 * m + 1] = lev >= 0 ? wgt * fd : 0.0;
+      reduce8(v, lane);
+      if ((lane & 7) == 0) {
+        const int j = lane >> 3, lev = top - (j >> 1);
+        if (lev >= 0) out[(j & 1) * (L + 1) + lev] = v[0];
+      }
     }
-    reduce8(v, lane);
-    if ((lane & 7) == 0) {
-      const int j = lane >> 3, lev = top - (j >> 1);
-      if (lev >= 0) out[(j & 1) * (L + 1) + lev] = v[0];
+  } else {
+    // surface level, then segments bottom-up: recompute the segment's
+    // layers from its checkpoint into registers, back-substitute them
+    {
+      double v[8];
+#pragma unroll
+      for (int m = 0; m < 4; m++) { v[2 * m] = 0.0; v[2 * m + 1] = 0.0; }
+      v[0] = wgt * fu;
+      v[1] = wgt * fd;
+      reduce8(v, lane);
+      if ((lane & 7) == 0 && (lane >> 3) < 2) out[(lane >> 3) * (L + 1) + L] = v[0];
+    }
+    const int nseg = (L + kSeg - 1) / kSeg;
+    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+      const int i0 = sgi * kSeg;
+      double sa[kSeg], sb[kSeg], sr[kSeg], sdn[kSeg];
+      if (active) {
+        double fdir_s = st[sgi][0], rho_s = st[sgi][1], dn_s = st[sgi][2];
+#pragma unroll
+        for (int k = 0; k < kSeg; k++) {
+          const int i = i0 + k;
+          sr[k] = rho_s; sdn[k] = dn_s; sa[k] = 0.0; sb[k] = 0.0;
+          if (i < L) {
+            double cv = isub > 0 ? csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
+            layer_step<MIXED>(c, t, rec + i * SG_NQ, br + i * SG_NBQ, cv, cop + i * 4, fdir_s, rho_s,
+                              dn_s, sa[k], sb[k]);
+          }
+        }
+      }
+      double v[8];
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const int k = kSeg - 1 - m;   // levels i0+3 .. i0
+        const int lev = i0 + k;
+        if (lev < L && active) {
+          fu = sa[k] * fu + sb[k];
+          fd = sr[k] * fu + sdn[k];
+        }
+        v[2 * m] = lev < L ? wgt * fu : 0.0;
+        v[2 * m + 1] = lev < L ? wgt * fd : 0.0;
+      }
+      reduce8(v, lane);
+      if ((lane & 7) == 0) {
+        const int j = lane >> 3, lev = i0 + kSeg - 1 - (j >> 1);
+        if (lev < L) out[(j & 1) * (L + 1) + lev] = v[0];
+      }
     }
   }
 }
@@ -906,12 +1074,14 @@ struct Context {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::map<std::string, std::unique_ptr<SpecEntry>> spectra;
-  DevBuf colrec, tlev, colscal, bandrec, bandscal, csub, partial, krange, minor;
+  DevBuf colrec, tlev, colscal, bandrec, bandscal, csub, partial, krange, minor, cldopt;
   DevBuf in_stage, out_stage;
   double last_total_ms = 0.0, last_solve_ms = 0.0;
   std::mutex mu;
 };
 static Context g_ctx;
+// solve-kernel tile per precision form ([0] FP64, [1] mixed)
+static SocgpuTile g_tile[2] = {{0, 2}, {0, 2}};
 
 static int ensure_init() {
   if (g_ctx.device >= 0) return 0;
@@ -1091,6 +1261,7 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
       g_ctx.partial.ensure((size_t)P * d.nwpc * 3 * (L + 1) * 8) || g_ctx.krange.ensure(64) ||
       g_ctx.minor.ensure((size_t)P * (S.n_minor > 0 ? S.n_minor : 1) * L * 2 * 8) ||
       (d.mcica && g_ctx.csub.ensure((size_t)P * L * SG_MAXSUBCOL * 8)) ||
+      (d.mcica && g_ctx.cldopt.ensure((size_t)P * NB * L * 4 * 8)) ||
       g_ctx.out_stage.ensure((size_t)P * (L + 1) * 3 * 8 + (size_t)P * L * 8))
     return fail(SOCGPU_ENOMEM, "device buffer allocation failed");
   double *oup = g_ctx.out_stage.as<double>();
@@ -1116,27 +1287,25 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
                                      g_ctx.csub.as<double>(), g_ctx.colscal.as<double>());
   }
   if (S.n_minor > 0)
-    minor_kernel<<<(P * S.n_minor + 63) / 64, 64, 0, s>>>(d, e.dev, g_ctx.colrec.as<double>(), g_ctx.tlev.as<double>(),
+    minor_kernel<<<(P * S.n_minor + kMinorWaves - 1) / kMinorWaves, 64 * kMinorWaves, 0, s>>>(d, e.dev, g_ctx.colrec.as<double>(), g_ctx.tlev.as<double>(),
                                                           g_ctx.colscal.as<double>(), g_ctx.bandscal.as<double>(),
                                                           g_ctx.minor.as<double>());
   prep_kernel<<<(P * NB * L + T - 1) / T, T, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
                                                      g_ctx.tlev.as<double>(), g_ctx.colscal.as<double>(),
                                                      g_ctx.minor.as<double>(), g_ctx.bandrec.as<double>(),
-                                                     g_ctx.bandscal.as<double>());
+                                                     g_ctx.bandscal.as<double>(), g_ctx.cldopt.as<double>());
   HIPCHK(hipEventRecord(g_ctx.ev1, s));
   int waves = P * d.nwpc;
-  if (precision == SOCGPU_MIXED)
-    solve_kernel<true><<<(waves + 3) / 4, 256, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
-                                                       g_ctx.colscal.as<double>(), g_ctx.bandrec.as<double>(),
-                                                       g_ctx.bandscal.as<double>(),
-                                                       d.mcica ? g_ctx.csub.as<double>() : nullptr,
-                                                       g_ctx.partial.as<double>());
-  else
-    solve_kernel<false><<<(waves + 3) / 4, 256, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
-                                                        g_ctx.colscal.as<double>(), g_ctx.bandrec.as<double>(),
-                                                        g_ctx.bandscal.as<double>(),
-                                                        d.mcica ? g_ctx.csub.as<double>() : nullptr,
-                                                        g_ctx.partial.as<double>());
+  const SocgpuTile &tile = g_tile[precision == SOCGPU_MIXED ? 1 : 0];
+  const int wpb = tile.waves_per_block;
+  const int grid = (waves + wpb - 1) / wpb;
+  auto *kern = precision == SOCGPU_MIXED
+                   ? (tile.recompute ? solve_kernel<true, true> : solve_kernel<true, false>)
+                   : (tile.recompute ? solve_kernel<false, true> : solve_kernel<false, false>);
+  kern<<<grid, wpb * kWave, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(), g_ctx.colscal.as<double>(),
+                                    g_ctx.bandrec.as<double>(), g_ctx.bandscal.as<double>(),
+                                    d.mcica ? g_ctx.csub.as<double>() : nullptr,
+                                    d.mcica ? g_ctx.cldopt.as<double>() : nullptr, g_ctx.partial.as<double>());
   HIPCHK(hipEventRecord(g_ctx.ev2, s));
   finish_kernel<<<(nlev + T - 1) / T, T, 0, s>>>(d, g_ctx.partial.as<double>(), g_ctx.bandrec.as<double>(),
                                                 g_ctx.bandscal.as<double>(), oup, odn, odir);
@@ -1157,6 +1326,24 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   HIPCHK(hipEventElapsedTime(&t_solve, g_ctx.ev1, g_ctx.ev2));
   g_ctx.last_total_ms = t_all;
   g_ctx.last_solve_ms = t_solve;
+  return 0;
+}
+
+int socgpu_set_tile(int precision, const SocgpuTile *tile) {
+  if (precision != SOCGPU_FP64 && precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision %d", precision);
+  if (!tile) return fail(SOCGPU_EINVAL, "null tile");
+  const int w = tile->waves_per_block;
+  if (w != 1 && w != 2 && w != 4) return fail(SOCGPU_EINVAL, "waves_per_block must be 1, 2 or 4");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  g_tile[precision] = *tile;
+  g_tile[precision].recompute = tile->recompute != 0;
+  return 0;
+}
+
+int socgpu_get_tile(int precision, SocgpuTile *tile) {
+  if (precision != SOCGPU_FP64 && precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision %d", precision);
+  if (!tile) return fail(SOCGPU_EINVAL, "null tile");
+  *tile = g_tile[precision];
   return 0;
 }
 
@@ -1302,7 +1489,8 @@ int socgpu_finalize(void) {
   }
   g_ctx.spectra.clear();
   for (DevBuf *b : {&g_ctx.colrec, &g_ctx.tlev, &g_ctx.colscal, &g_ctx.bandrec, &g_ctx.bandscal, &g_ctx.csub,
-                    &g_ctx.partial, &g_ctx.krange, &g_ctx.minor, &g_ctx.in_stage, &g_ctx.out_stage}) {
+                    &g_ctx.partial, &g_ctx.krange, &g_ctx.minor, &g_ctx.cldopt, &g_ctx.in_stage,
+                    &g_ctx.out_stage}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->n = 0;

@@ -295,3 +295,21 @@ def two_stream_case(tsc, precision="fp64", out=None):
                       **{k: tsc[k] for k in _TS_INPUTS},
                       l_scale_solar=tsc["l_scale_solar"],
                       l_ir_source_quad=tsc["l_ir_source_quad"], precision=precision, out=out)
+
+
+class Tile(ctypes.Structure):
+    """Launch shape of the column-solve kernel (SocgpuTile)."""
+    _fields_ = [("recompute", ctypes.c_int), ("waves_per_block", ctypes.c_int)]
+
+
+def set_tile(precision, recompute, waves_per_block):
+    init()
+    t = Tile(int(bool(recompute)), int(waves_per_block))
+    _check(lib().socgpu_set_tile(_PREC[precision], ctypes.byref(t)))
+
+
+def get_tile(precision):
+    init()
+    t = Tile()
+    _check(lib().socgpu_get_tile(_PREC[precision], ctypes.byref(t)))
+    return dict(recompute=t.recompute, waves_per_block=t.waves_per_block)

@@ -147,3 +147,22 @@ def test_runes_full_size(src):
                spread=oracle_spread(case, src, True))
     gpu = R.runes_case(case, src, precision="mixed")
     check_mixed(gpu, ref, "mixed %s full" % src)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_tiles_bit_identical(precision):
+    """Every solve-kernel tile (stored or recomputed elimination state, 1-4
+    waves per block) gives the same bits."""
+    case = synth.make_case(24, 70, "sw", NBAND["sw"], seed=515, cloudy=True)
+    saved = R.get_tile(precision)
+    try:
+        outs = []
+        for rec in (0, 1):
+            for wpb in (1, 2, 4):
+                R.set_tile(precision, rec, wpb)
+                outs.append(R.runes_case(case, "sw", precision=precision))
+    finally:
+        R.set_tile(precision, saved["recompute"], saved["waves_per_block"])
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(o[k], outs[0][k]), k
