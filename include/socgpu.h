@@ -164,6 +164,75 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req,
                      const double *p_layer, int n1, int n2, const double *xcw,
                      double *c_sub, int *n_subcol_cld);
 
+/* One radiation call of a batch: the arguments socgpu_runes takes. */
+typedef struct SocgpuRequest {
+  const char *spectrum_name;
+  const SocgpuRunesArgs *args;
+  SocgpuDiag *diag;
+} SocgpuRequest;
+
+/* Several radiation calls, each with its own spectrum, source, column count,
+ * layer count and options (a ragged batch), packed into one launch
+ * sequence: every kernel strides over the concatenated work of all calls.
+ * Each call's results are those socgpu_runes gives for it alone (each call
+ * keeps its own cloud top and base, as runes does per call,
+ * socrates_cloud_gen.F90:224-245). No reference counterpart: the reference
+ * runs one call at a time (socrates_runes.F90:47). Synchronous. */
+int socgpu_runes_batch(int n_request, const SocgpuRequest *requests,
+                       int precision);
+
+/* Persistent form of socgpu_runes_batch for a host model that calls
+ * radiation every time step: the batch object keeps capacity-sized device
+ * buffers and a captured HIP graph of the kernel sequence, and replays the
+ * graph for every batch that fits (capacity grows, and the graph is
+ * re-captured, only when a batch does not fit or the solve tile changes). */
+typedef struct SocgpuBatch SocgpuBatch;
+int socgpu_batch_create(SocgpuBatch **batch, int precision);
+int socgpu_batch_run(SocgpuBatch *batch, int n_request,
+                     const SocgpuRequest *requests);
+/* Device time of the last replay (ms), graph captures and replays so far. */
+int socgpu_batch_stats(const SocgpuBatch *batch, double *last_ms,
+                       int *captures, int *replays);
+int socgpu_batch_destroy(SocgpuBatch *batch);
+
+/* Serving: a request loop for callers that issue radiation calls
+ * continuously (many model instances, or one model's column segments from
+ * several threads). socgpu_serve_submit queues a call (its argument structs
+ * are copied; the arrays they point to must stay valid until the call
+ * completes); a server thread admits queued calls while the device
+ * footprint of all admitted calls fits hbm_budget_bytes, packs everything
+ * admitted into one batch per free slot (a stream with its own device
+ * buffers) and launches it, so calls that arrive while other batches are on
+ * the GPU are admitted as memory and slots free up and run alongside them.
+ * Results are those of the same call made alone. No reference
+ * counterpart. */
+typedef struct SocgpuServeConfig {
+  int precision;              /* SOCGPU_FP64 or SOCGPU_MIXED */
+  int n_slots;                /* batches in flight (1-8) */
+  int max_batch_requests;     /* calls per batch */
+  int pad;
+  size_t hbm_budget_bytes;    /* device bytes admitted calls may hold */
+} SocgpuServeConfig;
+
+typedef struct SocgpuServeStats {
+  long long completed, batches, solves;  /* solves: column x g-point */
+  double mean_batch;                     /* calls per batch */
+  double p50_ms, p99_ms, max_ms;         /* submit-to-done latency */
+  double elapsed_s;                      /* since start or reset */
+} SocgpuServeStats;
+
+typedef struct SocgpuServer SocgpuServer;
+int socgpu_serve_start(SocgpuServer **server, const SocgpuServeConfig *cfg);
+int socgpu_serve_submit(SocgpuServer *server, const SocgpuRequest *request,
+                        long long *ticket);
+/* Blocks until the call is done (outputs written) or timeout_s passes. */
+int socgpu_serve_wait(SocgpuServer *server, long long ticket,
+                      double timeout_s);
+int socgpu_serve_stats(SocgpuServer *server, SocgpuServeStats *stats);
+int socgpu_serve_reset_stats(SocgpuServer *server);
+/* Drains the queue, stops the server thread and frees its buffers. */
+int socgpu_serve_stop(SocgpuServer *server);
+
 /* Launch shape of the column-solve kernel, per precision form (no reference
  * counterpart; the autotuner in socrates_amd/tune.py picks it):
  *   recompute        0: keep each layer's elimination state in lane-private
@@ -172,13 +241,22 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req,
  *                       4 layers and recompute each segment in registers on
  *                       the way up (twice the arithmetic, a sixth of the
  *                       traffic). Results are bit-identical either way.
- *   waves_per_block  1, 2 or 4 column chunks (64 items each) per block. */
+ *   waves_per_block  1, 2 or 4 column chunks (64 items each) per block.
+ *   waves_per_simd   3 or 4: the register budget the kernel is compiled for
+ *                    (168 or 128 registers per lane). */
 typedef struct SocgpuTile {
   int recompute;
   int waves_per_block;
+  int waves_per_simd;
 } SocgpuTile;
 
 int socgpu_set_tile(int precision, const SocgpuTile *tile);
+/* Autotuner: times every tile (recompute 0/1 x waves_per_block 1/2/4 x
+ * waves_per_simd 3/4) on the given batch with HIP events around the solve
+ * kernel (best of `repeats` runs after a warm-up), keeps the fastest for the
+ * precision form and returns it. */
+int socgpu_autotune(int n_request, const SocgpuRequest *requests,
+                    int precision, int repeats, SocgpuTile *best);
 int socgpu_get_tile(int precision, SocgpuTile *tile);
 
 /* Message of the last failure on this host thread. */

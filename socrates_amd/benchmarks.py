@@ -1,0 +1,172 @@
+"""Benchmarks beyond the headline one (bench.py at the repository root):
+
+  ragged  - 10 000 columns with layer counts uniform in 30..137, half LW
+            (9 bands), half SW (6 bands), all-sky McICA, grouped into one
+            radiation call per (source, layer count) and run as one packed,
+            graph-captured batch per step (socgpu_batch_run); FP64 and mixed.
+  serve   - the serving loop under Poisson arrivals: each call carries 1..4096
+            columns (log-uniform) with 40..137 layers, LW or SW, all-sky McICA;
+            p50/p99 submit-to-done latency and throughput per arrival rate.
+
+Inputs are synthetic (socrates_amd/synth.py, seeded). Throughput is in
+column.g-point solves/s, and also normalised to 90 layers (x L/90).
+
+    python -m socrates_amd.benchmarks ragged [--steps 10]
+    python -m socrates_amd.benchmarks serve [--rates 50,200,1000,2000] [--seconds 10]
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from socrates_amd import runes as R  # noqa: E402
+from socrates_amd import synth  # noqa: E402
+
+NBAND = {"lw": 9, "sw": 6}
+NG = {"lw": 50, "sw": 40}
+FIELDS = ("p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3", "p_level",
+          "t_level", "t_ground", "solar_irrad", "albedo_diff", "albedo_dir", "cloud_frac", "liq_mmr",
+          "liq_dim", "rand_seed")
+
+
+def _device_case(torch, case, dev):
+    arrs = {k: torch.from_numpy(np.ascontiguousarray(getattr(case, k))).to(dev) for k in FIELDS}
+    arrs["cos_zenith_angle"] = torch.from_numpy(np.ascontiguousarray(case.cos_zen)).to(dev)
+    L, P = case.n_layer, case.n_profile
+    out = {k: torch.empty(((L + 1) if k != "heating_rate" else L, P), dtype=torch.float64, device=dev)
+           for k in ("flux_up", "flux_down", "flux_direct", "heating_rate")}
+    return arrs, out
+
+
+def ragged(steps=10, warmup=2, n_columns=10000, seed=20261206):
+    import torch
+    dev = torch.device("cuda:0")
+    R.init(0)
+    R.register_default_spectra()
+    rng = np.random.default_rng(seed)
+    levels = rng.integers(30, 138, n_columns)
+    source = np.where(np.arange(n_columns) < n_columns // 2, "lw", "sw")
+    calls, keep = [], []
+    solves = 0.0
+    for src in ("lw", "sw"):
+        for L in np.unique(levels[source == src]):
+            P = int(((levels == L) & (source == src)).sum())
+            case = synth.make_case(P, int(L), src, NBAND[src], seed=seed + 1000 * int(L) + (src == "sw"),
+                                   cloudy=True, first_column_id=int(L) * 100000)
+            arrs, out = _device_case(torch, case, dev)
+            keep.append((arrs, out))
+            calls.append(R.case_call(case, src, device_arrays=arrs, out=out))
+            solves += P * NG[src] * L / 90.0
+    res = {}
+    for prec in ("fp64", "mixed"):
+        b = R.Batch(prec)
+        for _ in range(warmup):
+            b.run(calls)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev_ms = 0.0
+        for _ in range(steps):
+            b.run(calls)
+            dev_ms += b.stats()["last_ms"]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        st = b.stats()
+        b.close()
+        res[prec] = dict(ms_per_call=dt * 1e3, device_ms=dev_ms / steps,
+                         solves_per_s_l90=solves / dt, captures=st["captures"], replays=st["replays"])
+    line = {"benchmark": "ragged batch, one graph-captured launch per radiation call",
+            "columns": n_columns, "calls_per_batch": len(calls), "layers": "U{30..137}",
+            "split": "50% LW (9 bands, 50 g) / 50% SW (6 bands, 40 g), all-sky McICA",
+            "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_over_fp64": res["mixed"]["solves_per_s_l90"] / res["fp64"]["solves_per_s_l90"]}
+    print(json.dumps(line), flush=True)
+    return line
+
+
+def serve(rates=(50, 200, 1000, 2000), seconds=10.0, pool=48, seed=20261207, precisions=("fp64", "mixed"),
+          max_requests=4000):
+    from socrates_amd.serve import Server
+    R.init(0)
+    R.register_default_spectra()
+    rng = np.random.default_rng(seed)
+    templates = []
+    for k in range(pool):
+        src = "lw" if k % 2 == 0 else "sw"
+        P = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
+        L = int(rng.integers(40, 138))
+        case = synth.make_case(max(P, 1), L, src, NBAND[src], seed=seed + k, cloudy=True,
+                               first_column_id=100000 * k)
+        templates.append((src, case))
+    lines = []
+    for prec in precisions:
+        for rate in rates:
+            srv = Server(precision=prec, hbm_budget_gb=96.0, n_slots=2, max_batch_requests=512)
+            # warm the slots' buffers
+            srv.wait(srv.submit(*R.case_call(templates[0][1], templates[0][0])))
+            srv.reset_stats()
+            n = int(min(max_requests, rate * seconds))
+            gaps = rng.exponential(1.0 / rate, n)
+            picks = rng.integers(0, pool, n)
+            solves = 0.0
+            # a consumer waits on the tickets as they come, releasing the
+            # outputs of finished calls
+            import queue
+            q = queue.Queue()
+
+            def consume():
+                while True:
+                    t = q.get()
+                    if t is None:
+                        return
+                    srv.wait(t, 600.0)
+            th = threading.Thread(target=consume, daemon=True)
+            th.start()
+            t_start = time.perf_counter()
+            t_next = t_start
+            for i in range(n):
+                t_next += gaps[i]
+                now = time.perf_counter()
+                if t_next > now:
+                    time.sleep(t_next - now)
+                src, case = templates[picks[i]]
+                q.put(srv.submit(*R.case_call(case, src)))
+                solves += case.n_profile * NG[src] * case.n_layer / 90.0
+            q.put(None)
+            th.join()
+            wall = time.perf_counter() - t_start
+            st = srv.stats()
+            srv.close()
+            line = {"benchmark": "serving loop, Poisson arrivals", "precision": prec, "rate_req_s": rate,
+                    "requests": n, "offered_s": float(gaps.sum()), "wall_s": wall,
+                    "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+                    "mean_batch": st["mean_batch"], "batches": st["batches"],
+                    "solves_per_s_l90": solves / wall}
+            print(json.dumps(line), flush=True)
+            lines.append(line)
+    return lines
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["ragged", "serve"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--seconds", type=float, default=10.0)
+    ap.add_argument("--max-requests", type=int, default=4000)
+    a = ap.parse_args()
+    if a.what == "ragged":
+        ragged(steps=a.steps)
+    else:
+        serve(rates=[int(x) for x in a.rates.split(",")], seconds=a.seconds, max_requests=a.max_requests)
+
+
+if __name__ == "__main__":
+    main()

@@ -14,12 +14,17 @@
 // The spectral tables live on the device from registration on.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <chrono>
+#include <deque>
+#include <condition_variable>
+#include <thread>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -137,10 +142,10 @@ struct CallDims {
 // threshold 0.001, type-fraction normalisation), set_cld_dim
 // (socrates_set_cld_dim.F90:214-220: radius clamp), cond_rsd of
 // set_cld_mcica (socrates_set_cld_mcica.F90:168-180).
-__global__ void pack_kernel(CallDims d, const SgSpec *__restrict__ spec, SocgpuRunesArgs a,
-                            double *__restrict__ colrec, double *__restrict__ tlev,
-                            double *__restrict__ colscal, double *__restrict__ bandscal) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pack_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                          const SocgpuRunesArgs &a, double *__restrict__ colrec,
+                                          double *__restrict__ tlev, double *__restrict__ colscal,
+                                          double *__restrict__ bandscal, int idx) {
   const int P = d.P, L = d.L;
   if (idx < P * L) {
     int col = idx % P, i = idx / P;
@@ -218,27 +223,6 @@ __global__ void pack_kernel(CallDims d, const SgSpec *__restrict__ spec, SocgpuR
 // Cloud top and base over all columns of the call, as cloud_gen finds them
 // (socrates_cloud_gen.F90:224-245): first layer with any w > epsilon, last
 // layer with any w >= epsilon. krange[0] = min, krange[1] = max (1-based).
-__global__ void cldrange_kernel(CallDims d, const double *__restrict__ colrec, int *krange) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  int kmin = INT32_MAX, kmax = 0;
-  if (idx < d.P * d.L) {
-    int col = idx / d.L, i = idx % d.L;
-    double w = colrec[((size_t)col * d.L + i) * SG_NQ + SG_Q_WCLD];
-    if (w > kEps) kmin = i + 1;
-    if (w >= kEps) kmax = i + 1;
-  }
-  // one atomic per wave: the whole call shares one cloud top and base
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    kmin = min(kmin, __shfl_xor(kmin, o));
-    kmax = max(kmax, __shfl_xor(kmax, o));
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    if (kmin != INT32_MAX) atomicMin(&krange[0], kmin);
-    if (kmax != 0) atomicMax(&krange[1], kmax);
-  }
-}
-
 __global__ void fill_int_kernel(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -270,15 +254,14 @@ __device__ double zcw_lookup(double y, double sigma, int n1, int n2, const doubl
   return ((t1 + t2) + t3) + t4;
 }
 
-__global__ __launch_bounds__(64) void cloudgen_kernel(CallDims d, const double *__restrict__ colrec,
-                                                      const int *__restrict__ seeds,
-                                                      double dp_cf, double dp_cw,
-                                                      const int *__restrict__ krange,
-                                                      const double *__restrict__ xcw, int n1, int n2,
-                                                      double *__restrict__ c_sub,
-                                                      double *__restrict__ colscal) {
+__device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *__restrict__ colrec,
+                                              const int *__restrict__ seeds, double dp_cf,
+                                              double dp_cw, const int *__restrict__ krange,
+                                              const double *__restrict__ xcw, int n1, int n2,
+                                              double *__restrict__ c_sub,
+                                              double *__restrict__ colscal, int col, int lane) {
 #pragma clang fp contract(off)
-  const int col = blockIdx.x, lane = threadIdx.x, L = d.L, n_gen = d.n_gen;
+  const int L = d.L, n_gen = d.n_gen;
   const double rm = 1.0 / (double)kLcgM;
   const int seed_max = (2147483647 - kLcgC) / kLcgA;
   double *cs = c_sub + (size_t)col * L * SG_MAXSUBCOL;
@@ -397,15 +380,13 @@ __device__ __forceinline__ void scan_affine(double &A, double &C, int lane) {
 // differences of order 1e-16 in the weights).
 constexpr int kMinorWaves = 4;
 
-__global__ __launch_bounds__(64 * kMinorWaves) void minor_kernel(CallDims d, const SgSpec *__restrict__ spec,
-                                                                const double *__restrict__ colrec,
-                                                                const double *__restrict__ tlev,
-                                                                const double *__restrict__ colscal,
-                                                                const double *__restrict__ bandscal,
-                                                                double *__restrict__ minor) {
-  __shared__ double sh[kMinorWaves][4][SG_MAXL];
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int wave = blockIdx.x * kMinorWaves + wv;
+__device__ __forceinline__ void minor_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                           const double *__restrict__ colrec,
+                                           const double *__restrict__ tlev,
+                                           const double *__restrict__ colscal,
+                                           const double *__restrict__ bandscal,
+                                           double *__restrict__ minor, double (*sh)[SG_MAXL],
+                                           int wave, int lane) {
   const int NT = spec->n_minor, L = d.L;
   if (wave >= d.P * NT) return;
   const int col = wave / NT, ti = wave % NT;
@@ -413,7 +394,7 @@ __global__ __launch_bounds__(64 * kMinorWaves) void minor_kernel(CallDims d, con
   const SgBand &B = spec->band[t.band];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   double *o = minor + ((size_t)col * NT + ti) * L * 2;
-  double *kk = sh[wv][0];
+  double *kk = sh[0];
   const int nch = (L + kWave - 1) / kWave;
   if (d.isolir == SOCGPU_SOURCE_SOLAR) {
     const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
@@ -434,7 +415,7 @@ __global__ __launch_bounds__(64 * kMinorWaves) void minor_kernel(CallDims d, con
     __syncwarp();
     for (int i = lane; i < L; i += kWave) o[2 * i + 1] = kk[i] * fdt;
   } else {
-    double *tr = sh[wv][1], *su = sh[wv][2], *fdl = sh[wv][3];
+    double *tr = sh[1], *su = sh[2], *fdl = sh[3];
     const double *tl = tlev + (size_t)col * (L + 1);
     const double alb = bandscal[((size_t)col * d.NB + t.band) * SG_NBQ + 2];
     const double sqe = 1.4901161193847656e-08;
@@ -490,15 +471,14 @@ __global__ __launch_bounds__(64 * kMinorWaves) void minor_kernel(CallDims d, con
 // the per-term sums (solve_band_k_eqv_scl.F90:826-846, 995-1019), the solar
 // direct-beam adjustment (ibid. :848-891), and the Planck source terms
 // (diff_planck_source_mod.F90:100-134). bandrec[col][band][layer][4].
-__global__ __launch_bounds__(256) void prep_kernel(CallDims d, const SgSpec *__restrict__ spec,
-                                                   int l_rayleigh, const double *__restrict__ colrec,
-                                                   const double *__restrict__ tlev,
-                                                   const double *__restrict__ colscal,
-                                                   const double *__restrict__ minor,
-                                                   double *__restrict__ bandrec,
-                                                   double *__restrict__ bandscal,
-                                                   double *__restrict__ cldopt) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                          int l_rayleigh, const double *__restrict__ colrec,
+                                          const double *__restrict__ tlev,
+                                          const double *__restrict__ colscal,
+                                          const double *__restrict__ minor,
+                                          double *__restrict__ bandrec,
+                                          double *__restrict__ bandscal,
+                                          double *__restrict__ cldopt, int idx) {
   const int NB = d.NB, L = d.L, NT = spec->n_minor;
   if (idx >= d.P * NB * L) return;
   const int i = idx % L, b = (idx / L) % NB, col = idx / (L * NB);
@@ -603,14 +583,48 @@ struct LaneCtx {
   double sec0, ksc_clear;
 };
 
+// One lane's inputs for one layer, loaded a layer ahead of their use so
+// that the loads of layer i+1 are in flight while layer i is computed.
+struct LayerIn {
+  double lnp, lnt, mmr, mass, frac1;  // column record (wave-uniform but mmr)
+  double kgt, q1, q2;                 // band record: grey extinction, adjust or Planck terms
+  double cv;                          // the lane's sub-column condensate scaling
+  double co[4];                       // droplet optics at unit scaling
+};
+
+struct LaneSrc {
+  const double *rec, *br, *csub, *cop;
+  int gas, isub;
+};
+
+__device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) {
+  const double *r = s.rec + i * SG_NQ;
+  x.lnp = r[SG_Q_LNP];
+  x.lnt = r[SG_Q_LNT];
+  x.mmr = r[SG_Q_MMR + s.gas];
+  x.mass = r[SG_Q_MASS];
+  x.frac1 = r[SG_Q_FRAC1];
+  const double *q = s.br + i * SG_NBQ;
+  x.kgt = q[0];
+  x.q1 = q[1];
+  x.q2 = q[2];
+  x.cv = s.isub > 0 ? s.csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
+  if (x.cv * x.frac1 > 0.0) {
+    const double *co = s.cop + i * 4;
+    x.co[0] = co[0]; x.co[1] = co[1]; x.co[2] = co[2]; x.co[3] = co[3];
+  } else {
+    x.co[0] = 0.0; x.co[1] = 0.0; x.co[2] = 0.0; x.co[3] = 0.0;
+  }
+}
+
 // co: the layer's droplet optics at unit condensate scaling (kext, kscat,
 // first moment, forward scattering), from prep_kernel.
-__device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, const double *r,
-                                           double kgt, double csub, const double *co, Coef &o) {
-  double kgas = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-  const double frac1 = r[SG_Q_FRAC1];
+__device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
+                                           Coef &o) {
+  double kgas = k_term(t, x.lnp, x.lnt, x.mmr);
+  const double frac1 = x.frac1, kgt = x.kgt, csub = x.cv;
   if (csub * frac1 > 0.0) {
-    const double kext = co[0], kscat = co[1], phf = co[2], fwd = co[3];
+    const double kext = x.co[0], kscat = x.co[1], phf = x.co[2], fwd = x.co[3];
     double kg = kgt + kext * csub;
     double ks = c.ksc_clear + kscat * csub;
     double fw = fwd * csub;
@@ -618,7 +632,7 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
     pf = (pf - fw) / fmax(ks - fw, kEps);
     fw = fw / fmax(ks, kEps);
     double ktot = kg + kgas;
-    double tau = ktot * r[SG_Q_MASS];
+    double tau = ktot * x.mass;
     double om = fmin(ks / (ktot + kTiny), 1.0 - 32.0 * kEps);
     tau = tau * (1.0 - om * fw);
     om = om * (1.0 - fw) / (1.0 - om * fw);
@@ -626,7 +640,7 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
     o.T *= frac1; o.R *= frac1; o.T0 *= frac1; o.s1 *= frac1; o.s2 *= frac1;
   } else {
     double ktot = kgt + kgas;
-    double tau = ktot * r[SG_Q_MASS];
+    double tau = ktot * x.mass;
     double om = fmin(c.ksc_clear / (ktot + kTiny), 1.0 - 32.0 * kEps);
     two_coeff(c.solar, tau, om, 0.0, c.sec0, o);
   }
@@ -638,12 +652,14 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
 // accuracy; the reference's clamp omega <= 1 - 32 eps(FP64) is kept as
 // 1 - omega >= 32 eps(FP64). No FP64 fallback: two_coeff_f32 has no
 // resonance singularity.
-__device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm &t, const double *r,
-                                                double kgt, double csub, const double *co, Coef &o) {
-  const float kgas = k_term_f32(t, (float)r[SG_Q_LNP], (float)r[SG_Q_LNT], (float)r[SG_Q_MMR + t.gas]);
-  const float frac1 = (float)r[SG_Q_FRAC1];
-  const float kag = (float)(kgt - c.ksc_clear);  // grey absorption (equivalent extinction)
-  const float mass = (float)r[SG_Q_MASS];
+__device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
+                                                Coef &o) {
+  const float kgas = k_term_f32(t, (float)x.lnp, (float)x.lnt, (float)x.mmr);
+  const double csub = x.cv;
+  const double *co = x.co;
+  const float frac1 = (float)x.frac1;
+  const float kag = (float)(x.kgt - c.ksc_clear);  // grey absorption (equivalent extinction)
+  const float mass = (float)x.mass;
   const float omc_min = (float)(32.0 * kEps);
   float tau, om, omc, g;
   bool cloudy = csub * frac1 > 0.0;
@@ -688,22 +704,21 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
 // above (rho) and the diffuse flux it sends down (dn). Returns the
 // back-substitution pair fu(i) = a fu(i+1) + bb.
 template <bool MIXED>
-__device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, const double *r,
-                                           const double *q, double cv, const double *co,
+__device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                            double &fdir, double &rho, double &dn, double &a,
                                            double &bb) {
   Coef o;
-  if (MIXED) layer_coef_mixed(c, t, r, q[0], cv, co, o);
-  else layer_coef(c, t, r, q[0], cv, co, o);
+  if (MIXED) layer_coef_mixed(c, t, x, o);
+  else layer_coef(c, t, x, o);
   double su, sd;
   if (c.solar) {
-    double fnew = fdir * o.T0 * q[1];
+    double fnew = fdir * o.T0 * x.q1;
     su = o.s1 * fdir;
     sd = (o.s2 - o.T0) * fdir + fnew;
     fdir = fnew;
   } else {
-    su = o.s1 * q[2] + o.s2 * q[1];
-    sd = -o.s1 * q[2] + o.s2 * q[1];
+    su = o.s1 * x.q2 + o.s2 * x.q1;
+    sd = -o.s1 * x.q2 + o.s2 * x.q1;
   }
   double bp;
   if (MIXED) {
@@ -742,17 +757,14 @@ __device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, co
 constexpr int kSeg = 4;
 
 template <bool MIXED, bool RECOMP>
-__global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec *__restrict__ spec,
-                                                    int l_rayleigh,
-                                                    const double *__restrict__ colrec,
-                                                    const double *__restrict__ colscal,
-                                                    const double *__restrict__ bandrec,
-                                                    const double *__restrict__ bandscal,
-                                                    const double *__restrict__ c_sub,
-                                                    const double *__restrict__ cldopt,
-                                                    double *__restrict__ partial) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int lane = threadIdx.x % kWave;
+__device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                           int l_rayleigh, const double *__restrict__ colrec,
+                                           const double *__restrict__ colscal,
+                                           const double *__restrict__ bandrec,
+                                           const double *__restrict__ bandscal,
+                                           const double *__restrict__ c_sub,
+                                           const double *__restrict__ cldopt,
+                                           double *__restrict__ partial, int wave, int lane) {
   const int L = d.L, NB = d.NB;
   const int col = wave / d.nwpc, chunk = wave % d.nwpc;
   if (col >= d.P) return;
@@ -787,8 +799,13 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
   c.solar = solar;
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
-  const double *cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
-  const double *csub = c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
+  LaneSrc src;
+  src.rec = rec;
+  src.br = br;
+  src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
+  src.csub = c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
+  src.gas = t.gas;
+  src.isub = isub;
   const double alb_diff = bs[2], alb_dir = bs[3];
 
   constexpr int kNst = RECOMP ? (SG_MAXL + kSeg - 1) / kSeg : SG_MAXL;
@@ -797,6 +814,8 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
   double fdir = solar ? finc : 0.0;
   double rho = 0.0, dn = finc;
   double vdir[8];
+  LayerIn cur, nxt;
+  load_layer(src, 0, cur);
   for (int i = 0; i < L; i++) {
     if (solar) {
       // direct flux at the top of layer i, summed 8 levels at a time
@@ -809,20 +828,21 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
         if ((lane & 7) == 0) out[2 * (L + 1) + i - 7 + (lane >> 3)] = vdir[0];
       }
     }
+    if (i + 1 < L) load_layer(src, i + 1, nxt);
     if (active) {
       if (RECOMP) {
         if (i % kSeg == 0) {
           st[i / kSeg][0] = fdir; st[i / kSeg][1] = rho; st[i / kSeg][2] = dn;
         }
       }
-      double cv = isub > 0 ? csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
       double a, bb;
       const double rho0 = rho, dn0 = dn;
-      layer_step<MIXED>(c, t, rec + i * SG_NQ, br + i * SG_NBQ, cv, cop + i * 4, fdir, rho, dn, a, bb);
+      layer_step<MIXED>(c, t, cur, fdir, rho, dn, a, bb);
       if (!RECOMP) {
         st[i][0] = a; st[i][1] = bb; st[i][2] = rho0; st[i][3] = dn0;
       }
     }
+    cur = nxt;
   }
   if (solar) {
     // remaining direct-flux levels (the partial block and the surface)
@@ -889,9 +909,9 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
           const int i = i0 + k;
           sr[k] = rho_s; sdn[k] = dn_s; sa[k] = 0.0; sb[k] = 0.0;
           if (i < L) {
-            double cv = isub > 0 ? csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
-            layer_step<MIXED>(c, t, rec + i * SG_NQ, br + i * SG_NBQ, cv, cop + i * 4, fdir_s, rho_s,
-                              dn_s, sa[k], sb[k]);
+            LayerIn x;
+            load_layer(src, i, x);
+            layer_step<MIXED>(c, t, x, fdir_s, rho_s, dn_s, sa[k], sb[k]);
           }
         }
       }
@@ -919,11 +939,11 @@ __global__ __launch_bounds__(256, 4) void solve_kernel(CallDims d, const SgSpec 
 // Sum of the per-wave partials, the Planck flux added back for the
 // infra-red (adjust_ir_radiance.F90:186-200, weight_band = 1), output in the
 // caller's [level][profile] layout.
-__global__ void finish_kernel(CallDims d, const double *__restrict__ partial,
-                              const double *__restrict__ bandrec,
-                              const double *__restrict__ bandscal, double *__restrict__ up,
-                              double *__restrict__ down, double *__restrict__ direct) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void finish_body(const CallDims &d, const double *__restrict__ partial,
+                                            const double *__restrict__ bandrec,
+                                            const double *__restrict__ bandscal,
+                                            double *__restrict__ up, double *__restrict__ down,
+                                            double *__restrict__ direct, int idx) {
   const int P = d.P, L = d.L;
   if (idx >= P * (L + 1)) return;
   int col = idx % P, lev = idx / P;
@@ -950,15 +970,173 @@ __global__ void finish_kernel(CallDims d, const double *__restrict__ partial,
 }
 
 // Heating rate from flux divergence (socrates_set_diag.F90:140-210), K/s.
-__global__ void heating_kernel(int P, int L, const double *__restrict__ up,
-                               const double *__restrict__ down,
-                               const double *__restrict__ heat_cap, double *__restrict__ hr) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void heating_body(int P, int L, const double *__restrict__ up,
+                                             const double *__restrict__ down,
+                                             const double *__restrict__ heat_cap,
+                                             double *__restrict__ hr, int idx) {
   if (idx >= P * L) return;
   int col = idx % P, i = idx / P;  // layer i+1 lies between levels i and i+1
   double fdiv = down[(size_t)i * P + col] - down[(size_t)(i + 1) * P + col] +
                 up[(size_t)(i + 1) * P + col] - up[(size_t)i * P + col];
   hr[idx] = heat_cap ? fdiv / heat_cap[idx] : fdiv;
+}
+
+// ------------------------------------------------------------ batches
+// A batch is a list of radiation calls ("requests"), each with its own
+// spectrum, source, shape and options, packed into one launch sequence:
+// every kernel loops over the concatenated work of all requests and finds
+// the request of a work item by binary search over per-request starts. A
+// request's results are those of the same call made alone.
+enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, kNWork };
+
+struct ReqDesc {
+  CallDims d;
+  const SgSpec *spec;
+  const double *xcw;
+  int n1, n2, l_rayleigh, pad;
+  double dp_cf, dp_cw;
+  SocgpuRunesArgs a;            // device input pointers
+  double *up, *dn, *dir, *hr;   // device outputs (dir may be null)
+  double *colrec, *tlev, *colscal, *bandrec, *bandscal, *csub, *minor, *cldopt, *partial;
+  int *krange;
+  long long s[kNWork];          // first work item of this request in each space
+};
+
+struct BatchHdr {
+  int R, pad;
+  long long tot[kNWork];
+};
+
+__device__ __forceinline__ int find_req(const BatchHdr *h, const ReqDesc *rq, int w, long long g) {
+  int lo = 0, hi = h->R - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (rq[mid].s[w] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void b_init_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= h->R) return;
+  rq[r].krange[0] = rq[r].d.L + 1;
+  rq[r].krange[1] = 0;
+}
+
+__global__ void b_pack_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_LEV];
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
+       g += (long long)gridDim.x * blockDim.x) {
+    const ReqDesc &q = rq[find_req(h, rq, W_LEV, g)];
+    pack_body(q.d, q.spec, q.a, q.colrec, q.tlev, q.colscal, q.bandscal, (int)(g - q.s[W_LEV]));
+  }
+}
+
+// Cloud top and base over all columns of each request, as cloud_gen finds
+// them (socrates_cloud_gen.F90:224-245): first layer with any w > epsilon,
+// last layer with any w >= epsilon; krange[0] = min, krange[1] = max
+// (1-based). One atomic per wave where the wave lies within one request.
+__global__ void b_cldrange_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_LAY];
+  const int lane = threadIdx.x & (kWave - 1);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = (blockIdx.x * (long long)blockDim.x + threadIdx.x) - lane; base < tot;
+       base += stride) {
+    const long long g = base + lane;
+    int r = -1, kmin = INT32_MAX, kmax = 0;
+    if (g < tot) {
+      r = find_req(h, rq, W_LAY, g);
+      const ReqDesc &q = rq[r];
+      if (q.d.mcica) {
+        const int idx = (int)(g - q.s[W_LAY]);
+        const int col = idx / q.d.L, i = idx % q.d.L;
+        double w = q.colrec[((size_t)col * q.d.L + i) * SG_NQ + SG_Q_WCLD];
+        if (w > kEps) kmin = i + 1;
+        if (w >= kEps) kmax = i + 1;
+      }
+    }
+    const int r0 = __shfl(r, 0, kWave);
+    if (__all(r == r0 || r < 0)) {
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        kmin = min(kmin, __shfl_xor(kmin, o, kWave));
+        kmax = max(kmax, __shfl_xor(kmax, o, kWave));
+      }
+      if (lane == 0 && r0 >= 0) {
+        if (kmin != INT32_MAX) atomicMin(&rq[r0].krange[0], kmin);
+        if (kmax != 0) atomicMax(&rq[r0].krange[1], kmax);
+      }
+    } else if (r >= 0) {
+      if (kmin != INT32_MAX) atomicMin(&rq[r].krange[0], kmin);
+      if (kmax != 0) atomicMax(&rq[r].krange[1], kmax);
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void b_cloudgen_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_CG];
+  for (long long w = blockIdx.x; w < tot; w += gridDim.x) {
+    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_CG, w));
+    const ReqDesc &q = rq[r];
+    cloudgen_body(q.d, q.colrec, q.a.rand_seed, q.dp_cf, q.dp_cw, q.krange, q.xcw, q.n1, q.n2, q.csub,
+                  q.colscal, (int)(w - q.s[W_CG]), threadIdx.x);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64 * kMinorWaves) void b_minor_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  __shared__ double sh[kMinorWaves][4][SG_MAXL];
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const long long tot = h->tot[W_MINOR];
+  for (long long w = blockIdx.x * (long long)kMinorWaves + wv; w < tot; w += (long long)gridDim.x * kMinorWaves) {
+    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_MINOR, w));
+    const ReqDesc &q = rq[r];
+    minor_body(q.d, q.spec, q.colrec, q.tlev, q.colscal, q.bandscal, q.minor, sh[wv],
+               (int)(w - q.s[W_MINOR]), lane);
+    __syncwarp();
+  }
+}
+
+__global__ __launch_bounds__(256) void b_prep_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_PREP];
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
+       g += (long long)gridDim.x * blockDim.x) {
+    const ReqDesc &q = rq[find_req(h, rq, W_PREP, g)];
+    prep_body(q.d, q.spec, q.l_rayleigh, q.colrec, q.tlev, q.colscal, q.minor, q.bandrec, q.bandscal,
+              q.cldopt, (int)(g - q.s[W_PREP]));
+  }
+}
+
+template <bool MIXED, bool RECOMP, int OCC>
+__global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_SOLVE];
+  const int lane = threadIdx.x % kWave;
+  const long long nw = (long long)gridDim.x * (blockDim.x / kWave);
+  for (long long w = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; w < tot; w += nw) {
+    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_SOLVE, w));
+    const ReqDesc &q = rq[r];
+    solve_body<MIXED, RECOMP>(q.d, q.spec, q.l_rayleigh, q.colrec, q.colscal, q.bandrec, q.bandscal,
+                              q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
+  }
+}
+
+__global__ void b_finish_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_LEV];
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
+       g += (long long)gridDim.x * blockDim.x) {
+    const ReqDesc &q = rq[find_req(h, rq, W_LEV, g)];
+    finish_body(q.d, q.partial, q.bandrec, q.bandscal, q.up, q.dn, q.dir, (int)(g - q.s[W_LEV]));
+  }
+}
+
+__global__ void b_heating_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_LAY];
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
+       g += (long long)gridDim.x * blockDim.x) {
+    const ReqDesc &q = rq[find_req(h, rq, W_LAY, g)];
+    if (q.hr) heating_body(q.d.P, q.d.L, q.up, q.dn, q.a.layer_heat_capacity, q.hr, (int)(g - q.s[W_LAY]));
+  }
 }
 
 // Kernel-level two-stream solve on prescribed optical properties, one lane
@@ -1074,14 +1252,14 @@ struct Context {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::map<std::string, std::unique_ptr<SpecEntry>> spectra;
-  DevBuf colrec, tlev, colscal, bandrec, bandscal, csub, partial, krange, minor, cldopt;
+  DevBuf ws, desc;
   DevBuf in_stage, out_stage;
   double last_total_ms = 0.0, last_solve_ms = 0.0;
   std::mutex mu;
 };
 static Context g_ctx;
 // solve-kernel tile per precision form ([0] FP64, [1] mixed)
-static SocgpuTile g_tile[2] = {{0, 2}, {0, 2}};
+static SocgpuTile g_tile[2] = {{0, 2, 4}, {0, 2, 4}};
 
 static int ensure_init() {
   if (g_ctx.device >= 0) return 0;
@@ -1166,12 +1344,12 @@ int socgpu_mcica_register(const char *path, const char *sw_name, const char *lw_
   return 0;
 }
 
-static int stage_inputs(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB) {
-  *dv = *a;
-  if (a->l_inputs_on_device) return 0;
+// Input arrays of one call in the order they are staged; n = element counts.
+struct InField { const double **dst; const double *src; size_t n; };
+
+static size_t input_fields(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB, InField *f) {
   const size_t P = a->n_profile, L = a->n_layer;
-  struct F { const double **dst; const double *src; size_t n; };
-  F f[] = {
+  InField g[] = {
       {&dv->p_layer, a->p_layer, L * P}, {&dv->t_layer, a->t_layer, L * P},
       {&dv->mass, a->mass, L * P}, {&dv->density, a->density, L * P},
       {&dv->layer_heat_capacity, a->layer_heat_capacity, L * P},
@@ -1184,30 +1362,57 @@ static int stage_inputs(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB) {
       {&dv->liq_dim, a->liq_dim, L * P}, {&dv->conv_frac, a->conv_frac, L * P},
       {&dv->liq_conv_mmr, a->liq_conv_mmr, L * P}, {&dv->liq_conv_dim, a->liq_conv_dim, L * P},
   };
-  size_t total = 0;
-  for (auto &x : f) if (x.src) total += x.n;
-  size_t ints = a->rand_seed ? P : 0;
-  if (g_ctx.in_stage.ensure(total * sizeof(double) + ints * sizeof(int) + 256))
-    return fail(SOCGPU_ENOMEM, "input staging allocation failed");
-  char *base = g_ctx.in_stage.as<char>();
-  size_t off = 0;
-  for (auto &x : f) {
+  const size_t n = sizeof g / sizeof g[0];
+  for (size_t k = 0; k < n; k++) f[k] = g[k];
+  return n;
+}
+
+// Bytes of device staging one call's host inputs need (0 for device inputs).
+static size_t stage_bytes(const SocgpuRunesArgs *a, int NB) {
+  if (a->l_inputs_on_device) return 0;
+  SocgpuRunesArgs dv = *a;
+  InField f[32];
+  size_t n = input_fields(a, &dv, NB, f), total = 0;
+  for (size_t k = 0; k < n; k++) if (f[k].src) total += (f[k].n * 8 + 255) & ~(size_t)255;
+  if (a->rand_seed) total += ((size_t)a->n_profile * 4 + 255) & ~(size_t)255;
+  return total;
+}
+
+// Copies one call's host inputs to base + *off on stream s and points dv at
+// them; device inputs are used in place.
+static int stage_inputs_at(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB, char *base, size_t *off,
+                           hipStream_t s) {
+  *dv = *a;
+  if (a->l_inputs_on_device) return 0;
+  InField f[32];
+  size_t n = input_fields(a, dv, NB, f);
+  for (size_t k = 0; k < n; k++) {
+    InField &x = f[k];
     if (!x.src) { *x.dst = nullptr; continue; }
-    HIPCHK(hipMemcpyAsync(base + off, x.src, x.n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
-    *x.dst = (const double *)(base + off);
-    off += x.n * sizeof(double);
+    HIPCHK(hipMemcpyAsync(base + *off, x.src, x.n * sizeof(double), hipMemcpyHostToDevice, s));
+    *x.dst = (const double *)(base + *off);
+    *off += (x.n * 8 + 255) & ~(size_t)255;
   }
-  if (ints) {
-    HIPCHK(hipMemcpyAsync(base + off, a->rand_seed, P * sizeof(int), hipMemcpyHostToDevice, g_ctx.stream));
-    dv->rand_seed = (const int *)(base + off);
+  if (a->rand_seed) {
+    const size_t P = a->n_profile;
+    HIPCHK(hipMemcpyAsync(base + *off, a->rand_seed, P * sizeof(int), hipMemcpyHostToDevice, s));
+    dv->rand_seed = (const int *)(base + *off);
+    *off += (P * 4 + 255) & ~(size_t)255;
   }
   return 0;
 }
 
-int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag *diag, int precision) {
-  if (int rc = ensure_init()) return rc;
-  if (!spectrum_name || !a || !diag) return fail(SOCGPU_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(g_ctx.mu);
+static int stage_inputs(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB) {
+  if (g_ctx.in_stage.ensure(stage_bytes(a, NB) + 256)) return fail(SOCGPU_ENOMEM, "input staging allocation failed");
+  size_t off = 0;
+  return stage_inputs_at(a, dv, NB, g_ctx.in_stage.as<char>(), &off, g_ctx.stream);
+}
+
+// Checks one call's arguments and resolves its options (socrates_set_control
+// defaults on the two-stream path); fills the request's shapes.
+static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, int precision,
+                           SpecEntry **entry, CallDims *dout) {
+  if (!spectrum_name || !a) return fail(SOCGPU_EINVAL, "null argument");
   auto it = g_ctx.spectra.find(spectrum_name);
   if (it == g_ctx.spectra.end()) return fail(SOCGPU_ENOTFOUND, "Spectrum name not found: %s", spectrum_name);
   SpecEntry &e = *it->second;
@@ -1252,16 +1457,121 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   }
   const int items = S.n_g + (d.mcica ? S.n_sub_items[d.mode] : 0);
   d.nwpc = (items + kWave - 1) / kWave;
+  *entry = &e;
+  *dout = d;
+  return 0;
+}
+
+// Device workspace of one request, in doubles (8-byte units), and its work
+// item counts per space.
+static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
+  const size_t P = d.P, L = d.L, NB = d.NB;
+  size_t n = P * L * SG_NQ + P * (L + 1) + P * SG_NS + P * NB * L * SG_NBQ + P * NB * SG_NBQ +
+             P * (S.n_minor > 0 ? S.n_minor : 0) * L * 2 + P * d.nwpc * 3 * (L + 1) + 2;
+  if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * 4;
+  return n;
+}
+
+static void request_work(const CallDims &d, const SgSpec &S, long long w[kNWork]) {
+  const long long P = d.P, L = d.L;
+  w[W_LEV] = P * (L + 1);
+  w[W_LAY] = P * L;
+  w[W_CG] = d.mcica ? P : 0;
+  w[W_MINOR] = P * S.n_minor;
+  w[W_PREP] = P * d.NB * L;
+  w[W_SOLVE] = P * d.nwpc;
+}
+
+// Lays out the workspace of request r at *cursor (doubles, 32-byte aligned)
+// and fills its descriptor's pointers.
+static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cursor) {
+  const CallDims &d = q.d;
+  const size_t P = d.P, L = d.L, NB = d.NB;
+  auto take = [&](size_t n) {
+    double *p = base + *cursor;
+    *cursor += (n + 3) & ~(size_t)3;
+    return p;
+  };
+  q.colrec = take(P * L * SG_NQ);
+  q.tlev = take(P * (L + 1));
+  q.colscal = take(P * SG_NS);
+  q.bandrec = take(P * NB * L * SG_NBQ);
+  q.bandscal = take(P * NB * SG_NBQ);
+  q.minor = S.n_minor > 0 ? take(P * S.n_minor * L * 2) : nullptr;
+  q.partial = take(P * d.nwpc * 3 * (L + 1));
+  q.krange = (int *)take(2);
+  q.csub = d.mcica ? take(P * L * SG_MAXSUBCOL) : nullptr;
+  q.cldopt = d.mcica ? take(P * NB * L * 4) : nullptr;
+}
+
+static size_t padded_ws_doubles(const CallDims &d, const SgSpec &S) {
+  return request_ws_doubles(d, S) + 10 * 4;  // per-array alignment padding
+}
+
+static void fill_desc(ReqDesc &q, const CallDims &d, const SpecEntry &e, const SocgpuRunesArgs &dv,
+                      double *up, double *dn, double *dir, double *hr) {
+  std::memset(&q, 0, sizeof q);
+  q.d = d;
+  q.spec = e.dev;
+  q.xcw = e.xcw;
+  q.n1 = e.host.n1;
+  q.n2 = e.host.n2;
+  q.l_rayleigh = dv.l_rayleigh;
+  q.dp_cf = dv.cloud_vertical_decorr;
+  q.dp_cw = dv.cloud_vertical_decorr * 0.5;
+  q.a = dv;
+  q.up = up; q.dn = dn; q.dir = dir; q.hr = hr;
+}
+
+// Launches the kernel sequence of a packed batch on stream s. hdr/rq are
+// device copies of the batch header and descriptors; tot is the host copy of
+// the work totals (grid sizes); solve-kernel time between ev_a and ev_b.
+static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long tot[kNWork], int R,
+                        int precision, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b) {
+  const int T = 256;
+  auto blocks = [](long long n, int per) {
+    long long b = (n + per - 1) / per;
+    return (int)(b < 1 ? 1 : (b > 1048576 ? 1048576 : b));
+  };
+  b_init_kernel<<<(R + 63) / 64, 64, 0, s>>>(hdr, rq);
+  b_pack_kernel<<<blocks(tot[W_LEV], T), T, 0, s>>>(hdr, rq);
+  if (tot[W_CG] > 0) {
+    b_cldrange_kernel<<<blocks(tot[W_LAY], T), T, 0, s>>>(hdr, rq);
+    b_cloudgen_kernel<<<blocks(tot[W_CG], 1), 64, 0, s>>>(hdr, rq);
+  }
+  if (tot[W_MINOR] > 0)
+    b_minor_kernel<<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
+  b_prep_kernel<<<blocks(tot[W_PREP], T), T, 0, s>>>(hdr, rq);
+  if (ev_a) HIPCHK(hipEventRecord(ev_a, s));
+  const SocgpuTile &tile = g_tile[precision == SOCGPU_MIXED ? 1 : 0];
+  const int wpb = tile.waves_per_block;
+  auto *kern = precision == SOCGPU_MIXED
+                   ? (tile.recompute ? (tile.waves_per_simd == 3 ? b_solve_kernel<true, true, 3> : b_solve_kernel<true, true, 4>)
+                                     : (tile.waves_per_simd == 3 ? b_solve_kernel<true, false, 3> : b_solve_kernel<true, false, 4>))
+                   : (tile.recompute ? (tile.waves_per_simd == 3 ? b_solve_kernel<false, true, 3> : b_solve_kernel<false, true, 4>)
+                                     : (tile.waves_per_simd == 3 ? b_solve_kernel<false, false, 3> : b_solve_kernel<false, false, 4>));
+  kern<<<blocks(tot[W_SOLVE], wpb), wpb * kWave, 0, s>>>(hdr, rq);
+  if (ev_b) HIPCHK(hipEventRecord(ev_b, s));
+  b_finish_kernel<<<blocks(tot[W_LEV], T), T, 0, s>>>(hdr, rq);
+  b_heating_kernel<<<blocks(tot[W_LAY], T), T, 0, s>>>(hdr, rq);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag *diag, int precision) {
+  if (int rc = ensure_init()) return rc;
+  if (!spectrum_name || !a || !diag) return fail(SOCGPU_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  SpecEntry *e = nullptr;
+  CallDims d;
+  if (int rc = resolve_request(spectrum_name, a, precision, &e, &d)) return rc;
+  const SgSpec &S = e->host;
+  const int P = d.P, L = d.L, NB = d.NB;
 
   SocgpuRunesArgs dv;
   if (int rc = stage_inputs(a, &dv, NB)) return rc;
-  if (g_ctx.colrec.ensure((size_t)P * L * SG_NQ * 8) || g_ctx.tlev.ensure((size_t)P * (L + 1) * 8) ||
-      g_ctx.colscal.ensure((size_t)P * SG_NS * 8) || g_ctx.bandrec.ensure((size_t)P * NB * L * SG_NBQ * 8) ||
-      g_ctx.bandscal.ensure((size_t)P * NB * SG_NBQ * 8) ||
-      g_ctx.partial.ensure((size_t)P * d.nwpc * 3 * (L + 1) * 8) || g_ctx.krange.ensure(64) ||
-      g_ctx.minor.ensure((size_t)P * (S.n_minor > 0 ? S.n_minor : 1) * L * 2 * 8) ||
-      (d.mcica && g_ctx.csub.ensure((size_t)P * L * SG_MAXSUBCOL * 8)) ||
-      (d.mcica && g_ctx.cldopt.ensure((size_t)P * NB * L * 4 * 8)) ||
+  if (g_ctx.ws.ensure(padded_ws_doubles(d, S) * 8) ||
+      g_ctx.desc.ensure(sizeof(BatchHdr) + sizeof(ReqDesc)) ||
       g_ctx.out_stage.ensure((size_t)P * (L + 1) * 3 * 8 + (size_t)P * L * 8))
     return fail(SOCGPU_ENOMEM, "device buffer allocation failed");
   double *oup = g_ctx.out_stage.as<double>();
@@ -1272,46 +1582,24 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
     if (diag->flux_direct) odir = diag->flux_direct;
     if (diag->heating_rate) ohr = diag->heating_rate;
   }
+  struct {
+    BatchHdr h;
+    ReqDesc q;
+  } host;
+  std::memset(&host, 0, sizeof host);
+  fill_desc(host.q, d, *e, dv, oup, odn, odir, diag->heating_rate ? ohr : nullptr);
+  size_t cursor = 0;
+  place_request(host.q, S, g_ctx.ws.as<double>(), &cursor);
+  long long w[kNWork];
+  request_work(d, S, w);
+  host.h.R = 1;
+  for (int k = 0; k < kNWork; k++) { host.q.s[k] = 0; host.h.tot[k] = w[k]; }
   hipStream_t s = g_ctx.stream;
   HIPCHK(hipEventRecord(g_ctx.ev0, s));
-  const int T = 256;
-  int nlev = P * (L + 1);
-  pack_kernel<<<(nlev + T - 1) / T, T, 0, s>>>(d, e.dev, dv, g_ctx.colrec.as<double>(), g_ctx.tlev.as<double>(),
-                                              g_ctx.colscal.as<double>(), g_ctx.bandscal.as<double>());
-  if (d.mcica) {
-    fill_int_kernel<<<1, 64, 0, s>>>(g_ctx.krange.as<int>(), 1, L + 1);
-    fill_int_kernel<<<1, 64, 0, s>>>(g_ctx.krange.as<int>() + 1, 1, 0);
-    cldrange_kernel<<<(P * L + T - 1) / T, T, 0, s>>>(d, g_ctx.colrec.as<double>(), g_ctx.krange.as<int>());
-    cloudgen_kernel<<<P, 64, 0, s>>>(d, g_ctx.colrec.as<double>(), dv.rand_seed, a->cloud_vertical_decorr,
-                                     a->cloud_vertical_decorr * 0.5, g_ctx.krange.as<int>(), e.xcw, S.n1, S.n2,
-                                     g_ctx.csub.as<double>(), g_ctx.colscal.as<double>());
-  }
-  if (S.n_minor > 0)
-    minor_kernel<<<(P * S.n_minor + kMinorWaves - 1) / kMinorWaves, 64 * kMinorWaves, 0, s>>>(d, e.dev, g_ctx.colrec.as<double>(), g_ctx.tlev.as<double>(),
-                                                          g_ctx.colscal.as<double>(), g_ctx.bandscal.as<double>(),
-                                                          g_ctx.minor.as<double>());
-  prep_kernel<<<(P * NB * L + T - 1) / T, T, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(),
-                                                     g_ctx.tlev.as<double>(), g_ctx.colscal.as<double>(),
-                                                     g_ctx.minor.as<double>(), g_ctx.bandrec.as<double>(),
-                                                     g_ctx.bandscal.as<double>(), g_ctx.cldopt.as<double>());
-  HIPCHK(hipEventRecord(g_ctx.ev1, s));
-  int waves = P * d.nwpc;
-  const SocgpuTile &tile = g_tile[precision == SOCGPU_MIXED ? 1 : 0];
-  const int wpb = tile.waves_per_block;
-  const int grid = (waves + wpb - 1) / wpb;
-  auto *kern = precision == SOCGPU_MIXED
-                   ? (tile.recompute ? solve_kernel<true, true> : solve_kernel<true, false>)
-                   : (tile.recompute ? solve_kernel<false, true> : solve_kernel<false, false>);
-  kern<<<grid, wpb * kWave, 0, s>>>(d, e.dev, a->l_rayleigh, g_ctx.colrec.as<double>(), g_ctx.colscal.as<double>(),
-                                    g_ctx.bandrec.as<double>(), g_ctx.bandscal.as<double>(),
-                                    d.mcica ? g_ctx.csub.as<double>() : nullptr,
-                                    d.mcica ? g_ctx.cldopt.as<double>() : nullptr, g_ctx.partial.as<double>());
-  HIPCHK(hipEventRecord(g_ctx.ev2, s));
-  finish_kernel<<<(nlev + T - 1) / T, T, 0, s>>>(d, g_ctx.partial.as<double>(), g_ctx.bandrec.as<double>(),
-                                                g_ctx.bandscal.as<double>(), oup, odn, odir);
-  if (diag->heating_rate)
-    heating_kernel<<<(P * L + T - 1) / T, T, 0, s>>>(P, L, oup, odn, dv.layer_heat_capacity, ohr);
-  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g_ctx.desc.p, &host, sizeof host, hipMemcpyHostToDevice, s));
+  const BatchHdr *dh = (const BatchHdr *)g_ctx.desc.p;
+  const ReqDesc *dq = (const ReqDesc *)((char *)g_ctx.desc.p + offsetof(decltype(host), q));
+  if (int rc = launch_batch(dh, (ReqDesc *)dq, host.h.tot, 1, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
   if (!diag->l_outputs_on_device) {
     size_t nl = (size_t)P * (L + 1) * 8;
@@ -1329,11 +1617,597 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   return 0;
 }
 
+}  // extern "C"
+
+namespace sg {
+
+// One packed batch on the host: descriptors (ordered by decreasing layer
+// count, so that the long columns start first), totals and the outputs to
+// copy back to host memory after the launch.
+struct Plan {
+  std::vector<ReqDesc> rq;
+  BatchHdr h;
+  size_t ws_doubles = 0, in_bytes = 0, out_doubles = 0;
+  struct Back { double *dst; const double *src; size_t n; };
+  std::vector<Back> back;
+  long long need[kNWork];
+};
+
+struct Resolved {
+  SpecEntry *e;
+  CallDims d;
+};
+
+// Sizes of a batch (workspace, staging, outputs) without placing it.
+static int plan_sizes(int n, const SocgpuRequest *req, int precision, std::vector<Resolved> &res, Plan &pl) {
+  res.resize(n);
+  pl.ws_doubles = pl.in_bytes = pl.out_doubles = 0;
+  for (int k = 0; k < kNWork; k++) pl.need[k] = 0;
+  for (int i = 0; i < n; i++) {
+    if (!req[i].args || !req[i].diag) return fail(SOCGPU_EINVAL, "request %d: null args or diag", i);
+    if (int rc = resolve_request(req[i].spectrum_name, req[i].args, precision, &res[i].e, &res[i].d))
+      return rc;
+    const CallDims &d = res[i].d;
+    pl.ws_doubles += padded_ws_doubles(d, res[i].e->host);
+    pl.in_bytes += stage_bytes(req[i].args, d.NB);
+    if (!req[i].diag->l_outputs_on_device) pl.out_doubles += (size_t)d.P * (3 * (d.L + 1) + d.L) + 16;
+    long long w[kNWork];
+    request_work(d, res[i].e->host, w);
+    for (int k = 0; k < kNWork; k++) pl.need[k] += w[k];
+  }
+  return 0;
+}
+
+// Places a sized batch into the given buffers and stages host inputs on s.
+static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolved> &res, Plan &pl,
+                      double *ws, char *in_stage, double *out_stage, hipStream_t s) {
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return res[x].d.L > res[y].d.L; });
+  pl.rq.assign(n, ReqDesc());
+  pl.back.clear();
+  std::memset(&pl.h, 0, sizeof pl.h);
+  pl.h.R = n;
+  size_t ws_cur = 0, in_off = 0, out_cur = 0;
+  long long start[kNWork] = {0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < n; j++) {
+    const int i = order[j];
+    const CallDims &d = res[i].d;
+    const SpecEntry &e = *res[i].e;
+    SocgpuRunesArgs dv;
+    if (int rc = stage_inputs_at(req[i].args, &dv, d.NB, in_stage, &in_off, s)) return rc;
+    const SocgpuDiag *g = req[i].diag;
+    const size_t nl = (size_t)d.P * (d.L + 1), nh = (size_t)d.P * d.L;
+    double *up, *dn, *dir, *hr;
+    if (g->l_outputs_on_device) {
+      up = g->flux_up; dn = g->flux_down; dir = g->flux_direct; hr = g->heating_rate;
+      if (!up || !dn) return fail(SOCGPU_EINVAL, "request %d: device outputs need flux_up and flux_down", i);
+    } else {
+      double *b = out_stage + out_cur;
+      up = b; dn = b + nl; dir = b + 2 * nl; hr = b + 3 * nl;
+      out_cur += 3 * nl + nh + 16;
+      if (g->flux_up) pl.back.push_back({g->flux_up, up, nl});
+      if (g->flux_down) pl.back.push_back({g->flux_down, dn, nl});
+      if (g->flux_direct) pl.back.push_back({g->flux_direct, dir, nl});
+      if (g->heating_rate) pl.back.push_back({g->heating_rate, hr, nh});
+      if (!g->flux_direct) dir = nullptr;
+      if (!g->heating_rate) hr = nullptr;
+    }
+    ReqDesc &q = pl.rq[j];
+    fill_desc(q, d, e, dv, up, dn, dir, hr);
+    place_request(q, e.host, ws, &ws_cur);
+    long long w[kNWork];
+    request_work(d, e.host, w);
+    for (int k = 0; k < kNWork; k++) {
+      q.s[k] = start[k];
+      start[k] += w[k];
+    }
+  }
+  for (int k = 0; k < kNWork; k++) pl.h.tot[k] = start[k];
+  return 0;
+}
+
+static size_t desc_bytes(int n) { return sizeof(BatchHdr) + (size_t)n * sizeof(ReqDesc); }
+
+static void pack_desc(const Plan &pl, char *dst) {
+  std::memcpy(dst, &pl.h, sizeof(BatchHdr));
+  std::memcpy(dst + sizeof(BatchHdr), pl.rq.data(), pl.rq.size() * sizeof(ReqDesc));
+}
+
+static int copy_back(const Plan &pl, hipStream_t s) {
+  for (const auto &b : pl.back)
+    HIPCHK(hipMemcpyAsync(b.dst, b.src, b.n * 8, hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+}  // namespace sg
+
+// Persistent batch: capacity-sized buffers and a captured HIP graph of the
+// kernel sequence, replayed for every batch that fits (kernels read the
+// batch's shape from the device descriptors and stride over it).
+struct SocgpuBatch {
+  int precision = SOCGPU_FP64;
+  SocgpuTile tile{};
+  size_t cap_ws = 0, cap_in = 0, cap_out = 0, cap_desc = 0;
+  long long cap[sg::kNWork] = {0, 0, 0, 0, 0, 0};
+  sg::DevBuf ws, in_stage, out_stage, desc;
+  char *pinned = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double last_ms = 0.0;
+  int replays = 0, captures = 0;
+};
+
+extern "C" {
+
+int socgpu_runes_batch(int n_request, const SocgpuRequest *requests, int precision) {
+  if (int rc = ensure_init()) return rc;
+  if (n_request <= 0 || !requests) return fail(SOCGPU_EINVAL, "empty batch");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  std::vector<Resolved> res;
+  Plan pl;
+  if (int rc = plan_sizes(n_request, requests, precision, res, pl)) return rc;
+  if (g_ctx.ws.ensure(pl.ws_doubles * 8) || g_ctx.in_stage.ensure(pl.in_bytes + 256) ||
+      g_ctx.out_stage.ensure(pl.out_doubles * 8 + 64) || g_ctx.desc.ensure(desc_bytes(n_request)))
+    return fail(SOCGPU_ENOMEM, "device buffer allocation failed");
+  hipStream_t s = g_ctx.stream;
+  HIPCHK(hipEventRecord(g_ctx.ev0, s));
+  if (int rc = plan_place(n_request, requests, res, pl, g_ctx.ws.as<double>(), g_ctx.in_stage.as<char>(),
+                          g_ctx.out_stage.as<double>(), s))
+    return rc;
+  std::vector<char> blob(desc_bytes(n_request));
+  pack_desc(pl, blob.data());
+  HIPCHK(hipMemcpyAsync(g_ctx.desc.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+  const BatchHdr *dh = (const BatchHdr *)g_ctx.desc.p;
+  ReqDesc *dq = (ReqDesc *)((char *)g_ctx.desc.p + sizeof(BatchHdr));
+  if (int rc = launch_batch(dh, dq, pl.h.tot, n_request, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
+  HIPCHK(hipEventRecord(g_ctx.ev3, s));
+  if (int rc = copy_back(pl, s)) return rc;
+  HIPCHK(hipStreamSynchronize(s));
+  float t_all = 0.f, t_solve = 0.f;
+  HIPCHK(hipEventElapsedTime(&t_all, g_ctx.ev0, g_ctx.ev3));
+  HIPCHK(hipEventElapsedTime(&t_solve, g_ctx.ev1, g_ctx.ev2));
+  g_ctx.last_total_ms = t_all;
+  g_ctx.last_solve_ms = t_solve;
+  return 0;
+}
+
+int socgpu_batch_create(SocgpuBatch **batch, int precision) {
+  if (int rc = ensure_init()) return rc;
+  if (!batch) return fail(SOCGPU_EINVAL, "null batch");
+  if (precision != SOCGPU_FP64 && precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision %d", precision);
+  auto *b = new SocgpuBatch();
+  b->precision = precision;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    delete b;
+    return fail(SOCGPU_EDEVICE, "stream or event creation failed");
+  }
+  *batch = b;
+  return 0;
+}
+
+static void batch_drop_graph(SocgpuBatch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *requests) {
+  if (int rc = ensure_init()) return rc;
+  if (!b || n_request <= 0 || !requests) return fail(SOCGPU_EINVAL, "null batch or empty request list");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  std::vector<Resolved> res;
+  Plan pl;
+  if (int rc = plan_sizes(n_request, requests, b->precision, res, pl)) return rc;
+  const SocgpuTile &tile = g_tile[b->precision == SOCGPU_MIXED ? 1 : 0];
+  bool recapture = !b->exec || std::memcmp(&tile, &b->tile, sizeof tile) != 0;
+  // grow capacity by a quarter over the need; a larger batch re-captures
+  auto grow = [](size_t need, size_t &cap) {
+    if (need <= cap) return false;
+    cap = need + need / 4 + 64;
+    return true;
+  };
+  bool g_ws = grow(pl.ws_doubles, b->cap_ws), g_in = grow(pl.in_bytes + 256, b->cap_in);
+  bool g_out = grow(pl.out_doubles + 64, b->cap_out), g_desc = grow(desc_bytes(n_request), b->cap_desc);
+  for (int k = 0; k < kNWork; k++)
+    if (pl.need[k] > b->cap[k]) { b->cap[k] = pl.need[k] + pl.need[k] / 4 + 64; recapture = true; }
+  if (g_ws || g_desc) recapture = true;
+  if ((g_ws && b->ws.ensure(b->cap_ws * 8)) || (g_in && b->in_stage.ensure(b->cap_in)) ||
+      (g_out && b->out_stage.ensure(b->cap_out * 8)) || (g_desc && b->desc.ensure(b->cap_desc)))
+    return fail(SOCGPU_ENOMEM, "batch buffer allocation failed");
+  if (g_desc || !b->pinned) {
+    if (b->pinned) (void)hipHostFree(b->pinned);
+    b->pinned = nullptr;
+    HIPCHK(hipHostMalloc((void **)&b->pinned, b->cap_desc, hipHostMallocDefault));
+  }
+  hipStream_t s = b->stream;
+  // the previous replay has finished (every run ends synchronised), so the
+  // pinned descriptor block can be rewritten
+  if (int rc = plan_place(n_request, requests, res, pl, b->ws.as<double>(), b->in_stage.as<char>(),
+                          b->out_stage.as<double>(), s))
+    return rc;
+  pack_desc(pl, b->pinned);
+  HIPCHK(hipMemcpyAsync(b->desc.p, b->pinned, desc_bytes(n_request), hipMemcpyHostToDevice, s));
+  const BatchHdr *dh = (const BatchHdr *)b->desc.p;
+  ReqDesc *dq = (ReqDesc *)((char *)b->desc.p + sizeof(BatchHdr));
+  if (recapture) {
+    batch_drop_graph(b);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = launch_batch(dh, dq, b->cap, b->cap_desc / sizeof(ReqDesc) + 1, b->precision, s, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc) return rc;
+    if (ec != hipSuccess) return fail(SOCGPU_EDEVICE, "graph capture failed: %s", hipGetErrorString(ec));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->tile = tile;
+    b->captures++;
+  }
+  HIPCHK(hipEventRecord(b->ev0, s));
+  HIPCHK(hipGraphLaunch(b->exec, s));
+  HIPCHK(hipEventRecord(b->ev1, s));
+  if (int rc = copy_back(pl, s)) return rc;
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  b->replays++;
+  return 0;
+}
+
+int socgpu_batch_stats(const SocgpuBatch *b, double *last_ms, int *captures, int *replays) {
+  if (!b) return fail(SOCGPU_EINVAL, "null batch");
+  if (last_ms) *last_ms = b->last_ms;
+  if (captures) *captures = b->captures;
+  if (replays) *replays = b->replays;
+  return 0;
+}
+
+int socgpu_batch_destroy(SocgpuBatch *b) {
+  if (!b) return 0;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (DevBuf *x : {&b->ws, &b->in_stage, &b->out_stage, &b->desc})
+    if (x->p) (void)hipFree(x->p);
+  if (b->pinned) (void)hipHostFree(b->pinned);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- serving
+// A request loop shaped like a host model's radiation calls: callers submit
+// calls from any thread; a server thread admits them while their device
+// footprint fits the HBM budget, packs everything admitted into one batch
+// per free slot (a stream with its own buffers) and launches it, so calls
+// that arrive while others are on the GPU join the next batch and run
+// concurrently with the batch in flight (continuous batching).
+namespace sg {
+
+struct ServeItem {
+  long long ticket;
+  SocgpuRequest req;
+  SocgpuRunesArgs args;  // copies: the caller's structs may go out of scope
+  SocgpuDiag diag;
+  std::string name;
+  size_t footprint = 0;
+  std::chrono::steady_clock::time_point t_submit;
+  int status = 1;  // 1 pending, 0 done, < 0 error
+};
+
+struct ServeSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  DevBuf ws, in_stage, out_stage, desc;
+  char *pinned = nullptr;
+  size_t pinned_bytes = 0;
+  std::vector<std::shared_ptr<ServeItem>> items;
+  Plan plan;
+  bool busy = false;
+};
+
+}  // namespace sg
+
+struct SocgpuServer {
+  SocgpuServeConfig cfg{};
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv, cv_done;
+  bool stop = false;
+  long long next_ticket = 1;
+  std::deque<std::shared_ptr<sg::ServeItem>> pending, admitted;
+  std::map<long long, std::shared_ptr<sg::ServeItem>> all;
+  size_t used = 0;
+  std::vector<sg::ServeSlot> slots;
+  std::vector<double> lat_ms;
+  long long completed = 0, batches = 0, batched_requests = 0, solves = 0;
+  std::chrono::steady_clock::time_point t0;
+  bool started = false;
+};
+
+namespace sg {
+
+// Device bytes a call holds while admitted: workspace, staged inputs,
+// staged outputs and its descriptor.
+static size_t footprint_bytes(const SocgpuRequest &r, const CallDims &d, const SgSpec &S) {
+  return padded_ws_doubles(d, S) * 8 + stage_bytes(r.args, d.NB) + ((size_t)d.P * (4 * (d.L + 1)) + 16) * 8 +
+         sizeof(ReqDesc);
+}
+
+static void serve_finish_slot(SocgpuServer *sv, ServeSlot &sl, int status) {
+  const auto now = std::chrono::steady_clock::now();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  for (auto &it : sl.items) {
+    it->status = status;
+    sv->used -= it->footprint;
+    if (status == 0) {
+      sv->lat_ms.push_back(std::chrono::duration<double, std::milli>(now - it->t_submit).count());
+      sv->completed++;
+    }
+  }
+  sl.items.clear();
+  sl.busy = false;
+  sv->cv_done.notify_all();
+  sv->cv.notify_all();
+}
+
+// Plans and launches one batch on a slot (the caller holds g_ctx.mu).
+static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
+  const int n = (int)sl.items.size();
+  std::vector<SocgpuRequest> req(n);
+  for (int i = 0; i < n; i++) req[i] = sl.items[i]->req;
+  std::vector<Resolved> res;
+  if (int rc = plan_sizes(n, req.data(), sv->cfg.precision, res, sl.plan)) return rc;
+  if (sl.ws.ensure(sl.plan.ws_doubles * 8) || sl.in_stage.ensure(sl.plan.in_bytes + 256) ||
+      sl.out_stage.ensure(sl.plan.out_doubles * 8 + 64) || sl.desc.ensure(desc_bytes(n)))
+    return fail(SOCGPU_ENOMEM, "serving slot allocation failed");
+  if (sl.pinned_bytes < desc_bytes(n)) {
+    if (sl.pinned) (void)hipHostFree(sl.pinned);
+    sl.pinned = nullptr;
+    sl.pinned_bytes = desc_bytes(n) * 2;
+    HIPCHK(hipHostMalloc((void **)&sl.pinned, sl.pinned_bytes, hipHostMallocDefault));
+  }
+  hipStream_t s = sl.stream;
+  if (int rc = plan_place(n, req.data(), res, sl.plan, sl.ws.as<double>(), sl.in_stage.as<char>(),
+                          sl.out_stage.as<double>(), s))
+    return rc;
+  pack_desc(sl.plan, sl.pinned);
+  HIPCHK(hipMemcpyAsync(sl.desc.p, sl.pinned, desc_bytes(n), hipMemcpyHostToDevice, s));
+  const BatchHdr *dh = (const BatchHdr *)sl.desc.p;
+  ReqDesc *dq = (ReqDesc *)((char *)sl.desc.p + sizeof(BatchHdr));
+  if (int rc = launch_batch(dh, dq, sl.plan.h.tot, n, sv->cfg.precision, s, nullptr, nullptr)) return rc;
+  if (int rc = copy_back(sl.plan, s)) return rc;
+  HIPCHK(hipEventRecord(sl.done, s));
+  long long sol = 0;
+  for (int i = 0; i < n; i++) sol += (long long)res[i].d.P * res[i].e->host.n_g;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->batches++;
+  sv->batched_requests += n;
+  sv->solves += sol;
+  return 0;
+}
+
+static void serve_loop(SocgpuServer *sv) {
+  (void)hipSetDevice(g_ctx.device);
+  for (;;) {
+    // completions
+    for (auto &sl : sv->slots)
+      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) serve_finish_slot(sv, sl, 0);
+    ServeSlot *free_slot = nullptr;
+    std::vector<std::shared_ptr<ServeItem>> take;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      // admission by footprint, first come first served
+      while (!sv->pending.empty() && sv->used + sv->pending.front()->footprint <= sv->cfg.hbm_budget_bytes) {
+        auto it = sv->pending.front();
+        sv->pending.pop_front();
+        sv->used += it->footprint;
+        sv->admitted.push_back(it);
+      }
+      for (auto &sl : sv->slots)
+        if (!sl.busy) { free_slot = &sl; break; }
+      if (free_slot && !sv->admitted.empty()) {
+        while (!sv->admitted.empty() && (int)take.size() < sv->cfg.max_batch_requests) {
+          take.push_back(sv->admitted.front());
+          sv->admitted.pop_front();
+        }
+        free_slot->busy = true;
+      }
+      bool any_busy = false;
+      for (auto &sl : sv->slots) any_busy |= sl.busy;
+      if (sv->stop && sv->pending.empty() && sv->admitted.empty() && !any_busy && take.empty()) break;
+      if (take.empty()) {
+        if (any_busy) {
+          lk.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(50));
+        } else {
+          sv->cv.wait_for(lk, std::chrono::milliseconds(5));
+        }
+        continue;
+      }
+    }
+    free_slot->items = take;
+    int rc;
+    {
+      std::lock_guard<std::mutex> g(g_ctx.mu);
+      rc = serve_launch(sv, *free_slot);
+    }
+    if (rc) serve_finish_slot(sv, *free_slot, rc);
+  }
+}
+
+}  // namespace sg
+
+extern "C" {
+
+int socgpu_serve_start(SocgpuServer **server, const SocgpuServeConfig *cfg) {
+  if (int rc = ensure_init()) return rc;
+  if (!server || !cfg) return fail(SOCGPU_EINVAL, "null argument");
+  if (cfg->precision != SOCGPU_FP64 && cfg->precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision");
+  if (cfg->n_slots < 1 || cfg->n_slots > 8) return fail(SOCGPU_EINVAL, "n_slots must be 1..8");
+  if (cfg->max_batch_requests < 1) return fail(SOCGPU_EINVAL, "max_batch_requests must be positive");
+  auto *sv = new SocgpuServer();
+  sv->cfg = *cfg;
+  sv->slots.resize(cfg->n_slots);
+  for (auto &sl : sv->slots) {
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+      delete sv;
+      return fail(SOCGPU_EDEVICE, "stream or event creation failed");
+    }
+  }
+  sv->t0 = std::chrono::steady_clock::now();
+  sv->th = std::thread(serve_loop, sv);
+  *server = sv;
+  return 0;
+}
+
+int socgpu_serve_submit(SocgpuServer *sv, const SocgpuRequest *request, long long *ticket) {
+  if (!sv || !request || !request->args || !request->diag || !request->spectrum_name || !ticket)
+    return fail(SOCGPU_EINVAL, "null argument");
+  auto it = std::make_shared<ServeItem>();
+  it->args = *request->args;
+  it->diag = *request->diag;
+  it->name = request->spectrum_name;
+  {
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    SpecEntry *e = nullptr;
+    CallDims d;
+    if (int rc = resolve_request(it->name.c_str(), &it->args, sv->cfg.precision, &e, &d)) return rc;
+    SocgpuRequest r{it->name.c_str(), &it->args, &it->diag};
+    it->footprint = footprint_bytes(r, d, e->host);
+  }
+  it->req = SocgpuRequest{it->name.c_str(), &it->args, &it->diag};
+  if (it->footprint > sv->cfg.hbm_budget_bytes)
+    return fail(SOCGPU_ENOMEM, "request needs %zu bytes, above the serving budget of %zu", it->footprint,
+                sv->cfg.hbm_budget_bytes);
+  it->t_submit = std::chrono::steady_clock::now();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (sv->stop) return fail(SOCGPU_EINVAL, "server stopping");
+  it->ticket = sv->next_ticket++;
+  sv->all[it->ticket] = it;
+  sv->pending.push_back(it);
+  *ticket = it->ticket;
+  sv->cv.notify_all();
+  return 0;
+}
+
+int socgpu_serve_wait(SocgpuServer *sv, long long ticket, double timeout_s) {
+  if (!sv) return fail(SOCGPU_EINVAL, "null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto f = sv->all.find(ticket);
+  if (f == sv->all.end()) return fail(SOCGPU_EINVAL, "unknown ticket %lld", ticket);
+  auto it = f->second;
+  auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+  while (it->status == 1) {
+    if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && it->status == 1)
+      return fail(SOCGPU_EDEVICE, "ticket %lld not done after %.1f s", ticket, timeout_s);
+  }
+  sv->all.erase(ticket);
+  if (it->status != 0) return fail(it->status, "request %lld failed", ticket);
+  return 0;
+}
+
+int socgpu_serve_stats(SocgpuServer *sv, SocgpuServeStats *st) {
+  if (!sv || !st) return fail(SOCGPU_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  std::memset(st, 0, sizeof *st);
+  st->completed = sv->completed;
+  st->batches = sv->batches;
+  st->mean_batch = sv->batches ? (double)sv->batched_requests / sv->batches : 0.0;
+  st->solves = sv->solves;
+  st->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - sv->t0).count();
+  if (!sv->lat_ms.empty()) {
+    std::vector<double> v = sv->lat_ms;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)(p * (v.size() - 1) + 0.5))]; };
+    st->p50_ms = pct(0.50);
+    st->p99_ms = pct(0.99);
+    st->max_ms = v.back();
+  }
+  return 0;
+}
+
+int socgpu_serve_reset_stats(SocgpuServer *sv) {
+  if (!sv) return fail(SOCGPU_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->lat_ms.clear();
+  sv->completed = sv->batches = sv->batched_requests = sv->solves = 0;
+  sv->t0 = std::chrono::steady_clock::now();
+  return 0;
+}
+
+int socgpu_serve_stop(SocgpuServer *sv) {
+  if (!sv) return 0;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+    sv->cv.notify_all();
+  }
+  if (sv->th.joinable()) sv->th.join();
+  for (auto &sl : sv->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    for (DevBuf *x : {&sl.ws, &sl.in_stage, &sl.out_stage, &sl.desc})
+      if (x->p) (void)hipFree(x->p);
+    if (sl.pinned) (void)hipHostFree(sl.pinned);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  delete sv;
+  return 0;
+}
+
+int socgpu_autotune(int n_request, const SocgpuRequest *requests, int precision, int repeats,
+                    SocgpuTile *best) {
+  if (int rc = ensure_init()) return rc;
+  if (n_request <= 0 || !requests || !best) return fail(SOCGPU_EINVAL, "null or empty argument");
+  if (precision != SOCGPU_FP64 && precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision %d", precision);
+  const int slot = precision == SOCGPU_MIXED ? 1 : 0;
+  const SocgpuTile saved = g_tile[slot];
+  SocgpuTile win = saved;
+  double best_ms = 1e30;
+  if (repeats < 1) repeats = 1;
+  for (int rec = 0; rec <= 1; rec++)
+    for (int wpb : {1, 2, 4})
+      for (int occ : {3, 4}) {
+        {
+          std::lock_guard<std::mutex> lk(g_ctx.mu);
+          g_tile[slot] = SocgpuTile{rec, wpb, occ};
+        }
+        double t = 1e30;
+        for (int r = 0; r <= repeats; r++) {  // the first run warms up
+          if (int rc = socgpu_runes_batch(n_request, requests, precision)) {
+            std::lock_guard<std::mutex> lk(g_ctx.mu);
+            g_tile[slot] = saved;
+            return rc;
+          }
+          if (r > 0) t = std::min(t, g_ctx.last_solve_ms);
+        }
+        if (t < best_ms) { best_ms = t; win = SocgpuTile{rec, wpb, occ}; }
+      }
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  g_tile[slot] = win;
+  *best = win;
+  return 0;
+}
+
 int socgpu_set_tile(int precision, const SocgpuTile *tile) {
   if (precision != SOCGPU_FP64 && precision != SOCGPU_MIXED) return fail(SOCGPU_EINVAL, "unknown precision %d", precision);
   if (!tile) return fail(SOCGPU_EINVAL, "null tile");
   const int w = tile->waves_per_block;
   if (w != 1 && w != 2 && w != 4) return fail(SOCGPU_EINVAL, "waves_per_block must be 1, 2 or 4");
+  if (tile->waves_per_simd != 3 && tile->waves_per_simd != 4)
+    return fail(SOCGPU_EINVAL, "waves_per_simd must be 3 or 4");
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   g_tile[precision] = *tile;
   g_tile[precision].recompute = tile->recompute != 0;
@@ -1450,20 +2324,41 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
   CallDims d;
   std::memset(&d, 0, sizeof d);
   d.P = P; d.L = L; d.n_gen = n_subcol; d.n_fill = n_req < n_subcol ? n_req : n_subcol;
-  d.ipph = ipph; d.ioverlap = ioverlap;
-  DevBuf drec, dseed, dxcw, dcs, dscal, dk;
+  d.ipph = ipph; d.ioverlap = ioverlap; d.mcica = 1;
+  DevBuf drec, dseed, dxcw, dcs, dscal, dk, ddesc;
+  struct {
+    BatchHdr h;
+    ReqDesc q;
+  } host;
   if (drec.ensure(rec.size() * 8) || dseed.ensure((size_t)P * 4) || dxcw.ensure((size_t)n1 * n2 * 8) ||
-      dcs.ensure((size_t)P * L * SG_MAXSUBCOL * 8) || dscal.ensure((size_t)P * SG_NS * 8) || dk.ensure(8))
+      dcs.ensure((size_t)P * L * SG_MAXSUBCOL * 8) || dscal.ensure((size_t)P * SG_NS * 8) || dk.ensure(8) ||
+      ddesc.ensure(sizeof host))
     return fail(SOCGPU_ENOMEM, "allocation failed");
+  std::memset(&host, 0, sizeof host);
+  host.h.R = 1;
+  host.h.tot[W_LAY] = (long long)P * L;
+  host.h.tot[W_CG] = P;
+  host.q.d = d;
+  host.q.colrec = drec.as<double>();
+  host.q.a.rand_seed = dseed.as<int>();
+  host.q.dp_cf = rlc_cf[0];
+  host.q.dp_cw = rlc_cw[0];
+  host.q.krange = dk.as<int>();
+  host.q.xcw = dxcw.as<double>();
+  host.q.n1 = n1;
+  host.q.n2 = n2;
+  host.q.csub = dcs.as<double>();
+  host.q.colscal = dscal.as<double>();
   hipStream_t s = g_ctx.stream;
   HIPCHK(hipMemcpyAsync(drec.p, rec.data(), rec.size() * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dseed.p, rand_seed, (size_t)P * 4, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dxcw.p, xcw, (size_t)n1 * n2 * 8, hipMemcpyHostToDevice, s));
-  fill_int_kernel<<<1, 64, 0, s>>>(dk.as<int>(), 1, L + 1);
-  fill_int_kernel<<<1, 64, 0, s>>>(dk.as<int>() + 1, 1, 0);
-  cldrange_kernel<<<(P * L + 255) / 256, 256, 0, s>>>(d, drec.as<double>(), dk.as<int>());
-  cloudgen_kernel<<<P, 64, 0, s>>>(d, drec.as<double>(), dseed.as<int>(), rlc_cf[0], rlc_cw[0], dk.as<int>(),
-                                   dxcw.as<double>(), n1, n2, dcs.as<double>(), dscal.as<double>());
+  HIPCHK(hipMemcpyAsync(ddesc.p, &host, sizeof host, hipMemcpyHostToDevice, s));
+  const BatchHdr *dh = (const BatchHdr *)ddesc.p;
+  ReqDesc *dq = (ReqDesc *)((char *)ddesc.p + offsetof(decltype(host), q));
+  b_init_kernel<<<1, 64, 0, s>>>(dh, dq);
+  b_cldrange_kernel<<<(P * L + 255) / 256, 256, 0, s>>>(dh, dq);
+  b_cloudgen_kernel<<<P, 64, 0, s>>>(dh, dq);
   HIPCHK(hipGetLastError());
   std::vector<double> cs((size_t)P * L * SG_MAXSUBCOL), sc((size_t)P * SG_NS);
   HIPCHK(hipMemcpyAsync(cs.data(), dcs.p, cs.size() * 8, hipMemcpyDeviceToHost, s));
@@ -1475,7 +2370,7 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
       for (int j = 0; j < n_subcol; j++)
         c_sub[((size_t)j * L + i) * P + l] = cs[((size_t)l * L + i) * SG_MAXSUBCOL + j];
   }
-  for (DevBuf *b : {&drec, &dseed, &dxcw, &dcs, &dscal, &dk}) if (b->p) (void)hipFree(b->p);
+  for (DevBuf *b : {&drec, &dseed, &dxcw, &dcs, &dscal, &dk, &ddesc}) if (b->p) (void)hipFree(b->p);
   return 0;
 }
 
@@ -1488,9 +2383,7 @@ int socgpu_finalize(void) {
     if (kv.second->xcw) (void)hipFree(kv.second->xcw);
   }
   g_ctx.spectra.clear();
-  for (DevBuf *b : {&g_ctx.colrec, &g_ctx.tlev, &g_ctx.colscal, &g_ctx.bandrec, &g_ctx.bandscal, &g_ctx.csub,
-                    &g_ctx.partial, &g_ctx.krange, &g_ctx.minor, &g_ctx.cldopt, &g_ctx.in_stage,
-                    &g_ctx.out_stage}) {
+  for (DevBuf *b : {&g_ctx.ws, &g_ctx.desc, &g_ctx.in_stage, &g_ctx.out_stage}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->n = 0;

@@ -125,7 +125,7 @@ def _ptr(x, keep, dtype=np.float64):
     return a.ctypes.data_as(_I if dtype == np.int32 else _D)
 
 
-def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
+def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
           density=None, layer_heat_capacity=None, h2o=None, o3=None, co2_mix_ratio=0.0,
           p_level=None, t_level=None, t_ground=None, cos_zenith_angle=None,
           solar_irrad=None, albedo_diff=None, albedo_dir=None, l_grey_albedo=False,
@@ -134,9 +134,9 @@ def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
           liq_conv_dim=None, cloud_vertical_decorr=0.0, conv_vertical_decorr=0.0,
           cloud_horizontal_rsd=0.0, i_cloud_representation=0, i_overlap=0, i_inhom=0,
           i_mcica_sampling=0, i_st_water=5, i_cnv_water=5, n_subcol_gen=0,
-          rand_seed=None, precision="fp64", out=None):
-    """One radiation call. Returns dict(flux_up, flux_down, flux_direct,
-    heating_rate) in the input memory space (numpy or torch)."""
+          rand_seed=None, out=None):
+    """ctypes argument and output structures of one call (and the objects
+    that must stay alive while it runs)."""
     init()
     inputs = dict(p_layer=p_layer, t_layer=t_layer, mass=mass, density=density,
                   layer_heat_capacity=layer_heat_capacity, h2o=h2o, o3=o3, p_level=p_level,
@@ -184,6 +184,15 @@ def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
         for k in out:
             if not out[k].flags["C_CONTIGUOUS"]:
                 raise SocgpuError("output %s must be contiguous" % k)
+    return a, d, out, keep
+
+
+def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass, precision="fp64",
+          **kw):
+    """One radiation call (socgpu_runes). Returns dict(flux_up, flux_down,
+    flux_direct, heating_rate) in the input memory space (numpy or torch)."""
+    a, d, out, keep = _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
+                               **kw)
     _check(lib().socgpu_runes(spectrum_name.encode(), ctypes.byref(a), ctypes.byref(d),
                               _PREC[precision]))
     return out
@@ -192,33 +201,9 @@ def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
 def runes_case(case: Case, spectrum_name, precision="fp64", device_arrays=None, out=None):
     """runes() on a Case (socrates_amd.case_io). device_arrays: optional dict
     of torch tensors replacing the case's numpy arrays."""
-    src = dict(device_arrays) if device_arrays else {}
-
-    def g(name, attr=None):
-        return src.get(name, getattr(case, attr or name))
-
-    cloudy = bool(case.l_cloud)
-    return runes(
-        case.n_profile, case.n_layer, spectrum_name, case.i_source,
-        p_layer=g("p_layer"), t_layer=g("t_layer"), mass=g("mass"), density=g("density"),
-        layer_heat_capacity=g("layer_heat_capacity"), h2o=g("h2o"), o3=g("o3"),
-        co2_mix_ratio=case.co2_mix_ratio, p_level=g("p_level"), t_level=g("t_level"),
-        t_ground=g("t_ground"), cos_zenith_angle=g("cos_zenith_angle", "cos_zen"),
-        solar_irrad=g("solar_irrad"), albedo_diff=g("albedo_diff"), albedo_dir=g("albedo_dir"),
-        l_rayleigh=case.l_rayleigh,
-        cloud_frac=g("cloud_frac") if cloudy else None,
-        liq_mmr=g("liq_mmr") if cloudy else None, liq_dim=g("liq_dim") if cloudy else None,
-        conv_frac=g("conv_frac") if cloudy and case.i_solver_kind == 1 else None,
-        liq_conv_mmr=g("liq_conv_mmr") if cloudy and case.i_solver_kind == 1 else None,
-        liq_conv_dim=g("liq_conv_dim") if cloudy and case.i_solver_kind == 1 else None,
-        cloud_vertical_decorr=case.cloud_vertical_decorr,
-        conv_vertical_decorr=case.conv_vertical_decorr,
-        cloud_horizontal_rsd=case.cloud_horizontal_rsd,
-        i_cloud_representation=case.i_cloud_repr if cloudy else 0, i_overlap=case.i_overlap,
-        i_inhom=case.i_inhom, i_mcica_sampling=case.i_mcica_sampling,
-        i_st_water=case.i_st_water, i_cnv_water=case.i_st_water,
-        n_subcol_gen=case.n_subcol_gen, rand_seed=g("rand_seed") if cloudy else None,
-        precision=precision, out=out)
+    name, kw = case_call(case, spectrum_name, device_arrays, out)
+    P, L, src = kw.pop("n_profile"), kw.pop("n_layer"), kw.pop("i_source")
+    return runes(P, L, name, src, precision=precision, **kw)
 
 
 def register_default_spectra():
@@ -299,12 +284,13 @@ def two_stream_case(tsc, precision="fp64", out=None):
 
 class Tile(ctypes.Structure):
     """Launch shape of the column-solve kernel (SocgpuTile)."""
-    _fields_ = [("recompute", ctypes.c_int), ("waves_per_block", ctypes.c_int)]
+    _fields_ = [("recompute", ctypes.c_int), ("waves_per_block", ctypes.c_int),
+                ("waves_per_simd", ctypes.c_int)]
 
 
-def set_tile(precision, recompute, waves_per_block):
+def set_tile(precision, recompute, waves_per_block, waves_per_simd=4):
     init()
-    t = Tile(int(bool(recompute)), int(waves_per_block))
+    t = Tile(int(bool(recompute)), int(waves_per_block), int(waves_per_simd))
     _check(lib().socgpu_set_tile(_PREC[precision], ctypes.byref(t)))
 
 
@@ -312,4 +298,108 @@ def get_tile(precision):
     init()
     t = Tile()
     _check(lib().socgpu_get_tile(_PREC[precision], ctypes.byref(t)))
-    return dict(recompute=t.recompute, waves_per_block=t.waves_per_block)
+    return dict(recompute=t.recompute, waves_per_block=t.waves_per_block,
+                waves_per_simd=t.waves_per_simd)
+
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("spectrum_name", ctypes.c_char_p), ("args", ctypes.POINTER(RunesArgs)),
+                ("diag", ctypes.POINTER(Diag))]
+
+
+def _requests(calls):
+    """calls: list of (spectrum_name, kwargs of runes()) -> ctypes array, outputs, keep-alive."""
+    keep, outs = [], []
+    arr = (Request * len(calls))()
+    for k, (name, kw) in enumerate(calls):
+        kw = dict(kw)
+        P, L, src = kw.pop("n_profile"), kw.pop("n_layer"), kw.pop("i_source")
+        a, d, out, kp = _prepare(P, L, name, src, **kw)
+        nm = name.encode()
+        keep += [a, d, kp, nm]
+        arr[k].spectrum_name = nm
+        arr[k].args = ctypes.pointer(a)
+        arr[k].diag = ctypes.pointer(d)
+        outs.append(out)
+    return arr, outs, keep
+
+
+def runes_batch(calls, precision="fp64"):
+    """Several radiation calls in one packed launch (socgpu_runes_batch).
+    calls: list of (spectrum_name, dict of runes() keyword arguments
+    including n_profile, n_layer and i_source). Returns the list of output
+    dicts, in order."""
+    arr, outs, keep = _requests(calls)
+    _check(lib().socgpu_runes_batch(len(calls), arr, _PREC[precision]))
+    return outs
+
+
+def case_call(case, spectrum_name, device_arrays=None, out=None):
+    """(spectrum_name, kwargs) of runes() for a Case, as runes_case builds them."""
+    src = dict(device_arrays) if device_arrays else {}
+
+    def g(name, attr=None):
+        return src.get(name, getattr(case, attr or name))
+
+    cloudy = bool(case.l_cloud)
+    kw = dict(
+        n_profile=case.n_profile, n_layer=case.n_layer, i_source=case.i_source,
+        p_layer=g("p_layer"), t_layer=g("t_layer"), mass=g("mass"), density=g("density"),
+        layer_heat_capacity=g("layer_heat_capacity"), h2o=g("h2o"), o3=g("o3"),
+        co2_mix_ratio=case.co2_mix_ratio, p_level=g("p_level"), t_level=g("t_level"),
+        t_ground=g("t_ground"), cos_zenith_angle=g("cos_zenith_angle", "cos_zen"),
+        solar_irrad=g("solar_irrad"), albedo_diff=g("albedo_diff"), albedo_dir=g("albedo_dir"),
+        l_rayleigh=case.l_rayleigh,
+        cloud_frac=g("cloud_frac") if cloudy else None,
+        liq_mmr=g("liq_mmr") if cloudy else None, liq_dim=g("liq_dim") if cloudy else None,
+        cloud_vertical_decorr=case.cloud_vertical_decorr,
+        conv_vertical_decorr=case.conv_vertical_decorr,
+        cloud_horizontal_rsd=case.cloud_horizontal_rsd,
+        i_cloud_representation=case.i_cloud_repr if cloudy else 0, i_overlap=case.i_overlap,
+        i_inhom=case.i_inhom, i_mcica_sampling=case.i_mcica_sampling,
+        i_st_water=case.i_st_water, i_cnv_water=case.i_st_water,
+        n_subcol_gen=case.n_subcol_gen, rand_seed=g("rand_seed") if cloudy else None, out=out)
+    return spectrum_name, kw
+
+
+class Batch:
+    """Persistent graph-captured batch (socgpu_batch_*): replays one captured
+    launch sequence per radiation call while the batch fits its capacity."""
+
+    def __init__(self, precision="fp64"):
+        init()
+        self._h = ctypes.c_void_p()
+        _check(lib().socgpu_batch_create(ctypes.byref(self._h), _PREC[precision]))
+
+    def run(self, calls):
+        arr, outs, keep = _requests(calls)
+        _check(lib().socgpu_batch_run(self._h, len(calls), arr))
+        return outs
+
+    def stats(self):
+        ms, cap, rep = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().socgpu_batch_stats(self._h, ctypes.byref(ms), ctypes.byref(cap),
+                                        ctypes.byref(rep)))
+        return dict(last_ms=ms.value, captures=cap.value, replays=rep.value)
+
+    def close(self):
+        if self._h:
+            lib().socgpu_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(calls, precision="fp64", repeats=2):
+    """Pick the fastest solve tile for a representative batch of calls
+    (socgpu_autotune); it stays in effect for the precision form."""
+    arr, outs, keep = _requests(calls)
+    t = Tile()
+    _check(lib().socgpu_autotune(len(calls), arr, _PREC[precision], int(repeats), ctypes.byref(t)))
+    return dict(recompute=t.recompute, waves_per_block=t.waves_per_block,
+                waves_per_simd=t.waves_per_simd)

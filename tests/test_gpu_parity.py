@@ -159,10 +159,11 @@ def test_solve_tiles_bit_identical(precision):
         outs = []
         for rec in (0, 1):
             for wpb in (1, 2, 4):
-                R.set_tile(precision, rec, wpb)
-                outs.append(R.runes_case(case, "sw", precision=precision))
+                for occ in (3, 4):
+                    R.set_tile(precision, rec, wpb, occ)
+                    outs.append(R.runes_case(case, "sw", precision=precision))
     finally:
-        R.set_tile(precision, saved["recompute"], saved["waves_per_block"])
+        R.set_tile(precision, saved["recompute"], saved["waves_per_block"], saved["waves_per_simd"])
     for o in outs[1:]:
         for k in o:
             assert np.array_equal(o[k], outs[0][k]), k
