@@ -616,6 +616,259 @@ static void pade_2(const double *cp, double mmr, double re, double *k_ext_tot,
   *fwd = *phf * asym;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Triple-column cloud overlap: clear, stratiform and convective regions     */
+/* (ip_cloud_conv_strat with maximum-random overlap, ip_cloud_triple, and    */
+/* ip_solver_triple_hogan).                                                  */
+
+/* overlap_coupled.F90:150-453 for one profile with ip_cloud_triple (the
+ * correlation factor of every region is 1). Layers are 1-based as in the
+ * reference; w_cloud, frac_st, frac_cv are indexed by layer-1. ov holds the
+ * 18 coefficients of interfaces i = nct-1 .. L at ov[(i-(nct-1))*18 + idx-1]
+ * (idx 1-9: v, 10-18: u, as the reference numbers them). */
+void soc_oracle_overlap_triple(int L, int nct, const double *w_cloud, const double *frac_st,
+                               const double *frac_cv, double *ov);
+static void overlap_triple_profile(int L, int nct, const double *w_cloud, const double *frac_st,
+                                   const double *frac_cv, double *ov) {
+  const double tol_cloud = 1.0e+02 * DBL_EPSILON;
+  double area_upper[4], area_lower[4], ao[4][4], aru[4], arl[4];
+  area_upper[1] = 1.0; area_upper[2] = 0.0; area_upper[3] = 0.0;
+  for (int i = nct - 1; i <= L; i++) {
+    double *co = ov + (size_t)(i - (nct - 1)) * 18 - 1; /* co[idx] for idx 1..18 */
+    for (int k = 1; k <= 3; k++) {
+      if (i < L) {
+        if (k == 1) area_lower[k] = 1.0 - w_cloud[i];           /* w_free(i+1) */
+        else area_lower[k] = w_cloud[i] * (k == 2 ? frac_st[i] : frac_cv[i]);
+      } else {
+        area_lower[k] = k == 1 ? 1.0 : 0.0;
+      }
+      ao[k][k] = 1.0 * fmin(area_lower[k], area_upper[k]);
+    }
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j < k; j++) { ao[k][j] = 0.0; ao[j][k] = 0.0; }
+    double art = 1.0 - ao[1][1];
+    for (int k = 2; k <= 3; k++) art = art - ao[k][k];
+    for (int k = 1; k <= 3; k++) {
+      aru[k] = area_upper[k] - ao[k][k];
+      arl[k] = area_lower[k] - ao[k][k];
+    }
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++)
+        if (art > tol_cloud) ao[k][j] = ao[k][j] + aru[k] * arl[j] / art;
+    for (int k = 1; k <= 3; k++) {
+      co[3 * (k - 1) + k] = 1.0;
+      for (int j = 1; j < k; j++) { co[3 * (j - 1) + k] = 0.0; co[3 * (k - 1) + j] = 0.0; }
+    }
+    for (int k = 1; k <= 3; k++)
+      if (area_upper[k] > tol_cloud)
+        for (int j = 1; j <= 3; j++) co[3 * (j - 1) + k] = ao[k][j] / area_upper[k];
+    for (int k = 1; k <= 3; k++) {
+      co[3 * (3 + k - 1) + k] = 1.0;
+      for (int j = 1; j < k; j++) { co[3 * (3 + j - 1) + k] = 0.0; co[3 * (3 + k - 1) + j] = 0.0; }
+    }
+    for (int k = 1; k <= 3; k++)
+      if (area_lower[k] > tol_cloud)
+        for (int j = 1; j <= 3; j++) co[3 * (3 + j - 1) + k] = ao[j][k] / area_lower[k];
+    if (i < L)
+      for (int k = 1; k <= 3; k++) area_upper[k] = area_lower[k];
+  }
+}
+
+/* exported for the tests */
+void soc_oracle_overlap_triple(int L, int nct, const double *w_cloud, const double *frac_st,
+                               const double *frac_cv, double *ov) {
+  overlap_triple_profile(L, nct, w_cloud, frac_st, frac_cv, ov);
+}
+
+typedef struct {
+  int L, nct;
+  double *tr[3], *rf[3], *t0[3], *sc1[3], *sc2[3], *su[3], *sd[3];
+  double *binv[3], *gam[3], *h[3];
+  double *flux_direct; /* L+1 */
+  double *flux_total;  /* 2L+2, 1-based index k at [k-1] */
+} tri_t;
+
+/* one k-term of triple_column.F90:255-569 for one profile: two_coeff_region
+ * (two_coeff_region.F90:200-445), ir_source per region with the area
+ * weights (triple_column.F90), triple_solar_source (triple_solar_source.F90:
+ * 180-412), the surface sources and solver_triple_hogan
+ * (solver_triple_hogan.F90:207-411). Layer arrays are 0-based (layer i at
+ * [i-1]); region 0 clear, 1 stratiform, 2 convective. */
+static void solve_triple(int isolir, tri_t *t, const double *tau_c, const double *om_c,
+                         const double *tau_k[2], const double *om_k[2], const double *g_k[2],
+                         const double *frac_k[2], const double *w_cloud, const double *adjust,
+                         const double *dpl, const double *dpl2, double flux_inc_direct,
+                         double flux_inc_down, double alb_diff, double alb_dir, double dps,
+                         const double *ov, double zen_0) {
+  const int L = t->L, nct = t->nct;
+  const double tiny_frac = DBL_MIN;
+#define OV(i, idx) ov[(size_t)((i) - (nct - 1)) * 18 + (idx) - 1]
+  /* two_coeff_region: clear region, all layers */
+  for (int i = 1; i <= L; i++) {
+    coeff_t o;
+    two_coeff_one(isolir, tau_c[i - 1], om_c[i - 1], 0.0, zen_0, &o);
+    t->tr[0][i - 1] = o.trans; t->rf[0][i - 1] = o.reflect; t->t0[0][i - 1] = o.trans_0;
+    t->sc1[0][i - 1] = o.sc1; t->sc2[0][i - 1] = o.sc2;
+  }
+  for (int r = 1; r <= 2; r++)
+    for (int i = nct; i <= L; i++) {
+      t->tr[r][i - 1] = 0.0; t->rf[r][i - 1] = 0.0; t->t0[r][i - 1] = 0.0;
+      t->sc1[r][i - 1] = 0.0; t->sc2[r][i - 1] = 0.0;
+    }
+  for (int k = 0; k < 2; k++) {
+    const int r = k + 1; /* stratiform type -> stratiform region, convective likewise */
+    for (int i = nct; i <= L; i++) {
+      const double f = frac_k[k][i - 1];
+      if (!(f > tiny_frac)) continue;
+      coeff_t o;
+      two_coeff_one(isolir, tau_k[k][i - 1], om_k[k][i - 1], g_k[k][i - 1], zen_0, &o);
+      t->tr[r][i - 1] = t->tr[r][i - 1] + f * o.trans;
+      t->rf[r][i - 1] = t->rf[r][i - 1] + f * o.reflect;
+      t->sc1[r][i - 1] = t->sc1[r][i - 1] + f * o.sc1;
+      t->sc2[r][i - 1] = t->sc2[r][i - 1] + f * o.sc2;
+      if (isolir == IP_SOLAR) t->t0[r][i - 1] = t->t0[r][i - 1] + f * o.trans_0;
+    }
+  }
+  for (int r = 1; r <= 2; r++)
+    for (int i = nct; i <= L; i++) {
+      const double fr = frac_k[r - 1][i - 1]; /* frac_region = frac_cloud of the type */
+      if (!(fr > tiny_frac)) continue;
+      const double inv = 1.0 / fr;
+      t->tr[r][i - 1] = t->tr[r][i - 1] * inv;
+      t->rf[r][i - 1] = t->rf[r][i - 1] * inv;
+      if (isolir == IP_SOLAR) t->t0[r][i - 1] = t->t0[r][i - 1] * inv;
+      t->sc1[r][i - 1] = t->sc1[r][i - 1] * inv;
+      t->sc2[r][i - 1] = t->sc2[r][i - 1] * inv;
+    }
+  double sfg[3];
+  if (isolir == IP_INFRA_RED) {
+    for (int r = 0; r < 3; r++) {
+      const int n_top = r == 0 ? 1 : nct;
+      for (int i = n_top; i <= L; i++) {
+        t->su[r][i - 1] = t->sc1[r][i - 1] * dpl[i - 1] + t->sc2[r][i - 1] * dpl2[i - 1];
+        t->sd[r][i - 1] = -t->sc1[r][i - 1] * dpl[i - 1] + t->sc2[r][i - 1] * dpl2[i - 1];
+      }
+    }
+    for (int i = nct; i <= L; i++) {
+      const double wf = 1.0 - w_cloud[i - 1], wc = w_cloud[i - 1];
+      t->sd[0][i - 1] = wf * t->sd[0][i - 1];
+      t->su[0][i - 1] = wf * t->su[0][i - 1];
+      t->sd[1][i - 1] = wc * frac_k[0][i - 1] * t->sd[1][i - 1];
+      t->su[1][i - 1] = wc * frac_k[0][i - 1] * t->su[1][i - 1];
+      t->sd[2][i - 1] = wc * frac_k[1][i - 1] * t->sd[2][i - 1];
+      t->su[2][i - 1] = wc * frac_k[1][i - 1] * t->su[2][i - 1];
+    }
+    sfg[0] = OV(L, 10) * (1.0 - alb_diff) * dps;
+    sfg[1] = OV(L, 13) * (1.0 - alb_diff) * dps;
+    sfg[2] = OV(L, 16) * (1.0 - alb_diff) * dps;
+  } else {
+    /* triple_solar_source (l_scale_solar) */
+    double *fd = t->flux_direct;
+    fd[0] = flux_inc_direct;
+    for (int i = 1; i <= nct - 1; i++) {
+      fd[i] = fd[i - 1] * t->t0[0][i - 1] * adjust[i - 1];
+      t->su[0][i - 1] = t->sc1[0][i - 1] * fd[i - 1];
+      t->sd[0][i - 1] = (t->sc2[0][i - 1] - t->t0[0][i - 1]) * fd[i - 1] + fd[i];
+    }
+    double base[3] = {fd[nct - 1], 0.0, 0.0}, top[3];
+    for (int i = nct; i <= L; i++) {
+      for (int a = 0; a < 3; a++)
+        top[a] = OV(i - 1, 3 * a + 1) * base[0] + OV(i - 1, 3 * a + 2) * base[1] +
+                 OV(i - 1, 3 * a + 3) * base[2];
+      for (int r = 0; r < 3; r++) {
+        base[r] = top[r] * t->t0[r][i - 1] * adjust[i - 1];
+        t->su[r][i - 1] = t->sc1[r][i - 1] * top[r];
+        t->sd[r][i - 1] = (t->sc2[r][i - 1] - t->t0[r][i - 1]) * top[r] + base[r];
+      }
+      fd[i] = base[0] + base[1] + base[2];
+    }
+    for (int r = 0; r < 3; r++) sfg[r] = (alb_dir - alb_diff) * base[r];
+  }
+  /* solver_triple_hogan */
+  double a1 = 0, a2 = 0, a3 = 0, g1 = 0, g2 = 0, g3 = 0;
+  double **T = t->tr, **R = t->rf, **SU = t->su, **SD = t->sd;
+  for (int i = L + 1; i >= 1; i--) {
+    if (i < L + 1 && i >= nct) {
+      double th1 = a1 * OV(i, 1) + a2 * OV(i, 4) + a3 * OV(i, 7);
+      double th2 = a1 * OV(i, 2) + a2 * OV(i, 5) + a3 * OV(i, 8);
+      double th3 = a1 * OV(i, 3) + a2 * OV(i, 6) + a3 * OV(i, 9);
+      const int j = i - 1;
+      t->binv[0][j] = 1.0 / (1.0 - th1 * R[0][j]);
+      t->gam[0][j] = th1 * T[0][j];
+      t->h[0][j] = g1 + th1 * SD[0][j];
+      t->binv[1][j] = 1.0 / (1.0 - th2 * R[1][j]);
+      t->gam[1][j] = th2 * T[1][j];
+      t->h[1][j] = g2 + th2 * SD[1][j];
+      t->binv[2][j] = 1.0 / (1.0 - th3 * R[2][j]);
+      t->gam[2][j] = th3 * T[2][j];
+      t->h[2][j] = g3 + th3 * SD[2][j];
+      double l1 = SU[0][j] + t->h[0][j] * T[0][j] * t->binv[0][j];
+      double l2 = SU[1][j] + t->h[1][j] * T[1][j] * t->binv[1][j];
+      double l3 = SU[2][j] + t->h[2][j] * T[2][j] * t->binv[2][j];
+      a1 = R[0][j] + th1 * T[0][j] * T[0][j] * t->binv[0][j];
+      g1 = OV(i - 1, 10) * l1 + OV(i - 1, 11) * l2 + OV(i - 1, 12) * l3;
+      a2 = R[1][j] + th2 * T[1][j] * T[1][j] * t->binv[1][j];
+      g2 = OV(i - 1, 13) * l1 + OV(i - 1, 14) * l2 + OV(i - 1, 15) * l3;
+      a3 = R[2][j] + th3 * T[2][j] * T[2][j] * t->binv[2][j];
+      g3 = OV(i - 1, 16) * l1 + OV(i - 1, 17) * l2 + OV(i - 1, 18) * l3;
+    } else if (i < nct - 1) {
+      const int j = i - 1;
+      t->binv[0][j] = 1.0 / (1.0 - a1 * R[0][j]);
+      t->gam[0][j] = a1 * T[0][j];
+      t->h[0][j] = g1 + a1 * SD[0][j];
+      double l1 = T[0][j] * t->binv[0][j];
+      a1 = R[0][j] + l1 * t->gam[0][j];
+      g1 = SU[0][j] + l1 * t->h[0][j];
+    } else if (i == nct - 1) {
+      const int j = i - 1;
+      double th1;
+      if (nct < L) th1 = a1 * OV(i, 1) + a2 * OV(i, 4) + a3 * OV(i, 7);
+      else th1 = a1;
+      t->binv[0][j] = 1.0 / (1.0 - th1 * R[0][j]);
+      t->gam[0][j] = th1 * T[0][j];
+      t->h[0][j] = g1 + th1 * SD[0][j];
+      double lam = T[0][j] * t->binv[0][j];
+      a1 = R[0][j] + lam * t->gam[0][j];
+      g1 = SU[0][j] + lam * t->h[0][j];
+    } else if (i == L + 1) {
+      a1 = alb_diff; a2 = alb_diff; a3 = alb_diff;
+      g1 = sfg[0]; g2 = sfg[1]; g3 = sfg[2];
+    }
+  }
+  double *ft = t->flux_total - 1; /* ft[k] for k = 1 .. 2L+2 */
+  ft[2] = flux_inc_down;
+  if (nct > 1) ft[1] = a1 * ft[2] + g1;
+  else ft[1] = g1 + flux_inc_down * (OV(0, 1) * a1 + OV(0, 4) * a2 + OV(0, 7) * a3);
+  for (int i = 1; i <= nct - 1; i++) {
+    const int j = i - 1;
+    ft[2 * i + 1] = (t->gam[0][j] * ft[2 * i] + t->h[0][j]) * t->binv[0][j];
+    ft[2 * i + 2] = T[0][j] * ft[2 * i] + R[0][j] * ft[2 * i + 1] + SD[0][j];
+  }
+  double fd1 = 0, fd2 = 0, fd3 = 0, fu1, fu2, fu3;
+  for (int i = nct; i <= L; i++) {
+    const int j = i - 1;
+    if (i > nct) {
+      double d1 = fd1, d2 = fd2, d3 = fd3;
+      fd1 = OV(i - 1, 1) * d1 + OV(i - 1, 2) * d2 + OV(i - 1, 3) * d3;
+      fd2 = OV(i - 1, 4) * d1 + OV(i - 1, 5) * d2 + OV(i - 1, 6) * d3;
+      fd3 = OV(i - 1, 7) * d1 + OV(i - 1, 8) * d2 + OV(i - 1, 9) * d3;
+    } else {
+      fd1 = OV(i - 1, 1) * ft[2 * i];
+      fd2 = OV(i - 1, 4) * ft[2 * i];
+      fd3 = OV(i - 1, 7) * ft[2 * i];
+    }
+    fu1 = (t->gam[0][j] * fd1 + t->h[0][j]) * t->binv[0][j];
+    fu2 = (t->gam[1][j] * fd2 + t->h[1][j]) * t->binv[1][j];
+    fu3 = (t->gam[2][j] * fd3 + t->h[2][j]) * t->binv[2][j];
+    fd1 = T[0][j] * fd1 + R[0][j] * fu1 + SD[0][j];
+    fd2 = T[1][j] * fd2 + R[1][j] * fu2 + SD[1][j];
+    fd3 = T[2][j] * fd3 + R[2][j] * fu3 + SD[2][j];
+    ft[2 * i + 1] = fu1 + fu2 + fu3;
+    ft[2 * i + 2] = fd1 + fd2 + fd3;
+  }
+#undef OV
+}
+
 int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const soc_case *c,
                      double *flux_up, double *flux_down, double *flux_direct,
                      double *heating_rate) {
@@ -628,7 +881,11 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   if (rc) return rc;
   const int NBAND = sp.n_band;
   int mcica = c->l_cloud && c->i_inhom == 2;
-  if (c->l_cloud && !mcica) return -10; /* triple-column restated separately */
+  /* ip_cloud_conv_strat without McICA: triple-column overlap with maximum-
+   * random overlap (socrates_set_control.F90:338-356) */
+  int triple = c->l_cloud && !mcica && c->i_cloud_repr == 3;
+  if (c->l_cloud && !mcica && !triple) return -10;
+  if (triple && c->i_overlap != 0) return -10;
   if (mcica) {
     rc = read_mcica(mcica_file, &mc);
     if (rc) return rc;
@@ -701,6 +958,67 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     free(rsd); free(dp); free(dpc);
   }
 
+  /* --- set_cld / set_cld_dim for the triple-column representation ----- */
+  double *tw = NULL, *tfs = NULL, *tfc = NULL, *tms = NULL, *tmc = NULL, *trs = NULL, *trc = NULL;
+  double *ovl = NULL;
+  int nct = L;
+  if (triple) {
+    const double min_cloud_fraction = 0.001;
+    tw = (double *)calloc((size_t)7 * L * P, sizeof(double));
+    tfs = tw + (size_t)L * P; tfc = tfs + (size_t)L * P; tms = tfc + (size_t)L * P;
+    tmc = tms + (size_t)L * P; trs = tmc + (size_t)L * P; trc = trs + (size_t)L * P;
+    const int its = c->i_st_water, itc = c->i_st_water;  /* the case carries one droplet type for both */
+    for (int i = 0; i < L; i++)
+      for (int l = 0; l < P; l++) {
+        /* type fractions and condensate (socrates_set_cld.F90:567-581, 284-330) */
+        double fs = fmax(LAY(c->cloud_frac, i, l), 0.0), fcv = fmax(LAY(c->conv_frac, i, l), 0.0);
+        double ms = fmax(LAY(c->liq_mmr, i, l), 0.0), mcv = fmax(LAY(c->liq_conv_mmr, i, l), 0.0);
+        /* in-cloud mixing ratios (:817-826) */
+        LAY(tms, i, l) = fs > eps ? ms / fs : 0.0;
+        LAY(tmc, i, l) = fcv > eps ? mcv / fcv : 0.0;
+        /* w_cloud and normalised type fractions (:828-852) */
+        double wc = fs + fcv;
+        if (wc > min_cloud_fraction) { fs = fs / wc; fcv = fcv / wc; }
+        else { wc = 0.0; fs = 0.0; fcv = 0.0; }
+        if (wc > 1.0 + min_cloud_fraction) return -11;
+        else if (wc > 1.0) wc = 1.0;
+        LAY(tw, i, l) = wc; LAY(tfs, i, l) = fs; LAY(tfc, i, l) = fcv;
+        /* radius clamps (socrates_set_cld_dim.F90:214-220) */
+        LAY(trs, i, l) = fmin(fmax(fmax(LAY(c->liq_dim, i, l), 0.0), sp.drop_min[its]), sp.drop_max[its]);
+        LAY(trc, i, l) = fmin(fmax(fmax(LAY(c->liq_conv_dim, i, l), 0.0), sp.drop_min[itc]), sp.drop_max[itc]);
+      }
+    /* cloud top over the whole call (set_cloud_geometry.F90:52-72) */
+    nct = 1;
+    for (;;) {
+      int any = 0;
+      for (int l = 0; l < P; l++) any |= LAY(tw, nct - 1, l) > 0.0;
+      if (any || nct >= L) break;
+      nct++;
+    }
+    ovl = (double *)malloc(sizeof(double) * (size_t)(L - nct + 2) * 18);
+  }
+  tri_t tri;
+  double *tbuf = NULL;
+  double *kcs[2][4], *tau_k[2], *om_k[2], *g_k[2], *frk[2], *wcol = NULL;
+  if (triple) {
+    tbuf = (double *)malloc(sizeof(double) * (size_t)(60 * (L + 2) + 2 * L + 4));
+    double *qq = tbuf;
+    tri.L = L; tri.nct = nct;
+    for (int r = 0; r < 3; r++) {
+      tri.tr[r] = qq; qq += L; tri.rf[r] = qq; qq += L; tri.t0[r] = qq; qq += L;
+      tri.sc1[r] = qq; qq += L; tri.sc2[r] = qq; qq += L; tri.su[r] = qq; qq += L;
+      tri.sd[r] = qq; qq += L; tri.binv[r] = qq; qq += L; tri.gam[r] = qq; qq += L;
+      tri.h[r] = qq; qq += L;
+    }
+    for (int k = 0; k < 2; k++) {
+      for (int m = 0; m < 4; m++) { kcs[k][m] = qq; qq += L; }
+      tau_k[k] = qq; qq += L; om_k[k] = qq; qq += L; g_k[k] = qq; qq += L; frk[k] = qq; qq += L;
+    }
+    wcol = qq; qq += L;
+    tri.flux_direct = qq; qq += L + 1;
+    tri.flux_total = qq; qq += 2 * L + 2;
+  }
+
   /* --- per-column radiative transfer ------------------------------------ */
   work_t w, wc;
   double *buf = (double *)malloc(sizeof(double) * (size_t)(30 * (L + 2) + 16 * (2 * L + 2)));
@@ -727,6 +1045,13 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   const double temp_max = log(1.0 / eps);
 
   for (int l = 0; l < P; l++) {
+    if (triple) {
+      double *fs1 = frk[0], *fc1 = frk[1];
+      for (int i = 0; i < L; i++) {
+        wcol[i] = LAY(tw, i, l); fs1[i] = LAY(tfs, i, l); fc1[i] = LAY(tfc, i, l);
+      }
+      overlap_triple_profile(L, nct, wcol, fs1, fc1, ovl);
+    }
     double zen_0 = 1.0, sol = 0.0;
     if (isolir == IP_SOLAR) {
       /* set_bound (socrates_set_bound.F90:611-618) */
@@ -858,6 +1183,42 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         }
       }
 
+      /* triple column: per cloud type, the grey totals with the droplet
+       * optics (grey_opt_prop.F90:975-1000, 1185-1230), the equivalent
+       * extinction (solve_band_k_eqv_scl.F90:1038-1045), normalised moments
+       * (grey_opt_prop.F90:1474-1500) and the delta-rescaled phase function
+       * (rescale_phase_fnc.F90, radiance_calc.F90:1027-1040) */
+      if (triple) {
+        for (int k = 0; k < 2; k++) {
+          const int it = k == 0 ? c->i_st_water : c->i_st_water;
+          const double *mm = k == 0 ? tms : tmc, *rr = k == 0 ? trs : trc;
+          for (int i = 0; i < L; i++) {
+            const double ray = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
+            double kgt = 0.0 + ray, ksc = ray, phf = 0.0, fwd = 0.0;
+            if (i + 1 >= nct && LAY(tw, i, l) > 0.0) {
+              double ct, cs, cp, cf;
+              pade_2(sp.drop_parm[it][b], LAY(mm, i, l), LAY(rr, i, l), &ct, &cs, &cp, &cf);
+              kgt = kgt + ct;
+              ksc = ksc + cs;
+              phf = phf + cp;
+              fwd = fwd + cf;
+              kgt = kgt + 0.0;   /* the type's ice component, zero mixing ratio */
+              ksc = ksc + 0.0;
+              phf = phf + 0.0;
+              fwd = fwd + 0.0;
+            }
+            kgt = kgt + keqv[i] + kgrey[i];
+            if (ksc > tiny) {
+              double inv = 1.0 / ksc;
+              fwd = fwd * inv;
+              phf = phf * inv;
+            }
+            phf = (phf - fwd) / (1.0 - fwd);
+            kcs[k][0][i] = kgt; kcs[k][1][i] = ksc; kcs[k][2][i] = phf; kcs[k][3][i] = fwd;
+          }
+        }
+      }
+
       /* major k-term loop (solve_band_k_eqv_scl.F90:1051-1337) */
       const int gmaj = sp.index_absorb[b][0] - 1;
       const int nkmaj = sp.i_band_k[b][gmaj];
@@ -890,9 +1251,32 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           w.trans[i] = o.trans; w.reflect[i] = o.reflect; w.trans_0[i] = o.trans_0;
           w.sc1[i] = o.sc1; w.sc2[i] = o.sc2;
         }
-        solve_column(isolir, &w, adjust, dpl, dpl2, flux_inc_direct, flux_inc_down, alb_diff,
-                     alb_dir, dps);
-        double *F = w.flux_total, *FD = w.flux_direct;
+        double *F, *FD;
+        if (triple) {
+          /* single_scattering_all.F90 and rescale_tau_omega.F90 per type */
+          for (int k = 0; k < 2; k++)
+            for (int i = 0; i < L; i++) {
+              double k_total = kcs[k][0][i] + KABS(0, iex, i);
+              double tau = k_total * LAY(c->mass, i, l);
+              double om = kcs[k][1][i] / (k_total + tiny);
+              om = fmin(om, 1.0 - 3.2e+01 * eps);
+              const double fw = kcs[k][3][i];
+              tau = tau * (1.0 - om * fw);
+              om = om * (1.0 - fw) / (1.0 - om * fw);
+              tau_k[k][i] = tau; om_k[k][i] = om; g_k[k][i] = kcs[k][2][i];
+            }
+          const double *tk[2] = {tau_k[0], tau_k[1]}, *ok[2] = {om_k[0], om_k[1]};
+          const double *gk[2] = {g_k[0], g_k[1]}, *fk[2] = {frk[0], frk[1]};
+          solve_triple(isolir, &tri, tau_clr, om_clr, tk, ok, gk, fk, wcol, adjust, dpl, dpl2,
+                       flux_inc_direct, flux_inc_down, alb_diff, alb_dir, dps, ovl, zen_0);
+          F = tri.flux_total;
+          FD = tri.flux_direct;
+        } else {
+          solve_column(isolir, &w, adjust, dpl, dpl2, flux_inc_direct, flux_inc_down, alb_diff,
+                       alb_dir, dps);
+          F = w.flux_total;
+          FD = w.flux_direct;
+        }
         if (mcica) {
           /* mcica_sample.F90:453-930 */
           memcpy(clr_total, w.flux_total, sizeof(double) * (2 * L + 2));
@@ -996,6 +1380,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     }
   }
   free(kabs); free(buf);
+  if (triple) { free(tw); free(ovl); free(tbuf); }
   if (mcica) {
     free(w_cloud); free(cond_mmr); free(re); free(c_sub); free(n_subcol_cld);
     free(frac_cloudy); free(subcol_reorder); free(mc.xcw); mc.xcw = NULL;

@@ -104,6 +104,12 @@ void soc_oracle_two_stream(int n_profile, int n_layer, int isolir, int l_scale_s
                            const double *adjust_solar_ke, double *flux_direct,
                            double *flux_total);
 
+/* overlap_coupled (overlap_coupled.F90:150-453) for one profile with the
+ * triple-column maximum-random overlap: the 18 transfer coefficients of
+ * interfaces nct-1 .. L (1-based layers) at ov[(i-(nct-1))*18 + idx-1]. */
+void soc_oracle_overlap_triple(int L, int nct, const double *w_cloud, const double *frac_st,
+                               const double *frac_cv, double *ov);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,16 +36,19 @@ CASES = {
     "lw_mcica": (6, 60, "lw", True, 20261205),
     "sw_mcica": (6, 60, "sw", True, 20261206),
     "sw_mcica_l90": (4, 90, "sw", True, 20261207),
+    "lw_triple": (6, 60, "lw", "triple", 20261212),
+    "sw_triple": (6, 60, "sw", "triple", 20261213),
 }
 
 
 def run_case(name, P, L, src, cloudy, seed, tmp):
-    case = synth.make_case(P, L, src, 9 if src == "lw" else 6, seed, cloudy=cloudy)
+    overlap = "triple" if cloudy == "triple" else "mcica"
+    case = synth.make_case(P, L, src, 9 if src == "lw" else 6, seed, cloudy=bool(cloudy), overlap=overlap)
     fin, fout = os.path.join(tmp, name + ".bin"), os.path.join(tmp, name + ".out")
     case.write(fin)
     subprocess.run([os.path.join(REF, "soc_runes_case"), os.path.join(DATA, "sp_sw_synth"),
                     os.path.join(DATA, "sp_lw_synth"),
-                    os.path.join(DATA, "mcica_synth") if cloudy else "-", fin, fout],
+                    os.path.join(DATA, "mcica_synth") if cloudy is True else "-", fin, fout],
                    check=True, capture_output=True)
     out = read_output(fout, P, L)
     np.savez_compressed(os.path.join(HERE, "runes_%s.npz" % name), **case_to_arrays(case),

@@ -35,7 +35,8 @@ def spec_for(case):
 def test_oracle_matches_reference_golden(path):
     d = np.load(path)
     case = case_from_arrays(d)
-    out = oracle.runes(case, spec_for(case), os.path.join(DATA, "mcica_synth") if case.l_cloud else None)
+    mcica = case.l_cloud and case.i_inhom == 2
+    out = oracle.runes(case, spec_for(case), os.path.join(DATA, "mcica_synth") if mcica else None)
     for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
         r = d["ref_" + k]
         err = np.abs(out[k] - r)
@@ -85,9 +86,12 @@ def test_oracle_two_stream_matches_compiled_reference(isolir, lss, lq):
 
 
 @pytest.mark.skipif(not os.path.exists(REF), reason="reference not built here (oracle/_ref)")
-@pytest.mark.parametrize("src,cloudy", [("lw", False), ("sw", True), ("lw", True)])
+@pytest.mark.parametrize("src,cloudy", [("lw", False), ("sw", True), ("lw", True),
+                                        ("lw", "triple"), ("sw", "triple")])
 def test_oracle_matches_compiled_reference(src, cloudy):
-    case = synth.make_case(7, 45, src, 9 if src == "lw" else 6, seed=777, cloudy=cloudy)
+    case = synth.make_case(7, 45, src, 9 if src == "lw" else 6, seed=777, cloudy=bool(cloudy),
+                           overlap="triple" if cloudy == "triple" else "mcica")
+    cloudy = cloudy is True  # McICA data only for the McICA representation
     with tempfile.TemporaryDirectory() as tmp:
         fin, fout = os.path.join(tmp, "c.bin"), os.path.join(tmp, "c.out")
         case.write(fin)
@@ -159,3 +163,50 @@ def test_solar_exp_conditioning():
         res[src] = max((np.abs(a[k] - b[k]) / np.maximum(np.abs(a[k]), 1.0)).max()
                        for k in ("flux_up", "flux_down"))
     assert 1e-11 < res["sw"] < 2e-8, res
+
+
+@pytest.mark.parametrize("src", ["lw", "sw"])
+def test_triple_column_without_cloud_is_the_clear_solve(src):
+    """solver_triple_hogan with zero cloud in every layer reduces to the
+    homogeneous solver: the triple-column oracle on a cloud-free call equals
+    the clear-sky oracle to rounding (the elimination orders differ)."""
+    case = synth.make_case(5, 50, src, 9 if src == "lw" else 6, seed=31, cloudy=True, overlap="triple")
+    case.cloud_frac[:] = 0.0
+    case.conv_frac[:] = 0.0
+    tri = oracle.runes(case, spec_for(case), None)
+    clear = synth.make_case(5, 50, src, 9 if src == "lw" else 6, seed=31, cloudy=False)
+    ref = oracle.runes(clear, spec_for(clear), None)
+    for k in ("flux_up", "flux_down"):
+        scale = np.abs(ref[k]).max()
+        assert np.abs(tri[k] - ref[k]).max() <= 1e-12 * scale, k
+
+
+def test_triple_overlap_coefficients_conserve_area():
+    """Maximum-random overlap transfer coefficients (overlap_coupled.F90:
+    150-453) conserve area: at every interface the downward coefficient
+    v(j<-k) (index 3(j-1)+k) summed over the lower regions j is one for each
+    upper region k, and the upward u likewise over the upper regions; all
+    coefficients lie in [0, 1]. A transposed index breaks the sums."""
+    P, L = 3, 40
+    case = synth.make_case(P, L, "lw", 9, seed=77, cloudy=True, overlap="triple")
+    for l in range(P):
+        st, cv = case.cloud_frac[:, l], case.conv_frac[:, l]
+        tot = st + cv
+        ok = tot > 0.001
+        w = np.where(ok, np.minimum(tot, 1.0), 0.0)
+        fs = np.where(ok, st / np.where(ok, tot, 1.0), 0.0)
+        fc = np.where(ok, cv / np.where(ok, tot, 1.0), 0.0)
+        assert (w > 0).sum() >= 3
+        nct = 1
+        ov = np.zeros((L - nct + 2) * 18)
+        keep = []
+        oracle.lib().soc_oracle_overlap_triple(L, nct, oracle._ptr(np.ascontiguousarray(w), keep),
+                                               oracle._ptr(np.ascontiguousarray(fs), keep),
+                                               oracle._ptr(np.ascontiguousarray(fc), keep),
+                                               oracle._ptr(ov, keep))
+        ov = ov.reshape(-1, 18)
+        v = ov[:, :9].reshape(-1, 3, 3)   # v[i, j-1, k-1]
+        u = ov[:, 9:].reshape(-1, 3, 3)
+        assert np.allclose(v.sum(axis=1), 1.0, atol=1e-12)
+        assert np.allclose(u.sum(axis=1), 1.0, atol=1e-12)
+        assert (ov >= -1e-15).all() and (ov <= 1 + 1e-12).all()
