@@ -10,12 +10,21 @@ sampling, exponential-random overlap; synthetic columns and spectral files
 Metric: column.g-point flux solves per second at 90 levels (one solve = one
 (column, major k-term) flux profile including its sampled sub-columns).
 
-Also reported: the roofline of the solve kernel (FP64/FP32 VALU rate from
-the algorithmic operation count, HIP-event kernel time on the library
-stream) and a CPU baseline: the reference Fortran (compiled from its own
-sources into oracle/_ref) on a bounded sample, on this host's cores.
+The step runs as one packed batch (socgpu_batch_run: the SW and LW calls
+in one launch sequence, replayed from a captured HIP graph) with the solve
+tile picked by the autotuner (socgpu_autotune). The default form is the
+mixed FP32/FP64 one; the FP64 form is timed beside it (fp64_value,
+mixed_over_fp64).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+Also reported: the roofline of the solve kernel (FP64/FP32 VALU rate from
+the algorithmic operation count over the solve kernel's HIP-event time on
+the library stream), a short serving run (Poisson arrivals, p50/p99
+submit-to-done latency) and a CPU baseline: the reference Fortran (compiled
+from its own sources into oracle/_ref) on a bounded sample, on this host's
+cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
+                      [--no-autotune] [--serve-seconds S] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -40,6 +49,10 @@ OPS_PER_LAYER = {"sw": 94.0, "lw": 79.0}
 BYTES_PER_SOLVE = {"sw": 235.0, "lw": 188.0}
 PEAK = {"fp64": 78.6, "mixed": 157.3}      # TFLOP/s vector
 HBM_PEAK = 8000.0                          # GB/s
+# HBM bytes per step moved by the solve kernel (both calls), from rocprofv3
+# FETCH_SIZE (x2, the factor a plain copy of known size reads) and
+# WRITE_SIZE (x1) runs; profiles/c2_4096x90_traffic.md
+TRAFFIC = {}
 
 
 def make_inputs(torch, dev):
@@ -56,12 +69,6 @@ def make_inputs(torch, dev):
         arrs["rand_seed"] = torch.from_numpy(np.ascontiguousarray(c.rand_seed)).to(dev)
         cases[src] = (c, arrs)
     return cases
-
-
-def subcolumns_per_gpoint(R, src, mode=1):
-    """Mean number of sampled sub-columns per g-point (subcol_k)."""
-    import ctypes  # noqa: F401
-    return 1.0  # refined below from the McICA data when available
 
 
 def _raise_stack_limit():
@@ -117,13 +124,60 @@ def cpu_baseline(budget_s=15.0):
                       % (cores, reps, ncol)}
 
 
+def serving_line(seconds, rate=200.0, precision="mixed", seed=20261208):
+    """A short serving run (socrates_amd.serve): Poisson arrivals of calls
+    with 1..4096 columns (log-uniform), 40..137 layers, LW or SW, all-sky."""
+    import queue
+    import threading
+    from socrates_amd import runes as R
+    from socrates_amd import synth
+    from socrates_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(16):
+        src = "lw" if k % 2 == 0 else "sw"
+        n = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
+        pool.append((src, synth.make_case(max(n, 1), int(rng.integers(40, 138)), src, 9 if src == "lw" else 6,
+                                          seed + k, cloudy=True, first_column_id=100000 * k)))
+    srv = Server(precision=precision, hbm_budget_gb=64.0, n_slots=2)
+    srv.wait(srv.submit(*R.case_call(pool[0][1], pool[0][0])))
+    srv.reset_stats()
+    n = int(rate * seconds)
+    gaps = rng.exponential(1.0 / rate, n)
+    picks = rng.integers(0, len(pool), n)
+    q = queue.Queue()
+    th = threading.Thread(target=lambda: [srv.wait(t) for t in iter(q.get, None)], daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    solves = 0.0
+    for i in range(n):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        src, c = pool[picks[i]]
+        q.put(srv.submit(*R.case_call(c, src)))
+        solves += c.n_profile * G[src] * c.n_layer / 90.0
+    q.put(None)
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"rate_req_s": rate, "requests": n, "precision": precision, "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"],
+            "solves_per_s_l90": solves / wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the path shards by columns with no collective, "
@@ -142,37 +196,45 @@ def main():
                          flux_down=torch.empty((L1, P), dtype=torch.float64, device=dev),
                          flux_direct=torch.empty((L1, P), dtype=torch.float64, device=dev),
                          heating_rate=torch.empty((L, P), dtype=torch.float64, device=dev))
+    # one step = one SW and one LW radiation call, packed into one batch
+    calls = [R.case_call(cases[src][0], src, device_arrays=cases[src][1], out=outs[src])
+             for src in ("sw", "lw")]
 
-    def step(prec):
-        t = {}
-        for src in ("sw", "lw"):
-            c, arrs = cases[src]
-            R.runes_case(c, src, precision=prec, device_arrays=arrs, out=outs[src])
-            t[src] = R.last_timing()
-        return t
-
+    precs = [a.precision] + (["fp64"] if a.precision != "fp64" else [])
+    tiles = {}
     results = {}
-    for prec in ([a.precision] + (["fp64"] if a.precision != "fp64" else [])):
+    for prec in precs:
+        if not a.no_autotune:
+            tiles[prec] = R.autotune(calls, prec)
+        else:
+            tiles[prec] = R.get_tile(prec)
+        # solve-kernel time (HIP events on the library stream) from the
+        # stream form of the same batch
+        ks = []
+        for _ in range(3):
+            R.runes_batch(calls, precision=prec)
+            ks.append(R.last_timing()[1])
+        solve_ms = sorted(ks)[1]
+        b = R.Batch(prec)
         for _ in range(a.warmup):
-            step(prec)
+            b.run(calls)
         torch.cuda.synchronize()
-        solve_ms = {"sw": 0.0, "lw": 0.0}
         t0 = time.perf_counter()
         for _ in range(a.steps):
-            t = step(prec)
-            for s in t:
-                solve_ms[s] += t[s][1]
+            b.run(calls)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        results[prec] = dict(dt=dt, solve_ms={s: v / a.steps for s, v in solve_ms.items()})
+        st = b.stats()
+        b.close()
+        results[prec] = dict(dt=dt, solve_ms=solve_ms, graph=st)
     main_r = results[a.precision]
     ms_per_step = main_r["dt"] / a.steps * 1e3
     solves_per_step = P * (G["sw"] + G["lw"])
     value = solves_per_step / (main_r["dt"] / a.steps)
-    # roofline of the solve kernel (SW and LW launches together)
+    # roofline of the solve kernel (SW and LW calls of one step together)
     flops = sum(P * G[s] * L * OPS_PER_LAYER[s] for s in G)
     bytes_alg = sum(P * G[s] * BYTES_PER_SOLVE[s] for s in G)
-    kt = sum(main_r["solve_ms"].values()) * 1e-3
+    kt = main_r["solve_ms"] * 1e-3
     achieved = flops / kt / 1e12
     hbm = bytes_alg / kt / 1e9
     line = {
@@ -182,16 +244,22 @@ def main():
         "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic columns and spectral files (seeded)",
         "config": {"workload": "4096 columns x 90 layers, all-sky SW (6 bands, 40 g) + LW "
-                               "(9 bands, 50 g), McICA 64 sub-columns, exp-random overlap",
-                   "precision": a.precision},
+                               "(9 bands, 50 g), McICA 64 sub-columns, exp-random overlap; "
+                               "one step = the SW and LW calls as one graph-replayed batch",
+                   "precision": a.precision, "tile": tiles[a.precision]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[a.precision],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[a.precision], "traffic": None,
-                     "kernel": "solve_kernel", "kernel_ms_per_step": kt * 1e3,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[a.precision],
+                     "traffic": TRAFFIC.get(a.precision),
+                     "kernel": "b_solve_kernel", "kernel_ms_per_step": kt * 1e3,
                      "hbm_achieved_GBps": hbm, "hbm_frac": hbm / HBM_PEAK},
     }
-    if "fp64" in results and a.precision != "fp64":
+    if a.precision != "fp64":
         line["fp64_value"] = solves_per_step / (results["fp64"]["dt"] / a.steps)
         line["mixed_over_fp64"] = value / line["fp64_value"]
+        line["fp64_solve_kernel_ms_per_step"] = results["fp64"]["solve_ms"]
+        line["fp64_tile"] = tiles["fp64"]
+    if a.serve_seconds > 0:
+        line["serving"] = serving_line(a.serve_seconds, precision=a.precision)
     if not a.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline()
