@@ -42,7 +42,11 @@ extern "C" {
 #define SOCGPU_SOURCE_THERMAL 2      /* ip_infra_red / ip_source_thermal */
 #define SOCGPU_CLOUD_OFF 0           /* ip_cloud_off */
 #define SOCGPU_CLOUD_HOMOGEN 1       /* ip_cloud_homogen */
-#define SOCGPU_CLOUD_CONV_STRAT 3    /* ip_cloud_conv_strat */
+#define SOCGPU_CLOUD_CONV_STRAT 3    /* ip_cloud_conv_strat: stratiform and
+                                        convective cloud as triple-column
+                                        regions (ip_cloud_triple with
+                                        ip_solver_triple_hogan; maximum-random
+                                        overlap only) */
 #define SOCGPU_OVERLAP_MAX_RAND 0    /* ip_max_rand */
 #define SOCGPU_OVERLAP_RANDOM 1      /* ip_rand */
 #define SOCGPU_OVERLAP_EXP_RAND 3    /* ip_exponential_rand */
@@ -234,14 +238,15 @@ int socgpu_serve_reset_stats(SocgpuServer *server);
 int socgpu_serve_stop(SocgpuServer *server);
 
 /* Launch shape of the column-solve kernel, per precision form (no reference
- * counterpart; the autotuner in socrates_amd/tune.py picks it):
+ * counterpart; socgpu_autotune picks it):
  *   recompute        0: keep each layer's elimination state in lane-private
  *                       memory between the two sweeps (64 B per lane and
  *                       layer of memory traffic); 1: keep a checkpoint every
  *                       4 layers and recompute each segment in registers on
  *                       the way up (twice the arithmetic, a sixth of the
  *                       traffic). Results are bit-identical either way.
- *   waves_per_block  1, 2 or 4 column chunks (64 items each) per block.
+ *   waves_per_block  1, 2 or 4 waves (64 items each: k-terms and McICA
+ *                    samples of consecutive columns) per block.
  *   waves_per_simd   3 or 4: the register budget the kernel is compiled for
  *                    (168 or 128 registers per lane). */
 typedef struct SocgpuTile {

@@ -880,10 +880,10 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   int rc = read_spectrum(spectral_file, &sp);
   if (rc) return rc;
   const int NBAND = sp.n_band;
-  int mcica = c->l_cloud && c->i_inhom == 2;
-  /* ip_cloud_conv_strat without McICA: triple-column overlap with maximum-
-   * random overlap (socrates_set_control.F90:338-356) */
-  int triple = c->l_cloud && !mcica && c->i_cloud_repr == 3;
+  /* ip_cloud_conv_strat: triple-column overlap with maximum-random overlap;
+   * McICA is switched off for it (socrates_set_control.F90:338-356) */
+  int triple = c->l_cloud && c->i_cloud_repr == 3;
+  int mcica = c->l_cloud && !triple && c->i_inhom == 2;
   if (c->l_cloud && !mcica && !triple) return -10;
   if (triple && c->i_overlap != 0) return -10;
   if (mcica) {

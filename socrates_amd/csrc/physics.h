@@ -127,9 +127,13 @@ SG_HD int lcg_step(int x) {
 
 }  // namespace sg
 
+// FP32 transcendentals of the mixed form: single hardware instructions on
+// the device (v_exp_f32 after the scaling, v_rcp_f32 then a multiply, about
+// 1.5 ulp, v_sqrt_f32); the host build used by the CPU tests has the
+// correctly rounded library forms.
 #ifdef __HIP_DEVICE_COMPILE__
 #define SG_EXPF(x) __expf(x)
-#define SG_FDIV(a, b) __fdividef((a), (b))
+#define SG_FDIV(a, b) ((a) * __builtin_amdgcn_rcpf(b))
 #define SG_SQRTF(x) __builtin_amdgcn_sqrtf(x)
 #else
 #define SG_EXPF(x) expf(x)
@@ -267,33 +271,6 @@ namespace sg {
 SG_HD float k_term_f32(const SgTerm &t, float lnp, float lnt, float mmr) {
   float s = SG_EXPF((float)t.a * (lnp - (float)t.lnpref) + (float)t.b * (lnt - (float)t.lntref));
   return (float)t.k * fmaxf(0.0f, s * mmr);
-}
-
-// FP32 Pade-2 droplet optics with the absorption returned directly
-// (kext * X with X the single-scattering co-albedo fit), so that 1 - omega
-// stays accurate for weakly absorbing droplets.
-SG_HD void pade2_f32(const double *cpd, float mmr, float re, float &kext, float &kabs,
-                     float &phf, float &fwd) {
-  float cp[16];
-  for (int i = 0; i < 16; i++) cp[i] = (float)cpd[i];
-  kext = mmr * (cp[0] + re * (cp[1] + re * cp[2])) / (1.0f + re * (cp[3] + re * (cp[4] + re * cp[5])));
-  float X = (cp[6] + re * (cp[7] + re * cp[8])) / (1.0f + re * (cp[9] + re * cp[10]));
-  kabs = kext * X;
-  float kscat = kext * (1.0f - X);
-  float g = (cp[11] + re * (cp[12] + re * cp[13])) / (1.0f + re * (cp[14] + re * cp[15]));
-  phf = kscat * g;
-  fwd = phf * g;
-}
-
-SG_HD void slingo_f32(const double *cpd, float mmr, float re, float &kext, float &kabs,
-                      float &phf, float &fwd) {
-  kext = mmr * ((float)cpd[0] + (float)cpd[1] / re);
-  float X = (float)cpd[2] + (float)cpd[3] * re;
-  kabs = kext * X;
-  float kscat = kext * (1.0f - X);
-  float g = (float)cpd[4] + (float)cpd[5] * re;
-  phf = kscat * g;
-  fwd = kscat * g * g;
 }
 
 }  // namespace sg

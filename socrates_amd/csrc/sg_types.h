@@ -85,7 +85,12 @@ struct SgSpec {
 #define SG_Q_P 11      // layer pressure
 #define SG_Q_T 12      // layer temperature
 #define SG_Q_RSD 13    // condensate relative standard deviation (McICA)
-#define SG_NQ 14
+// triple-column representation: slots WCLD, LWC, RE and FRAC1 carry the
+// total cloud fraction and the stratiform type; these the convective one
+#define SG_Q_LWC2 14   // in-cloud convective liquid mass mixing ratio
+#define SG_Q_RE2 15    // convective effective radius clamped to the fit range
+#define SG_Q_FRAC2 16  // frac_cloud of the convective type
+#define SG_NQ 17
 
 // per-column scalars, [col][SG_NS]
 #define SG_S_SEC0 0    // zen_0 (secant of the solar zenith angle, or 1)

@@ -132,9 +132,13 @@ __device__ __forceinline__ void reduce8(double (&v)[8], int lane) {
 struct CallDims {
   int P, L, NB, isolir;
   int mcica, mode;   // McICA on, sampling table (0 single, 1 optimal)
-  int drop;          // droplet type index (0-based)
-  int nwpc;          // waves per column in the solve kernel
+  int drop;          // droplet type index (0-based; the stratiform type)
+  int nwpc;          // waves per column of the triple-column solve
+  int n_items;       // items per column of the packed solve (k-terms, then sub-column samples)
+  int n_waves;       // waves of the packed solve
   int n_gen, n_fill, ipph, ioverlap;
+  int triple, drop2; // triple-column representation; convective droplet type
+  int raw_range;     // cloud range on the fractions as given (kernel-level cloud_gen)
 };
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
@@ -142,15 +146,11 @@ struct CallDims {
 // threshold 0.001, type-fraction normalisation), set_cld_dim
 // (socrates_set_cld_dim.F90:214-220: radius clamp), cond_rsd of
 // set_cld_mcica (socrates_set_cld_mcica.F90:168-180).
-__device__ __forceinline__ void pack_body(const CallDims &d, const SgSpec *__restrict__ spec,
-                                          const SocgpuRunesArgs &a, double *__restrict__ colrec,
-                                          double *__restrict__ tlev, double *__restrict__ colscal,
-                                          double *__restrict__ bandscal, int idx) {
-  const int P = d.P, L = d.L;
-  if (idx < P * L) {
-    int col = idx % P, i = idx / P;
+// Record of layer i of column col (caller index idx = i P + col) into r.
+__device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__restrict__ spec,
+                                           const SocgpuRunesArgs &a, int idx, double *r) {
+  {
     double p = a.p_layer[idx], t = a.t_layer[idx];
-    double *r = colrec + ((size_t)col * L + i) * SG_NQ;
     r[SG_Q_LNP] = log(p);
     r[SG_Q_LNT] = log(t);
     r[SG_Q_MASS] = a.mass[idx];
@@ -161,7 +161,27 @@ __device__ __forceinline__ void pack_body(const CallDims &d, const SgSpec *__res
     r[SG_Q_P] = p;
     r[SG_Q_T] = t;
     double w = 0.0, lwc = 0.0, re = 0.0, f1 = 0.0, rsd = 0.0;
-    if (a.cloud_frac) {
+    double lwc2 = 0.0, re2 = 0.0, f2 = 0.0;
+    if (d.triple) {
+      // stratiform and convective types (socrates_set_cld.F90:284-330,
+      // 567-581): in-cloud mixing ratios over the raw type fractions,
+      // w_cloud = sum of the types, fractions normalised by it above the
+      // 0.001 threshold (:817-852)
+      const double eps = kEps;
+      double fs = fmax(a.cloud_frac[idx], 0.0);
+      double fc = a.conv_frac ? fmax(a.conv_frac[idx], 0.0) : 0.0;
+      double ms = a.liq_mmr ? fmax(a.liq_mmr[idx], 0.0) : 0.0;
+      double mc = a.liq_conv_mmr ? fmax(a.liq_conv_mmr[idx], 0.0) : 0.0;
+      lwc = fs > eps ? ms / fs : 0.0;
+      lwc2 = fc > eps ? mc / fc : 0.0;
+      w = fs + fc;
+      if (w > 0.001) { f1 = fs / w; f2 = fc / w; }
+      else { w = 0.0; f1 = 0.0; f2 = 0.0; }
+      if (w > 1.0) w = 1.0;
+      const SgDrop &ds = spec->drop[d.drop], &dc = spec->drop[d.drop2];
+      re = fmin(fmax(a.liq_dim ? fmax(a.liq_dim[idx], 0.0) : 0.0, ds.dim_min), ds.dim_max);
+      re2 = fmin(fmax(a.liq_conv_dim ? fmax(a.liq_conv_dim[idx], 0.0) : 0.0, dc.dim_min), dc.dim_max);
+    } else if (a.cloud_frac) {
       const double eps = kEps;
       double frac = fmax(a.cloud_frac[idx], 0.0);
       double mmr = a.liq_mmr ? fmax(a.liq_mmr[idx], 0.0) : 0.0;
@@ -188,13 +208,17 @@ __device__ __forceinline__ void pack_body(const CallDims &d, const SgSpec *__res
     r[SG_Q_RE] = re;
     r[SG_Q_FRAC1] = f1;
     r[SG_Q_RSD] = rsd;
+    r[SG_Q_LWC2] = lwc2;
+    r[SG_Q_RE2] = re2;
+    r[SG_Q_FRAC2] = f2;
   }
-  if (idx < P * (L + 1)) {
-    int col = idx % P, i = idx / P;
-    tlev[(size_t)col * (L + 1) + i] = a.t_level ? a.t_level[idx] : 0.0;
-  }
-  if (idx < P) {
-    int col = idx;
+}
+
+// Per-column scalars and per-(column, band) albedos.
+__device__ __forceinline__ void pack_column(const CallDims &d, const SocgpuRunesArgs &a, int col,
+                                            double *__restrict__ colscal, double *__restrict__ bandscal) {
+  const int P = d.P;
+  {
     double sec0 = 1.0, sol = 0.0;
     if (d.isolir == SOCGPU_SOURCE_SOLAR) {
       // set_bound (socrates_set_bound.F90:611-618)
@@ -218,14 +242,6 @@ __device__ __forceinline__ void pack_body(const CallDims &d, const SgSpec *__res
       }
     }
   }
-}
-
-// Cloud top and base over all columns of the call, as cloud_gen finds them
-// (socrates_cloud_gen.F90:224-245): first layer with any w > epsilon, last
-// layer with any w >= epsilon. krange[0] = min, krange[1] = max (1-based).
-__global__ void fill_int_kernel(int *p, int n, int v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
 }
 
 // McICA sub-columns, one wave per column, one lane per generated
@@ -332,12 +348,10 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
     for (int j = 0; j < want; j++) m &= m - 1;
     src = __ffsll((long long)m) - 1;
   }
-  __syncthreads();
   for (int k = 0; k < L; k++) {
-    double v = src >= 0 ? cs[(size_t)k * SG_MAXSUBCOL + src] : 0.0;
-    __syncthreads();
-    cs[(size_t)k * SG_MAXSUBCOL + lane] = v;
-    __syncthreads();
+    const double own = cs[(size_t)k * SG_MAXSUBCOL + lane];
+    const double v = __shfl(own, src >= 0 ? src : 0, kWave);
+    cs[(size_t)k * SG_MAXSUBCOL + lane] = src >= 0 ? v : 0.0;
   }
   if (lane == 0) {
     colscal[(size_t)col * SG_NS + SG_S_NCLD] = (double)ncld;
@@ -478,7 +492,8 @@ __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__res
                                           const double *__restrict__ minor,
                                           double *__restrict__ bandrec,
                                           double *__restrict__ bandscal,
-                                          double *__restrict__ cldopt, int idx) {
+                                          double *__restrict__ cldopt, const int *__restrict__ krange,
+                                          int idx) {
   const int NB = d.NB, L = d.L, NT = spec->n_minor;
   if (idx >= d.P * NB * L) return;
   const int i = idx % L, b = (idx / L) % NB, col = idx / (L * NB);
@@ -548,6 +563,41 @@ __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__res
   }
   const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
   br[0] = (0.0 + ray) + keq;
+  if (d.triple) {
+    // grey optics of each cloud type (grey_opt_prop.F90:975-1000,
+    // 1185-1230: Rayleigh, then the type's droplets and its ice component of
+    // zero mixing ratio, at layers from the call's cloud top where the column
+    // has cloud), the equivalent extinction (solve_band_k_eqv_scl.F90:
+    // 1038-1045), moments normalised by the scattering (grey_opt_prop.F90:
+    // 1474-1500) and the delta-rescaled phase function (rescale_phase_fnc.F90)
+    const int nct = krange[0] < L ? krange[0] : L;
+    double *co = cldopt + (((size_t)col * NB + b) * L + i) * 8;
+    for (int k = 0; k < 2; k++) {
+      double kgt = 0.0 + ray, ksc = ray, phf = 0.0, fwd = 0.0;
+      if (i + 1 >= nct && r[SG_Q_WCLD] > 0.0) {
+        const SgDrop &dr = spec->drop[k == 0 ? d.drop : d.drop2];
+        const double mmr = r[k == 0 ? SG_Q_LWC : SG_Q_LWC2], re = r[k == 0 ? SG_Q_RE : SG_Q_RE2];
+        double ct, cs, cp, cf;
+        if (dr.scheme == 5) pade2(dr.parm[b], mmr, re, ct, cs, cp, cf);
+        else slingo(dr.parm[b], mmr, re, ct, cs, cp, cf);
+        kgt = kgt + ct;
+        ksc = ksc + cs;
+        phf = phf + cp;
+        fwd = fwd + cf;
+      }
+      kgt = kgt + keq;
+      if (ksc > kTiny) {
+        const double inv = 1.0 / ksc;
+        fwd = fwd * inv;
+        phf = phf * inv;
+      }
+      phf = (phf - fwd) / (1.0 - fwd);
+      co[4 * k + 0] = kgt;
+      co[4 * k + 1] = ksc;
+      co[4 * k + 2] = phf;
+      co[4 * k + 3] = fwd;
+    }
+  }
   if (solar) {
     const double temp_max = 36.043653389117154;  // log(1/DBL_EPSILON)
     br[1] = adjust * exp(fmin(keq * r[SG_Q_MASS] * sec0, temp_max));
@@ -737,14 +787,19 @@ __device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, co
   rho = rho_n;
 }
 
-// The column solve, one wave per column (or per chunk of 64 items of a
-// column), one lane per item: a clear-sky k-term (weight w_k (1 - frac_cloudy))
-// or a McICA sub-column sample (weight w_k frac_cloudy / subcol_k)
-// (mcica_sample.F90:453-930; monochromatic_radiance.F90:429-691;
-// mcica_column.F90:255-622; solver_homogen_direct.F90:93-127). The two-stream
+// The column solve, one lane per item: a clear-sky k-term (weight
+// w_k (1 - frac_cloudy)) or a McICA sub-column sample (weight
+// w_k frac_cloudy / subcol_k) (mcica_sample.F90:453-930;
+// monochromatic_radiance.F90:429-691; mcica_column.F90:255-622;
+// solver_homogen_direct.F90:93-127). A call's items are numbered column by
+// column (n_items per column: the k-terms, then the sub-column samples) and
+// cut into waves of 64 consecutive items, so a wave spans up to kMaxSeg
+// columns ("segments") and no lane idles between columns. The two-stream
 // system is eliminated top-down (reflectance of the atmosphere above and the
 // downward diffuse flux it emits) and back-substituted bottom-up. Output:
-// partial[col][chunk][3][L+1] (up, down, direct) of this wave's weighted sum.
+// partial[wave][segment][3][L+1] (up, down, direct) of the segment's
+// weighted sums, the sums formed by butterflies over the wave with the other
+// segments' lanes masked.
 //
 // RECOMP = false keeps every layer's elimination state (a, bb, rho, dn) in
 // lane-private memory between the sweeps: 64 B per lane and layer of
@@ -755,6 +810,17 @@ __device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, co
 // place of the memory traffic, which pays where the coefficients are cheap
 // (the mixed form's FP32 arithmetic).
 constexpr int kSeg = 4;
+constexpr int kMaxSeg = 4;     // columns a wave can span (n_items >= kMinItems)
+constexpr int kMinItems = 22;  // floor(63 / 22) + 2 = 4
+
+// Sum over the lanes of segment s of each of v[0..7] (other lanes count
+// zero); lanes 8j (j = 0..7) receive the sum of v[j].
+__device__ __forceinline__ void seg_reduce8(const double (&v)[8], int seg, int s, int nseg, double (&o)[8],
+                                            int lane) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) o[k] = (nseg == 1 || seg == s) ? v[k] : 0.0;
+  reduce8(o, lane);
+}
 
 template <bool MIXED, bool RECOMP>
 __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__restrict__ spec,
@@ -765,31 +831,40 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
                                            const double *__restrict__ c_sub,
                                            const double *__restrict__ cldopt,
                                            double *__restrict__ partial, int wave, int lane) {
-  const int L = d.L, NB = d.NB;
-  const int col = wave / d.nwpc, chunk = wave % d.nwpc;
-  if (col >= d.P) return;
+  const int L = d.L, NB = d.NB, IM = d.n_items;
+  const long long g0 = (long long)wave * kWave, gmax = (long long)d.P * IM;
+  if (g0 >= gmax) return;
+  const int colA = (int)(g0 / IM);
+  const long long glast = (g0 + kWave < gmax ? g0 + kWave : gmax) - 1;
+  const int nseg = (int)(glast / IM) - colA + 1;  // wave-uniform, 1..kMaxSeg
+  const long long g = g0 + lane;
+  const bool inrange = g < gmax;
+  const int col = inrange ? (int)(g / IM) : colA;
+  const int item = inrange ? (int)(g - (long long)col * IM) : 0;
+  const int seg = col - colA;
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
   const double *cs = colscal + (size_t)col * SG_NS;
   const double fc = d.mcica ? cs[SG_S_FC] : 0.0;
   const int G = spec->n_g;
-  const int n_clear = fc < 1.0 ? G : 0;
-  const int n_sub = (d.mcica && fc > 0.0) ? spec->n_sub_items[d.mode] : 0;
-  const int item = chunk * kWave + lane;
-  const bool active = item < n_clear + n_sub;
-  double *out = partial + ((size_t)col * d.nwpc + chunk) * 3 * (L + 1);
   int gi = 0, isub = 0;
   double wgt = 0.0;
-  if (active) {
-    if (item < n_clear) {
-      gi = item;
-      wgt = spec->g[gi].w * (1.0 - fc);
-    } else {
-      int s = item - n_clear;
-      gi = spec->sub_g[d.mode][s];
-      isub = spec->sub_isub[d.mode][s];
-      wgt = spec->g[gi].w * (fc * spec->sub_inv_s[d.mode][s]);
+  bool active = false;
+  if (inrange) {
+    if (item < G) {
+      if (fc < 1.0) {
+        active = true;
+        gi = item;
+        wgt = spec->g[gi].w * (1.0 - fc);
+      }
+    } else if (fc > 0.0 && item - G < (d.mcica ? spec->n_sub_items[d.mode] : 0)) {
+      const int sI = item - G;
+      active = true;
+      gi = spec->sub_g[d.mode][sI];
+      isub = spec->sub_isub[d.mode][sI];
+      wgt = spec->g[gi].w * (fc * spec->sub_inv_s[d.mode][sI]);
     }
   }
+  double *out = partial + (size_t)wave * kMaxSeg * 3 * (L + 1);
   const SgTerm t = spec->g[gi];
   const int b = t.band;
   const double *rec = colrec + (size_t)col * L * SG_NQ;
@@ -813,7 +888,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down, flux_inc_direct
   double fdir = solar ? finc : 0.0;
   double rho = 0.0, dn = finc;
-  double vdir[8];
+  double vdir[8], o8[8];
   LayerIn cur, nxt;
   load_layer(src, 0, cur);
   for (int i = 0; i < L; i++) {
@@ -824,8 +899,10 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
 #pragma unroll
       for (int k = 0; k < 8; k++) if (k == m) vdir[k] = x;
       if (m == 7) {
-        reduce8(vdir, lane);
-        if ((lane & 7) == 0) out[2 * (L + 1) + i - 7 + (lane >> 3)] = vdir[0];
+        for (int sg = 0; sg < nseg; sg++) {
+          seg_reduce8(vdir, seg, sg, nseg, o8, lane);
+          if ((lane & 7) == 0) out[(sg * 3 + 2) * (L + 1) + i - 7 + (lane >> 3)] = o8[0];
+        }
       }
     }
     if (i + 1 < L) load_layer(src, i + 1, nxt);
@@ -854,16 +931,19 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
       if (k == m) vdir[k] = x;
       else if (k > m) vdir[k] = 0.0;
     }
-    reduce8(vdir, lane);
-    if ((lane & 7) == 0 && base + (lane >> 3) <= L) out[2 * (L + 1) + base + (lane >> 3)] = vdir[0];
-  } else if (lane < 64) {
-    for (int i = lane; i <= L; i += kWave) out[2 * (L + 1) + i] = 0.0;
+    for (int sg = 0; sg < nseg; sg++) {
+      seg_reduce8(vdir, seg, sg, nseg, o8, lane);
+      if ((lane & 7) == 0 && base + (lane >> 3) <= L) out[(sg * 3 + 2) * (L + 1) + base + (lane >> 3)] = o8[0];
+    }
+  } else {
+    for (int sg = 0; sg < nseg; sg++)
+      for (int i = lane; i <= L; i += kWave) out[(sg * 3 + 2) * (L + 1) + i] = 0.0;
   }
   double fu = 0.0, fd = 0.0;
   if (active) {
-    double sg = solar ? (alb_dir - alb_diff) * fdir
-                      : (1.0 - alb_diff) * (bs[1] - br[(L - 1) * SG_NBQ + 3]);
-    fu = (alb_diff * dn + sg) / (1.0 - alb_diff * rho);
+    double sgs = solar ? (alb_dir - alb_diff) * fdir
+                       : (1.0 - alb_diff) * (bs[1] - br[(L - 1) * SG_NBQ + 3]);
+    fu = (alb_diff * dn + sgs) / (1.0 - alb_diff * rho);
     fd = rho * fu + dn;
   }
   if (!RECOMP) {
@@ -880,10 +960,12 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
         v[2 * m] = lev >= 0 ? wgt * fu : 0.0;
         v[2 * m + 1] = lev >= 0 ? wgt * fd : 0.0;
       }
-      reduce8(v, lane);
-      if ((lane & 7) == 0) {
-        const int j = lane >> 3, lev = top - (j >> 1);
-        if (lev >= 0) out[(j & 1) * (L + 1) + lev] = v[0];
+      for (int sg = 0; sg < nseg; sg++) {
+        seg_reduce8(v, seg, sg, nseg, o8, lane);
+        if ((lane & 7) == 0) {
+          const int j = lane >> 3, lev = top - (j >> 1);
+          if (lev >= 0) out[(sg * 3 + (j & 1)) * (L + 1) + lev] = o8[0];
+        }
       }
     }
   } else {
@@ -895,11 +977,13 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
       for (int m = 0; m < 4; m++) { v[2 * m] = 0.0; v[2 * m + 1] = 0.0; }
       v[0] = wgt * fu;
       v[1] = wgt * fd;
-      reduce8(v, lane);
-      if ((lane & 7) == 0 && (lane >> 3) < 2) out[(lane >> 3) * (L + 1) + L] = v[0];
+      for (int sg = 0; sg < nseg; sg++) {
+        seg_reduce8(v, seg, sg, nseg, o8, lane);
+        if ((lane & 7) == 0 && (lane >> 3) < 2) out[(sg * 3 + (lane >> 3)) * (L + 1) + L] = o8[0];
+      }
     }
-    const int nseg = (L + kSeg - 1) / kSeg;
-    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+    const int nsegL = (L + kSeg - 1) / kSeg;
+    for (int sgi = nsegL - 1; sgi >= 0; sgi--) {
       const int i0 = sgi * kSeg;
       double sa[kSeg], sb[kSeg], sr[kSeg], sdn[kSeg];
       if (active) {
@@ -927,10 +1011,12 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
         v[2 * m] = lev < L ? wgt * fu : 0.0;
         v[2 * m + 1] = lev < L ? wgt * fd : 0.0;
       }
-      reduce8(v, lane);
-      if ((lane & 7) == 0) {
-        const int j = lane >> 3, lev = i0 + kSeg - 1 - (j >> 1);
-        if (lev < L) out[(j & 1) * (L + 1) + lev] = v[0];
+      for (int sg = 0; sg < nseg; sg++) {
+        seg_reduce8(v, seg, sg, nseg, o8, lane);
+        if ((lane & 7) == 0) {
+          const int j = lane >> 3, lev = i0 + kSeg - 1 - (j >> 1);
+          if (lev < L) out[(sg * 3 + (j & 1)) * (L + 1) + lev] = o8[0];
+        }
       }
     }
   }
@@ -938,21 +1024,32 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
 
 // Sum of the per-wave partials, the Planck flux added back for the
 // infra-red (adjust_ir_radiance.F90:186-200, weight_band = 1), output in the
-// caller's [level][profile] layout.
+// caller's [level][profile] layout. A column's partials come from the waves
+// its items fall in (packed solve: partial[wave][segment], in wave order) or
+// from its nwpc waves (triple-column solve: partial[col][chunk]).
 __device__ __forceinline__ void finish_body(const CallDims &d, const double *__restrict__ partial,
                                             const double *__restrict__ bandrec,
-                                            const double *__restrict__ bandscal,
-                                            double *__restrict__ up, double *__restrict__ down,
-                                            double *__restrict__ direct, int idx) {
-  const int P = d.P, L = d.L;
-  if (idx >= P * (L + 1)) return;
-  int col = idx % P, lev = idx / P;
-  double su = 0.0, sd = 0.0, sx = 0.0;
-  for (int c = 0; c < d.nwpc; c++) {
-    const double *o = partial + ((size_t)col * d.nwpc + c) * 3 * (L + 1);
-    su += o[lev];
-    sd += o[(L + 1) + lev];
-    sx += o[2 * (L + 1) + lev];
+                                            const double *__restrict__ bandscal, int col, int lev,
+                                            double &su, double &sd, double &sx) {
+  const int L = d.L;
+  su = 0.0; sd = 0.0; sx = 0.0;
+  if (d.triple) {
+    for (int c = 0; c < d.nwpc; c++) {
+      const double *o = partial + ((size_t)col * d.nwpc + c) * 3 * (L + 1);
+      su += o[lev];
+      sd += o[(L + 1) + lev];
+      sx += o[2 * (L + 1) + lev];
+    }
+  } else {
+    const long long IM = d.n_items;
+    const long long wa = (long long)col * IM / kWave, wb = ((long long)col * IM + IM - 1) / kWave;
+    for (long long w = wa; w <= wb; w++) {
+      const int sg = col - (int)(w * kWave / IM);
+      const double *o = partial + ((size_t)w * kMaxSeg + sg) * 3 * (L + 1);
+      su += o[lev];
+      sd += o[(L + 1) + lev];
+      sx += o[2 * (L + 1) + lev];
+    }
   }
   if (d.isolir == SOCGPU_SOURCE_THERMAL) {
     double bsum = 0.0;
@@ -964,9 +1061,6 @@ __device__ __forceinline__ void finish_body(const CallDims &d, const double *__r
     su += bsum;
     sd += bsum;
   }
-  up[idx] = su;
-  down[idx] = sd;
-  if (direct) direct[idx] = sx;
 }
 
 // Heating rate from flux divergence (socrates_set_diag.F90:140-210), K/s.
@@ -981,13 +1075,402 @@ __device__ __forceinline__ void heating_body(int P, int L, const double *__restr
   hr[idx] = heat_cap ? fdiv / heat_cap[idx] : fdiv;
 }
 
+// ------------------------------------------------------------ triple column
+// Maximum-random overlap transfer coefficients of the triple-column
+// representation (overlap_coupled.F90:150-453 with full correlation; the
+// region fractions are those of the cloud types, aggregate_cloud.F90:
+// 106-166) at interface j of one column (0 the top of the atmosphere, L the
+// surface), regions 0 clear, 1 stratiform, 2 convective: ov[3a + b] is the
+// fraction of the downward flux in region b above the interface that enters
+// region a below, ov[9 + 3a + b] the fraction of the upward flux in region b
+// below that enters region a above. Interfaces above the call's cloud top
+// minus one are not used. ovl[col][L+1][18].
+constexpr int kOv = 18;
+
+__device__ __forceinline__ void overlap_body(const CallDims &d, const double *__restrict__ colrec,
+                                             const int *__restrict__ krange, double *__restrict__ ovl,
+                                             int idx) {
+  const int P = d.P, L = d.L;
+  if (idx >= P * (L + 1)) return;
+  const int col = idx % P, j = idx / P;
+  const int nct = krange[0] < L ? krange[0] : L;
+  if (j < nct - 1) return;
+  const double *rec = colrec + (size_t)col * L * SG_NQ;
+  double up[3] = {1.0, 0.0, 0.0}, lo[3] = {1.0, 0.0, 0.0};
+  if (j >= nct) {  // areas of layer j (1-based), the layer above
+    const double *r = rec + (size_t)(j - 1) * SG_NQ;
+    up[0] = 1.0 - r[SG_Q_WCLD];
+    up[1] = r[SG_Q_WCLD] * r[SG_Q_FRAC1];
+    up[2] = r[SG_Q_WCLD] * r[SG_Q_FRAC2];
+  }
+  if (j < L) {     // areas of layer j+1, the layer below
+    const double *r = rec + (size_t)j * SG_NQ;
+    lo[0] = 1.0 - r[SG_Q_WCLD];
+    lo[1] = r[SG_Q_WCLD] * r[SG_Q_FRAC1];
+    lo[2] = r[SG_Q_WCLD] * r[SG_Q_FRAC2];
+  }
+  const double tol = 100.0 * kEps;
+  double ao[3][3];  // ao[k][m]: area of region k above overlapping region m below
+  for (int k = 0; k < 3; k++)
+    for (int m = 0; m < 3; m++) ao[k][m] = k == m ? fmin(lo[k], up[k]) : 0.0;
+  double art = 1.0 - ao[0][0];
+  art = art - ao[1][1];
+  art = art - ao[2][2];
+  double aru[3], arl[3];
+  for (int k = 0; k < 3; k++) {
+    aru[k] = up[k] - ao[k][k];
+    arl[k] = lo[k] - ao[k][k];
+  }
+  if (art > tol)
+    for (int k = 0; k < 3; k++)
+      for (int m = 0; m < 3; m++) ao[k][m] = ao[k][m] + aru[k] * arl[m] / art;
+  double *o = ovl + ((size_t)col * (L + 1) + j) * kOv;
+  for (int b = 0; b < 3; b++)    // region b above
+    for (int a = 0; a < 3; a++) o[3 * a + b] = up[b] > tol ? ao[b][a] / up[b] : (a == b ? 1.0 : 0.0);
+  for (int b = 0; b < 3; b++)    // region b below
+    for (int a = 0; a < 3; a++) o[9 + 3 * a + b] = lo[b] > tol ? ao[a][b] / lo[b] : (a == b ? 1.0 : 0.0);
+}
+
+// Two-stream coefficients of one region of one layer for one lane: single
+// scattering and delta-rescaling (single_scattering_all.F90,
+// rescale_tau_omega.F90:73-80) of the region's grey optics kg (extinction,
+// with the equivalent extinction), ks (scattering), g (rescaled asymmetry),
+// fw (forward scattering) plus the lane's gas absorption kgas, then
+// two_coeff (FP64) or two_coeff_f32 (mixed form, with the absorption formed
+// in FP64 before rounding, as in layer_coef_mixed).
+template <bool MIXED>
+__device__ __forceinline__ void region_coef(const LaneCtx &c, double kgas, double mass, double kg, double ks,
+                                            double g, double fw, Coef &o) {
+  if (!MIXED) {
+    double ktot = kg + kgas;
+    double tau = ktot * mass;
+    double om = fmin(ks / (ktot + kTiny), 1.0 - 32.0 * kEps);
+    tau = tau * (1.0 - om * fw);
+    om = om * (1.0 - fw) / (1.0 - om * fw);
+    two_coeff(c.solar, tau, om, g, c.sec0, o);
+  } else {
+    const float ktot = (float)(kg + kgas);
+    const float kab = (float)((kg - ks) + kgas);
+    const float omc_min = (float)(32.0 * kEps);
+    float tau = ktot * (float)mass;
+    float omc = ktot > 0.0f ? fmaxf(SG_FDIV(kab, ktot), omc_min) : 1.0f;
+    float om = 1.0f - omc;
+    const float sc = 1.0f - om * (float)fw;
+    tau = tau * sc;
+    omc = SG_FDIV(omc, sc);
+    om = 1.0f - omc;
+    CoefF f;
+    two_coeff_f32(c.solar, tau, om, omc, (float)g, (float)c.sec0, f);
+    coef_from_f32(f, o);
+  }
+}
+
+// Region coefficients weighted as two_coeff_region.F90:200-445 forms them:
+// the type's coefficients times its fraction, summed from zero, times the
+// reciprocal of the region fraction (the same fraction here); zero where
+// the type is absent.
+__device__ __forceinline__ void region_mix(double f, Coef &o) {
+  if (f > kTiny) {
+    const double inv = 1.0 / f;
+    o.T = (0.0 + f * o.T) * inv;
+    o.R = (0.0 + f * o.R) * inv;
+    o.T0 = (0.0 + f * o.T0) * inv;
+    o.s1 = (0.0 + f * o.s1) * inv;
+    o.s2 = (0.0 + f * o.s2) * inv;
+  } else {
+    o.T = 0.0; o.R = 0.0; o.T0 = 0.0; o.s1 = 0.0; o.s2 = 0.0;
+  }
+}
+
+template <bool MIXED>
+__device__ __forceinline__ double lane_kgas(const SgTerm &t, const double *r) {
+  if (MIXED) return (double)k_term_f32(t, (float)r[SG_Q_LNP], (float)r[SG_Q_LNT], (float)r[SG_Q_MMR + t.gas]);
+  return k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
+}
+
+// Coefficients of the three regions of layer i (0-based) for one lane; the
+// cloud regions only from the cloud top (1-based nct) down.
+template <bool MIXED>
+__device__ __forceinline__ void tri_layer_coef(const LaneCtx &c, double kgas, const double *r,
+                                               const double *bq, const double *co, bool cloud_rows,
+                                               Coef (&o)[3]) {
+  const double mass = r[SG_Q_MASS];
+  region_coef<MIXED>(c, kgas, mass, bq[0], c.ksc_clear, 0.0, 0.0, o[0]);
+  if (cloud_rows) {
+    const double f1 = r[SG_Q_FRAC1], f2 = r[SG_Q_FRAC2];
+    if (f1 > kTiny) region_coef<MIXED>(c, kgas, mass, co[0], co[1], co[2], co[3], o[1]);
+    region_mix(f1, o[1]);
+    if (f2 > kTiny) region_coef<MIXED>(c, kgas, mass, co[4], co[5], co[6], co[7], o[2]);
+    region_mix(f2, o[2]);
+  }
+}
+
+// The triple-column solve of one column for one lane per major k-term
+// (triple_column.F90:255-569): region coefficients (two_coeff_region), the
+// direct beam carried through the overlap (triple_solar_source.F90:180-412)
+// or the area-weighted infra-red sources (triple_column.F90:298-349), the
+// surface sources, and solver_triple_hogan (solver_triple_hogan.F90:
+// 207-411): an upward elimination of the region albedos and sources coupled
+// through the overlap coefficients, then a downward substitution. Three
+// sweeps over the layers: down (solar only; the direct flux entering each
+// region of each layer, kept in tws), up (coefficients, sources and the
+// elimination; per layer and region the substitution pair fu = c1 fd + c2,
+// fd' = e1 fd + e2 kept in tws) and down again (fluxes). tws holds kTriQ
+// doubles per layer and lane, lane-interleaved: [wave][layer][q][64].
+// Output: partial[col][chunk][3][L+1] of this wave's k-term-weighted sums.
+constexpr int kTriQ = 12;
+
+template <bool MIXED>
+__device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                               int l_rayleigh, const double *__restrict__ colrec,
+                                               const double *__restrict__ colscal,
+                                               const double *__restrict__ bandrec,
+                                               const double *__restrict__ bandscal,
+                                               const double *__restrict__ cldopt,
+                                               const double *__restrict__ ovl,
+                                               const int *__restrict__ krange, double *__restrict__ tws,
+                                               double *__restrict__ partial, int wave, int lane) {
+  const int L = d.L, NB = d.NB;
+  const int col = wave / d.nwpc, chunk = wave % d.nwpc;
+  if (col >= d.P) return;
+  const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
+  const int item = chunk * kWave + lane;
+  const bool active = item < spec->n_g;
+  const int gi = active ? item : 0;
+  const double wgt = active ? spec->g[gi].w : 0.0;
+  const SgTerm t = spec->g[gi];
+  const int b = t.band;
+  const int nct = krange[0] < L ? krange[0] : L;  // 1-based cloud top of the call
+  const double *cs = colscal + (size_t)col * SG_NS;
+  const double *rec = colrec + (size_t)col * L * SG_NQ;
+  const double *br = bandrec + ((size_t)col * NB + b) * L * SG_NBQ;
+  const double *bs = bandscal + ((size_t)col * NB + b) * SG_NBQ;
+  const double *cop = cldopt + ((size_t)col * NB + b) * L * 8;
+  const double *ov = ovl + (size_t)col * (L + 1) * kOv;
+  double *ws = tws + (size_t)wave * L * kTriQ * kWave + lane;
+  double *out = partial + ((size_t)col * d.nwpc + chunk) * 3 * (L + 1);
+  LaneCtx c;
+  c.solar = solar;
+  c.sec0 = cs[SG_S_SEC0];
+  c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
+  const double alb_diff = bs[2], alb_dir = bs[3];
+  const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down (= flux_inc_direct)
+#define TWS(i, q) ws[((size_t)(i) * kTriQ + (q)) * kWave]
+
+  // --- sweep 1 (solar): direct flux entering each region of each layer
+  double base0 = finc, base1 = 0.0, base2 = 0.0;  // direct flux leaving the layer above, per region
+  if (solar) {
+    double vdir[8];
+    for (int i = 0; i < L; i++) {
+      {  // direct flux at level i, summed 8 levels at a time
+        const int m = i & 7;
+        const double x = wgt * (base0 + base1 + base2);
+#pragma unroll
+        for (int k = 0; k < 8; k++) if (k == m) vdir[k] = x;
+        if (m == 7) {
+          reduce8(vdir, lane);
+          if ((lane & 7) == 0) out[2 * (L + 1) + i - 7 + (lane >> 3)] = vdir[0];
+        }
+      }
+      const double *r = rec + (size_t)i * SG_NQ;
+      const double *bq = br + (size_t)i * SG_NBQ;
+      const double adj = bq[1];
+      const double kgas = lane_kgas<MIXED>(t, r);
+      Coef o[3];
+      const bool cloud_rows = i + 1 >= nct;
+      tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, cloud_rows, o);
+      double top0, top1 = 0.0, top2 = 0.0;
+      if (!cloud_rows) {
+        top0 = base0;
+        base0 = top0 * o[0].T0 * adj;
+      } else {
+        const double *v = ov + (size_t)i * kOv;  // interface at the top of layer i+1
+        top0 = v[0] * base0 + v[1] * base1 + v[2] * base2;
+        top1 = v[3] * base0 + v[4] * base1 + v[5] * base2;
+        top2 = v[6] * base0 + v[7] * base1 + v[8] * base2;
+        base0 = top0 * o[0].T0 * adj;
+        base1 = top1 * o[1].T0 * adj;
+        base2 = top2 * o[2].T0 * adj;
+      }
+      TWS(i, 0) = top0;
+      TWS(i, 1) = top1;
+      TWS(i, 2) = top2;
+    }
+    {  // remaining direct-flux levels (the partial block and the surface)
+      const int bl = L & ~7, m = L & 7;
+      const double x = wgt * (base0 + base1 + base2);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (k == m) vdir[k] = x;
+        else if (k > m) vdir[k] = 0.0;
+      }
+      reduce8(vdir, lane);
+      if ((lane & 7) == 0 && bl + (lane >> 3) <= L) out[2 * (L + 1) + bl + (lane >> 3)] = vdir[0];
+    }
+  } else {
+    for (int i = lane; i <= L; i += kWave) out[2 * (L + 1) + i] = 0.0;
+  }
+
+  // --- sweep 2: upward elimination (solver_triple_hogan.F90:207-312)
+  double a0 = alb_diff, a1 = alb_diff, a2 = alb_diff, g0, g1, g2;
+  if (solar) {
+    g0 = (alb_dir - alb_diff) * base0;
+    g1 = (alb_dir - alb_diff) * base1;
+    g2 = (alb_dir - alb_diff) * base2;
+  } else {
+    const double dps = bs[1] - br[(size_t)(L - 1) * SG_NBQ + 3];
+    const double *u = ov + (size_t)L * kOv + 9;
+    g0 = u[0] * (1.0 - alb_diff) * dps;
+    g1 = u[3] * (1.0 - alb_diff) * dps;
+    g2 = u[6] * (1.0 - alb_diff) * dps;
+  }
+  for (int i = L - 1; i >= 0; i--) {
+    const double *r = rec + (size_t)i * SG_NQ;
+    const double *bq = br + (size_t)i * SG_NBQ;
+    const double kgas = lane_kgas<MIXED>(t, r);
+    const bool cloud_rows = i + 1 >= nct;
+    Coef o[3];
+    tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, cloud_rows, o);
+    double su[3], sd[3];
+    const int nreg = cloud_rows ? 3 : 1;
+    if (solar) {
+      const double adj = bq[1];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (k < nreg) {
+          const double top = TWS(i, k);
+          const double base = top * o[k].T0 * adj;
+          su[k] = o[k].s1 * top;
+          sd[k] = (o[k].s2 - o[k].T0) * top + base;
+        }
+      }
+    } else {
+      const double dpl2 = bq[1], dpl = bq[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (k < nreg) {
+          su[k] = o[k].s1 * dpl + o[k].s2 * dpl2;
+          sd[k] = -o[k].s1 * dpl + o[k].s2 * dpl2;
+        }
+      }
+      if (cloud_rows) {
+        const double w = r[SG_Q_WCLD], wf = 1.0 - w;
+        const double w1 = w * r[SG_Q_FRAC1], w2 = w * r[SG_Q_FRAC2];
+        sd[0] = wf * sd[0]; su[0] = wf * su[0];
+        sd[1] = w1 * sd[1]; su[1] = w1 * su[1];
+        sd[2] = w2 * sd[2]; su[2] = w2 * su[2];
+      }
+    }
+    if (cloud_rows) {
+      const double *vb = ov + (size_t)(i + 1) * kOv;  // interface at the bottom of layer i+1
+      const double th[3] = {a0 * vb[0] + a1 * vb[3] + a2 * vb[6], a0 * vb[1] + a1 * vb[4] + a2 * vb[7],
+                            a0 * vb[2] + a1 * vb[5] + a2 * vb[8]};
+      const double gg[3] = {g0, g1, g2};
+      double lsrc[3], aa[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double T = o[k].T, R = o[k].R;
+        const double binv = 1.0 / (1.0 - th[k] * R);
+        const double gam = th[k] * T;
+        const double h = gg[k] + th[k] * sd[k];
+        lsrc[k] = su[k] + h * T * binv;
+        aa[k] = R + th[k] * T * T * binv;
+        const double c1 = gam * binv, c2 = h * binv;
+        TWS(i, 4 * k + 0) = c1;
+        TWS(i, 4 * k + 1) = c2;
+        TWS(i, 4 * k + 2) = T + R * c1;
+        TWS(i, 4 * k + 3) = R * c2 + sd[k];
+      }
+      const double *ut = ov + (size_t)i * kOv + 9;    // interface at the top of layer i+1
+      a0 = aa[0]; a1 = aa[1]; a2 = aa[2];
+      g0 = ut[0] * lsrc[0] + ut[1] * lsrc[1] + ut[2] * lsrc[2];
+      g1 = ut[3] * lsrc[0] + ut[4] * lsrc[1] + ut[5] * lsrc[2];
+      g2 = ut[6] * lsrc[0] + ut[7] * lsrc[1] + ut[8] * lsrc[2];
+    } else {
+      double th = a0;
+      if (i + 1 == nct - 1 && nct < L) {
+        const double *vb = ov + (size_t)(i + 1) * kOv;
+        th = a0 * vb[0] + a1 * vb[3] + a2 * vb[6];
+      }
+      const double T = o[0].T, R = o[0].R;
+      const double binv = 1.0 / (1.0 - th * R);
+      const double gam = th * T;
+      const double h = g0 + th * sd[0];
+      const double lam = T * binv;
+      a0 = R + lam * gam;
+      g0 = su[0] + lam * h;
+      const double c1 = gam * binv, c2 = h * binv;
+      TWS(i, 0) = c1;
+      TWS(i, 1) = c2;
+      TWS(i, 2) = T + R * c1;
+      TWS(i, 3) = R * c2 + sd[0];
+    }
+  }
+
+  // --- sweep 3: downward substitution, up and down fluxes at every level,
+  // summed over the wave 4 levels at a time
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = 0.0;
+  auto put = [&](int lev, double fu, double fd) {
+    const int m = lev & 3;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (k == m) { v[2 * k] = wgt * fu; v[2 * k + 1] = wgt * fd; }
+      else if (k > m && lev == L) { v[2 * k] = 0.0; v[2 * k + 1] = 0.0; }
+    }
+    if (m == 3 || lev == L) {
+      reduce8(v, lane);
+      if ((lane & 7) == 0) {
+        const int j = lane >> 3, lv = (lev & ~3) + (j >> 1);
+        if (lv <= L) out[(j & 1) * (L + 1) + lv] = v[0];
+      }
+    }
+  };
+  double fdt = finc;
+  {
+    const double up0 = nct > 1 ? a0 * finc + g0 : g0 + finc * (ov[0] * a0 + ov[3] * a1 + ov[6] * a2);
+    put(0, up0, finc);
+  }
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int i = 0; i < L; i++) {
+    double fu, fd;
+    if (i + 1 < nct) {
+      fu = TWS(i, 0) * fdt + TWS(i, 1);
+      fdt = TWS(i, 2) * fdt + TWS(i, 3);
+      fd = fdt;
+    } else {
+      const double *vt = ov + (size_t)i * kOv;
+      if (i + 1 == nct) {
+        d0 = vt[0] * fdt; d1 = vt[3] * fdt; d2 = vt[6] * fdt;
+      } else {
+        const double x0 = d0, x1 = d1, x2 = d2;
+        d0 = vt[0] * x0 + vt[1] * x1 + vt[2] * x2;
+        d1 = vt[3] * x0 + vt[4] * x1 + vt[5] * x2;
+        d2 = vt[6] * x0 + vt[7] * x1 + vt[8] * x2;
+      }
+      const double u0 = TWS(i, 0) * d0 + TWS(i, 1);
+      const double u1 = TWS(i, 4) * d1 + TWS(i, 5);
+      const double u2 = TWS(i, 8) * d2 + TWS(i, 9);
+      d0 = TWS(i, 2) * d0 + TWS(i, 3);
+      d1 = TWS(i, 6) * d1 + TWS(i, 7);
+      d2 = TWS(i, 10) * d2 + TWS(i, 11);
+      fu = u0 + u1 + u2;
+      fd = d0 + d1 + d2;
+      fdt = fd;
+    }
+    put(i + 1, fu, fd);
+  }
+#undef TWS
+}
+
 // ------------------------------------------------------------ batches
 // A batch is a list of radiation calls ("requests"), each with its own
 // spectrum, source, shape and options, packed into one launch sequence:
 // every kernel loops over the concatenated work of all requests and finds
 // the request of a work item by binary search over per-request starts. A
 // request's results are those of the same call made alone.
-enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, kNWork };
+enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, kNWork };
 
 struct ReqDesc {
   CallDims d;
@@ -998,6 +1481,7 @@ struct ReqDesc {
   SocgpuRunesArgs a;            // device input pointers
   double *up, *dn, *dir, *hr;   // device outputs (dir may be null)
   double *colrec, *tlev, *colscal, *bandrec, *bandscal, *csub, *minor, *cldopt, *partial;
+  double *ovl, *tws;            // triple column: overlap coefficients, solve workspace
   int *krange;
   long long s[kNWork];          // first work item of this request in each space
 };
@@ -1024,21 +1508,66 @@ __global__ void b_init_kernel(const BatchHdr *h, const ReqDesc *rq) {
   rq[r].krange[1] = 0;
 }
 
-__global__ void b_pack_kernel(const BatchHdr *h, const ReqDesc *rq) {
-  const long long tot = h->tot[W_LEV];
-  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
-       g += (long long)gridDim.x * blockDim.x) {
-    const ReqDesc &q = rq[find_req(h, rq, W_LEV, g)];
-    pack_body(q.d, q.spec, q.a, q.colrec, q.tlev, q.colscal, q.bandscal, (int)(g - q.s[W_LEV]));
+// Caller arrays ([layer][profile]) to column records ([col][layer][SG_NQ]):
+// one tile of kPackC columns x kPackL levels per block and iteration, read
+// along the profiles, transposed through LDS and written as runs of
+// kPackL records per column.
+constexpr int kPackC = 64, kPackL = 4;
+
+__host__ __device__ inline int pack_tiles(const CallDims &d) {
+  return ((d.P + kPackC - 1) / kPackC) * ((d.L + 1 + kPackL - 1) / kPackL);
+}
+
+__global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  __shared__ double sh[kPackC * kPackL * SG_NQ];
+  __shared__ double shl[kPackC * kPackL];
+  const long long tot = h->tot[W_PTILE];
+  const int tx = threadIdx.x % kPackC, ty = threadIdx.x / kPackC;
+  for (long long tile = blockIdx.x; tile < tot; tile += gridDim.x) {
+    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_PTILE, tile));
+    const ReqDesc &q = rq[r];
+    const int P = q.d.P, L = q.d.L;
+    const int t = (int)(tile - q.s[W_PTILE]);
+    const int ntc = (P + kPackC - 1) / kPackC;
+    const int c0 = (t % ntc) * kPackC, i0 = (t / ntc) * kPackL;
+    const int col = c0 + tx, i = i0 + ty;
+    if (col < P && i < L) pack_layer(q.d, q.spec, q.a, i * P + col, sh + (tx * kPackL + ty) * SG_NQ);
+    if (col < P && i <= L) shl[tx * kPackL + ty] = q.a.t_level ? q.a.t_level[(size_t)i * P + col] : 0.0;
+    if (col < P && i == 0) pack_column(q.d, q.a, col, q.colscal, q.bandscal);
+    __syncthreads();
+    const int nl = L - i0 < kPackL ? L - i0 : kPackL;
+    if (nl > 0)
+      for (int e = threadIdx.x; e < kPackC * kPackL * SG_NQ; e += kPackC * kPackL) {
+        const int cc = e / (kPackL * SG_NQ), off = e - cc * (kPackL * SG_NQ);
+        if (c0 + cc < P && off < nl * SG_NQ)
+          q.colrec[((size_t)(c0 + cc) * L + i0) * SG_NQ + off] = sh[cc * kPackL * SG_NQ + off];
+      }
+    {
+      const int e = threadIdx.x, cc = e / kPackL, off = e % kPackL;
+      if (c0 + cc < P && i0 + off <= L) q.tlev[(size_t)(c0 + cc) * (L + 1) + i0 + off] = shl[e];
+    }
+    __syncthreads();
   }
 }
 
 // Cloud top and base over all columns of each request, as cloud_gen finds
 // them (socrates_cloud_gen.F90:224-245): first layer with any w > epsilon,
 // last layer with any w >= epsilon; krange[0] = min, krange[1] = max
-// (1-based). One atomic per wave where the wave lies within one request.
+// (1-based). For the triple-column representation krange[0] is also the
+// call's cloud top (set_cloud_geometry.F90:93-99: first layer with any
+// w > 0, L when there is none, applied by the callers). w is 0 or above
+// the 0.001 threshold, so both tests read "the layer has cloud", evaluated
+// here on the caller's fractions exactly as the pack kernel forms w. One
+// thread per column, reading along the profiles; one atomic per wave where
+// the wave lies within one request.
+__device__ __forceinline__ bool layer_cloudy(const CallDims &d, const SocgpuRunesArgs &a, size_t idx) {
+  double w = fmax(a.cloud_frac[idx], 0.0);
+  if (d.triple) w = w + (a.conv_frac ? fmax(a.conv_frac[idx], 0.0) : 0.0);
+  return w > 0.001;
+}
+
 __global__ void b_cldrange_kernel(const BatchHdr *h, const ReqDesc *rq) {
-  const long long tot = h->tot[W_LAY];
+  const long long tot = h->tot[W_CCOL];
   const int lane = threadIdx.x & (kWave - 1);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long base = (blockIdx.x * (long long)blockDim.x + threadIdx.x) - lane; base < tot;
@@ -1046,14 +1575,22 @@ __global__ void b_cldrange_kernel(const BatchHdr *h, const ReqDesc *rq) {
     const long long g = base + lane;
     int r = -1, kmin = INT32_MAX, kmax = 0;
     if (g < tot) {
-      r = find_req(h, rq, W_LAY, g);
+      r = find_req(h, rq, W_CCOL, g);
       const ReqDesc &q = rq[r];
-      if (q.d.mcica) {
-        const int idx = (int)(g - q.s[W_LAY]);
-        const int col = idx / q.d.L, i = idx % q.d.L;
-        double w = q.colrec[((size_t)col * q.d.L + i) * SG_NQ + SG_Q_WCLD];
-        if (w > kEps) kmin = i + 1;
-        if (w >= kEps) kmax = i + 1;
+      const int col = (int)(g - q.s[W_CCOL]), P = q.d.P, L = q.d.L;
+      if (q.d.raw_range) {
+        for (int i = 0; i < L; i++) {
+          const double w = q.a.cloud_frac[(size_t)i * P + col];
+          if (w > kEps && kmin == INT32_MAX) kmin = i + 1;
+          if (w >= kEps) kmax = i + 1;
+        }
+      } else {
+        for (int i = 0; i < L; i++) {
+          if (layer_cloudy(q.d, q.a, (size_t)i * P + col)) {
+            kmin = kmin < i + 1 ? kmin : i + 1;
+            kmax = i + 1;
+          }
+        }
       }
     }
     const int r0 = __shfl(r, 0, kWave);
@@ -1074,13 +1611,32 @@ __global__ void b_cldrange_kernel(const BatchHdr *h, const ReqDesc *rq) {
   }
 }
 
-__global__ __launch_bounds__(64) void b_cloudgen_kernel(const BatchHdr *h, const ReqDesc *rq) {
+// kCgWaves columns per block and iteration, one wave each; the condensate
+// table (xcw, n1 x n2 <= kXcwMax) of the block's first column is kept in
+// LDS and reloaded only when it changes; a wave whose column uses another
+// table reads it from global memory.
+constexpr int kCgWaves = 4, kXcwMax = 4096;
+
+__global__ __launch_bounds__(64 * kCgWaves) void b_cloudgen_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  __shared__ double sx[kXcwMax];
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const long long tot = h->tot[W_CG];
-  for (long long w = blockIdx.x; w < tot; w += gridDim.x) {
-    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_CG, w));
-    const ReqDesc &q = rq[r];
-    cloudgen_body(q.d, q.colrec, q.a.rand_seed, q.dp_cf, q.dp_cw, q.krange, q.xcw, q.n1, q.n2, q.csub,
-                  q.colscal, (int)(w - q.s[W_CG]), threadIdx.x);
+  const double *loaded = nullptr;
+  for (long long base = (long long)blockIdx.x * kCgWaves; base < tot; base += (long long)gridDim.x * kCgWaves) {
+    const ReqDesc &q0 = rq[__builtin_amdgcn_readfirstlane(find_req(h, rq, W_CG, base))];
+    const int nx0 = q0.n1 * q0.n2;
+    if (q0.xcw != loaded && nx0 <= kXcwMax) {
+      for (int e = threadIdx.x; e < nx0; e += blockDim.x) sx[e] = q0.xcw[e];
+      loaded = q0.xcw;
+    }
+    __syncthreads();
+    const long long w = base + wv;
+    if (w < tot) {
+      const ReqDesc &q = rq[__builtin_amdgcn_readfirstlane(find_req(h, rq, W_CG, w))];
+      const double *tab = (q.xcw == loaded && q.n1 * q.n2 <= kXcwMax) ? sx : q.xcw;
+      cloudgen_body(q.d, q.colrec, q.a.rand_seed, q.dp_cf, q.dp_cw, q.krange, tab, q.n1, q.n2, q.csub,
+                    q.colscal, (int)(w - q.s[W_CG]), lane);
+    }
     __syncthreads();
   }
 }
@@ -1104,7 +1660,7 @@ __global__ __launch_bounds__(256) void b_prep_kernel(const BatchHdr *h, const Re
        g += (long long)gridDim.x * blockDim.x) {
     const ReqDesc &q = rq[find_req(h, rq, W_PREP, g)];
     prep_body(q.d, q.spec, q.l_rayleigh, q.colrec, q.tlev, q.colscal, q.minor, q.bandrec, q.bandscal,
-              q.cldopt, (int)(g - q.s[W_PREP]));
+              q.cldopt, q.krange, (int)(g - q.s[W_PREP]));
   }
 }
 
@@ -1121,12 +1677,65 @@ __global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, co
   }
 }
 
-__global__ void b_finish_kernel(const BatchHdr *h, const ReqDesc *rq) {
-  const long long tot = h->tot[W_LEV];
+// Outputs in tiles of kFinC columns x kFinL levels per block and iteration:
+// partials read along the levels of a column, transposed through LDS,
+// written along the profiles of a level.
+constexpr int kFinC = 16, kFinL = 16;
+
+__host__ __device__ inline int finish_tiles(const CallDims &d) {
+  return ((d.P + kFinC - 1) / kFinC) * ((d.L + 1 + kFinL - 1) / kFinL);
+}
+
+__global__ __launch_bounds__(kFinC * kFinL) void b_finish_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  __shared__ double sh[3][kFinC][kFinL + 1];
+  const long long tot = h->tot[W_FTILE];
+  const int ta = threadIdx.x % kFinL, tb = threadIdx.x / kFinL;
+  for (long long tile = blockIdx.x; tile < tot; tile += gridDim.x) {
+    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_FTILE, tile));
+    const ReqDesc &q = rq[r];
+    const int P = q.d.P, L = q.d.L;
+    const int t = (int)(tile - q.s[W_FTILE]);
+    const int ntc = (P + kFinC - 1) / kFinC;
+    const int c0 = (t % ntc) * kFinC, l0 = (t / ntc) * kFinL;
+    {
+      const int col = c0 + tb, lev = l0 + ta;   // levels fastest
+      if (col < P && lev <= L)
+        finish_body(q.d, q.partial, q.bandrec, q.bandscal, col, lev, sh[0][tb][ta], sh[1][tb][ta],
+                    sh[2][tb][ta]);
+    }
+    __syncthreads();
+    {
+      const int col = c0 + ta, lev = l0 + tb;   // profiles fastest
+      if (col < P && lev <= L) {
+        const size_t idx = (size_t)lev * P + col;
+        q.up[idx] = sh[0][ta][tb];
+        q.dn[idx] = sh[1][ta][tb];
+        if (q.dir) q.dir[idx] = sh[2][ta][tb];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void b_overlap_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_TOVL];
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
        g += (long long)gridDim.x * blockDim.x) {
-    const ReqDesc &q = rq[find_req(h, rq, W_LEV, g)];
-    finish_body(q.d, q.partial, q.bandrec, q.bandscal, q.up, q.dn, q.dir, (int)(g - q.s[W_LEV]));
+    const ReqDesc &q = rq[find_req(h, rq, W_TOVL, g)];
+    overlap_body(q.d, q.colrec, q.krange, q.ovl, (int)(g - q.s[W_TOVL]));
+  }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void b_tsolve_kernel(const BatchHdr *h, const ReqDesc *rq) {
+  const long long tot = h->tot[W_TSOLVE];
+  const int lane = threadIdx.x % kWave;
+  const long long nw = (long long)gridDim.x * (blockDim.x / kWave);
+  for (long long w = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; w < tot; w += nw) {
+    const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_TSOLVE, w));
+    const ReqDesc &q = rq[r];
+    tri_solve_body<MIXED>(q.d, q.spec, q.l_rayleigh, q.colrec, q.colscal, q.bandrec, q.bandscal, q.cldopt,
+                          q.ovl, q.krange, q.tws, q.partial, (int)(w - q.s[W_TSOLVE]), lane);
   }
 }
 
@@ -1302,8 +1911,14 @@ int socgpu_spectrum_register(const char *name, const char *path) {
   HIPCHK(hipMemcpy(e->dev, &e->host, sizeof(SgSpec), hipMemcpyHostToDevice));
   auto it = g_ctx.spectra.find(name);
   if (it != g_ctx.spectra.end()) {
+    // the replaced entry's tables; its condensate table may be shared with
+    // another spectrum (socgpu_mcica_register) and is freed with its last user
+    double *xcw = it->second->xcw;
     (void)hipFree(it->second->dev);
-    if (it->second->xcw) (void)hipFree(it->second->xcw);
+    it->second->xcw = nullptr;
+    bool shared = false;
+    for (auto &kv : g_ctx.spectra) shared |= kv.second->xcw == xcw;
+    if (xcw && !shared) (void)hipFree(xcw);
   }
   g_ctx.spectra[name] = std::move(e);
   return 0;
@@ -1330,16 +1945,30 @@ int socgpu_mcica_register(const char *path, const char *sw_name, const char *lw_
   std::string err;
   if (read_mcica(path, &m, &err)) return fail(SOCGPU_EIO, "%s: %s", path, err.c_str());
   const char *names[2] = {sw_name, lw_name};
+  for (int k = 0; k < 2; k++)
+    if (names[k] && g_ctx.spectra.find(names[k]) == g_ctx.spectra.end())
+      return fail(SOCGPU_ENOTFOUND, "spectrum '%s' not registered", names[k]);
+  // one device copy of the condensate table, shared by the spectra it is
+  // associated with (the cloud generator keeps one table per block in LDS)
+  double *xcw = nullptr;
+  HIPCHK(hipMalloc(&xcw, m.xcw.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(xcw, m.xcw.data(), m.xcw.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double *> old;
   for (int k = 0; k < 2; k++) {
     if (!names[k]) continue;
-    auto it = g_ctx.spectra.find(names[k]);
-    if (it == g_ctx.spectra.end()) return fail(SOCGPU_ENOTFOUND, "spectrum '%s' not registered", names[k]);
-    SpecEntry &e = *it->second;
+    SpecEntry &e = *g_ctx.spectra.find(names[k])->second;
     if (attach_mcica(&e.host, m, k == 0, &err)) return fail(SOCGPU_EUNSUPPORTED, "%s", err.c_str());
     HIPCHK(hipMemcpy(e.dev, &e.host, sizeof(SgSpec), hipMemcpyHostToDevice));
-    if (e.xcw) (void)hipFree(e.xcw);
-    HIPCHK(hipMalloc(&e.xcw, m.xcw.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(e.xcw, m.xcw.data(), m.xcw.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (e.xcw) old.push_back(e.xcw);
+    e.xcw = xcw;
+  }
+  for (double *p : old) {
+    bool used = false;
+    for (auto &kv : g_ctx.spectra) used |= kv.second->xcw == p;
+    if (!used) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
   }
   return 0;
 }
@@ -1432,7 +2061,24 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   std::memset(&d, 0, sizeof d);
   d.P = P; d.L = L; d.NB = NB; d.isolir = a->i_source;
   int cloud = a->i_cloud_representation;
-  if (cloud != SOCGPU_CLOUD_OFF) {
+  auto check_drop = [&](int t, const char *what) -> int {
+    if (t <= 0) return fail(SOCGPU_EUNSUPPORTED, "Prescribed liquid cloud not yet implemented.");
+    if (t > SG_NDROP || !S.drop[t - 1].present) return fail(SOCGPU_EINVAL, "%s cloud type not in spectral file.", what);
+    if (S.drop[t - 1].scheme != 5 && S.drop[t - 1].scheme != 1) return fail(SOCGPU_EUNSUPPORTED, "droplet scheme %d", S.drop[t - 1].scheme);
+    return 0;
+  };
+  if (cloud == SOCGPU_CLOUD_CONV_STRAT) {
+    // stratiform and convective cloud: triple-column regions; McICA does not
+    // apply to this representation (socrates_set_control.F90:338-356)
+    if (a->i_overlap != SOCGPU_OVERLAP_MAX_RAND)
+      return fail(SOCGPU_EUNSUPPORTED, "overlap %d with convective and stratiform cloud is not supported", a->i_overlap);
+    if (!a->cloud_frac || !a->conv_frac) return fail(SOCGPU_EINVAL, "Cloud fraction not provided.");
+    if (int rc = check_drop(a->i_st_water, "Liquid")) return rc;
+    if (int rc = check_drop(a->i_cnv_water, "Convective liquid")) return rc;
+    d.triple = 1;
+    d.drop = a->i_st_water - 1;
+    d.drop2 = a->i_cnv_water - 1;
+  } else if (cloud != SOCGPU_CLOUD_OFF) {
     if (cloud != SOCGPU_CLOUD_HOMOGEN || a->i_inhom != SOCGPU_INHOM_MCICA)
       return fail(SOCGPU_EUNSUPPORTED, "cloud representation %d with i_inhom %d is not on the McICA path", cloud, a->i_inhom);
     if (!S.has_mcica) return fail(SOCGPU_ENOTFOUND, "Spectrum name not associated with MCICA data.");
@@ -1443,8 +2089,7 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     if (!a->cloud_frac || !a->liq_mmr || !a->liq_dim) return fail(SOCGPU_EINVAL, "Cloud fraction not provided.");
     if (!a->rand_seed) return fail(SOCGPU_EINVAL, "rand_seed is required: default seeds depend on the position in the call");
     int drop = a->i_st_water;
-    if (drop < 1 || drop > SG_NDROP || !S.drop[drop - 1].present) return fail(SOCGPU_EINVAL, "Liquid cloud type not in spectral file.");
-    if (S.drop[drop - 1].scheme != 5 && S.drop[drop - 1].scheme != 1) return fail(SOCGPU_EUNSUPPORTED, "droplet scheme %d", S.drop[drop - 1].scheme);
+    if (int rc = check_drop(drop, "Liquid")) return rc;
     d.mcica = 1;
     d.mode = a->i_mcica_sampling == SOCGPU_MCICA_OPTIMAL ? 1 : 0;
     if (S.n_req[d.mode] <= 0) return fail(SOCGPU_EUNSUPPORTED, "McICA data has no tables for this sampling");
@@ -1457,9 +2102,17 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   }
   const int items = S.n_g + (d.mcica ? S.n_sub_items[d.mode] : 0);
   d.nwpc = (items + kWave - 1) / kWave;
+  d.n_items = items > kMinItems ? items : kMinItems;
+  d.n_waves = (int)(((long long)P * d.n_items + kWave - 1) / kWave);
   *entry = &e;
   *dout = d;
   return 0;
+}
+
+// Per-wave partial sums of a request's solve.
+static size_t partial_doubles(const CallDims &d) {
+  if (d.triple) return (size_t)d.P * d.nwpc * 3 * (d.L + 1);
+  return (size_t)d.n_waves * kMaxSeg * 3 * (d.L + 1);
 }
 
 // Device workspace of one request, in doubles (8-byte units), and its work
@@ -1467,8 +2120,9 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
 static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
   const size_t P = d.P, L = d.L, NB = d.NB;
   size_t n = P * L * SG_NQ + P * (L + 1) + P * SG_NS + P * NB * L * SG_NBQ + P * NB * SG_NBQ +
-             P * (S.n_minor > 0 ? S.n_minor : 0) * L * 2 + P * d.nwpc * 3 * (L + 1) + 2;
+             P * (S.n_minor > 0 ? S.n_minor : 0) * L * 2 + partial_doubles(d) + 2;
   if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * 4;
+  if (d.triple) n += P * NB * L * 8 + P * (L + 1) * kOv + P * d.nwpc * L * kTriQ * kWave;
   return n;
 }
 
@@ -1479,7 +2133,12 @@ static void request_work(const CallDims &d, const SgSpec &S, long long w[kNWork]
   w[W_CG] = d.mcica ? P : 0;
   w[W_MINOR] = P * S.n_minor;
   w[W_PREP] = P * d.NB * L;
-  w[W_SOLVE] = P * d.nwpc;
+  w[W_SOLVE] = d.triple ? 0 : d.n_waves;
+  w[W_TOVL] = d.triple ? P * (L + 1) : 0;
+  w[W_TSOLVE] = d.triple ? P * d.nwpc : 0;
+  w[W_PTILE] = pack_tiles(d);
+  w[W_CCOL] = (d.mcica || d.triple) ? P : 0;
+  w[W_FTILE] = finish_tiles(d);
 }
 
 // Lays out the workspace of request r at *cursor (doubles, 32-byte aligned)
@@ -1498,14 +2157,16 @@ static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cur
   q.bandrec = take(P * NB * L * SG_NBQ);
   q.bandscal = take(P * NB * SG_NBQ);
   q.minor = S.n_minor > 0 ? take(P * S.n_minor * L * 2) : nullptr;
-  q.partial = take(P * d.nwpc * 3 * (L + 1));
+  q.partial = take(partial_doubles(d));
   q.krange = (int *)take(2);
   q.csub = d.mcica ? take(P * L * SG_MAXSUBCOL) : nullptr;
-  q.cldopt = d.mcica ? take(P * NB * L * 4) : nullptr;
+  q.cldopt = d.mcica ? take(P * NB * L * 4) : (d.triple ? take(P * NB * L * 8) : nullptr);
+  q.ovl = d.triple ? take(P * (L + 1) * kOv) : nullptr;
+  q.tws = d.triple ? take((size_t)P * d.nwpc * L * kTriQ * kWave) : nullptr;
 }
 
 static size_t padded_ws_doubles(const CallDims &d, const SgSpec &S) {
-  return request_ws_doubles(d, S) + 10 * 4;  // per-array alignment padding
+  return request_ws_doubles(d, S) + 12 * 4;  // per-array alignment padding
 }
 
 static void fill_desc(ReqDesc &q, const CallDims &d, const SpecEntry &e, const SocgpuRunesArgs &dv,
@@ -1534,14 +2195,13 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
     return (int)(b < 1 ? 1 : (b > 1048576 ? 1048576 : b));
   };
   b_init_kernel<<<(R + 63) / 64, 64, 0, s>>>(hdr, rq);
-  b_pack_kernel<<<blocks(tot[W_LEV], T), T, 0, s>>>(hdr, rq);
-  if (tot[W_CG] > 0) {
-    b_cldrange_kernel<<<blocks(tot[W_LAY], T), T, 0, s>>>(hdr, rq);
-    b_cloudgen_kernel<<<blocks(tot[W_CG], 1), 64, 0, s>>>(hdr, rq);
-  }
+  b_pack_kernel<<<blocks(tot[W_PTILE], 1), kPackC * kPackL, 0, s>>>(hdr, rq);
+  if (tot[W_CCOL] > 0) b_cldrange_kernel<<<blocks(tot[W_CCOL], T), T, 0, s>>>(hdr, rq);
+  if (tot[W_CG] > 0) b_cloudgen_kernel<<<blocks(tot[W_CG], kCgWaves), 64 * kCgWaves, 0, s>>>(hdr, rq);
   if (tot[W_MINOR] > 0)
     b_minor_kernel<<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
   b_prep_kernel<<<blocks(tot[W_PREP], T), T, 0, s>>>(hdr, rq);
+  if (tot[W_TOVL] > 0) b_overlap_kernel<<<blocks(tot[W_TOVL], T), T, 0, s>>>(hdr, rq);
   if (ev_a) HIPCHK(hipEventRecord(ev_a, s));
   const SocgpuTile &tile = g_tile[precision == SOCGPU_MIXED ? 1 : 0];
   const int wpb = tile.waves_per_block;
@@ -1550,9 +2210,13 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
                                      : (tile.waves_per_simd == 3 ? b_solve_kernel<true, false, 3> : b_solve_kernel<true, false, 4>))
                    : (tile.recompute ? (tile.waves_per_simd == 3 ? b_solve_kernel<false, true, 3> : b_solve_kernel<false, true, 4>)
                                      : (tile.waves_per_simd == 3 ? b_solve_kernel<false, false, 3> : b_solve_kernel<false, false, 4>));
-  kern<<<blocks(tot[W_SOLVE], wpb), wpb * kWave, 0, s>>>(hdr, rq);
+  if (tot[W_SOLVE] > 0) kern<<<blocks(tot[W_SOLVE], wpb), wpb * kWave, 0, s>>>(hdr, rq);
+  if (tot[W_TSOLVE] > 0) {
+    auto *tk = precision == SOCGPU_MIXED ? b_tsolve_kernel<true> : b_tsolve_kernel<false>;
+    tk<<<blocks(tot[W_TSOLVE], 4), 4 * kWave, 0, s>>>(hdr, rq);
+  }
   if (ev_b) HIPCHK(hipEventRecord(ev_b, s));
-  b_finish_kernel<<<blocks(tot[W_LEV], T), T, 0, s>>>(hdr, rq);
+  b_finish_kernel<<<blocks(tot[W_FTILE], 1), kFinC * kFinL, 0, s>>>(hdr, rq);
   b_heating_kernel<<<blocks(tot[W_LAY], T), T, 0, s>>>(hdr, rq);
   HIPCHK(hipGetLastError());
   return 0;
@@ -1669,7 +2333,7 @@ static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolve
   std::memset(&pl.h, 0, sizeof pl.h);
   pl.h.R = n;
   size_t ws_cur = 0, in_off = 0, out_cur = 0;
-  long long start[kNWork] = {0, 0, 0, 0, 0, 0};
+  long long start[kNWork] = {};
   for (int j = 0; j < n; j++) {
     const int i = order[j];
     const CallDims &d = res[i].d;
@@ -1729,7 +2393,7 @@ struct SocgpuBatch {
   int precision = SOCGPU_FP64;
   SocgpuTile tile{};
   size_t cap_ws = 0, cap_in = 0, cap_out = 0, cap_desc = 0;
-  long long cap[sg::kNWork] = {0, 0, 0, 0, 0, 0};
+  long long cap[sg::kNWork] = {};
   sg::DevBuf ws, in_stage, out_stage, desc;
   char *pinned = nullptr;
   hipStream_t stream = nullptr;
@@ -2324,20 +2988,22 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
   CallDims d;
   std::memset(&d, 0, sizeof d);
   d.P = P; d.L = L; d.n_gen = n_subcol; d.n_fill = n_req < n_subcol ? n_req : n_subcol;
-  d.ipph = ipph; d.ioverlap = ioverlap; d.mcica = 1;
-  DevBuf drec, dseed, dxcw, dcs, dscal, dk, ddesc;
+  d.ipph = ipph; d.ioverlap = ioverlap; d.mcica = 1; d.raw_range = 1;
+  DevBuf drec, dseed, dxcw, dcs, dscal, dk, ddesc, dcf;
   struct {
     BatchHdr h;
     ReqDesc q;
   } host;
   if (drec.ensure(rec.size() * 8) || dseed.ensure((size_t)P * 4) || dxcw.ensure((size_t)n1 * n2 * 8) ||
       dcs.ensure((size_t)P * L * SG_MAXSUBCOL * 8) || dscal.ensure((size_t)P * SG_NS * 8) || dk.ensure(8) ||
-      ddesc.ensure(sizeof host))
+      ddesc.ensure(sizeof host) || dcf.ensure((size_t)P * L * 8))
     return fail(SOCGPU_ENOMEM, "allocation failed");
   std::memset(&host, 0, sizeof host);
   host.h.R = 1;
   host.h.tot[W_LAY] = (long long)P * L;
   host.h.tot[W_CG] = P;
+  host.h.tot[W_CCOL] = P;
+  host.q.a.cloud_frac = dcf.as<double>();
   host.q.d = d;
   host.q.colrec = drec.as<double>();
   host.q.a.rand_seed = dseed.as<int>();
@@ -2352,13 +3018,14 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
   hipStream_t s = g_ctx.stream;
   HIPCHK(hipMemcpyAsync(drec.p, rec.data(), rec.size() * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dseed.p, rand_seed, (size_t)P * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dcf.p, avg_cf, (size_t)P * L * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dxcw.p, xcw, (size_t)n1 * n2 * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(ddesc.p, &host, sizeof host, hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)ddesc.p;
   ReqDesc *dq = (ReqDesc *)((char *)ddesc.p + offsetof(decltype(host), q));
   b_init_kernel<<<1, 64, 0, s>>>(dh, dq);
-  b_cldrange_kernel<<<(P * L + 255) / 256, 256, 0, s>>>(dh, dq);
-  b_cloudgen_kernel<<<P, 64, 0, s>>>(dh, dq);
+  b_cldrange_kernel<<<(P + 255) / 256, 256, 0, s>>>(dh, dq);
+  b_cloudgen_kernel<<<(P + kCgWaves - 1) / kCgWaves, 64 * kCgWaves, 0, s>>>(dh, dq);
   HIPCHK(hipGetLastError());
   std::vector<double> cs((size_t)P * L * SG_MAXSUBCOL), sc((size_t)P * SG_NS);
   HIPCHK(hipMemcpyAsync(cs.data(), dcs.p, cs.size() * 8, hipMemcpyDeviceToHost, s));
@@ -2370,7 +3037,7 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
       for (int j = 0; j < n_subcol; j++)
         c_sub[((size_t)j * L + i) * P + l] = cs[((size_t)l * L + i) * SG_MAXSUBCOL + j];
   }
-  for (DevBuf *b : {&drec, &dseed, &dxcw, &dcs, &dscal, &dk, &ddesc}) if (b->p) (void)hipFree(b->p);
+  for (DevBuf *b : {&drec, &dseed, &dxcw, &dcs, &dscal, &dk, &ddesc, &dcf}) if (b->p) (void)hipFree(b->p);
   return 0;
 }
 
@@ -2378,10 +3045,13 @@ int socgpu_finalize(void) {
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   if (g_ctx.device < 0) return 0;
   (void)hipStreamSynchronize(g_ctx.stream);
+  std::vector<double *> xcws;
   for (auto &kv : g_ctx.spectra) {
     (void)hipFree(kv.second->dev);
-    if (kv.second->xcw) (void)hipFree(kv.second->xcw);
+    if (kv.second->xcw && std::find(xcws.begin(), xcws.end(), kv.second->xcw) == xcws.end())
+      xcws.push_back(kv.second->xcw);
   }
+  for (double *p : xcws) (void)hipFree(p);
   g_ctx.spectra.clear();
   for (DevBuf *b : {&g_ctx.ws, &g_ctx.desc, &g_ctx.in_stage, &g_ctx.out_stage}) {
     if (b->p) (void)hipFree(b->p);

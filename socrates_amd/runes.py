@@ -308,8 +308,20 @@ class Request(ctypes.Structure):
                 ("diag", ctypes.POINTER(Diag))]
 
 
+class Requests:
+    """A list of radiation calls marshalled once into the C request array
+    (SocgpuRequest[]) with its outputs, for callers that run the same calls
+    (the same input and output buffers) every time step."""
+
+    def __init__(self, calls):
+        self.arr, self.outs, self._keep = _requests(calls)
+        self.n = len(calls)
+
+
 def _requests(calls):
     """calls: list of (spectrum_name, kwargs of runes()) -> ctypes array, outputs, keep-alive."""
+    if isinstance(calls, Requests):
+        return calls.arr, calls.outs, calls._keep
     keep, outs = [], []
     arr = (Request * len(calls))()
     for k, (name, kw) in enumerate(calls):
@@ -331,7 +343,7 @@ def runes_batch(calls, precision="fp64"):
     including n_profile, n_layer and i_source). Returns the list of output
     dicts, in order."""
     arr, outs, keep = _requests(calls)
-    _check(lib().socgpu_runes_batch(len(calls), arr, _PREC[precision]))
+    _check(lib().socgpu_runes_batch(len(outs), arr, _PREC[precision]))
     return outs
 
 
@@ -343,6 +355,7 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         return src.get(name, getattr(case, attr or name))
 
     cloudy = bool(case.l_cloud)
+    conv = cloudy and case.i_cloud_repr == 3   # stratiform + convective (triple column)
     kw = dict(
         n_profile=case.n_profile, n_layer=case.n_layer, i_source=case.i_source,
         p_layer=g("p_layer"), t_layer=g("t_layer"), mass=g("mass"), density=g("density"),
@@ -353,6 +366,9 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         l_rayleigh=case.l_rayleigh,
         cloud_frac=g("cloud_frac") if cloudy else None,
         liq_mmr=g("liq_mmr") if cloudy else None, liq_dim=g("liq_dim") if cloudy else None,
+        conv_frac=g("conv_frac") if conv else None,
+        liq_conv_mmr=g("liq_conv_mmr") if conv else None,
+        liq_conv_dim=g("liq_conv_dim") if conv else None,
         cloud_vertical_decorr=case.cloud_vertical_decorr,
         conv_vertical_decorr=case.conv_vertical_decorr,
         cloud_horizontal_rsd=case.cloud_horizontal_rsd,
@@ -373,8 +389,9 @@ class Batch:
         _check(lib().socgpu_batch_create(ctypes.byref(self._h), _PREC[precision]))
 
     def run(self, calls):
+        """calls: a list of (spectrum_name, kwargs) or a Requests built once."""
         arr, outs, keep = _requests(calls)
-        _check(lib().socgpu_batch_run(self._h, len(calls), arr))
+        _check(lib().socgpu_batch_run(self._h, len(outs), arr))
         return outs
 
     def stats(self):
@@ -400,6 +417,6 @@ def autotune(calls, precision="fp64", repeats=2):
     (socgpu_autotune); it stays in effect for the precision form."""
     arr, outs, keep = _requests(calls)
     t = Tile()
-    _check(lib().socgpu_autotune(len(calls), arr, _PREC[precision], int(repeats), ctypes.byref(t)))
+    _check(lib().socgpu_autotune(len(outs), arr, _PREC[precision], int(repeats), ctypes.byref(t)))
     return dict(recompute=t.recompute, waves_per_block=t.waves_per_block,
                 waves_per_simd=t.waves_per_simd)

@@ -77,3 +77,16 @@ def test_batch_rejects_bad_request():
     kw["n_layer"] = 500
     with pytest.raises(R.SocgpuError):
         R.runes_batch([(name, kw)])
+
+
+def test_reregistering_spectra_keeps_results():
+    """Registering the spectra and McICA data again (the condensate table is
+    shared by the spectra it is associated with, and freed with its last
+    user) leaves the library usable and the results unchanged."""
+    c = synth.make_case(9, 44, "sw", NBAND["sw"], seed=12, cloudy=True)
+    before = R.runes_case(c, "sw", precision="fp64")
+    for _ in range(3):
+        R.register_default_spectra()
+    after = R.runes_case(c, "sw", precision="fp64")
+    for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+        assert np.array_equal(np.asarray(before[k]), np.asarray(after[k])), k
