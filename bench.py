@@ -41,18 +41,22 @@ sys.path.insert(0, ROOT)
 
 P, L = 4096, 90
 G = {"sw": 40, "lw": 50}
-# algorithmic operations per layer of one (column, g-point, sub-column) solve,
-# counted from the reference formulas (exp, sqrt and div count as one op)
+# sampled McICA sub-columns per column (optimal sampling of the McICA data
+# file: subcol_need_optimal = 52, one per (band, k-term) item)
+NSUB = {"sw": 52, "lw": 52}
+# algorithmic operations per layer of one two-stream profile solve (a clear
+# k-term or one sampled sub-column), counted from the reference formulas
+# (exp, sqrt and div count as one op)
 OPS_PER_LAYER = {"sw": 94.0, "lw": 79.0}
 # compulsory HBM bytes per (column, g-point) solve: FP64 column inputs read
 # once and outputs written once, divided by the g-points of the file
 BYTES_PER_SOLVE = {"sw": 235.0, "lw": 188.0}
 PEAK = {"fp64": 78.6, "mixed": 157.3}      # TFLOP/s vector
 HBM_PEAK = 8000.0                          # GB/s
-# HBM bytes per step moved by the solve kernel (both calls), from rocprofv3
-# FETCH_SIZE (x2, the factor a plain copy of known size reads) and
-# WRITE_SIZE (x1) runs; profiles/c2_4096x90_traffic.md
-TRAFFIC = {}
+# HBM bytes per step moved by the solve kernel (one launch covers both
+# calls), from rocprofv3 FETCH_SIZE (x2, the factor a plain 512 MiB copy
+# reads) and WRITE_SIZE (x1) runs of this script; profiles/c2_solve_traffic.md
+TRAFFIC = {"mixed": 2.573e9, "fp64": 4.317e9}
 
 
 def make_inputs(torch, dev):
@@ -234,8 +238,11 @@ def main():
     ms_per_step = main_r["dt"] / a.steps * 1e3
     solves_per_step = P * (G["sw"] + G["lw"])
     value = solves_per_step / (main_r["dt"] / a.steps)
-    # roofline of the solve kernel (SW and LW calls of one step together)
-    flops = sum(P * G[s] * L * OPS_PER_LAYER[s] for s in G)
+    # roofline of the solve kernel (SW and LW calls of one step together):
+    # every clear k-term and sampled sub-column profile of the reference's
+    # mcica_sample (columns entirely clear or overcast skip one of the two,
+    # here and in the reference, so this slightly overstates the work)
+    flops = sum(P * (G[s] + NSUB[s]) * L * OPS_PER_LAYER[s] for s in G)
     bytes_alg = sum(P * G[s] * BYTES_PER_SOLVE[s] for s in G)
     kt = main_r["solve_ms"] * 1e-3
     achieved = flops / kt / 1e12
@@ -251,9 +258,11 @@ def main():
                                "(9 bands, 50 g), McICA 64 sub-columns, exp-random overlap; "
                                "one step = the SW and LW calls as one graph-replayed batch",
                    "precision": a.precision, "tile": tiles[a.precision]},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[a.precision],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[a.precision],
                      "unit": "TFLOP/s", "frac": achieved / PEAK[a.precision],
                      "traffic": TRAFFIC.get(a.precision),
+                     "engine": "VALU: the path has no matrix-shaped work; the compute roof "
+                               "is the published FP64 (FP32 for the mixed form) rate",
                      "kernel": "b_solve_kernel", "kernel_ms_per_step": kt * 1e3,
                      "hbm_achieved_GBps": hbm, "hbm_frac": hbm / HBM_PEAK},
     }

@@ -127,6 +127,11 @@ SG_HD int lcg_step(int x) {
 
 }  // namespace sg
 
+// The mixed form's functions fuse a*b+c into FMAs within one expression
+// only (contract(on)): the backend then makes the same choice wherever the
+// code is inlined, so a layer recomputed by the recomputing solve tile gives
+// the bits of its first computation.
+//
 // FP32 transcendentals of the mixed form: single hardware instructions on
 // the device (v_exp_f32 after the scaling, v_rcp_f32 then a multiply, about
 // 1.5 ulp, v_sqrt_f32); the host build used by the CPU tests has the
@@ -154,6 +159,7 @@ struct CoefF {
 // where the subtraction loses under 2 bits. Both sides are evaluated and one
 // selected, so lanes of a wave do not diverge.
 SG_HD float one_minus_exp(float x, float e) {
+#pragma clang fp contract(on)
   float p = x * (1.0f - x * (0.5f - x * (0.16666667f - x * (0.041666668f - x * (0.008333334f -
             x * (0.0013888889f - x * 1.984127e-4f))))));
   return x < 0.2f ? p : 1.0f - e;
@@ -189,6 +195,7 @@ SG_HD float one_minus_exp(float x, float e) {
 // 1 / (lambda - mu) amplification, so FP32 needs no FP64 fallback.
 SG_HD void two_coeff_f32(bool solar, float tau, float om, float omc, float g, float sec0,
                          CoefF &c) {
+#pragma clang fp contract(on)
   float S, D, smd;  // smd = S - D, formed from omega without cancellation
   if (solar) {
     S = 2.0f - 1.5f * om * g - 0.5f * om;
@@ -269,6 +276,7 @@ namespace sg {
 
 // FP32 k-term (the mixed form's k-term exponent sum and exponential)
 SG_HD float k_term_f32(const SgTerm &t, float lnp, float lnt, float mmr) {
+#pragma clang fp contract(on)
   float s = SG_EXPF((float)t.a * (lnp - (float)t.lnpref) + (float)t.b * (lnt - (float)t.lntref));
   return (float)t.k * fmaxf(0.0f, s * mmr);
 }
@@ -286,6 +294,7 @@ namespace sg {
 // their flux divergence, and so their heating rates, and thick
 // near-conservative layers (R ~ 1) keep their transmission.
 SG_HD void coef_from_f32(const CoefF &f, Coef &o) {
+#pragma clang fp contract(on)
   if (f.T < f.R) {
     o.T = f.T;
     o.R = (1.0 - (double)f.T) - (double)f.u;

@@ -54,6 +54,8 @@ struct SgDrop {
 
 struct SgSpec {
   int n_band, n_g, n_minor, n_deg;
+  int n_mband;                   // bands with minor gases, and which
+  int mband[SG_MAXB];
   int n_absorb;                  // gaseous absorbers of the file and their
   int type_absorb[SG_MAXABS];    // identifiers (rad_pcf.F90 ip_h2o = 1, ...)
   double t_ref_planck;

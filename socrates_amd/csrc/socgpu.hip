@@ -139,6 +139,7 @@ struct CallDims {
   int n_gen, n_fill, ipph, ioverlap;
   int triple, drop2; // triple-column representation; convective droplet type
   int raw_range;     // cloud range on the fractions as given (kernel-level cloud_gen)
+  int f32;           // mixed form: McICA c_sub and droplet optics kept in FP32
 };
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
@@ -270,6 +271,53 @@ __device__ double zcw_lookup(double y, double sigma, int n1, int n2, const doubl
   return ((t1 + t2) + t3) + t4;
 }
 
+// One layer k (1-based) of one sub-column's draws (cloud_gen:253-405):
+// updates the x stream (cloud presence) and, with Y, the y stream
+// (condensate rank). The x stream never depends on the y stream.
+template <bool Y>
+__device__ __forceinline__ void cg_layer(int ioverlap, int k, const double *__restrict__ rec, double dp_cf,
+                                         double dp_cw, int &ix, int &iy, double &x, double &y) {
+#pragma clang fp contract(off)
+  const double rm = 1.0 / (double)kLcgM;
+  if (ioverlap == 0) {
+    // maximum-random (cloud_gen:253-298)
+    ix = lcg_step(ix); x = (double)ix * rm;
+    if (Y) { iy = lcg_step(iy); y = (double)iy * rm; }
+    if (k == 1) {
+      ix = lcg_step(ix); x = (double)ix * rm;
+      if (Y) { iy = lcg_step(iy); y = (double)iy * rm; }
+    } else {
+      double cfa = rec[(k - 2) * SG_NQ + SG_Q_WCLD];
+      if (x < (1.0 - cfa)) {
+        ix = lcg_step(ix); double x1 = (double)ix * rm;
+        x = x1 * (1.0 - cfa);
+        if (Y) { iy = lcg_step(iy); y = (double)iy * rm; }
+      }
+    }
+  } else {
+    // random / exponential-random (cloud_gen:339-405)
+    double alpha = 0.0, rcorr = 0.0;
+    if (ioverlap != 1 && k - 1 >= 1) {
+      double cfa = rec[(k - 2) * SG_NQ + SG_Q_WCLD];
+      if (cfa > 0.0) {
+        double dz = rec[(k - 2) * SG_NQ + SG_Q_P] - rec[(k - 1) * SG_NQ + SG_Q_P];
+        alpha = dp_cf > 0.0 ? exp(dz / dp_cf) : 0.0;
+        if (Y) rcorr = dp_cw > 0.0 ? exp(dz / dp_cw) : 0.0;
+      }
+    }
+    ix = lcg_step(ix); double x1 = (double)ix * rm;
+    if (x1 >= alpha) { ix = lcg_step(ix); x = (double)ix * rm; }
+    if (Y) {
+      iy = lcg_step(iy); double y1 = (double)iy * rm;
+      if (y1 >= rcorr) { iy = lcg_step(iy); y = (double)iy * rm; }
+    }
+  }
+}
+
+// Two passes over the layers: the x streams alone give each sub-column's
+// cloud presence and so the cloudy-first order with the cyclic fill; then
+// each lane replays, in full, the sub-column that lands in its slot and
+// writes its condensate scalings in place (no second pass over c_sub).
 __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *__restrict__ colrec,
                                               const int *__restrict__ seeds, double dp_cf,
                                               double dp_cw, const int *__restrict__ krange,
@@ -278,7 +326,6 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
                                               double *__restrict__ colscal, int col, int lane) {
 #pragma clang fp contract(off)
   const int L = d.L, n_gen = d.n_gen;
-  const double rm = 1.0 / (double)kLcgM;
   const int seed_max = (2147483647 - kLcgC) / kLcgA;
   double *cs = c_sub + (size_t)col * L * SG_MAXSUBCOL;
   const double *rec = colrec + (size_t)col * L * SG_NQ;
@@ -290,53 +337,20 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
   int sx = 0, sy = 0;
   for (int t = 0; t < n_gen; t++) { rs = lcg_step(rs); if (t == lane) sx = rs; }
   for (int t = 0; t < n_gen; t++) { rs = lcg_step(rs); if (t == lane) sy = rs; }
-  int ix = sx, iy = sy;
-  double x = 0.0, y = 0.0;
-  bool cloudy = false;
   const bool any = k_top <= L;
-  for (int k = 1; k <= L; k++) {
-    double val = 0.0;
-    double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
-    if (valid && any && k >= k_top && k <= k_base && cf > 0.0) {
-      if (d.ioverlap == 0) {
-        // maximum-random (cloud_gen:253-298)
-        ix = lcg_step(ix); x = (double)ix * rm;
-        iy = lcg_step(iy); y = (double)iy * rm;
-        if (k == 1) {
-          ix = lcg_step(ix); x = (double)ix * rm;
-          iy = lcg_step(iy); y = (double)iy * rm;
-        } else {
-          double cfa = rec[(k - 2) * SG_NQ + SG_Q_WCLD];
-          if (x < (1.0 - cfa)) {
-            ix = lcg_step(ix); double x1 = (double)ix * rm;
-            iy = lcg_step(iy); double y1 = (double)iy * rm;
-            x = x1 * (1.0 - cfa);
-            y = y1;
-          }
-        }
-      } else {
-        // random / exponential-random (cloud_gen:339-405)
-        double alpha = 0.0, rcorr = 0.0;
-        if (d.ioverlap != 1 && k - 1 >= 1) {
-          double cfa = rec[(k - 2) * SG_NQ + SG_Q_WCLD];
-          if (cfa > 0.0) {
-            double dz = rec[(k - 2) * SG_NQ + SG_Q_P] - rec[(k - 1) * SG_NQ + SG_Q_P];
-            alpha = dp_cf > 0.0 ? exp(dz / dp_cf) : 0.0;
-            rcorr = dp_cw > 0.0 ? exp(dz / dp_cw) : 0.0;
-          }
-        }
-        ix = lcg_step(ix); double x1 = (double)ix * rm;
-        iy = lcg_step(iy); double y1 = (double)iy * rm;
-        if (x1 >= alpha) { ix = lcg_step(ix); x = (double)ix * rm; }
-        if (y1 >= rcorr) { iy = lcg_step(iy); y = (double)iy * rm; }
-      }
-      if (x < cf) {
-        val = d.ipph == 0 ? 1.0
-                          : zcw_lookup(y, rec[(k - 1) * SG_NQ + SG_Q_RSD], n1, n2, xcw) * 1.0;
-        cloudy = true;
+  const int k0 = any ? k_top : 1, k1 = any ? k_base : 0;
+  // pass 1: cloud presence of this lane's sub-column
+  bool cloudy = false;
+  {
+    int ix = sx, iy = 0;
+    double x = 0.0, y = 0.0;
+    for (int k = k0; k <= k1; k++) {
+      const double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
+      if (valid && cf > 0.0) {
+        cg_layer<false>(d.ioverlap, k, rec, dp_cf, dp_cw, ix, iy, x, y);
+        if (x < cf) cloudy = true;
       }
     }
-    cs[(size_t)(k - 1) * SG_MAXSUBCOL + lane] = val;
   }
   // cloudy sub-columns first, then the cyclic fill up to n_fill
   unsigned long long mask = __ballot(cloudy);
@@ -348,10 +362,19 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
     for (int j = 0; j < want; j++) m &= m - 1;
     src = __ffsll((long long)m) - 1;
   }
-  for (int k = 0; k < L; k++) {
-    const double own = cs[(size_t)k * SG_MAXSUBCOL + lane];
-    const double v = __shfl(own, src >= 0 ? src : 0, kWave);
-    cs[(size_t)k * SG_MAXSUBCOL + lane] = src >= 0 ? v : 0.0;
+  // pass 2: the sub-column of slot `lane` (sub-column src), both streams
+  int ix = __shfl(sx, src >= 0 ? src : 0, kWave), iy = __shfl(sy, src >= 0 ? src : 0, kWave);
+  double x = 0.0, y = 0.0;
+  for (int k = 1; k <= L; k++) {
+    double val = 0.0;
+    const double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
+    if (src >= 0 && k >= k0 && k <= k1 && cf > 0.0) {
+      cg_layer<true>(d.ioverlap, k, rec, dp_cf, dp_cw, ix, iy, x, y);
+      if (x < cf)
+        val = d.ipph == 0 ? 1.0 : zcw_lookup(y, rec[(k - 1) * SG_NQ + SG_Q_RSD], n1, n2, xcw) * 1.0;
+    }
+    if (d.f32) ((float *)c_sub + (size_t)col * L * SG_MAXSUBCOL)[(size_t)(k - 1) * SG_MAXSUBCOL + lane] = (float)val;
+    else cs[(size_t)(k - 1) * SG_MAXSUBCOL + lane] = val;
   }
   if (lane == 0) {
     colscal[(size_t)col * SG_NS + SG_S_NCLD] = (double)ncld;
@@ -382,109 +405,207 @@ __device__ __forceinline__ void scan_affine(double &A, double &C, int lane) {
   }
 }
 
-// Minor-gas weights for the equivalent extinction, one wave per (column,
-// minor k-term), lanes over layers (64 at a time with a carry). Solar: the
-// direct-beam transmission of the term down to each layer
-// (solve_band_k_eqv_scl.F90:654-739). Infra-red: the non-scattering fluxes
-// of monochromatic_gas_flux (monochromatic_gas_flux.F90:115-196,
-// diffusivity 1.66) as two affine recurrences, solved by wave scans.
-// minor[col][term][layer][2]: solar (fdt, k fdt(surface)), infra-red
-// (k |F| sum, |F| sum) with the term's weight. The scans associate the
-// products differently from the reference's left-to-right sweep (relative
-// differences of order 1e-16 in the weights).
+// Equivalent extinction of a band's minor gases (solve_band_k_eqv_scl.F90:
+// 572-1047), one wave per (column, band with minor gases), lanes over
+// layers (64 at a time, at most kKeqChunks chunks, with carries between
+// them), the band's gases and their k-terms in turn:
+//  - solar: each term's direct-beam transmission down the column (a wave
+//    product scan, :654-739); k_eqv = sum_k k T_k(surface) / sum_k
+//    T_k(surface) per gas (:826-846), and adjust_solar_ke from the per-layer
+//    ratios of the summed transmissions (:848-891);
+//  - infra-red: each term's non-scattering fluxes (monochromatic_gas_flux.F90:
+//    115-196, diffusivity 1.66) as two affine recurrences solved by wave
+//    scans; k_eqv = sum k |F| / sum |F| over the layer's two levels
+//    (:995-1019);
+//  - where the weighting flux vanishes, the smallest k of the gas's terms.
+// Per lane, the layer records and (infra-red) the band's Planck fluxes at
+// the layer's levels are read or evaluated once for all terms.
+// keqv[col][band][layer][2] = (sum over gases of k_eqv, adjust_solar_ke).
+// The scans associate the products differently from the reference's
+// left-to-right sweeps (relative differences of order 1e-16).
+// NCH: chunks of 64 layers the lane arrays hold (kernels for NCH = 2 and 3
+// each take the calls with that many chunks, so a call of at most 128
+// layers runs with the smaller register footprint).
 constexpr int kMinorWaves = 4;
+constexpr int kKeqMaxChunks = (SG_MAXL + kWave - 1) / kWave;
 
-__device__ __forceinline__ void minor_body(const CallDims &d, const SgSpec *__restrict__ spec,
-                                           const double *__restrict__ colrec,
-                                           const double *__restrict__ tlev,
-                                           const double *__restrict__ colscal,
-                                           const double *__restrict__ bandscal,
-                                           double *__restrict__ minor, double (*sh)[SG_MAXL],
-                                           int wave, int lane) {
-  const int NT = spec->n_minor, L = d.L;
-  if (wave >= d.P * NT) return;
-  const int col = wave / NT, ti = wave % NT;
-  const SgTerm &t = spec->minor[ti];
-  const SgBand &B = spec->band[t.band];
+template <int kKeqChunks>
+__device__ __forceinline__ void keqv_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                          const double *__restrict__ colrec,
+                                          const double *__restrict__ tlev,
+                                          const double *__restrict__ colscal,
+                                          const double *__restrict__ bandscal,
+                                          double *__restrict__ keqv, int wave, int lane) {
+  const int L = d.L, NMB = spec->n_mband;
+  if (wave >= d.P * NMB) return;
+  const int col = wave / NMB, b = spec->mband[wave % NMB];
+  const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
-  double *o = minor + ((size_t)col * NT + ti) * L * 2;
-  double *kk = sh[0];
   const int nch = (L + kWave - 1) / kWave;
-  if (d.isolir == SOCGPU_SOURCE_SOLAR) {
-    const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
-    double fdt = t.w;
-    for (int ch = 0; ch < nch; ch++) {
-      const int i = ch * kWave + lane;
-      double x = 1.0;
-      if (i < L) {
-        const double *r = rec + i * SG_NQ;
-        double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-        kk[i] = k;
-        x = exp(-k * r[SG_Q_MASS] * sec0);
+  const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
+  double lnp[kKeqChunks], lnt[kKeqChunks], mass[kKeqChunks], b0[kKeqChunks], b1[kKeqChunks];
+  double keq[kKeqChunks], adj[kKeqChunks];
+  const double *tl = tlev + (size_t)col * (L + 1);
+#pragma unroll
+  for (int c = 0; c < kKeqChunks; c++) {
+    const int i = c * kWave + lane;
+    lnp[c] = 0.0; lnt[c] = 0.0; mass[c] = 0.0; b0[c] = 0.0; b1[c] = 0.0;
+    keq[c] = 0.0; adj[c] = 1.0;
+    if (c < nch && i < L) {
+      const double *r = rec + (size_t)i * SG_NQ;
+      lnp[c] = r[SG_Q_LNP]; lnt[c] = r[SG_Q_LNT]; mass[c] = r[SG_Q_MASS];
+      if (!solar) {
+        b0[c] = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i]);
+        b1[c] = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
       }
-      double f = fdt * scan_mul(x, lane);
-      if (i < L) o[2 * i] = f;
-      fdt = __shfl(f, kWave - 1, kWave);
     }
-    __syncwarp();
-    for (int i = lane; i < L; i += kWave) o[2 * i + 1] = kk[i] * fdt;
-  } else {
-    double *tr = sh[1], *su = sh[2], *fdl = sh[3];
-    const double *tl = tlev + (size_t)col * (L + 1);
-    const double alb = bandscal[((size_t)col * d.NB + t.band) * SG_NBQ + 2];
-    const double sqe = 1.4901161193847656e-08;
-    const double b_top = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
-    double F = -b_top;  // differential downward flux at the top of the chunk
-    for (int ch = 0; ch < nch; ch++) {
-      const int i = ch * kWave + lane;
-      double A = 1.0, C = 0.0;
-      if (i < L) {
-        const double *r = rec + i * SG_NQ;
-        double k = k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
-        double tau = k * r[SG_Q_MASS];
-        double b0 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i]);
-        double b1 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
-        double trans = exp(-kDiffMinor * tau);
-        double s = (1.0 - trans + sqe) * (b1 - b0) / (kDiffMinor * tau + sqe);
-        kk[i] = k; tr[i] = trans; su[i] = s;
-        A = trans;
-        C = -s;
-      }
-      scan_affine(A, C, lane);
-      double below = A * F + C;                       // F at the bottom of layer i
-      double above = __shfl_up(below, 1, kWave);      // F at its top
-      if (lane == 0) above = F;
-      if (i < L) fdl[i] = above;
-      F = __shfl(below, kWave - 1, kWave);
+  }
+  const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
+  const double alb = bandscal[((size_t)col * d.NB + b) * SG_NBQ + 2];
+  double b_top = 0.0, b_bot = 0.0, bg = 0.0;
+  if (!solar) {
+    b_top = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
+    b_bot = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[L]);
+    bg = planck_band(B, spec->n_deg, spec->t_ref_planck, colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
+  }
+  const double sqe = 1.4901161193847656e-08;
+  int t0 = B.minor0;
+  for (int j = 0; j < B.n_minor_gas; j++) {
+    const int nk = B.minor_nk[j];
+    const int gas = spec->minor[t0].gas;
+    double mmr[kKeqChunks], num[kKeqChunks], den[kKeqChunks], kmin[kKeqChunks];
+#pragma unroll
+    for (int c = 0; c < kKeqChunks; c++) {
+      const int i = c * kWave + lane;
+      mmr[c] = (c < nch && i < L) ? rec[(size_t)i * SG_NQ + SG_Q_MMR + gas] : 0.0;
+      num[c] = 0.0; den[c] = 0.0; kmin[c] = DBL_MAX;
     }
-    const double b_bot = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[L]);
-    const double bg = planck_band(B, spec->n_deg, spec->t_ref_planck,
-                                  colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
-    double U = (bg - b_bot) + alb * F;                // upward flux at the surface
-    __syncwarp();
-    for (int ch = 0; ch < nch; ch++) {
-      const int i = L - 1 - (ch * kWave + lane);      // bottom-up
-      double A = 1.0, C = 0.0;
-      if (i >= 0) { A = tr[i]; C = su[i]; }
-      scan_affine(A, C, lane);
-      double above = A * U + C;                       // F up at the top of layer i
-      double below = __shfl_up(above, 1, kWave);      // at its bottom
-      if (lane == 0) below = U;
-      if (i >= 0) {
-        double a = t.w * fabs(fdl[i]), bb = t.w * fabs(below);
-        o[2 * i] = kk[i] * a + kk[i] * bb;
-        o[2 * i + 1] = a + bb;
+    double sflux = 0.0;
+    for (int q = 0; q < nk; q++) {
+      const SgTerm &t = spec->minor[t0 + q];
+      double kk[kKeqChunks];
+#pragma unroll
+      for (int c = 0; c < kKeqChunks; c++) {
+        kk[c] = (c < nch && c * kWave + lane < L) ? k_term(t, lnp[c], lnt[c], mmr[c]) : 0.0;
+        kmin[c] = fmin(kmin[c], kk[c]);
       }
-      U = __shfl(above, kWave - 1, kWave);
+      if (solar) {
+        // direct-beam transmission: f = w prod exp(-k m sec0) down to the
+        // bottom of each layer; den accumulates f (summed over terms, fg)
+        double fdt = t.w, f[kKeqChunks];
+#pragma unroll
+        for (int c = 0; c < kKeqChunks; c++) {
+          f[c] = 0.0;
+          if (c < nch) {
+            const int i = c * kWave + lane;
+            const double x = i < L ? exp(-kk[c] * mass[c] * sec0) : 1.0;
+            f[c] = fdt * scan_mul(x, lane);
+            fdt = __shfl(f[c], kWave - 1, kWave);
+          }
+        }
+        // lanes below layer L-1 carry the identity, so fdt is now the
+        // transmission to the surface (the value at layer L-1)
+        const double fs = fdt;
+#pragma unroll
+        for (int c = 0; c < kKeqChunks; c++) {
+          num[c] = num[c] + kk[c] * fs;
+          den[c] = den[c] + f[c];
+        }
+        sflux = sflux + fs;
+      } else {
+        // downward differential flux F at the top of each layer
+        double F = -b_top, tr[kKeqChunks], su[kKeqChunks], fdl[kKeqChunks];
+#pragma unroll
+        for (int c = 0; c < kKeqChunks; c++) {
+          tr[c] = 1.0; su[c] = 0.0; fdl[c] = 0.0;
+          if (c < nch) {
+            const int i = c * kWave + lane;
+            double A = 1.0, C = 0.0;
+            if (i < L) {
+              const double tau = kk[c] * mass[c];
+              tr[c] = exp(-kDiffMinor * tau);
+              su[c] = (1.0 - tr[c] + sqe) * (b1[c] - b0[c]) / (kDiffMinor * tau + sqe);
+              A = tr[c];
+              C = -su[c];
+            }
+            scan_affine(A, C, lane);
+            const double below = A * F + C;
+            double above = __shfl_up(below, 1, kWave);
+            if (lane == 0) above = F;
+            fdl[c] = above;
+            F = __shfl(below, kWave - 1, kWave);
+          }
+        }
+        // lanes below layer L-1 carry the identity, so F is the downward
+        // flux at the surface
+        double U = (bg - b_bot) + alb * F;   // upward flux at the surface
+#pragma unroll
+        for (int cc = kKeqChunks - 1; cc >= 0; cc--) {
+          if (cc < nch) {
+            // bottom-up over this chunk: lane with layer i = cc*64 + lane,
+            // scanned in reverse lane order
+            const int rl = kWave - 1 - lane;     // reverse position
+            const int i = cc * kWave + rl;
+            double A = 1.0, C = 0.0;
+            const double trr = __shfl(tr[cc], rl, kWave), sur = __shfl(su[cc], rl, kWave);
+            if (i < L) { A = trr; C = sur; }
+            scan_affine(A, C, lane);
+            const double above = A * U + C;                  // F up at the top of layer i
+            double belowu = __shfl_up(above, 1, kWave);      // at its bottom
+            if (lane == 0) belowu = U;
+            // back to natural lane order: lane l holds layer cc*64 + l
+            const double bl = __shfl(belowu, rl, kWave);
+            const int il = cc * kWave + lane;
+            if (il < L) {
+              const double a = t.w * fabs(fdl[cc]), bb = t.w * fabs(bl);
+              num[cc] = num[cc] + (kk[cc] * a + kk[cc] * bb);
+              den[cc] = den[cc] + (a + bb);
+            }
+            U = __shfl(above, kWave - 1, kWave);
+          }
+        }
+      }
+    }
+    // this gas's equivalent extinction and (solar) direct-beam adjustment
+#pragma unroll
+    for (int c = 0; c < kKeqChunks; c++) {
+      if (c < nch) {
+        const int i = c * kWave + lane;
+        double kq;
+        if (solar) {
+          kq = sflux > 0.0 ? num[c] / sflux : kmin[c];
+          // flux_gas at the top of layer i: the summed transmission one layer
+          // up (1 above the first layer)
+          double fgp = __shfl_up(den[c], 1, kWave);
+          const double carry = __shfl(den[c > 0 ? c - 1 : 0], kWave - 1, kWave);
+          if (lane == 0) fgp = c > 0 ? carry : 1.0;
+          if (i == 0) fgp = 1.0;
+          if (i < L && fgp > 0.0) adj[c] = adj[c] * den[c] / fgp;
+        } else {
+          kq = den[c] > 0.0 ? num[c] / den[c] : kmin[c];
+        }
+        if (i < L) keq[c] = keq[c] + kq;
+      }
+    }
+    t0 += nk;
+  }
+  double *o = keqv + ((size_t)col * d.NB + b) * L * 2;
+#pragma unroll
+  for (int c = 0; c < kKeqChunks; c++) {
+    const int i = c * kWave + lane;
+    if (c < nch && i < L) {
+      o[2 * i] = keq[c];
+      o[2 * i + 1] = adj[c];
     }
   }
 }
 
 // Per (column, band, layer): the grey extinction (Rayleigh, grey_opt_prop.F90:
-// 476-541, 816-835) plus the equivalent extinction of the minor gases from
-// the per-term sums (solve_band_k_eqv_scl.F90:826-846, 995-1019), the solar
-// direct-beam adjustment (ibid. :848-891), and the Planck source terms
-// (diff_planck_source_mod.F90:100-134). bandrec[col][band][layer][4].
+// 476-541, 816-835) plus the equivalent extinction of the minor gases
+// (keqv_body), the solar direct-beam adjustment with its exponential of the
+// equivalent optical depth (solve_band_k_eqv_scl.F90:848-891), the Planck
+// source terms (diff_planck_source_mod.F90:100-134) and the cloud optics.
+// bandrec[col][band][layer][4].
 __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__restrict__ spec,
                                           int l_rayleigh, const double *__restrict__ colrec,
                                           const double *__restrict__ tlev,
@@ -494,9 +615,11 @@ __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__res
                                           double *__restrict__ bandscal,
                                           double *__restrict__ cldopt, const int *__restrict__ krange,
                                           int idx) {
-  const int NB = d.NB, L = d.L, NT = spec->n_minor;
+  const int NB = d.NB, L = d.L;
   if (idx >= d.P * NB * L) return;
-  const int i = idx % L, b = (idx / L) % NB, col = idx / (L * NB);
+  // bands fastest: the lanes of a wave share a few consecutive layer
+  // records (one read each) and write runs of consecutive layers per band
+  const int b = idx % NB, i = (idx / NB) % L, col = idx / (L * NB);
   const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   const double *r = rec + i * SG_NQ;
@@ -504,62 +627,33 @@ __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__res
     // droplet optics of the layer's in-cloud water at unit condensate
     // scaling (they depend on the band, not on the k-term; the solve scales
     // them by each sub-column's c_sub)
-    double *co = cldopt + (((size_t)col * NB + b) * L + i) * 4;
-    double kext = 0.0, kscat = 0.0, phf = 0.0, fwd = 0.0;
+    // (written only where the layer has cloud: the solve reads them only
+    // for cloudy layers of a sub-column)
     if (r[SG_Q_FRAC1] > 0.0) {
+      double *co = cldopt + (((size_t)col * NB + b) * L + i) * 4;
+      double kext, kscat, phf, fwd;
       const SgDrop &dr = spec->drop[d.drop];
       if (dr.scheme == 5) pade2(dr.parm[b], r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
       else slingo(dr.parm[b], r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
+      if (d.f32) {
+        // mixed form: (kext, kext - kscat, phf, fwd) in FP32, the absorption
+        // formed here in FP64
+        float4 *cf = (float4 *)((float *)cldopt + (((size_t)col * NB + b) * L + i) * 4);
+        *cf = make_float4((float)kext, (float)(kext - kscat), (float)phf, (float)fwd);
+      } else {
+        co[0] = kext; co[1] = kscat; co[2] = phf; co[3] = fwd;
+      }
     }
-    co[0] = kext; co[1] = kscat; co[2] = phf; co[3] = fwd;
   }
   double *br = bandrec + (((size_t)col * NB + b) * L + i) * SG_NBQ;
   double *bs = bandscal + ((size_t)col * NB + b) * SG_NBQ;
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
   const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
-  const double *mc = minor + (size_t)col * NT * L * 2;
   double keq = 0.0, adjust = 1.0;
-  int t0 = B.minor0;
-  for (int j = 0; j < B.n_minor_gas; j++) {
-    const int nk = B.minor_nk[j];
-    if (solar) {
-      double num = 0.0, sflux = 0.0, fg = 0.0, fgp = 0.0;
-      for (int q = 0; q < nk; q++) {
-        const double *o = mc + (size_t)(t0 + q) * L * 2;
-        num += o[2 * i + 1];
-        sflux += o[2 * (L - 1)];
-        fg += o[2 * i];
-        fgp += i > 0 ? o[2 * (i - 1)] : 0.0;
-      }
-      if (i == 0) fgp = 1.0;
-      double kq;
-      if (sflux > 0.0) kq = num / sflux;
-      else {
-        kq = DBL_MAX;
-        for (int q = 0; q < nk; q++)
-          kq = fmin(kq, k_term(spec->minor[t0 + q], r[SG_Q_LNP], r[SG_Q_LNT],
-                               r[SG_Q_MMR + spec->minor[t0 + q].gas]));
-      }
-      keq += kq;
-      if (fgp > 0.0) adjust = adjust * fg / fgp;
-    } else {
-      double num = 0.0, den = 0.0;
-      for (int q = 0; q < nk; q++) {
-        const double *o = mc + ((size_t)(t0 + q) * L + i) * 2;
-        num += o[0];
-        den += o[1];
-      }
-      double kq;
-      if (den > 0.0) kq = num / den;
-      else {
-        kq = DBL_MAX;
-        for (int q = 0; q < nk; q++)
-          kq = fmin(kq, k_term(spec->minor[t0 + q], r[SG_Q_LNP], r[SG_Q_LNT],
-                               r[SG_Q_MMR + spec->minor[t0 + q].gas]));
-      }
-      keq += kq;
-    }
-    t0 += nk;
+  if (B.n_minor_gas > 0) {
+    const double *kq = minor + (((size_t)col * NB + b) * L + i) * 2;
+    keq = kq[0];
+    adjust = kq[1];
   }
   const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
   br[0] = (0.0 + ray) + keq;
@@ -644,9 +738,13 @@ struct LayerIn {
 
 struct LaneSrc {
   const double *rec, *br, *csub, *cop;
+  const float *csubf, *copf;  // FP32 forms (mixed form)
   int gas, isub;
 };
 
+// MIXED reads the FP32 c_sub and droplet optics (kext, kabs, first moment,
+// forward scattering), the FP64 form (kext, kscat, ...) in FP64.
+template <bool MIXED>
 __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) {
   const double *r = s.rec + i * SG_NQ;
   x.lnp = r[SG_Q_LNP];
@@ -658,10 +756,16 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
   x.kgt = q[0];
   x.q1 = q[1];
   x.q2 = q[2];
-  x.cv = s.isub > 0 ? s.csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
+  if (MIXED) x.cv = s.isub > 0 ? (double)s.csubf[(size_t)i * SG_MAXSUBCOL] : 0.0;
+  else x.cv = s.isub > 0 ? s.csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
   if (x.cv * x.frac1 > 0.0) {
-    const double *co = s.cop + i * 4;
-    x.co[0] = co[0]; x.co[1] = co[1]; x.co[2] = co[2]; x.co[3] = co[3];
+    if (MIXED) {
+      const float4 co = *(const float4 *)(s.copf + i * 4);
+      x.co[0] = co.x; x.co[1] = co.y; x.co[2] = co.z; x.co[3] = co.w;
+    } else {
+      const double *co = s.cop + i * 4;
+      x.co[0] = co[0]; x.co[1] = co[1]; x.co[2] = co[2]; x.co[3] = co[3];
+    }
   } else {
     x.co[0] = 0.0; x.co[1] = 0.0; x.co[2] = 0.0; x.co[3] = 0.0;
   }
@@ -704,6 +808,7 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
 // resonance singularity.
 __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                                 Coef &o) {
+#pragma clang fp contract(on)
   const float kgas = k_term_f32(t, (float)x.lnp, (float)x.lnt, (float)x.mmr);
   const double csub = x.cv;
   const double *co = x.co;
@@ -714,9 +819,9 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   float tau, om, omc, g;
   bool cloudy = csub * frac1 > 0.0;
   if (cloudy) {
-    // absorption kext - kscat formed in FP64 (relative accuracy ~1e-10 even
-    // for weakly absorbing droplets), then rounded
-    const float kext = (float)co[0], kabs = (float)(co[0] - co[1]);
+    // absorption kext - kscat formed in FP64 by the prep kernel (relative
+    // accuracy ~1e-10 even for weakly absorbing droplets), kept in FP32
+    const float kext = (float)co[0], kabs = (float)co[1];
     const float phf = (float)co[2], fwd = (float)co[3];
     const float cs = (float)csub;
     float ks = (float)c.ksc_clear + (kext - kabs) * cs;
@@ -757,6 +862,7 @@ template <bool MIXED>
 __device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                            double &fdir, double &rho, double &dn, double &a,
                                            double &bb) {
+#pragma clang fp contract(on)
   Coef o;
   if (MIXED) layer_coef_mixed(c, t, x, o);
   else layer_coef(c, t, x, o);
@@ -831,6 +937,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
                                            const double *__restrict__ c_sub,
                                            const double *__restrict__ cldopt,
                                            double *__restrict__ partial, int wave, int lane) {
+#pragma clang fp contract(on)
   const int L = d.L, NB = d.NB, IM = d.n_items;
   const long long g0 = (long long)wave * kWave, gmax = (long long)d.P * IM;
   if (g0 >= gmax) return;
@@ -879,6 +986,8 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   src.br = br;
   src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
   src.csub = c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
+  src.copf = d.mcica ? (const float *)cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
+  src.csubf = (const float *)c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
   src.gas = t.gas;
   src.isub = isub;
   const double alb_diff = bs[2], alb_dir = bs[3];
@@ -890,7 +999,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   double rho = 0.0, dn = finc;
   double vdir[8], o8[8];
   LayerIn cur, nxt;
-  load_layer(src, 0, cur);
+  load_layer<MIXED>(src, 0, cur);
   for (int i = 0; i < L; i++) {
     if (solar) {
       // direct flux at the top of layer i, summed 8 levels at a time
@@ -905,7 +1014,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
         }
       }
     }
-    if (i + 1 < L) load_layer(src, i + 1, nxt);
+    if (i + 1 < L) load_layer<MIXED>(src, i + 1, nxt);
     if (active) {
       if (RECOMP) {
         if (i % kSeg == 0) {
@@ -994,7 +1103,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
           sr[k] = rho_s; sdn[k] = dn_s; sa[k] = 0.0; sb[k] = 0.0;
           if (i < L) {
             LayerIn x;
-            load_layer(src, i, x);
+            load_layer<MIXED>(src, i, x);
             layer_step<MIXED>(c, t, x, fdir_s, rho_s, dn_s, sa[k], sb[k]);
           }
         }
@@ -1141,6 +1250,7 @@ __device__ __forceinline__ void overlap_body(const CallDims &d, const double *__
 template <bool MIXED>
 __device__ __forceinline__ void region_coef(const LaneCtx &c, double kgas, double mass, double kg, double ks,
                                             double g, double fw, Coef &o) {
+#pragma clang fp contract(on)
   if (!MIXED) {
     double ktot = kg + kgas;
     double tau = ktot * mass;
@@ -1470,7 +1580,9 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
 // every kernel loops over the concatenated work of all requests and finds
 // the request of a work item by binary search over per-request starts. A
 // request's results are those of the same call made alone.
-enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, kNWork };
+// W_LONG counts the equivalent-extinction waves of calls with more than 128
+// layers (it only decides whether their kernel is launched).
+enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, W_LONG, kNWork };
 
 struct ReqDesc {
   CallDims d;
@@ -1558,8 +1670,9 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
 // w > 0, L when there is none, applied by the callers). w is 0 or above
 // the 0.001 threshold, so both tests read "the layer has cloud", evaluated
 // here on the caller's fractions exactly as the pack kernel forms w. One
-// thread per column, reading along the profiles; one atomic per wave where
-// the wave lies within one request.
+// thread per (column, chunk of kRangeChunk layers), reading along the
+// profiles; one atomic per wave where the wave lies within one request.
+constexpr int kRangeChunk = 8;
 __device__ __forceinline__ bool layer_cloudy(const CallDims &d, const SocgpuRunesArgs &a, size_t idx) {
   double w = fmax(a.cloud_frac[idx], 0.0);
   if (d.triple) w = w + (a.conv_frac ? fmax(a.conv_frac[idx], 0.0) : 0.0);
@@ -1577,15 +1690,17 @@ __global__ void b_cldrange_kernel(const BatchHdr *h, const ReqDesc *rq) {
     if (g < tot) {
       r = find_req(h, rq, W_CCOL, g);
       const ReqDesc &q = rq[r];
-      const int col = (int)(g - q.s[W_CCOL]), P = q.d.P, L = q.d.L;
+      const int idx = (int)(g - q.s[W_CCOL]), P = q.d.P, L = q.d.L;
+      const int col = idx % P, i0 = (idx / P) * kRangeChunk;
+      const int i1 = i0 + kRangeChunk < L ? i0 + kRangeChunk : L;
       if (q.d.raw_range) {
-        for (int i = 0; i < L; i++) {
+        for (int i = i0; i < i1; i++) {
           const double w = q.a.cloud_frac[(size_t)i * P + col];
           if (w > kEps && kmin == INT32_MAX) kmin = i + 1;
           if (w >= kEps) kmax = i + 1;
         }
       } else {
-        for (int i = 0; i < L; i++) {
+        for (int i = i0; i < i1; i++) {
           if (layer_cloudy(q.d, q.a, (size_t)i * P + col)) {
             kmin = kmin < i + 1 ? kmin : i + 1;
             kmax = i + 1;
@@ -1641,16 +1756,17 @@ __global__ __launch_bounds__(64 * kCgWaves) void b_cloudgen_kernel(const BatchHd
   }
 }
 
-__global__ __launch_bounds__(64 * kMinorWaves) void b_minor_kernel(const BatchHdr *h, const ReqDesc *rq) {
-  __shared__ double sh[kMinorWaves][4][SG_MAXL];
+template <int NCH>
+__global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? 4 : 2) void b_keqv_kernel(const BatchHdr *h, const ReqDesc *rq) {
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const long long tot = h->tot[W_MINOR];
   for (long long w = blockIdx.x * (long long)kMinorWaves + wv; w < tot; w += (long long)gridDim.x * kMinorWaves) {
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_MINOR, w));
     const ReqDesc &q = rq[r];
-    minor_body(q.d, q.spec, q.colrec, q.tlev, q.colscal, q.bandscal, q.minor, sh[wv],
-               (int)(w - q.s[W_MINOR]), lane);
-    __syncwarp();
+    const int nch = (q.d.L + kWave - 1) / kWave;
+    if ((NCH == 2 && nch <= 2) || (NCH > 2 && nch > 2))
+      keqv_body<NCH>(q.d, q.spec, q.colrec, q.tlev, q.colscal, q.bandscal, q.minor, (int)(w - q.s[W_MINOR]),
+                     lane);
   }
 }
 
@@ -1868,7 +1984,9 @@ struct Context {
 };
 static Context g_ctx;
 // solve-kernel tile per precision form ([0] FP64, [1] mixed)
-static SocgpuTile g_tile[2] = {{0, 2, 4}, {0, 2, 4}};
+// default solve tiles (FP64, mixed): the autotuner's picks on the
+// 4096-column x 90-layer all-sky SW + LW batch
+static SocgpuTile g_tile[2] = {{0, 1, 3}, {1, 1, 3}};
 
 static int ensure_init() {
   if (g_ctx.device >= 0) return 0;
@@ -2091,6 +2209,7 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     int drop = a->i_st_water;
     if (int rc = check_drop(drop, "Liquid")) return rc;
     d.mcica = 1;
+    d.f32 = precision == SOCGPU_MIXED;
     d.mode = a->i_mcica_sampling == SOCGPU_MCICA_OPTIMAL ? 1 : 0;
     if (S.n_req[d.mode] <= 0) return fail(SOCGPU_EUNSUPPORTED, "McICA data has no tables for this sampling");
     d.drop = drop - 1;
@@ -2120,7 +2239,7 @@ static size_t partial_doubles(const CallDims &d) {
 static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
   const size_t P = d.P, L = d.L, NB = d.NB;
   size_t n = P * L * SG_NQ + P * (L + 1) + P * SG_NS + P * NB * L * SG_NBQ + P * NB * SG_NBQ +
-             P * (S.n_minor > 0 ? S.n_minor : 0) * L * 2 + partial_doubles(d) + 2;
+             (S.n_mband > 0 ? P * NB * L * 2 : 0) + partial_doubles(d) + 2;
   if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * 4;
   if (d.triple) n += P * NB * L * 8 + P * (L + 1) * kOv + P * d.nwpc * L * kTriQ * kWave;
   return n;
@@ -2131,13 +2250,14 @@ static void request_work(const CallDims &d, const SgSpec &S, long long w[kNWork]
   w[W_LEV] = P * (L + 1);
   w[W_LAY] = P * L;
   w[W_CG] = d.mcica ? P : 0;
-  w[W_MINOR] = P * S.n_minor;
+  w[W_MINOR] = P * S.n_mband;
+  w[W_LONG] = L > 2 * kWave ? P * S.n_mband : 0;
   w[W_PREP] = P * d.NB * L;
   w[W_SOLVE] = d.triple ? 0 : d.n_waves;
   w[W_TOVL] = d.triple ? P * (L + 1) : 0;
   w[W_TSOLVE] = d.triple ? P * d.nwpc : 0;
   w[W_PTILE] = pack_tiles(d);
-  w[W_CCOL] = (d.mcica || d.triple) ? P : 0;
+  w[W_CCOL] = (d.mcica || d.triple) ? P * ((L + kRangeChunk - 1) / kRangeChunk) : 0;
   w[W_FTILE] = finish_tiles(d);
 }
 
@@ -2156,7 +2276,7 @@ static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cur
   q.colscal = take(P * SG_NS);
   q.bandrec = take(P * NB * L * SG_NBQ);
   q.bandscal = take(P * NB * SG_NBQ);
-  q.minor = S.n_minor > 0 ? take(P * S.n_minor * L * 2) : nullptr;
+  q.minor = S.n_mband > 0 ? take(P * NB * L * 2) : nullptr;
   q.partial = take(partial_doubles(d));
   q.krange = (int *)take(2);
   q.csub = d.mcica ? take(P * L * SG_MAXSUBCOL) : nullptr;
@@ -2199,7 +2319,13 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
   if (tot[W_CCOL] > 0) b_cldrange_kernel<<<blocks(tot[W_CCOL], T), T, 0, s>>>(hdr, rq);
   if (tot[W_CG] > 0) b_cloudgen_kernel<<<blocks(tot[W_CG], kCgWaves), 64 * kCgWaves, 0, s>>>(hdr, rq);
   if (tot[W_MINOR] > 0)
-    b_minor_kernel<<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
+  {
+    // calls of up to 128 layers, then the longer ones (each kernel skips the
+    // other's waves)
+    b_keqv_kernel<2><<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
+    if (tot[W_LONG] > 0)
+      b_keqv_kernel<kKeqMaxChunks><<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
+  }
   b_prep_kernel<<<blocks(tot[W_PREP], T), T, 0, s>>>(hdr, rq);
   if (tot[W_TOVL] > 0) b_overlap_kernel<<<blocks(tot[W_TOVL], T), T, 0, s>>>(hdr, rq);
   if (ev_a) HIPCHK(hipEventRecord(ev_a, s));
@@ -3002,7 +3128,7 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
   host.h.R = 1;
   host.h.tot[W_LAY] = (long long)P * L;
   host.h.tot[W_CG] = P;
-  host.h.tot[W_CCOL] = P;
+  host.h.tot[W_CCOL] = (long long)P * ((L + kRangeChunk - 1) / kRangeChunk);
   host.q.a.cloud_frac = dcf.as<double>();
   host.q.d = d;
   host.q.colrec = drec.as<double>();
@@ -3024,7 +3150,7 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
   const BatchHdr *dh = (const BatchHdr *)ddesc.p;
   ReqDesc *dq = (ReqDesc *)((char *)ddesc.p + offsetof(decltype(host), q));
   b_init_kernel<<<1, 64, 0, s>>>(dh, dq);
-  b_cldrange_kernel<<<(P + 255) / 256, 256, 0, s>>>(dh, dq);
+  b_cldrange_kernel<<<(int)((host.h.tot[W_CCOL] + 255) / 256), 256, 0, s>>>(dh, dq);
   b_cloudgen_kernel<<<(P + kCgWaves - 1) / kCgWaves, 64 * kCgWaves, 0, s>>>(dh, dq);
   HIPCHK(hipGetLastError());
   std::vector<double> cs((size_t)P * L * SG_MAXSUBCOL), sc((size_t)P * SG_NS);

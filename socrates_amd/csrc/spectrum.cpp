@@ -266,6 +266,9 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
   }
   s->n_g = ng;
   s->n_minor = nm;
+  s->n_mband = 0;
+  for (int b = 0; b < s->n_band; b++)
+    if (s->band[b].n_minor_gas > 0) s->mband[s->n_mband++] = b;
   (void)have6;
   return 0;
 }
