@@ -27,7 +27,7 @@ class _Case(ctypes.Structure):
         "p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
         "p_level", "t_level", "t_ground", "cos_zen", "solar_irrad", "albedo_diff",
         "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr",
-        "liq_conv_dim")] + [("rand_seed", _I)]
+        "liq_conv_dim")] + [("rand_seed", _I), ("wm_mix_ratio", ctypes.c_double * 10)]
 
 
 def build():
@@ -68,6 +68,8 @@ def runes(case, spectral_file, mcica_file=None, variant=None):
         v = getattr(case, f)
         if f == "rand_seed":
             setattr(cs, f, _ptr(v, keep, np.int32))
+        elif f == "wm_mix_ratio":
+            setattr(cs, f, (ctypes.c_double * 10)(*[float(x) for x in v]))
         elif isinstance(v, np.ndarray):
             setattr(cs, f, _ptr(v, keep))
         else:

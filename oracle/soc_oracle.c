@@ -9,6 +9,7 @@
  * results agree with the compiled reference to rounding.
  */
 #include "soc_oracle.h"
+#include "../socrates_amd/csrc/wenyi_tables.h"
 
 #include <float.h>
 #include <math.h>
@@ -31,15 +32,28 @@ static double jit_exp(double x) {
 #endif
 
 #define NB 16      /* bands */
-#define NG 8       /* gases per file */
+#define NG 16      /* gases per file */
 #define NK 32      /* k-terms per gas and band */
+#define NCONT 4    /* continua per band */
+#define NPRE 64    /* pressures of the k-term look-up table */
+#define NTMP 8     /* temperatures of the k-term look-up table */
 #define NDROP 16   /* droplet types */
 #define NPARM 20   /* parameters per droplet fit */
 
 /* option codes, rad_pcf.F90 */
 enum { IP_SOLAR = 1, IP_INFRA_RED = 2 };
-enum { IP_H2O = 1, IP_CO2 = 2, IP_O3 = 3 };
-enum { IP_SCALE_TERM = 2, IP_SCALE_POWER_LAW = 1 };
+/* gas identifiers (gas_list_pcf.F90) */
+enum { IP_H2O = 1, IP_CO2 = 2, IP_O3 = 3, IP_N2O = 4, IP_CH4 = 6, IP_O2 = 7, IP_SO2 = 9,
+       IP_CFC11 = 14, IP_CFC12 = 15, IP_CFC113 = 16, IP_HCFC22 = 17, IP_HFC134A = 19,
+       IP_OCS = 25 };
+enum { IP_SCALE_TERM = 2 };
+/* scaling functions (rad_pcf.F90:509-532) */
+enum { IP_SCALE_POWER_LAW = 1, IP_SCALE_POWER_QUAD = 2, IP_SCALE_WENYI = 4, IP_SCALE_LOOKUP = 9 };
+/* continua (rad_pcf.F90) */
+enum { IP_SELF_CONTINUUM = 1, IP_FRN_CONTINUUM = 2 };
+/* rad_ccf.F90:42,58 */
+static const double MOL_WEIGHT_AIR = 28.966e-03;
+static const double REPSILON = 18.0153 / 28.966;
 enum { IP_DROP_PADE_2 = 5, IP_SLINGO_SCHRECKER = 1 };
 
 static const double ELSASSER = 1.66;  /* diffusivity_factor.F90 */
@@ -57,7 +71,19 @@ typedef struct {
   int index_absorb[NB][NG]; /* 1-based gas index */
   int i_band_k[NB][NG], i_scale_k[NB][NG], i_scale_fnc[NB][NG];
   double p_ref[NB][NG], t_ref[NB][NG];
-  double k[NB][NG][NK], w[NB][NG][NK], scale[NB][NG][NK][2];
+  double k[NB][NG][NK], w[NB][NG][NK], scale[NB][NG][NK][3];
+  /* P/T look-up tables of ip_scale_lookup terms (read_spectrum.F90:1404-1507):
+   * ln p_lookup, t_lookup(it, ip) as [ip][it], k_lookup per (band, gas) as
+   * [term][ip][it] */
+  int n_pre, n_tmp;
+  double p_lookup[NPRE], t_lookup[NPRE][NTMP];
+  double *k_lookup[NB][NG];
+  /* continua (blocks 8 and 9, read_spectrum.F90:1881-1997): stored by the
+   * continuum's index number, as the reader stores them */
+  int n_band_continuum[NB], index_continuum[NB][NCONT], index_water;
+  int i_scale_fnc_cont[NB][NCONT + 1];
+  double k_cont[NB][NCONT + 1], scale_cont[NB][NCONT + 1][3];
+  double p_ref_cont[NCONT + 1][NB], t_ref_cont[NCONT + 1][NB];
   int n_deg_fit;
   double t_ref_planck;
   double thermal_coeff[NB][8];
@@ -99,24 +125,84 @@ static char *next_line(FILE *f, char *ln, int n) {
   return r;
 }
 
-/* Text readers for the blocks used by the synthetic files: a restatement of
- * read_spectrum.F90 block readers 0_0_1 (557-621), 2_0_0 (764-789),
- * 3_0_0 (872-900), 4_0_0 (951-992), 5_0_1 (1218-1641, power-law terms),
- * 6_0_0 (1746-1777), 10_0_2 (2245-2331). Other blocks are skipped. */
+/* Numbers of a '(6(1PE13.6))' record set: n values, 6 to a line. */
+static int read_e13(FILE *f, char *ln, int lnsz, double *out, int n) {
+  int got = 0;
+  while (got < n) {
+    if (!next_line(f, ln, lnsz)) return -1;
+    for (int q = 0; q < 6 && got < n; q++) out[got++] = fld(ln, 13 * q, 13);
+  }
+  return 0;
+}
+
+/* The k-term look-up tables of the '_k' file (read_spectrum.F90:1404-1507):
+ * the block whose header names 'k-table', two header records, the P/T grid
+ * (ln p stored), then per band and per gas with ip_scale_lookup (in the
+ * order block 5 lists them) two header records and per term and pressure
+ * one record set of n_tmp values. */
+static int read_k_table(const char *path, spec_t *s, int order[NB][NG], const int *n_order) {
+  char kp[1100], ln[512];
+  snprintf(kp, sizeof kp, "%s_k", path);
+  FILE *f = fopen(kp, "r");
+  if (!f) return -20;
+  int found = 0;
+  while (next_line(f, ln, sizeof ln))
+    if (strncmp(ln, "*BLOCK", 6) == 0 && strncmp(ln + 8, "k-table", 7) == 0) { found = 1; break; }
+  if (!found) { fclose(f); return -21; }
+  next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+  double row[NTMP + 1];
+  for (int ip = 0; ip < s->n_pre; ip++) {
+    if (read_e13(f, ln, sizeof ln, row, s->n_tmp + 1)) { fclose(f); return -22; }
+    s->p_lookup[ip] = log(row[0]);
+    for (int it = 0; it < s->n_tmp; it++) s->t_lookup[ip][it] = row[it + 1];
+  }
+  for (int b = 0; b < s->n_band; b++)
+    for (int j = 0; j < n_order[b]; j++) {
+      int g = order[b][j];
+      if (s->i_scale_fnc[b][g] != IP_SCALE_LOOKUP) continue;
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      int nt = s->i_band_k[b][g];
+      double *kl = (double *)malloc(sizeof(double) * (size_t)nt * s->n_pre * s->n_tmp);
+      s->k_lookup[b][g] = kl;
+      for (int k = 0; k < nt; k++)
+        for (int ip = 0; ip < s->n_pre; ip++)
+          if (read_e13(f, ln, sizeof ln, kl + ((size_t)k * s->n_pre + ip) * s->n_tmp, s->n_tmp)) {
+            fclose(f);
+            return -22;
+          }
+    }
+  fclose(f);
+  return 0;
+}
+
+/* Text readers for the blocks on the flux path: a restatement of
+ * read_spectrum.F90 block readers 0_0_1/0_0_2 (557-728), 2_0_0 (764-789),
+ * 3_0_0 (872-900), 4_0_0/4_0_1 (951-1036), 5_0_1 (1218-1641: power-law,
+ * power-quadratic, Wenyi and look-up-table terms), 6_0_0 (1746-1777),
+ * 8_0_0 (1881-1935), 9_0_0 (1939-1997), 10_0_2 (2245-2331). Other blocks
+ * are skipped. */
 static int read_spectrum(const char *path, spec_t *s) {
   FILE *f = fopen(path, "r");
   char ln[512];
   int t, st, v;
+  int order[NB][NG], n_order[NB];
+  int any_lookup = 0;
   if (!f) return -1;
+  for (int b = 0; b < NB; b++)
+    for (int g = 0; g < NG; g++) free(s->k_lookup[b][g]);
   memset(s, 0, sizeof(*s));
+  memset(n_order, 0, sizeof n_order);
   while (next_line(f, ln, sizeof ln)) {
     if (!block_header(ln, &t, &st, &v)) continue;
     if (t == 0) {
+      /* summary lines are labelled; version 2 adds optional ones */
+      while (next_line(f, ln, sizeof ln)) {
+        if (strncmp(ln, "Number of spectral bands", 24) == 0) s->n_band = atoi(strchr(ln, '=') + 1);
+        else if (strncmp(ln, "Total number of gaseous absorbers", 33) == 0)
+          s->n_absorb = atoi(strchr(ln, '=') + 1);
+        else if (strncmp(ln, "List of indexing numbers and absorbers.", 39) == 0) break;
+      }
       next_line(f, ln, sizeof ln);
-      next_line(f, ln, sizeof ln); s->n_band = (int)fld(ln, 27, 5);
-      next_line(f, ln, sizeof ln); s->n_absorb = (int)fld(ln, 36, 5);
-      next_line(f, ln, sizeof ln);
-      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
       for (int i = 0; i < s->n_absorb; i++) {
         next_line(f, ln, sizeof ln);
         s->type_absorb[i] = (int)fld(ln, 12, 5);
@@ -136,6 +222,15 @@ static int read_spectrum(const char *path, spec_t *s) {
       for (int b = 0; b < s->n_band; b++) {
         next_line(f, ln, sizeof ln);
         s->n_band_absorb[b] = (int)fld(ln, 12, 5);
+        if (v == 1) {
+          /* '(5x, 12(2x, i3))' (:1022) */
+          if (s->n_band_absorb[b] > 0) {
+            next_line(f, ln, sizeof ln);
+            for (int q = 0; q < s->n_band_absorb[b]; q++)
+              s->index_absorb[b][q] = (int)fld(ln, 5 + 5 * q + 2, 3);
+          }
+          continue;
+        }
         int got = 0, first = 1;
         while (got < s->n_band_absorb[b]) {
           next_line(f, ln, sizeof ln);
@@ -154,25 +249,81 @@ static int read_spectrum(const char *path, spec_t *s) {
           next_line(f, ln, sizeof ln);
           int band = (int)fld(ln, 0, 5) - 1, gas = (int)fld(ln, 12, 5) - 1;
           int nt = (int)fld(ln, 24, 5);
+          int fnc = (int)fld(ln, 48, 5);
           s->i_band_k[band][gas] = nt;
           s->i_scale_k[band][gas] = (int)fld(ln, 36, 5);
-          s->i_scale_fnc[band][gas] = (int)fld(ln, 48, 5);
+          s->i_scale_fnc[band][gas] = fnc;
+          order[band][n_order[band]++] = gas;
           next_line(f, ln, sizeof ln);
-          s->p_ref[band][gas] = fld(ln, 6, 16);
-          s->t_ref[band][gas] = fld(ln, 28, 16);
+          if (fnc == IP_SCALE_LOOKUP) {
+            /* '(2(7x, i5))' num_ref_p, num_ref_t (:1333-1336) */
+            s->n_pre = (int)fld(ln, 7, 5);
+            s->n_tmp = (int)fld(ln, 19, 5);
+            any_lookup = 1;
+            if (s->n_pre > NPRE || s->n_tmp > NTMP) { fclose(f); return -23; }
+          } else {
+            s->p_ref[band][gas] = fld(ln, 6, 16);
+            s->t_ref[band][gas] = fld(ln, 28, 16);
+          }
+          /* n_scale_variable(fnc) parameters (rad_pcf.F90:505-506): the
+           * first two on the k-term's record, a third on the next one */
+          const int nsv = fnc == IP_SCALE_POWER_LAW ? 2 : fnc == IP_SCALE_POWER_QUAD ? 3 : 0;
           for (int k = 0; k < nt; k++) {
             next_line(f, ln, sizeof ln);
             s->k[band][gas][k] = fld(ln, 3, 16);
             s->w[band][gas][k] = fld(ln, 22, 16);
-            s->scale[band][gas][k][0] = fld(ln, 41, 16);
-            s->scale[band][gas][k][1] = fld(ln, 60, 16);
+            if (nsv >= 1) s->scale[band][gas][k][0] = fld(ln, 41, 16);
+            if (nsv >= 2) s->scale[band][gas][k][1] = fld(ln, 60, 16);
+            if (nsv >= 3) {
+              next_line(f, ln, sizeof ln);
+              s->scale[band][gas][k][2] = fld(ln, 41, 16);
+            }
+          }
+        }
+    } else if (t == 8) {
+      for (int j = 0; j < 5; j++) next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++) {
+        next_line(f, ln, sizeof ln);
+        s->n_band_continuum[b] = (int)fld(ln, 12, 5);
+        if (s->n_band_continuum[b] > NCONT) { fclose(f); return -24; }
+        if (s->n_band_continuum[b] > 0) {
+          next_line(f, ln, sizeof ln);
+          for (int q = 0; q < s->n_band_continuum[b]; q++)
+            s->index_continuum[b][q] = (int)fld(ln, 5 + 5 * q + 2, 3);
+        }
+      }
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      s->index_water = (int)fld(ln, 22, 5);
+    } else if (t == 9) {
+      next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
+      for (int b = 0; b < s->n_band; b++)
+        for (int j = 0; j < s->n_band_continuum[b]; j++) {
+          next_line(f, ln, sizeof ln);
+          int band = (int)fld(ln, 0, 5) - 1, ic = (int)fld(ln, 12, 5), fnc = (int)fld(ln, 24, 5);
+          if (ic < 1 || ic > NCONT) { fclose(f); return -24; }
+          s->i_scale_fnc_cont[band][ic] = fnc;
+          next_line(f, ln, sizeof ln);
+          s->p_ref_cont[ic][band] = fld(ln, 6, 16);
+          s->t_ref_cont[ic][band] = fld(ln, 28, 16);
+          /* '(6x, 1pe16.9, (t23, 2(6x, 1pe16.9)))' */
+          next_line(f, ln, sizeof ln);
+          s->k_cont[band][ic] = fld(ln, 6, 16);
+          const int nsv = fnc == IP_SCALE_POWER_LAW ? 2 : fnc == IP_SCALE_POWER_QUAD ? 3 : 0;
+          if (nsv >= 1) s->scale_cont[band][ic][0] = fld(ln, 28, 16);
+          if (nsv >= 2) s->scale_cont[band][ic][1] = fld(ln, 50, 16);
+          if (nsv >= 3) {
+            next_line(f, ln, sizeof ln);
+            s->scale_cont[band][ic][2] = fld(ln, 28, 16);
           }
         }
     } else if (t == 6) {
+      /* version 1 (read_block_6_0_1, :1781-1877) adds a 'Type of data' line */
       next_line(f, ln, sizeof ln);
+      if (v == 1) next_line(f, ln, sizeof ln);
       next_line(f, ln, sizeof ln);
-      s->n_deg_fit = (int)fld(ln, 23, 5);
-      s->t_ref_planck = fld(ln, 54, 16);
+      /* '(23x, i5, 26x, 1pe16.9)' (:1757), '(24x, i5, 26x, 1pe16.9)' (:1802) */
+      s->n_deg_fit = (int)fld(ln, v == 1 ? 24 : 23, 5);
+      s->t_ref_planck = fld(ln, v == 1 ? 55 : 54, 16);
       next_line(f, ln, sizeof ln); next_line(f, ln, sizeof ln);
       for (int b = 0; b < s->n_band; b++) {
         int got = 0;
@@ -220,7 +371,9 @@ static int read_spectrum(const char *path, spec_t *s) {
     }
   }
   fclose(f);
-  return s->n_band > 0 ? 0 : -2;
+  if (s->n_band <= 0) return -2;
+  if (any_lookup) return read_k_table(path, s, order, n_order);
+  return 0;
 }
 
 /* read_mcica_data, def_mcica.F90:61-190 */
@@ -869,6 +1022,65 @@ static void solve_triple(int isolir, tri_t *t, const double *tau_c, const double
 #undef OV
 }
 
+/* atm%gas_mix_ratio of one absorber (socrates_set_atm.F90:227-415, 535-622):
+ * full fields clipped at zero, well-mixed values as given, absent gases 0 */
+static double gas_mix_ratio(const soc_case *c, int type, int i, int l) {
+  const int P = c->n_profile;
+  switch (type) {
+    case IP_H2O: return fmax(c->h2o[(size_t)i * P + l], 0.0);
+    case IP_O3: return fmax(c->o3[(size_t)i * P + l], 0.0);
+    case IP_CO2: return c->co2_mix_ratio;
+    case IP_N2O: return c->wm_mix_ratio[SOC_WM_N2O];
+    case IP_CH4: return c->wm_mix_ratio[SOC_WM_CH4];
+    case IP_O2: return c->wm_mix_ratio[SOC_WM_O2];
+    case IP_SO2: return c->wm_mix_ratio[SOC_WM_SO2];
+    case IP_OCS: return c->wm_mix_ratio[SOC_WM_OCS];
+    case IP_CFC11: return c->wm_mix_ratio[SOC_WM_CFC11];
+    case IP_CFC12: return c->wm_mix_ratio[SOC_WM_CFC12];
+    case IP_CFC113: return c->wm_mix_ratio[SOC_WM_CFC113];
+    case IP_HCFC22: return c->wm_mix_ratio[SOC_WM_HCFC22];
+    case IP_HFC134A: return c->wm_mix_ratio[SOC_WM_HFC134A];
+    default: return 0.0;
+  }
+}
+
+/* MINLOC(x - tab, mask x - tab >= 0) over n ascending entries (0-based) */
+static int minloc_below(double x, const double *tab, int n, int stride) {
+  int best = -1;
+  double db = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double d = x - tab[(size_t)k * stride];
+    if (d >= 0.0 && (best < 0 || d < db)) { best = k; db = d; }
+  }
+  return best;
+}
+
+/* inter_pt_lookup.F90:147-191 for one layer: indices (0-based) and factors
+ * fac00, fac10, fac01, fac11 */
+static void inter_pt_lookup(const spec_t *s, double p, double t, int *jp, int *jt, int *jtt,
+                            double fac[4]) {
+  const double eps = DBL_EPSILON;
+  const int np = s->n_pre, nt = s->n_tmp;
+  const double pmin = s->p_lookup[0] + fmax(fabs(s->p_lookup[0]) * eps, eps);
+  const double pmax = s->p_lookup[np - 1] - fmax(fabs(s->p_lookup[np - 1]) * eps, eps);
+  const double pl = fmin(fmax(log(p), pmin), pmax);
+  const int ip = minloc_below(pl, s->p_lookup, np, 1);
+  const double fp = (pl - s->p_lookup[ip]) / (s->p_lookup[ip + 1] - s->p_lookup[ip]);
+  double tl = fmin(fmax(t, s->t_lookup[ip][0] * (1.0 + eps)), s->t_lookup[ip][nt - 1] * (1.0 - eps));
+  const int it = minloc_below(tl, s->t_lookup[ip], nt, 1);
+  const double ft = (tl - s->t_lookup[ip][it]) / (s->t_lookup[ip][it + 1] - s->t_lookup[ip][it]);
+  tl = fmin(fmax(t, s->t_lookup[ip + 1][0] * (1.0 + eps)), s->t_lookup[ip + 1][nt - 1] * (1.0 - eps));
+  const int itt = minloc_below(tl, s->t_lookup[ip + 1], nt, 1);
+  const double ftt =
+      (tl - s->t_lookup[ip + 1][itt]) / (s->t_lookup[ip + 1][itt + 1] - s->t_lookup[ip + 1][itt]);
+  const double compfp = 1.0 - fp;
+  fac[0] = compfp * (1.0 - ft);
+  fac[1] = fp * (1.0 - ftt);
+  fac[2] = compfp * ft;
+  fac[3] = fp * ftt;
+  *jp = ip; *jt = it; *jtt = itt;
+}
+
 int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const soc_case *c,
                      double *flux_up, double *flux_down, double *flux_direct,
                      double *heating_rate) {
@@ -880,6 +1092,16 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   int rc = read_spectrum(spectral_file, &sp);
   if (rc) return rc;
   const int NBAND = sp.n_band;
+  /* the scalings restated here: per-term power law, power-quadratic, look-up
+   * table, and Wenyi's CO2 tables in band 4 */
+  for (int b = 0; b < NBAND; b++)
+    for (int j = 0; j < sp.n_band_absorb[b]; j++) {
+      const int g = sp.index_absorb[b][j] - 1, fnc = sp.i_scale_fnc[b][g];
+      if (sp.i_scale_k[b][g] != IP_SCALE_TERM) return -12;
+      if (fnc != IP_SCALE_POWER_LAW && fnc != IP_SCALE_POWER_QUAD && fnc != IP_SCALE_LOOKUP &&
+          !(fnc == IP_SCALE_WENYI && b == 3))
+        return -12;
+    }
   /* ip_cloud_conv_strat: triple-column overlap with maximum-random overlap;
    * McICA is switched off for it (socrates_set_control.F90:338-356) */
   int triple = c->l_cloud && c->i_cloud_repr == 3;
@@ -1021,7 +1243,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
 
   /* --- per-column radiative transfer ------------------------------------ */
   work_t w, wc;
-  double *buf = (double *)malloc(sizeof(double) * (size_t)(30 * (L + 2) + 16 * (2 * L + 2)));
+  double *buf = (double *)malloc(sizeof(double) * (size_t)(31 * (L + 2) + 16 * (2 * L + 2)));
   double *q = buf;
 #define TAKE(n) (q += (n), q - (n))
   w.L = wc.L = L;
@@ -1037,6 +1259,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   double *kc_tot = TAKE(L), *kc_scat = TAKE(L), *kc_phf = TAKE(L), *kc_fwd = TAKE(L);
   double *sum_flux = TAKE(2 * L + 3), *sum_k_flux = TAKE(2 * L + 3);
   double *ft = TAKE(2 * L + 2), *tg = TAKE(L), *src = TAKE(L);
+  double *kg0 = TAKE(L);  /* clear-air grey extinction before k_eqv */
   double *acc_total = TAKE(2 * L + 2), *acc_direct = TAKE(L + 1);
   double *clr_total = TAKE(2 * L + 2), *clr_direct = TAKE(L + 1);
   double *up = TAKE(L + 1), *down = TAKE(L + 1), *dir = TAKE(L + 1);
@@ -1044,7 +1267,13 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
 #define KABS(j, k, i) kabs[((size_t)(j) * NK + (k)) * L + (i)]
   const double temp_max = log(1.0 / eps);
 
+  int *lu_jp = (int *)malloc(sizeof(int) * (size_t)3 * L), *lu_jt = lu_jp + L, *lu_jtt = lu_jt + L;
+  double (*lu_fac)[4] = (double (*)[4])malloc(sizeof(double) * 4 * (size_t)L);
   for (int l = 0; l < P; l++) {
+    if (sp.n_pre > 0)
+      for (int i = 0; i < L; i++)
+        inter_pt_lookup(&sp, LAY(c->p_layer, i, l), LAY(c->t_layer, i, l), &lu_jp[i], &lu_jt[i], &lu_jtt[i],
+                        lu_fac[i]);
     if (triple) {
       double *fs1 = frk[0], *fc1 = frk[1];
       for (int i = 0; i < L; i++) {
@@ -1068,22 +1297,88 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       for (int j = 0; j < ngas; j++) {
         int g = sp.index_absorb[b][j] - 1;
         int type = sp.type_absorb[g];
+        const int fnc = sp.i_scale_fnc[b][g];
         for (int k = 0; k < sp.i_band_k[b][g]; k++) {
           double p_inv = 1.0 / (sp.p_ref[b][g] + 0.0), t_inv = 1.0 / sp.t_ref[b][g];
+          const double *sc = sp.scale[b][g][k];
           for (int i = 0; i < L; i++) {
-            double pwk = pow((LAY(c->p_layer, i, l) + 0.0) * p_inv, sp.scale[b][g][k][0]);
-            double twk = pow(LAY(c->t_layer, i, l) * t_inv, sp.scale[b][g][k][1]);
-            double frac = pwk * twk;
-            double mr = type == IP_H2O ? fmax(LAY(c->h2o, i, l), 0.0)
-                      : type == IP_O3 ? fmax(LAY(c->o3, i, l), 0.0) : c->co2_mix_ratio;
+            const double p = LAY(c->p_layer, i, l), tt = LAY(c->t_layer, i, l);
+            const double mr = gas_mix_ratio(c, type, i, l);
+            if (fnc == IP_SCALE_LOOKUP) {
+              /* radiance_calc.F90:1136-1152, then :1432-1442 */
+              const double *kl = sp.k_lookup[b][g] + (size_t)k * sp.n_pre * sp.n_tmp;
+              const int ip = lu_jp[i], jt = lu_jt[i], jtt = lu_jtt[i], nt = sp.n_tmp;
+              double ke = fmax(0.0, lu_fac[i][0] * kl[ip * nt + jt] + lu_fac[i][1] * kl[(ip + 1) * nt + jtt] +
+                                        lu_fac[i][2] * kl[ip * nt + jt + 1] +
+                                        lu_fac[i][3] * kl[(ip + 1) * nt + jtt + 1]);
+              KABS(j, k, i) = fmax(0.0, ke * mr);
+              continue;
+            }
+            double frac;
+            if (fnc == IP_SCALE_POWER_LAW) {
+              /* scale_absorb.F90:137-152 */
+              double pwk = pow((p + 0.0) * p_inv, sc[0]);
+              double twk = pow(tt * t_inv, sc[1]);
+              frac = pwk * twk;
+            } else if (fnc == IP_SCALE_POWER_QUAD) {
+              /* scale_absorb.F90:176-193 */
+              double pwk = pow((p + 0.0) * p_inv, sc[0]);
+              double tmp = tt * t_inv - 1.0;
+              frac = pwk * (1.0 + tmp * sc[1] + sc[2] * tmp * tmp);
+            } else {
+              /* ip_scale_wenyi in band 4 (scale_absorb.F90:238-259) */
+              double cgp = fmax(-5.5, log(p / 100.0));
+              int jp = (int)((5.5 + cgp) * 2.0) + 1;
+              jp = jp < 1 ? 1 : jp > 26 ? 26 : jp;
+              int jp1 = jp + 1 < 26 ? jp + 1 : 26;
+              int jw = (int)((tt - 240.0) / 20.0 + 7.0);
+              jw = jw < 1 ? 1 : jw > 10 ? 10 : jw;
+              int jw1 = jw + 1 < 10 ? jw + 1 : 10;
+              const double gkpb = kWenyiGk4[jw - 1][jp - 1][k] +
+                                  (kWenyiGk4[jw - 1][jp1 - 1][k] - kWenyiGk4[jw - 1][jp - 1][k]) *
+                                      (cgp - kWenyiPlg[jp - 1]) * 2.0;
+              const double gkpc = kWenyiGk4[jw1 - 1][jp - 1][k] +
+                                  (kWenyiGk4[jw1 - 1][jp1 - 1][k] - kWenyiGk4[jw1 - 1][jp - 1][k]) *
+                                      (cgp - kWenyiPlg[jp - 1]) * 2.0;
+              frac = (gkpb + (gkpc - gkpb) * (tt - kWenyiTtb[jw - 1]) / 20.0) / kWenyiGk250b[k];
+            }
             KABS(j, k, i) = sp.k[b][g][k] * fmax(0.0, frac * mr);
           }
         }
       }
-      /* grey optics: Rayleigh (grey_opt_prop.F90:476-541, 816-835) */
+      /* grey optics: continua (rescale_continuum.F90:107-179 via
+       * radiance_calc.F90:897-923; grey_opt_prop.F90:801-816), then Rayleigh
+       * (grey_opt_prop.F90:476-541, 816-835). The continuum data are taken
+       * by the continuum's position in the band, as radiance_calc passes
+       * them. */
       for (int i = 0; i < L; i++) {
         k_ext_scat[i] = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
-        k_grey_tot[i] = 0.0 + k_ext_scat[i];
+        k_grey_tot[i] = 0.0;
+      }
+      for (int jc = 1; jc <= sp.n_band_continuum[b]; jc++) {
+        const int icont = sp.index_continuum[b][jc - 1];
+        const int fnc = sp.i_scale_fnc_cont[b][jc];
+        const double *sc = sp.scale_cont[b][jc];
+        const double pref = sp.p_ref_cont[jc][b], tref = sp.t_ref_cont[jc][b];
+        for (int i = 0; i < L; i++) {
+          const double p = LAY(c->p_layer, i, l), tt = LAY(c->t_layer, i, l);
+          const double rho = LAY(c->density, i, l), wf = fmax(LAY(c->h2o, i, l), 0.0);
+          double pwk = pow(p / pref, sc[0]), amount = 0.0;
+          if (fnc == IP_SCALE_POWER_LAW) amount = pwk * pow(tt / tref, sc[1]);
+          else if (fnc == IP_SCALE_POWER_QUAD)
+            amount = pwk * (1.0 + sc[1] * (tt / tref - 1.0) + sc[2] * ((tt / tref - 1.0) * (tt / tref - 1.0)));
+          if (icont == IP_SELF_CONTINUUM) {
+            double mdw = rho * wf / (REPSILON * MOL_WEIGHT_AIR);
+            amount = amount * mdw * wf;
+          } else if (icont == IP_FRN_CONTINUUM) {
+            amount = amount * rho * wf * (1.0 - wf) / MOL_WEIGHT_AIR;
+          }
+          k_grey_tot[i] = k_grey_tot[i] + sp.k_cont[b][jc] * amount;
+        }
+      }
+      for (int i = 0; i < L; i++) {
+        k_grey_tot[i] = k_grey_tot[i] + k_ext_scat[i];
+        kg0[i] = k_grey_tot[i];
       }
       /* Planck terms (diff_planck_source_mod.F90:100-134) */
       double flux_ground = 0.0;
@@ -1194,7 +1489,9 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           const double *mm = k == 0 ? tms : tmc, *rr = k == 0 ? trs : trc;
           for (int i = 0; i < L; i++) {
             const double ray = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
-            double kgt = 0.0 + ray, ksc = ray, phf = 0.0, fwd = 0.0;
+            /* every type starts from the clear-air totals (grey_opt_prop.F90:
+             * 983-990) */
+            double kgt = kg0[i], ksc = ray, phf = 0.0, fwd = 0.0;
             if (i + 1 >= nct && LAY(tw, i, l) > 0.0) {
               double ct, cs, cp, cf;
               pade_2(sp.drop_parm[it][b], LAY(mm, i, l), LAY(rr, i, l), &ct, &cs, &cp, &cf);
@@ -1379,7 +1676,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       heating_rate[(size_t)(i - 1) * P + l] = fdiv / LAY(c->layer_heat_capacity, i - 1, l);
     }
   }
-  free(kabs); free(buf);
+  free(kabs); free(buf); free(lu_jp); free(lu_fac);
   if (triple) { free(tw); free(ovl); free(tbuf); }
   if (mcica) {
     free(w_cloud); free(cond_mmr); free(re); free(c_sub); free(n_subcol_cld);

@@ -48,7 +48,13 @@ typedef struct {
       *albedo_diff, *albedo_dir, *cloud_frac, *liq_mmr, *liq_dim, *conv_frac,
       *liq_conv_mmr, *liq_conv_dim;
   const int *rand_seed;
+  /* well-mixed mass mixing ratios of the further gases (the *_mix_ratio
+   * arguments of runes, socrates_runes.F90:69-83), by SOC_WM_* slot */
+  double wm_mix_ratio[10];
 } soc_case;
+
+enum { SOC_WM_N2O, SOC_WM_CH4, SOC_WM_O2, SOC_WM_SO2, SOC_WM_OCS, SOC_WM_CFC11, SOC_WM_CFC12,
+       SOC_WM_CFC113, SOC_WM_HCFC22, SOC_WM_HFC134A };
 
 /* Full runes() restatement. Outputs [level][profile] (n_layer+1 rows) and
  * heating_rate [layer][profile] in K/s. Returns 0 or a negative error. */

@@ -7,12 +7,15 @@
   serve   - the serving loop under Poisson arrivals: each call carries 1..4096
             columns (log-uniform) with 40..137 layers, LW or SW, all-sky McICA;
             p50/p99 submit-to-done latency and throughput per arrival rate.
+  triple  - stratiform + convective cloud (triple-column regions) on 4096
+            columns x 90 layers, SW and LW in one graph-captured batch.
 
 Inputs are synthetic (socrates_amd/synth.py, seeded). Throughput is in
 column.g-point solves/s, and also normalised to 90 layers (x L/90).
 
     python -m socrates_amd.benchmarks ragged [--steps 10]
-    python -m socrates_amd.benchmarks serve [--rates 50,200,1000,2000] [--seconds 10]
+    python -m socrates_amd.benchmarks serve [--rates 50,200,1000,2000] [--seconds 10] [--inputs host|device]
+    python -m socrates_amd.benchmarks triple [--steps 10]
 """
 import argparse
 import json
@@ -95,7 +98,16 @@ def ragged(steps=10, warmup=2, n_columns=10000, seed=20261206):
 
 
 def serve(rates=(50, 200, 1000, 2000), seconds=10.0, pool=48, seed=20261207, precisions=("fp64", "mixed"),
-          max_requests=4000):
+          max_requests=4000, inputs="host"):
+    """inputs="host": each call hands over host arrays, which cross PCIe in
+    and out (a host model whose state lives in host memory). inputs="device":
+    the profiles already sit in HBM and each call writes fresh device output
+    arrays (a host model whose state is device-resident)."""
+    on_dev = inputs == "device"
+    if on_dev:   # torch takes the device before the library does
+        import torch
+        dev = torch.device("cuda:0")
+        torch.zeros(1, device=dev)
     from socrates_amd.serve import Server
     R.init(0)
     R.register_default_spectra()
@@ -107,13 +119,21 @@ def serve(rates=(50, 200, 1000, 2000), seconds=10.0, pool=48, seed=20261207, pre
         L = int(rng.integers(40, 138))
         case = synth.make_case(max(P, 1), L, src, NBAND[src], seed=seed + k, cloudy=True,
                                first_column_id=100000 * k)
-        templates.append((src, case))
+        templates.append((src, case, _device_case(torch, case, dev)[0] if on_dev else None))
+
+    def call(src, case, arrs):
+        if arrs is None:
+            return R.case_call(case, src)
+        L, P = case.n_layer, case.n_profile
+        out = {k: torch.empty(((L + 1) if k != "heating_rate" else L, P), dtype=torch.float64, device=dev)
+               for k in ("flux_up", "flux_down", "flux_direct", "heating_rate")}
+        return R.case_call(case, src, device_arrays=arrs, out=out)
     lines = []
     for prec in precisions:
         for rate in rates:
             srv = Server(precision=prec, hbm_budget_gb=96.0, n_slots=2, max_batch_requests=512)
             # warm the slots' buffers
-            srv.wait(srv.submit(*R.case_call(templates[0][1], templates[0][0])))
+            srv.wait(srv.submit(*call(*templates[0])))
             srv.reset_stats()
             n = int(min(max_requests, rate * seconds))
             gaps = rng.exponential(1.0 / rate, n)
@@ -139,15 +159,16 @@ def serve(rates=(50, 200, 1000, 2000), seconds=10.0, pool=48, seed=20261207, pre
                 now = time.perf_counter()
                 if t_next > now:
                     time.sleep(t_next - now)
-                src, case = templates[picks[i]]
-                q.put(srv.submit(*R.case_call(case, src)))
+                src, case, arrs = templates[picks[i]]
+                q.put(srv.submit(*call(src, case, arrs)))
                 solves += case.n_profile * NG[src] * case.n_layer / 90.0
             q.put(None)
             th.join()
             wall = time.perf_counter() - t_start
             st = srv.stats()
             srv.close()
-            line = {"benchmark": "serving loop, Poisson arrivals", "precision": prec, "rate_req_s": rate,
+            line = {"benchmark": "serving loop, Poisson arrivals", "inputs": inputs, "precision": prec,
+                    "rate_req_s": rate,
                     "requests": n, "offered_s": float(gaps.sum()), "wall_s": wall,
                     "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
                     "mean_batch": st["mean_batch"], "batches": st["batches"],
@@ -157,18 +178,60 @@ def serve(rates=(50, 200, 1000, 2000), seconds=10.0, pool=48, seed=20261207, pre
     return lines
 
 
+def triple(steps=10, warmup=2, P=4096, L=90, seed=20261209):
+    """Stratiform + convective cloud as triple-column regions
+    (solver_triple_hogan), SW and LW on P x L columns, one batch per step,
+    FP64 and mixed; (column, k-term) solves per second."""
+    import torch
+    dev = torch.device("cuda:0")
+    R.init(0)
+    R.register_default_spectra()
+    calls, keep = [], []
+    for src in ("sw", "lw"):
+        case = synth.make_case(P, L, src, NBAND[src], seed=seed + (src == "sw"), cloudy=True, overlap="triple")
+        arrs, out = _device_case(torch, case, dev)
+        for k in ("conv_frac", "liq_conv_mmr", "liq_conv_dim"):
+            arrs[k] = torch.from_numpy(np.ascontiguousarray(getattr(case, k))).to(dev)
+        keep.append((arrs, out))
+        calls.append(R.case_call(case, src, device_arrays=arrs, out=out))
+    reqs = R.Requests(calls)
+    solves = P * (NG["sw"] + NG["lw"])
+    res = {}
+    for prec in ("fp64", "mixed"):
+        b = R.Batch(prec)
+        for _ in range(warmup):
+            b.run(reqs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run(reqs)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        b.close()
+        res[prec] = dict(ms_per_step=dt * 1e3, solves_per_s=solves / dt)
+    line = {"benchmark": "triple-column cloud, one graph-captured batch of the SW and LW calls per step",
+            "columns": P, "layers": L, "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_over_fp64": res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"]}
+    print(json.dumps(line), flush=True)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["ragged", "serve"])
+    ap.add_argument("what", choices=["ragged", "serve", "triple"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--rates", default="50,200,1000,2000")
     ap.add_argument("--seconds", type=float, default=10.0)
+    ap.add_argument("--inputs", choices=["host", "device"], default="host")
     ap.add_argument("--max-requests", type=int, default=4000)
     a = ap.parse_args()
     if a.what == "ragged":
         ragged(steps=a.steps)
+    elif a.what == "triple":
+        triple(steps=a.steps)
     else:
-        serve(rates=[int(x) for x in a.rates.split(",")], seconds=a.seconds, max_requests=a.max_requests)
+        serve(rates=[int(x) for x in a.rates.split(",")], seconds=a.seconds, max_requests=a.max_requests,
+              inputs=a.inputs)
 
 
 if __name__ == "__main__":

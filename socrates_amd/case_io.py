@@ -14,7 +14,11 @@ import dataclasses
 import numpy as np
 
 MAGIC = 1397703491
-VERSION = 1
+VERSION = 2     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds
+
+# further gases runes takes as well-mixed mixing ratios (the *_mix_ratio
+# arguments, socrates_runes.F90:69-83), in this order
+WM_GASES = ("n2o", "ch4", "o2", "so2", "ocs", "cfc11", "cfc12", "cfc113", "hcfc22", "hfc134a")
 
 _HDR = ("n_profile", "n_layer", "n_band", "i_source", "l_cloud", "i_cloud_repr",
         "i_overlap", "i_inhom", "i_mcica_sampling", "l_rayleigh", "i_st_water",
@@ -70,6 +74,7 @@ class Case:
     liq_conv_mmr: np.ndarray = None
     liq_conv_dim: np.ndarray = None
     rand_seed: np.ndarray = None
+    wm_mix_ratio: np.ndarray = None   # by WM_GASES, 0 = absent
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile
@@ -79,6 +84,8 @@ class Case:
                         else np.full((L, P), 1.0e-5))
         if self.rand_seed is None:
             self.rand_seed = np.arange(P, dtype=np.int32)
+        if self.wm_mix_ratio is None:
+            self.wm_mix_ratio = np.zeros(len(WM_GASES))
         return self
 
     def subset(self, cols):
@@ -87,7 +94,7 @@ class Case:
         out = dataclasses.replace(self, n_profile=int(cols.size))
         for f in dataclasses.fields(self):
             v = getattr(self, f.name)
-            if isinstance(v, np.ndarray):
+            if isinstance(v, np.ndarray) and f.name != "wm_mix_ratio":
                 setattr(out, f.name, np.ascontiguousarray(v[..., cols]))
         return out
 
@@ -100,6 +107,7 @@ class Case:
             for k in _LAYER + _LEVEL + _COLUMN + _BAND + _CLOUD + _CONV:
                 np.ascontiguousarray(getattr(self, k), dtype=np.float64).tofile(f)
             np.ascontiguousarray(self.rand_seed, dtype=np.int32).tofile(f)
+            np.ascontiguousarray(self.wm_mix_ratio, dtype=np.float64).tofile(f)
 
     @classmethod
     def read(cls, path):
@@ -124,6 +132,9 @@ class Case:
                                         offset=off).reshape(shapes[k]).copy())
             off += 8 * n
         c.rand_seed = np.frombuffer(raw, dtype=np.int32, count=P, offset=off).copy()
+        off += 4 * P
+        if hdr[1] >= 2:
+            c.wm_mix_ratio = np.frombuffer(raw, dtype=np.float64, count=len(WM_GASES), offset=off).copy()
         return c
 
 
@@ -153,6 +164,8 @@ def case_to_arrays(case):
 def case_from_arrays(d):
     kw = {}
     for f in dataclasses.fields(Case):
+        if "case_" + f.name not in d:   # fields added after the arrays were written
+            continue
         v = d["case_" + f.name]
         if v.ndim == 0:
             v = v.item()

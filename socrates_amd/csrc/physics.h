@@ -118,6 +118,27 @@ SG_HD void slingo(const double *cp, double mmr, double re, double &kext, double 
 // McICA cloud-generator linear congruential step (socrates_cloud_gen.F90:48-50
 // constants, :196-205 step with the FP64 multiply-truncate of the reference)
 constexpr int kLcgM = 86436, kLcgA = 1093, kLcgC = 18257;
+// k steps of the generator from x in one: F^k(x) = (a_k x + c_k) mod m with
+// a_k = a^k mod m and c_k = (a c_(k-1) + c) mod m. lcg_step's FP64
+// multiply-truncate is the exact residue for every int32 argument (checked
+// over the whole range), so F^k is exactly k applications of lcg_step.
+constexpr int kLcgSkipMax = 2 * 64 + 64;
+struct LcgSkip {
+  int a[kLcgSkipMax + 1], c[kLcgSkipMax + 1];
+};
+constexpr LcgSkip make_lcg_skip() {
+  LcgSkip s{};
+  long long A = 1, C = 0;
+  s.a[0] = 1;
+  s.c[0] = 0;
+  for (int k = 1; k <= kLcgSkipMax; k++) {
+    A = (A * kLcgA) % kLcgM;
+    C = (C * kLcgA + kLcgC) % kLcgM;
+    s.a[k] = (int)A;
+    s.c[k] = (int)C;
+  }
+  return s;
+}
 SG_HD int lcg_step(int x) {
   const double rm = 1.0 / (double)kLcgM;
   int t = x * kLcgA + kLcgC;

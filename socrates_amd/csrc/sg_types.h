@@ -12,7 +12,7 @@
 
 #define SG_MAXB 16          // bands per file
 #define SG_MAXG 96          // major k-terms (g-points) per file
-#define SG_MAXMINOR_GAS 6   // minor gases per band
+#define SG_MAXMINOR_GAS 11  // minor gases per band
 #define SG_MAXMINOR 256     // minor k-terms per file
 #define SG_MAXSUB 192       // McICA sub-column items per file (sum of subcol_k)
 #define SG_NPL 8            // Planck polynomial coefficients
@@ -20,19 +20,37 @@
 #define SG_NDPARM 16        // droplet fit parameters
 #define SG_MAXL 160         // layers per column
 #define SG_MAXSUBCOL 64     // generated McICA sub-columns per column
+#define SG_MAXCONT 2        // continua per band
+#define SG_MAXPRE 64        // pressures of the k-term look-up table
+#define SG_MAXTMP 8         // temperatures per pressure of the look-up table
 
 // gas slots of the column record
 #define SG_GAS_H2O 0
 #define SG_GAS_CO2 1
 #define SG_GAS_O3 2
 #define SG_GAS_ZERO 3       // absorber the call supplies no amount for
+#define SG_GAS_WM0 4        // first of the well-mixed further gases (slot
+                            // SG_GAS_WM0 + k for the call's wm_mix_ratio[k])
+#define SG_NWM 10           // well-mixed further gases: n2o, ch4, o2, so2, ocs,
+                            // cfc11, cfc12, cfc113, hcfc22, hfc134a
+
+// scaling functions of the k-terms (rad_pcf.F90:509-532)
+#define SG_FNC_POWER_LAW 1
+#define SG_FNC_POWER_QUAD 2
+#define SG_FNC_WENYI 4      // ip_scale_wenyi, CO2 in band 4 only
+#define SG_FNC_LOOKUP 9
 
 struct SgTerm {             // one k-term (ESFT term)
   double k, w;              // absorption coefficient (m2/kg), weight
-  double a, b;              // power-law exponents (pressure, temperature)
+  double a, b, c;           // scaling parameters: power law (a, b), power
+                            // quadratic (a, b, c)
   double lnpref, lntref;    // log of the reference pressure and temperature
+  double tinv;              // 1 / reference temperature
   int band, gas;            // band, gas slot
   int n_sub, sub0;          // McICA: sampled sub-columns, first sub item
+  int fnc;                  // SG_FNC_*
+  int lut;                  // look-up table of the term (SG_FNC_LOOKUP), or
+                            // the term's index in the Wenyi tables
 };
 
 struct SgBand {
@@ -42,6 +60,13 @@ struct SgBand {
   double solar_flux;                   // normalised solar flux in the band
   double rayleigh;                     // Rayleigh mass scattering coefficient
   double planck[SG_NPL];               // thermal_coeff(0:n_deg)
+  // continua, in the band's order (radiance_calc.F90:897-923): continuum
+  // type (1 self-broadened water vapour, 2 foreign-broadened), scaling
+  // function, coefficient, reference pressure and temperature, parameters
+  int n_cont;
+  int cont_type[SG_MAXCONT], cont_fnc[SG_MAXCONT];
+  double cont_k[SG_MAXCONT], cont_pref[SG_MAXCONT], cont_tref[SG_MAXCONT];
+  double cont_s[SG_MAXCONT][3];
 };
 
 struct SgDrop {
@@ -59,6 +84,15 @@ struct SgSpec {
   int n_absorb;                  // gaseous absorbers of the file and their
   int type_absorb[SG_MAXABS];    // identifiers (rad_pcf.F90 ip_h2o = 1, ...)
   double t_ref_planck;
+  // look-up-table k-terms (read_spectrum.F90:1404-1507): the P/T grid (ln p,
+  // and t_lookup(it, ip) as [ip][it]) and the device tables, term t.lut at
+  // klut + t.lut * n_pre * n_tmp as [ip][it]; the Wenyi CO2 tables
+  // (10 x 26 x 14 gk4, then gk250b, plg, ttb) when a term uses them
+  int n_pre, n_tmp, has_lut, has_cont;
+  double lut_lnp[SG_MAXPRE];
+  double lut_t[SG_MAXPRE][SG_MAXTMP];
+  const double *klut;
+  const double *wenyi;
   SgBand band[SG_MAXB];
   SgTerm g[SG_MAXG];
   SgTerm minor[SG_MAXMINOR];
@@ -92,7 +126,23 @@ struct SgSpec {
 #define SG_Q_LWC2 14   // in-cloud convective liquid mass mixing ratio
 #define SG_Q_RE2 15    // convective effective radius clamped to the fit range
 #define SG_Q_FRAC2 16  // frac_cloud of the convective type
+// McICA representation: the cloud generator's exponential-random overlap
+// parameters of the layer and the one above (in the triple-column slots)
+#define SG_Q_ALPHA 14
+#define SG_Q_RCORR 15
 #define SG_NQ 17
+
+// per-layer gas record of calls whose spectrum has look-up-table terms or
+// continua, [col][layer][SG_NGQ]: the look-up interpolation factors
+// (inter_pt_lookup.F90:184-189), the indices jp, jt, jtt (0-based, packed
+// as jp + 256 jt + 65536 jtt) and the layer density
+#define SG_G_F00 0
+#define SG_G_F10 1
+#define SG_G_F01 2
+#define SG_G_F11 3
+#define SG_G_IDX 4
+#define SG_G_RHO 5
+#define SG_NGQ 6
 
 // per-column scalars, [col][SG_NS]
 #define SG_S_SEC0 0    // zen_0 (secant of the solar zenith angle, or 1)

@@ -139,6 +139,7 @@ struct CallDims {
   int n_gen, n_fill, ipph, ioverlap;
   int triple, drop2; // triple-column representation; convective droplet type
   int raw_range;     // cloud range on the fractions as given (kernel-level cloud_gen)
+  int cg_pre;        // overlap parameters precomputed by the pack kernel
   int f32;           // mixed form: McICA c_sub and droplet optics kept in FP32
 };
 
@@ -147,9 +148,11 @@ struct CallDims {
 // threshold 0.001, type-fraction normalisation), set_cld_dim
 // (socrates_set_cld_dim.F90:214-220: radius clamp), cond_rsd of
 // set_cld_mcica (socrates_set_cld_mcica.F90:168-180).
-// Record of layer i of column col (caller index idx = i P + col) into r.
+// Record of layer i of column col (caller index idx = i P + col) into r;
+// dp_cf, dp_cw: the call's decorrelation pressure scales (McICA).
 __device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__restrict__ spec,
-                                           const SocgpuRunesArgs &a, int idx, double *r) {
+                                           const SocgpuRunesArgs &a, int idx, double dp_cf, double dp_cw,
+                                           double *r) {
   {
     double p = a.p_layer[idx], t = a.t_layer[idx];
     r[SG_Q_LNP] = log(p);
@@ -212,6 +215,20 @@ __device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__re
     r[SG_Q_LWC2] = lwc2;
     r[SG_Q_RE2] = re2;
     r[SG_Q_FRAC2] = f2;
+    if (d.mcica) {
+      // exponential-random overlap parameters between this layer and the
+      // one above (socrates_cloud_gen.F90:359-364), as the generator forms
+      // them (same operations, no contraction)
+#pragma clang fp contract(off)
+      double alpha = 0.0, rcorr = 0.0;
+      if (idx >= d.P) {
+        const double dz = a.p_layer[idx - d.P] - p;
+        alpha = dp_cf > 0.0 ? exp(dz / dp_cf) : 0.0;
+        rcorr = dp_cw > 0.0 ? exp(dz / dp_cw) : 0.0;
+      }
+      r[SG_Q_ALPHA] = alpha;
+      r[SG_Q_RCORR] = rcorr;
+    }
   }
 }
 
@@ -271,12 +288,14 @@ __device__ double zcw_lookup(double y, double sigma, int n1, int n2, const doubl
   return ((t1 + t2) + t3) + t4;
 }
 
+__constant__ LcgSkip kLcgSkip = make_lcg_skip();
+
 // One layer k (1-based) of one sub-column's draws (cloud_gen:253-405):
 // updates the x stream (cloud presence) and, with Y, the y stream
 // (condensate rank). The x stream never depends on the y stream.
 template <bool Y>
-__device__ __forceinline__ void cg_layer(int ioverlap, int k, const double *__restrict__ rec, double dp_cf,
-                                         double dp_cw, int &ix, int &iy, double &x, double &y) {
+__device__ __forceinline__ void cg_layer(int ioverlap, bool pre, int k, const double *__restrict__ rec,
+                                         double dp_cf, double dp_cw, int &ix, int &iy, double &x, double &y) {
 #pragma clang fp contract(off)
   const double rm = 1.0 / (double)kLcgM;
   if (ioverlap == 0) {
@@ -300,9 +319,14 @@ __device__ __forceinline__ void cg_layer(int ioverlap, int k, const double *__re
     if (ioverlap != 1 && k - 1 >= 1) {
       double cfa = rec[(k - 2) * SG_NQ + SG_Q_WCLD];
       if (cfa > 0.0) {
-        double dz = rec[(k - 2) * SG_NQ + SG_Q_P] - rec[(k - 1) * SG_NQ + SG_Q_P];
-        alpha = dp_cf > 0.0 ? exp(dz / dp_cf) : 0.0;
-        if (Y) rcorr = dp_cw > 0.0 ? exp(dz / dp_cw) : 0.0;
+        if (pre) {
+          alpha = rec[(k - 1) * SG_NQ + SG_Q_ALPHA];
+          if (Y) rcorr = rec[(k - 1) * SG_NQ + SG_Q_RCORR];
+        } else {
+          double dz = rec[(k - 2) * SG_NQ + SG_Q_P] - rec[(k - 1) * SG_NQ + SG_Q_P];
+          alpha = dp_cf > 0.0 ? exp(dz / dp_cf) : 0.0;
+          if (Y) rcorr = dp_cw > 0.0 ? exp(dz / dp_cw) : 0.0;
+        }
       }
     }
     ix = lcg_step(ix); double x1 = (double)ix * rm;
@@ -331,12 +355,16 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   const int k_top = krange[0], k_base = krange[1];
   const bool valid = lane < n_gen;
-  // seeds of this sub-column (cloud_gen:180-205)
-  int rs = abs(seeds[col]) % seed_max;
-  for (int t = 0; t < n_gen; t++) rs = lcg_step(rs);
+  // seeds of this sub-column (cloud_gen:180-205): the stream from the
+  // column's seed, warmed up by n_gen steps; sub-column j takes the values
+  // after n_gen + j + 1 and 2 n_gen + j + 1 steps (taken in one skip each)
+  const long long rs0 = abs(seeds[col]) % seed_max;
   int sx = 0, sy = 0;
-  for (int t = 0; t < n_gen; t++) { rs = lcg_step(rs); if (t == lane) sx = rs; }
-  for (int t = 0; t < n_gen; t++) { rs = lcg_step(rs); if (t == lane) sy = rs; }
+  if (valid) {
+    const int kx = n_gen + 1 + lane, ky = 2 * n_gen + 1 + lane;
+    sx = (int)(((long long)kLcgSkip.a[kx] * rs0 + kLcgSkip.c[kx]) % kLcgM);
+    sy = (int)(((long long)kLcgSkip.a[ky] * rs0 + kLcgSkip.c[ky]) % kLcgM);
+  }
   const bool any = k_top <= L;
   const int k0 = any ? k_top : 1, k1 = any ? k_base : 0;
   // pass 1: cloud presence of this lane's sub-column
@@ -347,7 +375,7 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
     for (int k = k0; k <= k1; k++) {
       const double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
       if (valid && cf > 0.0) {
-        cg_layer<false>(d.ioverlap, k, rec, dp_cf, dp_cw, ix, iy, x, y);
+        cg_layer<false>(d.ioverlap, d.cg_pre, k, rec, dp_cf, dp_cw, ix, iy, x, y);
         if (x < cf) cloudy = true;
       }
     }
@@ -369,7 +397,7 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
     double val = 0.0;
     const double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
     if (src >= 0 && k >= k0 && k <= k1 && cf > 0.0) {
-      cg_layer<true>(d.ioverlap, k, rec, dp_cf, dp_cw, ix, iy, x, y);
+      cg_layer<true>(d.ioverlap, d.cg_pre, k, rec, dp_cf, dp_cw, ix, iy, x, y);
       if (x < cf)
         val = d.ipph == 0 ? 1.0 : zcw_lookup(y, rec[(k - 1) * SG_NQ + SG_Q_RSD], n1, n2, xcw) * 1.0;
     }
@@ -405,10 +433,11 @@ __device__ __forceinline__ void scan_affine(double &A, double &C, int lane) {
   }
 }
 
-// Equivalent extinction of a band's minor gases (solve_band_k_eqv_scl.F90:
-// 572-1047), one wave per (column, band with minor gases), lanes over
-// layers (64 at a time, at most kKeqChunks chunks, with carries between
-// them), the band's gases and their k-terms in turn:
+// One wave per (column, band), lanes over layers (64 at a time, at most
+// NCH chunks, with carries between them): the equivalent extinction of the
+// band's minor gases (solve_band_k_eqv_scl.F90:572-1047), then the band's
+// per-layer records (prep_layer). The band's gases and their k-terms in
+// turn:
 //  - solar: each term's direct-beam transmission down the column (a wave
 //    product scan, :654-739); k_eqv = sum_k k T_k(surface) / sum_k
 //    T_k(surface) per gas (:826-846), and adjust_solar_ke from the per-layer
@@ -419,10 +448,9 @@ __device__ __forceinline__ void scan_affine(double &A, double &C, int lane) {
 //    (:995-1019);
 //  - where the weighting flux vanishes, the smallest k of the gas's terms.
 // Per lane, the layer records and (infra-red) the band's Planck fluxes at
-// the layer's levels are read or evaluated once for all terms.
-// keqv[col][band][layer][2] = (sum over gases of k_eqv, adjust_solar_ke).
-// The scans associate the products differently from the reference's
-// left-to-right sweeps (relative differences of order 1e-16).
+// the layer's levels are read or evaluated once for all terms and for the
+// records. The scans associate the products differently from the
+// reference's left-to-right sweeps (relative differences of order 1e-16).
 // NCH: chunks of 64 layers the lane arrays hold (kernels for NCH = 2 and 3
 // each take the calls with that many chunks, so a call of at most 128
 // layers runs with the smaller register footprint).
@@ -430,15 +458,17 @@ constexpr int kMinorWaves = 4;
 constexpr int kKeqMaxChunks = (SG_MAXL + kWave - 1) / kWave;
 
 template <int kKeqChunks>
-__device__ __forceinline__ void keqv_body(const CallDims &d, const SgSpec *__restrict__ spec,
-                                          const double *__restrict__ colrec,
+__device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                          int l_rayleigh, const double *__restrict__ colrec,
                                           const double *__restrict__ tlev,
                                           const double *__restrict__ colscal,
-                                          const double *__restrict__ bandscal,
-                                          double *__restrict__ keqv, int wave, int lane) {
-  const int L = d.L, NMB = spec->n_mband;
-  if (wave >= d.P * NMB) return;
-  const int col = wave / NMB, b = spec->mband[wave % NMB];
+                                          double *__restrict__ bandrec,
+                                          double *__restrict__ bandscal,
+                                          double *__restrict__ cldopt, const int *__restrict__ krange,
+                                          int wave, int lane) {
+  const int L = d.L, NB = d.NB;
+  if (wave >= d.P * NB) return;
+  const int col = wave / NB, b = wave % NB;
   const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   const int nch = (L + kWave - 1) / kWave;
@@ -589,37 +619,31 @@ __device__ __forceinline__ void keqv_body(const CallDims &d, const SgSpec *__res
     }
     t0 += nk;
   }
-  double *o = keqv + ((size_t)col * d.NB + b) * L * 2;
 #pragma unroll
   for (int c = 0; c < kKeqChunks; c++) {
     const int i = c * kWave + lane;
-    if (c < nch && i < L) {
-      o[2 * i] = keq[c];
-      o[2 * i + 1] = adj[c];
-    }
+    if (c < nch && i < L)
+      prep_layer(d, spec, l_rayleigh, colrec, colscal, bandrec, bandscal, cldopt, krange, col, b, i, keq[c],
+                 adj[c], b0[c], b1[c]);
   }
 }
 
-// Per (column, band, layer): the grey extinction (Rayleigh, grey_opt_prop.F90:
-// 476-541, 816-835) plus the equivalent extinction of the minor gases
-// (keqv_body), the solar direct-beam adjustment with its exponential of the
-// equivalent optical depth (solve_band_k_eqv_scl.F90:848-891), the Planck
-// source terms (diff_planck_source_mod.F90:100-134) and the cloud optics.
+// Per (column, band, layer), given the layer's equivalent extinction and
+// direct-beam adjustment product: the grey extinction (Rayleigh,
+// grey_opt_prop.F90:476-541, 816-835, plus k_eqv), the solar direct-beam
+// adjustment with the exponential of the equivalent optical depth
+// (solve_band_k_eqv_scl.F90:848-891), the Planck source terms
+// (diff_planck_source_mod.F90:100-134) and the cloud optics.
 // bandrec[col][band][layer][4].
-__device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__restrict__ spec,
-                                          int l_rayleigh, const double *__restrict__ colrec,
-                                          const double *__restrict__ tlev,
-                                          const double *__restrict__ colscal,
-                                          const double *__restrict__ minor,
-                                          double *__restrict__ bandrec,
-                                          double *__restrict__ bandscal,
-                                          double *__restrict__ cldopt, const int *__restrict__ krange,
-                                          int idx) {
+__device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__restrict__ spec,
+                                           int l_rayleigh, const double *__restrict__ colrec,
+                                           const double *__restrict__ colscal,
+                                           double *__restrict__ bandrec,
+                                           double *__restrict__ bandscal,
+                                           double *__restrict__ cldopt, const int *__restrict__ krange,
+                                           int col, int b, int i, double keq, double adjust, double b0,
+                                           double b1) {
   const int NB = d.NB, L = d.L;
-  if (idx >= d.P * NB * L) return;
-  // bands fastest: the lanes of a wave share a few consecutive layer
-  // records (one read each) and write runs of consecutive layers per band
-  const int b = idx % NB, i = (idx / NB) % L, col = idx / (L * NB);
   const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
   const double *r = rec + i * SG_NQ;
@@ -649,12 +673,6 @@ __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__res
   double *bs = bandscal + ((size_t)col * NB + b) * SG_NBQ;
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
   const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
-  double keq = 0.0, adjust = 1.0;
-  if (B.n_minor_gas > 0) {
-    const double *kq = minor + (((size_t)col * NB + b) * L + i) * 2;
-    keq = kq[0];
-    adjust = kq[1];
-  }
   const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
   br[0] = (0.0 + ray) + keq;
   if (d.triple) {
@@ -699,9 +717,7 @@ __device__ __forceinline__ void prep_body(const CallDims &d, const SgSpec *__res
     br[3] = 0.0;
     if (i == 0) { bs[0] = colscal[(size_t)col * SG_NS + SG_S_SOL] * B.solar_flux; bs[1] = 0.0; }
   } else {
-    const double *tl = tlev + (size_t)col * (L + 1);
-    double b0 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i]);
-    double b1 = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[i + 1]);
+    // b0, b1: the band's Planck fluxes at the top and bottom of the layer
     double bt = planck_band(B, spec->n_deg, spec->t_ref_planck, r[SG_Q_T]);
     br[1] = 2.0 * (b1 + b0 - 2.0 * bt);  // diff_2
     br[2] = b1 - b0;                     // diff
@@ -1366,6 +1382,9 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   const double alb_diff = bs[2], alb_dir = bs[3];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down (= flux_inc_direct)
 #define TWS(i, q) ws[((size_t)(i) * kTriQ + (q)) * kWave]
+  // lanes past the last k-term neither store nor load the workspace (their
+  // results carry weight 0; zeros keep them finite)
+#define TWL(i, q) (active ? TWS(i, q) : 0.0)
 
   // --- sweep 1 (solar): direct flux entering each region of each layer
   double base0 = finc, base1 = 0.0, base2 = 0.0;  // direct flux leaving the layer above, per region
@@ -1402,9 +1421,9 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
         base1 = top1 * o[1].T0 * adj;
         base2 = top2 * o[2].T0 * adj;
       }
-      TWS(i, 0) = top0;
-      TWS(i, 1) = top1;
-      TWS(i, 2) = top2;
+      if (active) TWS(i, 0) = top0;
+      if (active) TWS(i, 1) = top1;
+      if (active) TWS(i, 2) = top2;
     }
     {  // remaining direct-flux levels (the partial block and the surface)
       const int bl = L & ~7, m = L & 7;
@@ -1448,7 +1467,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         if (k < nreg) {
-          const double top = TWS(i, k);
+          const double top = TWL(i, k);
           const double base = top * o[k].T0 * adj;
           su[k] = o[k].s1 * top;
           sd[k] = (o[k].s2 - o[k].T0) * top + base;
@@ -1486,10 +1505,10 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
         lsrc[k] = su[k] + h * T * binv;
         aa[k] = R + th[k] * T * T * binv;
         const double c1 = gam * binv, c2 = h * binv;
-        TWS(i, 4 * k + 0) = c1;
-        TWS(i, 4 * k + 1) = c2;
-        TWS(i, 4 * k + 2) = T + R * c1;
-        TWS(i, 4 * k + 3) = R * c2 + sd[k];
+        if (active) TWS(i, 4 * k + 0) = c1;
+        if (active) TWS(i, 4 * k + 1) = c2;
+        if (active) TWS(i, 4 * k + 2) = T + R * c1;
+        if (active) TWS(i, 4 * k + 3) = R * c2 + sd[k];
       }
       const double *ut = ov + (size_t)i * kOv + 9;    // interface at the top of layer i+1
       a0 = aa[0]; a1 = aa[1]; a2 = aa[2];
@@ -1510,10 +1529,10 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
       a0 = R + lam * gam;
       g0 = su[0] + lam * h;
       const double c1 = gam * binv, c2 = h * binv;
-      TWS(i, 0) = c1;
-      TWS(i, 1) = c2;
-      TWS(i, 2) = T + R * c1;
-      TWS(i, 3) = R * c2 + sd[0];
+      if (active) TWS(i, 0) = c1;
+      if (active) TWS(i, 1) = c2;
+      if (active) TWS(i, 2) = T + R * c1;
+      if (active) TWS(i, 3) = R * c2 + sd[0];
     }
   }
 
@@ -1546,8 +1565,8 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   for (int i = 0; i < L; i++) {
     double fu, fd;
     if (i + 1 < nct) {
-      fu = TWS(i, 0) * fdt + TWS(i, 1);
-      fdt = TWS(i, 2) * fdt + TWS(i, 3);
+      fu = TWL(i, 0) * fdt + TWL(i, 1);
+      fdt = TWL(i, 2) * fdt + TWL(i, 3);
       fd = fdt;
     } else {
       const double *vt = ov + (size_t)i * kOv;
@@ -1559,12 +1578,12 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
         d1 = vt[3] * x0 + vt[4] * x1 + vt[5] * x2;
         d2 = vt[6] * x0 + vt[7] * x1 + vt[8] * x2;
       }
-      const double u0 = TWS(i, 0) * d0 + TWS(i, 1);
-      const double u1 = TWS(i, 4) * d1 + TWS(i, 5);
-      const double u2 = TWS(i, 8) * d2 + TWS(i, 9);
-      d0 = TWS(i, 2) * d0 + TWS(i, 3);
-      d1 = TWS(i, 6) * d1 + TWS(i, 7);
-      d2 = TWS(i, 10) * d2 + TWS(i, 11);
+      const double u0 = TWL(i, 0) * d0 + TWL(i, 1);
+      const double u1 = TWL(i, 4) * d1 + TWL(i, 5);
+      const double u2 = TWL(i, 8) * d2 + TWL(i, 9);
+      d0 = TWL(i, 2) * d0 + TWL(i, 3);
+      d1 = TWL(i, 6) * d1 + TWL(i, 7);
+      d2 = TWL(i, 10) * d2 + TWL(i, 11);
       fu = u0 + u1 + u2;
       fd = d0 + d1 + d2;
       fdt = fd;
@@ -1572,6 +1591,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
     put(i + 1, fu, fd);
   }
 #undef TWS
+#undef TWL
 }
 
 // ------------------------------------------------------------ batches
@@ -1580,8 +1600,9 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
 // every kernel loops over the concatenated work of all requests and finds
 // the request of a work item by binary search over per-request starts. A
 // request's results are those of the same call made alone.
-// W_LONG counts the equivalent-extinction waves of calls with more than 128
-// layers (it only decides whether their kernel is launched).
+// W_MINOR: waves of the per-(column, band) kernel; W_LONG counts those of
+// calls with more than 128 layers (it only decides whether their kernel is
+// launched).
 enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, W_LONG, kNWork };
 
 struct ReqDesc {
@@ -1643,7 +1664,7 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
     const int ntc = (P + kPackC - 1) / kPackC;
     const int c0 = (t % ntc) * kPackC, i0 = (t / ntc) * kPackL;
     const int col = c0 + tx, i = i0 + ty;
-    if (col < P && i < L) pack_layer(q.d, q.spec, q.a, i * P + col, sh + (tx * kPackL + ty) * SG_NQ);
+    if (col < P && i < L) pack_layer(q.d, q.spec, q.a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ);
     if (col < P && i <= L) shl[tx * kPackL + ty] = q.a.t_level ? q.a.t_level[(size_t)i * P + col] : 0.0;
     if (col < P && i == 0) pack_column(q.d, q.a, col, q.colscal, q.bandscal);
     __syncthreads();
@@ -1757,7 +1778,7 @@ __global__ __launch_bounds__(64 * kCgWaves) void b_cloudgen_kernel(const BatchHd
 }
 
 template <int NCH>
-__global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? 4 : 2) void b_keqv_kernel(const BatchHdr *h, const ReqDesc *rq) {
+__global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? 3 : 2) void b_band_kernel(const BatchHdr *h, const ReqDesc *rq) {
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const long long tot = h->tot[W_MINOR];
   for (long long w = blockIdx.x * (long long)kMinorWaves + wv; w < tot; w += (long long)gridDim.x * kMinorWaves) {
@@ -1765,20 +1786,11 @@ __global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? 4 : 2) void b_keqv_ker
     const ReqDesc &q = rq[r];
     const int nch = (q.d.L + kWave - 1) / kWave;
     if ((NCH == 2 && nch <= 2) || (NCH > 2 && nch > 2))
-      keqv_body<NCH>(q.d, q.spec, q.colrec, q.tlev, q.colscal, q.bandscal, q.minor, (int)(w - q.s[W_MINOR]),
-                     lane);
+      band_body<NCH>(q.d, q.spec, q.l_rayleigh, q.colrec, q.tlev, q.colscal, q.bandrec, q.bandscal, q.cldopt,
+                     q.krange, (int)(w - q.s[W_MINOR]), lane);
   }
 }
 
-__global__ __launch_bounds__(256) void b_prep_kernel(const BatchHdr *h, const ReqDesc *rq) {
-  const long long tot = h->tot[W_PREP];
-  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
-       g += (long long)gridDim.x * blockDim.x) {
-    const ReqDesc &q = rq[find_req(h, rq, W_PREP, g)];
-    prep_body(q.d, q.spec, q.l_rayleigh, q.colrec, q.tlev, q.colscal, q.minor, q.bandrec, q.bandscal,
-              q.cldopt, q.krange, (int)(g - q.s[W_PREP]));
-  }
-}
 
 template <bool MIXED, bool RECOMP, int OCC>
 __global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, const ReqDesc *rq) {
@@ -1949,25 +1961,64 @@ __global__ __launch_bounds__(256) void two_stream_kernel(TwoStreamArgs a) {
 }
 
 // ---------------------------------------------------------------- host side
+// Device and pinned host buffers grow by at least half their size when they
+// have to grow (freeing either synchronises the device, so growth should
+// be rare in a long run) and are kept until released.
+inline size_t grown(size_t bytes, size_t n) { return bytes > n + n / 2 ? bytes : n + n / 2; }
+
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
   int ensure(size_t bytes) {
     if (bytes <= n) return 0;
+    const size_t want = grown(bytes, n);
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
-    if (hipMalloc(&p, bytes) != hipSuccess) return -1;
-    n = bytes;
+    if (hipMalloc(&p, want) != hipSuccess) return -1;
+    n = want;
     return 0;
   }
   template <class T> T *as() { return (T *)p; }
   ~DevBuf() {}
 };
 
+// Pinned host staging for small transfers: a call's (or batch's) host
+// inputs of at most kPinnedMax bytes are gathered here and cross PCIe in one
+// copy, and its outputs come back in one copy and are scattered to the
+// caller's arrays. Larger transfers are bandwidth-bound and go array by
+// array from the caller's memory (pinning hundreds of MB would cost more than
+// it saves). use() gives the buffer or nullptr.
+constexpr size_t kPinnedMax = 32u << 20;
+
+struct HostBuf {
+  char *p = nullptr;
+  size_t n = 0;
+  char *use(size_t bytes) {
+    if (bytes == 0 || bytes > kPinnedMax) return nullptr;
+    return ensure(bytes) == 0 ? p : nullptr;
+  }
+  int ensure(size_t bytes) {
+    if (bytes <= n) return 0;
+    const size_t want = grown(bytes, n);
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipHostMalloc((void **)&p, want, hipHostMallocDefault) != hipSuccess) return -1;
+    n = want;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
 struct SpecEntry {
   SgSpec host;
   SgSpec *dev = nullptr;
+  double *tables = nullptr;  // look-up k-tables and Wenyi tables on the device
   double *xcw = nullptr;  // McICA condensate table on the device
   bool solar_hint = false;
 };
@@ -1979,13 +2030,14 @@ struct Context {
   std::map<std::string, std::unique_ptr<SpecEntry>> spectra;
   DevBuf ws, desc;
   DevBuf in_stage, out_stage;
+  HostBuf in_host, out_host;
   double last_total_ms = 0.0, last_solve_ms = 0.0;
   std::mutex mu;
 };
 static Context g_ctx;
-// solve-kernel tile per precision form ([0] FP64, [1] mixed)
-// default solve tiles (FP64, mixed): the autotuner's picks on the
-// 4096-column x 90-layer all-sky SW + LW batch
+// solve-kernel tile per precision form ([0] FP64, [1] mixed); the defaults
+// are the autotuner's picks on the 4096-column x 90-layer all-sky SW + LW
+// batch
 static SocgpuTile g_tile[2] = {{0, 1, 3}, {1, 1, 3}};
 
 static int ensure_init() {
@@ -2023,8 +2075,15 @@ int socgpu_spectrum_register(const char *name, const char *path) {
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   auto e = std::make_unique<SpecEntry>();
   std::string err;
-  int rc = read_spectrum(path, &e->host, &err);
+  SpecTables tab;
+  int rc = read_spectrum(path, &e->host, &tab, &err);
   if (rc) return fail(rc == -3 ? SOCGPU_EUNSUPPORTED : SOCGPU_EIO, "%s: %s", path, err.c_str());
+  if (!tab.data.empty()) {
+    HIPCHK(hipMalloc(&e->tables, tab.data.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(e->tables, tab.data.data(), tab.data.size() * sizeof(double), hipMemcpyHostToDevice));
+    e->host.klut = e->host.has_lut ? e->tables : nullptr;
+    e->host.wenyi = tab.wenyi_off >= 0 ? e->tables + tab.wenyi_off : nullptr;
+  }
   HIPCHK(hipMalloc(&e->dev, sizeof(SgSpec)));
   HIPCHK(hipMemcpy(e->dev, &e->host, sizeof(SgSpec), hipMemcpyHostToDevice));
   auto it = g_ctx.spectra.find(name);
@@ -2033,6 +2092,7 @@ int socgpu_spectrum_register(const char *name, const char *path) {
     // another spectrum (socgpu_mcica_register) and is freed with its last user
     double *xcw = it->second->xcw;
     (void)hipFree(it->second->dev);
+    if (it->second->tables) (void)hipFree(it->second->tables);
     it->second->xcw = nullptr;
     bool shared = false;
     for (auto &kv : g_ctx.spectra) shared |= kv.second->xcw == xcw;
@@ -2125,10 +2185,12 @@ static size_t stage_bytes(const SocgpuRunesArgs *a, int NB) {
   return total;
 }
 
-// Copies one call's host inputs to base + *off on stream s and points dv at
-// them; device inputs are used in place.
-static int stage_inputs_at(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB, char *base, size_t *off,
-                           hipStream_t s) {
+// Gathers one call's host inputs into the pinned block hbase + *off and
+// points dv at the same offsets of the device block base (the caller copies
+// the whole block across at once); device inputs are used in place.
+// (hbase null: each array is copied across from the caller's memory on s.)
+static int stage_inputs_at(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB, char *base, char *hbase,
+                           size_t *off, hipStream_t s) {
   *dv = *a;
   if (a->l_inputs_on_device) return 0;
   InField f[32];
@@ -2136,13 +2198,15 @@ static int stage_inputs_at(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
   for (size_t k = 0; k < n; k++) {
     InField &x = f[k];
     if (!x.src) { *x.dst = nullptr; continue; }
-    HIPCHK(hipMemcpyAsync(base + *off, x.src, x.n * sizeof(double), hipMemcpyHostToDevice, s));
+    if (hbase) std::memcpy(hbase + *off, x.src, x.n * sizeof(double));
+    else HIPCHK(hipMemcpyAsync(base + *off, x.src, x.n * sizeof(double), hipMemcpyHostToDevice, s));
     *x.dst = (const double *)(base + *off);
     *off += (x.n * 8 + 255) & ~(size_t)255;
   }
   if (a->rand_seed) {
     const size_t P = a->n_profile;
-    HIPCHK(hipMemcpyAsync(base + *off, a->rand_seed, P * sizeof(int), hipMemcpyHostToDevice, s));
+    if (hbase) std::memcpy(hbase + *off, a->rand_seed, P * sizeof(int));
+    else HIPCHK(hipMemcpyAsync(base + *off, a->rand_seed, P * sizeof(int), hipMemcpyHostToDevice, s));
     dv->rand_seed = (const int *)(base + *off);
     *off += (P * 4 + 255) & ~(size_t)255;
   }
@@ -2150,9 +2214,13 @@ static int stage_inputs_at(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
 }
 
 static int stage_inputs(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB) {
-  if (g_ctx.in_stage.ensure(stage_bytes(a, NB) + 256)) return fail(SOCGPU_ENOMEM, "input staging allocation failed");
+  const size_t need = stage_bytes(a, NB) + 256;
+  if (g_ctx.in_stage.ensure(need)) return fail(SOCGPU_ENOMEM, "input staging allocation failed");
+  char *h = g_ctx.in_host.use(need);
   size_t off = 0;
-  return stage_inputs_at(a, dv, NB, g_ctx.in_stage.as<char>(), &off, g_ctx.stream);
+  if (int rc = stage_inputs_at(a, dv, NB, g_ctx.in_stage.as<char>(), h, &off, g_ctx.stream)) return rc;
+  if (h && off > 0) HIPCHK(hipMemcpyAsync(g_ctx.in_stage.p, h, off, hipMemcpyHostToDevice, g_ctx.stream));
+  return 0;
 }
 
 // Checks one call's arguments and resolves its options (socrates_set_control
@@ -2175,6 +2243,10 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   if (solar && (!a->cos_zenith_angle || !a->solar_irrad)) return fail(SOCGPU_EINVAL, "solar call needs cos_zenith_angle and solar_irrad");
   if (!solar && (!a->t_level || !a->t_ground)) return fail(SOCGPU_EINVAL, "thermal call needs t_level and t_ground");
   if (!solar && S.n_deg <= 0 && S.band[0].planck[0] == 0.0) return fail(SOCGPU_EUNSUPPORTED, "spectral file has no Planck data");
+  bool gx = S.has_lut || S.has_cont;
+  for (int t = 0; t < S.n_g; t++) gx |= S.g[t].fnc != SG_FNC_POWER_LAW;
+  for (int t = 0; t < S.n_minor; t++) gx |= S.minor[t].fnc != SG_FNC_POWER_LAW || S.minor[t].gas >= SG_GAS_WM0;
+  if (gx) return fail(SOCGPU_EUNSUPPORTED, "look-up-table, power-quadratic and Wenyi k-terms, continua and further gases are not yet on the device path");
   CallDims d;
   std::memset(&d, 0, sizeof d);
   d.P = P; d.L = L; d.NB = NB; d.isolir = a->i_source;
@@ -2209,6 +2281,7 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     int drop = a->i_st_water;
     if (int rc = check_drop(drop, "Liquid")) return rc;
     d.mcica = 1;
+    d.cg_pre = 1;
     d.f32 = precision == SOCGPU_MIXED;
     d.mode = a->i_mcica_sampling == SOCGPU_MCICA_OPTIMAL ? 1 : 0;
     if (S.n_req[d.mode] <= 0) return fail(SOCGPU_EUNSUPPORTED, "McICA data has no tables for this sampling");
@@ -2239,7 +2312,7 @@ static size_t partial_doubles(const CallDims &d) {
 static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
   const size_t P = d.P, L = d.L, NB = d.NB;
   size_t n = P * L * SG_NQ + P * (L + 1) + P * SG_NS + P * NB * L * SG_NBQ + P * NB * SG_NBQ +
-             (S.n_mband > 0 ? P * NB * L * 2 : 0) + partial_doubles(d) + 2;
+             partial_doubles(d) + 2;
   if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * 4;
   if (d.triple) n += P * NB * L * 8 + P * (L + 1) * kOv + P * d.nwpc * L * kTriQ * kWave;
   return n;
@@ -2250,8 +2323,8 @@ static void request_work(const CallDims &d, const SgSpec &S, long long w[kNWork]
   w[W_LEV] = P * (L + 1);
   w[W_LAY] = P * L;
   w[W_CG] = d.mcica ? P : 0;
-  w[W_MINOR] = P * S.n_mband;
-  w[W_LONG] = L > 2 * kWave ? P * S.n_mband : 0;
+  w[W_MINOR] = P * d.NB;
+  w[W_LONG] = L > 2 * kWave ? P * d.NB : 0;
   w[W_PREP] = P * d.NB * L;
   w[W_SOLVE] = d.triple ? 0 : d.n_waves;
   w[W_TOVL] = d.triple ? P * (L + 1) : 0;
@@ -2276,7 +2349,6 @@ static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cur
   q.colscal = take(P * SG_NS);
   q.bandrec = take(P * NB * L * SG_NBQ);
   q.bandscal = take(P * NB * SG_NBQ);
-  q.minor = S.n_mband > 0 ? take(P * NB * L * 2) : nullptr;
   q.partial = take(partial_doubles(d));
   q.krange = (int *)take(2);
   q.csub = d.mcica ? take(P * L * SG_MAXSUBCOL) : nullptr;
@@ -2318,15 +2390,13 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
   b_pack_kernel<<<blocks(tot[W_PTILE], 1), kPackC * kPackL, 0, s>>>(hdr, rq);
   if (tot[W_CCOL] > 0) b_cldrange_kernel<<<blocks(tot[W_CCOL], T), T, 0, s>>>(hdr, rq);
   if (tot[W_CG] > 0) b_cloudgen_kernel<<<blocks(tot[W_CG], kCgWaves), 64 * kCgWaves, 0, s>>>(hdr, rq);
-  if (tot[W_MINOR] > 0)
   {
     // calls of up to 128 layers, then the longer ones (each kernel skips the
     // other's waves)
-    b_keqv_kernel<2><<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
+    b_band_kernel<2><<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
     if (tot[W_LONG] > 0)
-      b_keqv_kernel<kKeqMaxChunks><<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
+      b_band_kernel<kKeqMaxChunks><<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
   }
-  b_prep_kernel<<<blocks(tot[W_PREP], T), T, 0, s>>>(hdr, rq);
   if (tot[W_TOVL] > 0) b_overlap_kernel<<<blocks(tot[W_TOVL], T), T, 0, s>>>(hdr, rq);
   if (ev_a) HIPCHK(hipEventRecord(ev_a, s));
   const SocgpuTile &tile = g_tile[precision == SOCGPU_MIXED ? 1 : 0];
@@ -2391,14 +2461,26 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   const ReqDesc *dq = (const ReqDesc *)((char *)g_ctx.desc.p + offsetof(decltype(host), q));
   if (int rc = launch_batch(dh, (ReqDesc *)dq, host.h.tot, 1, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
-  if (!diag->l_outputs_on_device) {
-    size_t nl = (size_t)P * (L + 1) * 8;
-    if (diag->flux_up) HIPCHK(hipMemcpyAsync(diag->flux_up, oup, nl, hipMemcpyDeviceToHost, s));
-    if (diag->flux_down) HIPCHK(hipMemcpyAsync(diag->flux_down, odn, nl, hipMemcpyDeviceToHost, s));
-    if (diag->flux_direct) HIPCHK(hipMemcpyAsync(diag->flux_direct, odir, nl, hipMemcpyDeviceToHost, s));
+  const size_t nout = (size_t)P * (L + 1) * 3 + (size_t)P * L;
+  const size_t nl = (size_t)P * (L + 1);
+  char *oh = diag->l_outputs_on_device ? nullptr : g_ctx.out_host.use(nout * 8);
+  if (oh) {
+    // one copy of the output block into pinned memory, scattered below
+    HIPCHK(hipMemcpyAsync(oh, g_ctx.out_stage.p, nout * 8, hipMemcpyDeviceToHost, s));
+  } else if (!diag->l_outputs_on_device) {
+    if (diag->flux_up) HIPCHK(hipMemcpyAsync(diag->flux_up, oup, nl * 8, hipMemcpyDeviceToHost, s));
+    if (diag->flux_down) HIPCHK(hipMemcpyAsync(diag->flux_down, odn, nl * 8, hipMemcpyDeviceToHost, s));
+    if (diag->flux_direct) HIPCHK(hipMemcpyAsync(diag->flux_direct, odir, nl * 8, hipMemcpyDeviceToHost, s));
     if (diag->heating_rate) HIPCHK(hipMemcpyAsync(diag->heating_rate, ohr, (size_t)P * L * 8, hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
+  if (oh) {
+    const double *h = (const double *)oh;
+    if (diag->flux_up) std::memcpy(diag->flux_up, h, nl * 8);
+    if (diag->flux_down) std::memcpy(diag->flux_down, h + nl, nl * 8);
+    if (diag->flux_direct) std::memcpy(diag->flux_direct, h + 2 * nl, nl * 8);
+    if (diag->heating_rate) std::memcpy(diag->heating_rate, h + 3 * nl, (size_t)P * L * 8);
+  }
   float t_all = 0.f, t_solve = 0.f;
   HIPCHK(hipEventElapsedTime(&t_all, g_ctx.ev0, g_ctx.ev3));
   HIPCHK(hipEventElapsedTime(&t_solve, g_ctx.ev1, g_ctx.ev2));
@@ -2417,7 +2499,7 @@ namespace sg {
 struct Plan {
   std::vector<ReqDesc> rq;
   BatchHdr h;
-  size_t ws_doubles = 0, in_bytes = 0, out_doubles = 0;
+  size_t ws_doubles = 0, in_bytes = 0, out_doubles = 0, out_used = 0, in_used = 0;
   struct Back { double *dst; const double *src; size_t n; };
   std::vector<Back> back;
   long long need[kNWork];
@@ -2448,9 +2530,11 @@ static int plan_sizes(int n, const SocgpuRequest *req, int precision, std::vecto
   return 0;
 }
 
-// Places a sized batch into the given buffers and stages host inputs on s.
+// Places a sized batch into the given buffers; host inputs are gathered into
+// the pinned block in_host and copied across in one copy on s, or (in_host
+// null) copied array by array.
 static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolved> &res, Plan &pl,
-                      double *ws, char *in_stage, double *out_stage, hipStream_t s) {
+                      double *ws, char *in_stage, char *in_host, double *out_stage, hipStream_t s) {
   std::vector<int> order(n);
   for (int i = 0; i < n; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return res[x].d.L > res[y].d.L; });
@@ -2465,7 +2549,7 @@ static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolve
     const CallDims &d = res[i].d;
     const SpecEntry &e = *res[i].e;
     SocgpuRunesArgs dv;
-    if (int rc = stage_inputs_at(req[i].args, &dv, d.NB, in_stage, &in_off, s)) return rc;
+    if (int rc = stage_inputs_at(req[i].args, &dv, d.NB, in_stage, in_host, &in_off, s)) return rc;
     const SocgpuDiag *g = req[i].diag;
     const size_t nl = (size_t)d.P * (d.L + 1), nh = (size_t)d.P * d.L;
     double *up, *dn, *dir, *hr;
@@ -2494,6 +2578,9 @@ static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolve
     }
   }
   for (int k = 0; k < kNWork; k++) pl.h.tot[k] = start[k];
+  pl.out_used = out_cur;
+  pl.in_used = in_off;
+  if (in_host && in_off > 0) HIPCHK(hipMemcpyAsync(in_stage, in_host, in_off, hipMemcpyHostToDevice, s));
   return 0;
 }
 
@@ -2504,10 +2591,23 @@ static void pack_desc(const Plan &pl, char *dst) {
   std::memcpy(dst + sizeof(BatchHdr), pl.rq.data(), pl.rq.size() * sizeof(ReqDesc));
 }
 
-static int copy_back(const Plan &pl, hipStream_t s) {
-  for (const auto &b : pl.back)
-    HIPCHK(hipMemcpyAsync(b.dst, b.src, b.n * 8, hipMemcpyDeviceToHost, s));
+// Host outputs: one copy of the used part of the output block into pinned
+// memory (issue), then, once the stream has reached it, the scatter to the
+// caller's arrays (finish); with out_host null, array by array directly.
+static int copy_back_issue(const Plan &pl, const double *out_stage, char *out_host, hipStream_t s) {
+  if (pl.back.empty()) return 0;
+  if (out_host) {
+    HIPCHK(hipMemcpyAsync(out_host, out_stage, pl.out_used * 8, hipMemcpyDeviceToHost, s));
+  } else {
+    for (const auto &b : pl.back) HIPCHK(hipMemcpyAsync(b.dst, b.src, b.n * 8, hipMemcpyDeviceToHost, s));
+  }
   return 0;
+}
+
+static void copy_back_finish(const Plan &pl, const double *out_stage, const char *out_host) {
+  if (!out_host) return;
+  const double *h = (const double *)out_host;
+  for (const auto &b : pl.back) std::memcpy(b.dst, h + (b.src - out_stage), b.n * 8);
 }
 
 }  // namespace sg
@@ -2521,6 +2621,7 @@ struct SocgpuBatch {
   size_t cap_ws = 0, cap_in = 0, cap_out = 0, cap_desc = 0;
   long long cap[sg::kNWork] = {};
   sg::DevBuf ws, in_stage, out_stage, desc;
+  sg::HostBuf in_host, out_host;
   char *pinned = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2542,10 +2643,11 @@ int socgpu_runes_batch(int n_request, const SocgpuRequest *requests, int precisi
   if (g_ctx.ws.ensure(pl.ws_doubles * 8) || g_ctx.in_stage.ensure(pl.in_bytes + 256) ||
       g_ctx.out_stage.ensure(pl.out_doubles * 8 + 64) || g_ctx.desc.ensure(desc_bytes(n_request)))
     return fail(SOCGPU_ENOMEM, "device buffer allocation failed");
+  char *ih = g_ctx.in_host.use(pl.in_bytes), *oh = g_ctx.out_host.use(pl.out_doubles * 8);
   hipStream_t s = g_ctx.stream;
   HIPCHK(hipEventRecord(g_ctx.ev0, s));
   if (int rc = plan_place(n_request, requests, res, pl, g_ctx.ws.as<double>(), g_ctx.in_stage.as<char>(),
-                          g_ctx.out_stage.as<double>(), s))
+                          ih, g_ctx.out_stage.as<double>(), s))
     return rc;
   std::vector<char> blob(desc_bytes(n_request));
   pack_desc(pl, blob.data());
@@ -2554,8 +2656,9 @@ int socgpu_runes_batch(int n_request, const SocgpuRequest *requests, int precisi
   ReqDesc *dq = (ReqDesc *)((char *)g_ctx.desc.p + sizeof(BatchHdr));
   if (int rc = launch_batch(dh, dq, pl.h.tot, n_request, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
-  if (int rc = copy_back(pl, s)) return rc;
+  if (int rc = copy_back_issue(pl, g_ctx.out_stage.as<double>(), oh, s)) return rc;
   HIPCHK(hipStreamSynchronize(s));
+  copy_back_finish(pl, g_ctx.out_stage.as<double>(), oh);
   float t_all = 0.f, t_solve = 0.f;
   HIPCHK(hipEventElapsedTime(&t_all, g_ctx.ev0, g_ctx.ev3));
   HIPCHK(hipEventElapsedTime(&t_solve, g_ctx.ev1, g_ctx.ev2));
@@ -2609,6 +2712,7 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
   if ((g_ws && b->ws.ensure(b->cap_ws * 8)) || (g_in && b->in_stage.ensure(b->cap_in)) ||
       (g_out && b->out_stage.ensure(b->cap_out * 8)) || (g_desc && b->desc.ensure(b->cap_desc)))
     return fail(SOCGPU_ENOMEM, "batch buffer allocation failed");
+  char *ih = b->in_host.use(pl.in_bytes), *oh = b->out_host.use(pl.out_doubles * 8);
   if (g_desc || !b->pinned) {
     if (b->pinned) (void)hipHostFree(b->pinned);
     b->pinned = nullptr;
@@ -2618,7 +2722,7 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
   // the previous replay has finished (every run ends synchronised), so the
   // pinned descriptor block can be rewritten
   if (int rc = plan_place(n_request, requests, res, pl, b->ws.as<double>(), b->in_stage.as<char>(),
-                          b->out_stage.as<double>(), s))
+                          ih, b->out_stage.as<double>(), s))
     return rc;
   pack_desc(pl, b->pinned);
   HIPCHK(hipMemcpyAsync(b->desc.p, b->pinned, desc_bytes(n_request), hipMemcpyHostToDevice, s));
@@ -2641,8 +2745,9 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
   HIPCHK(hipEventRecord(b->ev0, s));
   HIPCHK(hipGraphLaunch(b->exec, s));
   HIPCHK(hipEventRecord(b->ev1, s));
-  if (int rc = copy_back(pl, s)) return rc;
+  if (int rc = copy_back_issue(pl, b->out_stage.as<double>(), oh, s)) return rc;
   HIPCHK(hipStreamSynchronize(s));
+  copy_back_finish(pl, b->out_stage.as<double>(), oh);
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
@@ -2665,6 +2770,8 @@ int socgpu_batch_destroy(SocgpuBatch *b) {
   batch_drop_graph(b);
   for (DevBuf *x : {&b->ws, &b->in_stage, &b->out_stage, &b->desc})
     if (x->p) (void)hipFree(x->p);
+  b->in_host.release();
+  b->out_host.release();
   if (b->pinned) (void)hipHostFree(b->pinned);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -2699,6 +2806,7 @@ struct ServeSlot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   DevBuf ws, in_stage, out_stage, desc;
+  char *out_host_used = nullptr;  // the pinned output block of the batch in flight, or null
   char *pinned = nullptr;
   size_t pinned_bytes = 0;
   std::vector<std::shared_ptr<ServeItem>> items;
@@ -2735,6 +2843,7 @@ static size_t footprint_bytes(const SocgpuRequest &r, const CallDims &d, const S
 }
 
 static void serve_finish_slot(SocgpuServer *sv, ServeSlot &sl, int status) {
+  if (status == 0) copy_back_finish(sl.plan, sl.out_stage.as<double>(), sl.out_host_used);
   const auto now = std::chrono::steady_clock::now();
   std::lock_guard<std::mutex> lk(sv->mu);
   for (auto &it : sl.items) {
@@ -2761,6 +2870,12 @@ static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
   if (sl.ws.ensure(sl.plan.ws_doubles * 8) || sl.in_stage.ensure(sl.plan.in_bytes + 256) ||
       sl.out_stage.ensure(sl.plan.out_doubles * 8 + 64) || sl.desc.ensure(desc_bytes(n)))
     return fail(SOCGPU_ENOMEM, "serving slot allocation failed");
+  // host arrays cross PCIe array by array: gathering them into pinned
+  // staging here would put the copies on this thread, which also launches
+  // the next batch (the runtime's own pageable copies measured faster under
+  // load)
+  char *ih = nullptr;
+  sl.out_host_used = nullptr;
   if (sl.pinned_bytes < desc_bytes(n)) {
     if (sl.pinned) (void)hipHostFree(sl.pinned);
     sl.pinned = nullptr;
@@ -2769,14 +2884,14 @@ static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
   }
   hipStream_t s = sl.stream;
   if (int rc = plan_place(n, req.data(), res, sl.plan, sl.ws.as<double>(), sl.in_stage.as<char>(),
-                          sl.out_stage.as<double>(), s))
+                          ih, sl.out_stage.as<double>(), s))
     return rc;
   pack_desc(sl.plan, sl.pinned);
   HIPCHK(hipMemcpyAsync(sl.desc.p, sl.pinned, desc_bytes(n), hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)sl.desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)sl.desc.p + sizeof(BatchHdr));
   if (int rc = launch_batch(dh, dq, sl.plan.h.tot, n, sv->cfg.precision, s, nullptr, nullptr)) return rc;
-  if (int rc = copy_back(sl.plan, s)) return rc;
+  if (int rc = copy_back_issue(sl.plan, sl.out_stage.as<double>(), sl.out_host_used, s)) return rc;
   HIPCHK(hipEventRecord(sl.done, s));
   long long sol = 0;
   for (int i = 0; i < n; i++) sol += (long long)res[i].d.P * res[i].e->host.n_g;
@@ -2852,7 +2967,7 @@ int socgpu_serve_start(SocgpuServer **server, const SocgpuServeConfig *cfg) {
   for (auto &sl : sv->slots) {
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
-      delete sv;
+      socgpu_serve_stop(sv);  // releases what the earlier slots hold
       return fail(SOCGPU_EDEVICE, "stream or event creation failed");
     }
   }
@@ -3174,11 +3289,14 @@ int socgpu_finalize(void) {
   std::vector<double *> xcws;
   for (auto &kv : g_ctx.spectra) {
     (void)hipFree(kv.second->dev);
+    if (kv.second->tables) (void)hipFree(kv.second->tables);
     if (kv.second->xcw && std::find(xcws.begin(), xcws.end(), kv.second->xcw) == xcws.end())
       xcws.push_back(kv.second->xcw);
   }
   for (double *p : xcws) (void)hipFree(p);
   g_ctx.spectra.clear();
+  g_ctx.in_host.release();
+  g_ctx.out_host.release();
   for (DevBuf *b : {&g_ctx.ws, &g_ctx.desc, &g_ctx.in_stage, &g_ctx.out_stage}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;

@@ -2,12 +2,14 @@
 // device table layout of sg_types.h.
 //
 // Formats: the text blocks read by read_spectrum.F90 (block dispatch at
-// :212-240; readers 0_0_1 :557-621, 2_0_0 :764-789, 3_0_0 :872-900,
-// 4_0_0 :951-992, 5_0_0/5_0_1 :1040-1641, 6_0_0 :1746-1777, 6_0_1 polynomial
-// :1781-1877, 8_0_0 :1881-1935, 10_0_2 :2245-2331) and by read_mcica_data
-// (def_mcica.F90:61-190). Features off the two-stream flux path - k-term
-// look-up tables, continua, SES2, aerosol-only data - are rejected with a
-// clear error, never silently dropped.
+// :212-240; readers 0_0_1/0_0_2 :557-728, 2_0_0 :764-789, 3_0_0 :872-900,
+// 4_0_0/4_0_1 :951-1036, 5_0_0/5_0_1 :1040-1641 with the k-term look-up
+// tables of the '_k' file :1404-1507, 6_0_0 :1746-1777, 6_0_1 polynomial
+// :1781-1877, 8_0_0 :1881-1935, 9_0_0 :1939-1997, 10_0_2 :2245-2331) and by
+// read_mcica_data (def_mcica.F90:61-190). Features off the two-stream flux
+// path - SES2 and temperature-only look-up scaling, self-broadened look-up
+// tables, generalised continua, aerosol-only data - are rejected with a clear
+// error, never silently dropped.
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +22,7 @@
 
 #include "sg_types.h"
 #include "spectrum.h"
+#include "wenyi_tables.h"
 
 namespace sg {
 
@@ -71,33 +74,74 @@ struct Lines {
 };
 
 int gas_slot(int type_absorb) {
-  switch (type_absorb) {  // identifiers of rad_pcf.F90 (ip_h2o, ip_co2, ip_o3)
+  switch (type_absorb) {  // identifiers of gas_list_pcf.F90
     case 1: return SG_GAS_H2O;
     case 2: return SG_GAS_CO2;
     case 3: return SG_GAS_O3;
+    case 4: return SG_GAS_WM0 + 0;   // N2O
+    case 6: return SG_GAS_WM0 + 1;   // CH4
+    case 7: return SG_GAS_WM0 + 2;   // O2
+    case 9: return SG_GAS_WM0 + 3;   // SO2
+    case 25: return SG_GAS_WM0 + 4;  // OCS
+    case 14: return SG_GAS_WM0 + 5;  // CFC11
+    case 15: return SG_GAS_WM0 + 6;  // CFC12
+    case 16: return SG_GAS_WM0 + 7;  // CFC113
+    case 17: return SG_GAS_WM0 + 8;  // HCFC22
+    case 19: return SG_GAS_WM0 + 9;  // HFC134A
     default: return SG_GAS_ZERO;
   }
 }
 
-}  // namespace
+// n_scale_variable of a scaling function (rad_pcf.F90:505-506)
+int n_scale_variable(int fnc) {
+  static const int n[11] = {0, 2, 3, 4, 0, 0, 6, 8, 10, 0, 0};
+  return fnc >= 0 && fnc <= 10 ? n[fnc] : 0;
+}
 
-int read_spectrum(const char *path, SgSpec *s, std::string *err) {
-  std::ifstream f(path);
-  if (!f) { *err = std::string("cannot open spectral file ") + path; return -1; }
+Lines read_lines(const char *path, bool *ok) {
   Lines L;
+  std::ifstream f(path);
+  *ok = (bool)f;
   for (std::string ln; std::getline(f, ln);) {
     if (!ln.empty() && ln.back() == '\r') ln.pop_back();
     L.v.push_back(ln);
   }
+  return L;
+}
+
+// n values of '(6(1PE13.6))' records
+bool e13_values(Lines &L, int n, double *out) {
+  int got = 0;
+  while (got < n) {
+    if (!L.more()) return false;
+    const std::string &ln = L.next();
+    for (int q = 0; q < 6 && got < n; q++) out[got++] = fnum(ln, 13 * q, 13);
+  }
+  return true;
+}
+
+}  // namespace
+
+int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err) {
+  bool ok = false;
+  Lines L = read_lines(path, &ok);
+  if (!ok) { *err = std::string("cannot open spectral file ") + path; return -1; }
   std::memset(s, 0, sizeof(*s));
+  tab->data.clear();
+  tab->wenyi_off = -1;
   std::vector<int> type_absorb;
   std::vector<int> n_band_absorb;
   std::vector<std::vector<int>> index_absorb;
   // per (band, gas index): term list
   struct Raw { int nk = 0, scale = 0, fnc = 0; double pref = 0, tref = 0;
-               std::vector<double> k, w, a, b; };
+               std::vector<double> k, w, s1, s2, s3; int lut0 = -1; };
   std::vector<std::vector<Raw>> raw;
-  bool have5 = false, have6 = false;
+  std::vector<std::vector<int>> order;  // gases per band in block 5's order
+  // continua by band: listed types, and the block 9 data stored by type
+  std::vector<std::vector<int>> cont_index;
+  struct Cont { int fnc = 0; double k = 0, pref = 0, tref = 0, sc[3] = {0, 0, 0}; bool set = false; };
+  std::vector<std::vector<Cont>> cont;
+  bool have5 = false, have6 = false, lookup = false;
   while (L.more()) {
     const std::string ln = L.next();
     if (ln.compare(0, 6, "*BLOCK") != 0) continue;
@@ -134,6 +178,17 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
       for (int b = 0; b < s->n_band; b++) {
         const std::string h = L.next();
         n_band_absorb[b] = inum(h, 12, 5);
+        if (ver == 1) {
+          // '(i5, 2(7x, i5))' then '(5x, 12(2x, i3))' (:1012-1023); the
+          // per-band overlap method (third field) only matters with
+          // ip_overlap_hybrid, which the flux path does not use
+          if (n_band_absorb[b] > 12) { *err = "more than 12 absorbers in a band"; return -3; }
+          if (n_band_absorb[b] > 0) {
+            const std::string d = L.next();
+            for (int q = 0; q < n_band_absorb[b]; q++) index_absorb[b].push_back(inum(d, 5 + 5 * q + 2, 3));
+          }
+          continue;
+        }
         bool first = true;
         while ((int)index_absorb[b].size() < n_band_absorb[b]) {
           const std::string d = L.next();
@@ -146,9 +201,22 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
     } else if (type == 5) {
       if (n_band_absorb.empty()) { *err = "block 5 before block 4"; return -1; }
       raw.assign(s->n_band, std::vector<Raw>(type_absorb.size()));
+      order.assign(s->n_band, {});
       if (ver == 0) { L.next(); L.next(); L.next(); }
       else {
-        while (L.more() && L.next().compare(0, 34, "Band        Gas, Number of k-terms") != 0) {}
+        for (;;) {
+          if (!L.more()) { *err = "bad block 5"; return -1; }
+          const std::string h = L.next();
+          if (trim(h) == "Self-broadened indexing numbers of all absorbers.") {
+            L.next();
+            for (int q = 0; q < 6 && L.more(); q++)
+              if (inum(L.v[L.i], 2 + 5 * q, 3) > 0) {
+                *err = "self-broadened look-up tables are not on this path"; return -3;
+              }
+            L.next();
+          }
+          if (h.compare(0, 34, "Band        Gas, Number of k-terms") == 0) break;
+        }
         L.next(); L.next();
       }
       for (int b = 0; b < s->n_band; b++)
@@ -159,18 +227,37 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
             *err = "bad k-term header in block 5"; return -1;
           }
           Raw &r = raw[band][gas];
+          order[band].push_back(gas);
           r.nk = inum(h, 24, 5); r.scale = inum(h, 36, 5); r.fnc = inum(h, 48, 5);
-          if (r.fnc != 1 || r.scale != 2) {
-            *err = "only per-term power-law scaling (ip_scale_term, ip_scale_power_law) is on this path";
+          if (r.scale != 2) { *err = "only per-term scaling (ip_scale_term) is on this path"; return -3; }
+          if (r.fnc != SG_FNC_POWER_LAW && r.fnc != SG_FNC_POWER_QUAD && r.fnc != SG_FNC_LOOKUP &&
+              !(r.fnc == SG_FNC_WENYI && band == 3)) {
+            *err = "k-term scaling function not on this path (power law, power quadratic, look-up "
+                   "table, and Wenyi's CO2 tables in band 4 are)";
             return -3;
           }
           const std::string pt = L.next();
-          r.pref = fnum(pt, 6, 16); r.tref = fnum(pt, 28, 16);
+          if (r.fnc == SG_FNC_LOOKUP) {
+            // '(2(7x, i5))' num_ref_p, num_ref_t (:1333-1349)
+            const int np = inum(pt, 7, 5), nt = inum(pt, 19, 5);
+            if (s->n_pre == 0) { s->n_pre = np; s->n_tmp = nt; }
+            else if (np != s->n_pre || nt != s->n_tmp) { *err = "P/T lookup table size is not consistent"; return -1; }
+            if (np < 2 || nt < 2 || np > SG_MAXPRE || nt > SG_MAXTMP) {
+              *err = "P/T lookup table larger than 64 pressures x 8 temperatures"; return -3;
+            }
+            lookup = true;
+          } else {
+            r.pref = fnum(pt, 6, 16); r.tref = fnum(pt, 28, 16);
+          }
+          const int nsv = n_scale_variable(r.fnc);
           for (int k = 0; k < r.nk; k++) {
             const std::string d = L.next();
             r.k.push_back(fnum(d, 3, 16)); r.w.push_back(fnum(d, 22, 16));
-            r.a.push_back(fnum(d, 41, 16)); r.b.push_back(fnum(d, 60, 16));
+            r.s1.push_back(nsv >= 1 ? fnum(d, 41, 16) : 0.0);
+            r.s2.push_back(nsv >= 2 ? fnum(d, 60, 16) : 0.0);
+            r.s3.push_back(nsv >= 3 ? fnum(L.next(), 41, 16) : 0.0);
           }
+          if (r.fnc == SG_FNC_WENYI && r.nk > 14) { *err = "Wenyi scaling has 14 k-terms"; return -3; }
         }
       have5 = true;
     } else if (type == 6) {
@@ -199,12 +286,43 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
       }
       have6 = true;
     } else if (type == 8) {
+      // '(////)', per band '(i5, 7x, i5)' and '(5x, 4(2x, i3))' (:1895-1921)
       for (int j = 0; j < 5; j++) L.next();
+      cont_index.assign(s->n_band, {});
       for (int b = 0; b < s->n_band; b++) {
-        if (inum(L.next(), 12, 5) != 0) {
-          *err = "continuum absorption is not on this path"; return -3;
+        const int n = inum(L.next(), 12, 5);
+        if (n > SG_MAXCONT) { *err = "more than 2 continua in a band"; return -3; }
+        if (n > 0) {
+          const std::string d = L.next();
+          for (int q = 0; q < n; q++) cont_index[b].push_back(inum(d, 5 + 5 * q + 2, 3));
         }
       }
+    } else if (type == 9) {
+      if (ver != 0) { *err = "only block 9.0.0 continua are on this path"; return -3; }
+      if (cont_index.empty()) { *err = "block 9 before block 8"; return -1; }
+      cont.assign(s->n_band, std::vector<Cont>(SG_MAXCONT + 1));
+      L.next(); L.next(); L.next();
+      for (int b = 0; b < s->n_band; b++)
+        for (size_t j = 0; j < cont_index[b].size(); j++) {
+          const std::string h = L.next();
+          const int band = inum(h, 0, 5) - 1, ic = inum(h, 12, 5), fnc = inum(h, 24, 5);
+          if (band < 0 || band >= s->n_band || ic < 1 || ic > SG_MAXCONT) {
+            *err = "continuum outside the self- and foreign-broadened water vapour continua"; return -3;
+          }
+          if (fnc != SG_FNC_POWER_LAW && fnc != SG_FNC_POWER_QUAD) {
+            *err = "continuum scaling function not on this path"; return -3;
+          }
+          Cont &c = cont[band][ic];
+          c.fnc = fnc; c.set = true;
+          const std::string pt = L.next();
+          c.pref = fnum(pt, 6, 16); c.tref = fnum(pt, 28, 16);
+          // '(6x, 1pe16.9, (t23, 2(6x, 1pe16.9)))' (:1985-1989)
+          const std::string d = L.next();
+          c.k = fnum(d, 6, 16);
+          c.sc[0] = fnum(d, 28, 16);
+          c.sc[1] = fnum(d, 50, 16);
+          if (n_scale_variable(fnc) >= 3) c.sc[2] = fnum(L.next(), 28, 16);
+        }
     } else if (type == 10) {
       if (ver != 2) { *err = "droplet block version not supported"; return -3; }
       L.next();
@@ -227,22 +345,63 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
         }
       }
     }
-    // other blocks (1, 9 without continua, 11, 12, 14, ...) carry nothing the
-    // flux path reads; they are skipped to their *END by the scan above
+    // other blocks (1, 11, 12, 14, 17, ...) carry nothing the flux path
+    // reads; they are skipped to their *END by the scan above
   }
   if (!have5) { *err = "no k-distribution block (5)"; return -1; }
+  // k-term look-up tables from the '_k' file: the block headed 'k-table',
+  // two header records, the P/T grid, then per band and gas in block 5's
+  // order, two header records and per term and pressure n_tmp values
+  if (lookup) {
+    const std::string kp = std::string(path) + "_k";
+    Lines K = read_lines(kp.c_str(), &ok);
+    if (!ok) { *err = "k-term look-up tables: cannot open " + kp; return -1; }
+    bool found = false;
+    while (K.more()) {
+      const std::string &h = K.next();
+      if (h.compare(0, 6, "*BLOCK") == 0 && h.compare(8, 7, "k-table") == 0) { found = true; break; }
+    }
+    if (!found) { *err = "no k-table in " + kp; return -1; }
+    K.next(); K.next();
+    std::vector<double> row(s->n_tmp + 1);
+    for (int ip = 0; ip < s->n_pre; ip++) {
+      if (!e13_values(K, s->n_tmp + 1, row.data())) { *err = "truncated P/T grid in " + kp; return -1; }
+      s->lut_lnp[ip] = std::log(row[0]);
+      for (int it = 0; it < s->n_tmp; it++) s->lut_t[ip][it] = row[it + 1];
+    }
+    const size_t per = (size_t)s->n_pre * s->n_tmp;
+    for (int b = 0; b < s->n_band; b++)
+      for (int gas : order[b]) {
+        Raw &r = raw[b][gas];
+        if (r.fnc != SG_FNC_LOOKUP) continue;
+        K.next(); K.next();
+        r.lut0 = (int)(tab->data.size() / per);
+        const size_t at = tab->data.size();
+        tab->data.resize(at + per * r.nk);
+        for (int k = 0; k < r.nk; k++)
+          for (int ip = 0; ip < s->n_pre; ip++)
+            if (!e13_values(K, s->n_tmp, tab->data.data() + at + k * per + (size_t)ip * s->n_tmp)) {
+              *err = "truncated k-term look-up table in " + kp; return -1;
+            }
+      }
+    s->has_lut = 1;
+  }
   // assemble the term tables: major gas is the first absorber listed for the
   // band (radiance_calc.F90: index_abs(1) is the major absorber)
   int ng = 0, nm = 0;
+  bool wenyi = false;
   for (int b = 0; b < s->n_band; b++) {
     SgBand &B = s->band[b];
     if (n_band_absorb[b] < 1) { *err = "band without gaseous absorption"; return -3; }
     for (int j = 0; j < n_band_absorb[b]; j++) {
       int gi = index_absorb[b][j] - 1;
+      if (gi < 0 || gi >= (int)type_absorb.size()) { *err = "bad absorber index in block 4"; return -1; }
       const Raw &r = raw[b][gi];
       int slot = gas_slot(type_absorb[gi]);
-      if (j == 0) { B.n_major = r.nk; B.g0 = ng; }
-      else {
+      if (j == 0) {
+        if (slot >= SG_GAS_WM0) { *err = "a band's major gas other than H2O, CO2 or O3 is not on this path"; return -3; }
+        B.n_major = r.nk; B.g0 = ng;
+      } else {
         if (j - 1 >= SG_MAXMINOR_GAS) { *err = "too many minor gases in a band"; return -3; }
         if (j == 1) B.minor0 = nm;
         B.minor_nk[j - 1] = r.nk;
@@ -251,8 +410,14 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
       for (int k = 0; k < r.nk; k++) {
         SgTerm t;
         std::memset(&t, 0, sizeof t);
-        t.k = r.k[k]; t.w = r.w[k]; t.a = r.a[k]; t.b = r.b[k];
-        t.lnpref = std::log(r.pref); t.lntref = std::log(r.tref);
+        t.k = r.k[k]; t.w = r.w[k]; t.a = r.s1[k]; t.b = r.s2[k]; t.c = r.s3[k];
+        t.fnc = r.fnc;
+        if (r.fnc != SG_FNC_LOOKUP) {
+          t.lnpref = std::log(r.pref); t.lntref = std::log(r.tref);
+          t.tinv = 1.0 / r.tref;
+        }
+        if (r.fnc == SG_FNC_LOOKUP) t.lut = r.lut0 + k;
+        if (r.fnc == SG_FNC_WENYI) { t.lut = k; wenyi = true; }
         t.band = b; t.gas = slot;
         if (j == 0) {
           if (ng >= SG_MAXG) { *err = "too many major k-terms"; return -3; }
@@ -263,6 +428,33 @@ int read_spectrum(const char *path, SgSpec *s, std::string *err) {
         }
       }
     }
+    // continua in the band's order; radiance_calc.F90:910-922 passes the
+    // data stored under the continuum's position as its index number, which
+    // is the continuum's own data when the band lists them as 1, 2
+    if (!cont_index.empty()) {
+      B.n_cont = (int)cont_index[b].size();
+      for (int j = 0; j < B.n_cont; j++) {
+        if (cont_index[b][j] != j + 1) {
+          *err = "continua listed out of index order in a band are not on this path"; return -3;
+        }
+        if (cont.empty() || !cont[b][j + 1].set) { *err = "continuum without block 9 data"; return -1; }
+        const Cont &c = cont[b][j + 1];
+        B.cont_type[j] = cont_index[b][j];
+        B.cont_fnc[j] = c.fnc;
+        B.cont_k[j] = c.k; B.cont_pref[j] = c.pref; B.cont_tref[j] = c.tref;
+        for (int q = 0; q < 3; q++) B.cont_s[j][q] = c.sc[q];
+        s->has_cont = 1;
+      }
+    }
+  }
+  if (wenyi) {
+    // the Wenyi tables: gk4 (10 x 26 x 14), gk250b, plg, ttb
+    tab->wenyi_off = (long)tab->data.size();
+    const double *g4 = &kWenyiGk4[0][0][0];
+    tab->data.insert(tab->data.end(), g4, g4 + 10 * 26 * 14);
+    tab->data.insert(tab->data.end(), kWenyiGk250b, kWenyiGk250b + 14);
+    tab->data.insert(tab->data.end(), kWenyiPlg, kWenyiPlg + 26);
+    tab->data.insert(tab->data.end(), kWenyiTtb, kWenyiTtb + 10);
   }
   s->n_g = ng;
   s->n_minor = nm;

@@ -16,7 +16,15 @@ struct McicaData {
   std::vector<double> xcw;                                   // n1 x n2, column-major
 };
 
-int read_spectrum(const char *path, SgSpec *s, std::string *err);
+// Variable-size tables of a spectral file, uploaded beside the SgSpec: the
+// k-term look-up tables (term t.lut at t.lut * n_pre * n_tmp) and, from
+// wenyi_off (or -1), the Wenyi CO2 tables.
+struct SpecTables {
+  std::vector<double> data;
+  long wenyi_off = -1;
+};
+
+int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err);
 int read_mcica(const char *path, McicaData *m, std::string *err);
 int attach_mcica(SgSpec *s, const McicaData &m, bool solar, std::string *err);
 
