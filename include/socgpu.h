@@ -86,7 +86,25 @@ typedef struct SocgpuRunesArgs {
   const double *cloud_frac, *liq_mmr, *liq_dim;           /* [layer][profile] */
   const double *conv_frac, *liq_conv_mmr, *liq_conv_dim;  /* [layer][profile] */
   const int *rand_seed;                    /* [profile] McICA seeds */
+  /* Well-mixed mass mixing ratios of further gases, by SOCGPU_GAS_* index:
+   * runes' n2o_mix_ratio, ch4_mix_ratio, o2_mix_ratio, so2_mix_ratio,
+   * ocs_mix_ratio, cfc11_mix_ratio, cfc12_mix_ratio, cfc113_mix_ratio,
+   * hcfc22_mix_ratio, hfc134a_mix_ratio (socrates_runes.F90:69-83). 0 is an
+   * absent gas, as in set_atm (socrates_set_atm.F90:535-622). Read only for
+   * gases the spectral file lists. */
+  double gas_mix_ratio[10];
 } SocgpuRunesArgs;
+
+#define SOCGPU_GAS_N2O 0
+#define SOCGPU_GAS_CH4 1
+#define SOCGPU_GAS_O2 2
+#define SOCGPU_GAS_SO2 3
+#define SOCGPU_GAS_OCS 4
+#define SOCGPU_GAS_CFC11 5
+#define SOCGPU_GAS_CFC12 6
+#define SOCGPU_GAS_CFC113 7
+#define SOCGPU_GAS_HCFC22 8
+#define SOCGPU_GAS_HFC134A 9
 
 /* Outputs (StrDiag, socrates_def_diag.F90:31-76): caller-owned, NULL = not
  * requested. Fluxes [level][profile] in W m-2, heating rate [layer][profile]

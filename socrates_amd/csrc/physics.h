@@ -92,6 +92,112 @@ SG_HD double k_term(const SgTerm &t, double lnp, double lnt, double mmr) {
   return t.k * fmax(0.0, s * mmr);
 }
 
+// k_abs of one term at one layer for the other scalings on the path:
+//  - power quadratic, k * max(0, (p/p_ref)^a (1 + b x + c x^2) * mmr) with
+//    x = T/T_ref - 1 (scale_absorb.F90:176-193);
+//  - Wenyi's CO2 tables in band 4 (scale_absorb.F90:238-259): gk4 bilinear
+//    in ln(p/100 Pa) (step 0.5) and T (step 20 K), over gk250b; table block
+//    w = [gk4 (10 x 26 x 14) | gk250b (14) | plg (26) | ttb (10)];
+//  - P/T look-up table, max(0, max(0, sum of the four weighted entries) *
+//    mmr) (radiance_calc.F90:1136-1152, 1432-1442) with the layer's
+//    factors and indices g (inter_pt_lookup.F90:147-191).
+// p, tk: layer pressure and temperature.
+SG_HD double k_scaled(const SgTerm &t, const SgSpec *spec, double lnp, double p, double tk,
+                      const double *g, double mmr) {
+  if (t.fnc == SG_FNC_LOOKUP) {
+    const int idx = (int)g[SG_G_IDX];
+    const int jp = idx & 255, jt = (idx >> 8) & 255, jtt = idx >> 16, nt = spec->n_tmp;
+    const double *kl = spec->klut + (size_t)t.lut * spec->n_pre * nt;
+    const double ke = fmax(0.0, g[SG_G_F00] * kl[jp * nt + jt] + g[SG_G_F10] * kl[(jp + 1) * nt + jtt] +
+                                    g[SG_G_F01] * kl[jp * nt + jt + 1] +
+                                    g[SG_G_F11] * kl[(jp + 1) * nt + jtt + 1]);
+    return fmax(0.0, ke * mmr);
+  }
+  double s;
+  if (t.fnc == SG_FNC_POWER_QUAD) {
+    const double x = tk * t.tinv - 1.0;
+    s = exp(t.a * (lnp - t.lnpref)) * (1.0 + x * t.b + t.c * x * x);
+  } else {  // SG_FNC_WENYI
+    const double *gk4 = spec->wenyi, *gk250b = gk4 + 3640, *plg = gk250b + 14, *ttb = plg + 26;
+    const double cgp = fmax(-5.5, log(p / 100.0));
+    int jp = (int)((5.5 + cgp) * 2.0) + 1;
+    jp = jp < 1 ? 1 : (jp > 26 ? 26 : jp);
+    const int jp1 = jp + 1 < 26 ? jp + 1 : 26;
+    int jt = (int)((tk - 240.0) / 20.0 + 7.0);
+    jt = jt < 1 ? 1 : (jt > 10 ? 10 : jt);
+    const int jt1 = jt + 1 < 10 ? jt + 1 : 10;
+    const int ix = t.lut;
+#define SG_GK4(a, b) gk4[((a) - 1) * 364 + ((b) - 1) * 14 + ix]
+    const double dl = (cgp - plg[jp - 1]) * 2.0;
+    const double gkpb = SG_GK4(jt, jp) + (SG_GK4(jt, jp1) - SG_GK4(jt, jp)) * dl;
+    const double gkpc = SG_GK4(jt1, jp) + (SG_GK4(jt1, jp1) - SG_GK4(jt1, jp)) * dl;
+#undef SG_GK4
+    s = (gkpb + (gkpc - gkpb) * (tk - ttb[jt - 1]) / 20.0) / gk250b[ix];
+  }
+  return t.k * fmax(0.0, s * mmr);
+}
+
+// Water-vapour continuum absorption of band B at one layer: the sum over the
+// band's continua of k_cont * amount (grey_opt_prop.F90:801-816), amount by
+// rescale_continuum.F90:107-179 (self-broadened: times the molar density of
+// water and the water mixing ratio; foreign-broadened: times rho q (1 - q) /
+// M_air, l_mixing_ratio false), with the reference's constants
+// (rad_ccf.F90:42,58). h2o: the layer's water mixing ratio (clipped at 0).
+SG_HD double continuum_k(const SgBand &B, double p, double tk, double rho, double h2o) {
+  const double mol_weight_air = 28.966e-03, repsilon = 18.0153 / 28.966;
+  double kg = 0.0;
+  for (int j = 0; j < B.n_cont; j++) {
+    const double *sc = B.cont_s[j];
+    const double pwk = pow(p / B.cont_pref[j], sc[0]);
+    double amount;
+    if (B.cont_fnc[j] == SG_FNC_POWER_LAW) {
+      amount = pwk * pow(tk / B.cont_tref[j], sc[1]);
+    } else {
+      const double x = tk / B.cont_tref[j] - 1.0;
+      amount = pwk * (1.0 + sc[1] * x + sc[2] * (x * x));
+    }
+    if (B.cont_type[j] == 1) amount = amount * (rho * h2o / (repsilon * mol_weight_air)) * h2o;
+    else amount = amount * rho * h2o * (1.0 - h2o) / mol_weight_air;
+    kg = kg + B.cont_k[j] * amount;
+  }
+  return kg;
+}
+
+// P/T look-up interpolation of one layer (inter_pt_lookup.F90:147-191):
+// ln p clamped inside the table, jp the last table pressure at or below it
+// (MINLOC over the non-negative differences), T clamped per pressure row,
+// jt / jtt the rows' lower neighbours; writes the four factors and the
+// packed indices into g.
+SG_HD void lookup_factors(const SgSpec *spec, double p, double tk, double *g) {
+  const double eps = kEps;
+  const int np = spec->n_pre, nt = spec->n_tmp;
+  const double *lp = spec->lut_lnp;
+  const double pmin = lp[0] + fmax(fabs(lp[0]) * eps, eps);
+  const double pmax = lp[np - 1] - fmax(fabs(lp[np - 1]) * eps, eps);
+  const double pl = fmin(fmax(log(p), pmin), pmax);
+  int jp = 0;
+  for (int k = 1; k < np; k++)
+    if (pl - lp[k] >= 0.0) jp = k;
+  const double fp = (pl - lp[jp]) / (lp[jp + 1] - lp[jp]);
+  int jt[2];
+  double ft[2];
+  for (int r = 0; r < 2; r++) {
+    const double *tl = spec->lut_t[jp + r];
+    const double tc = fmin(fmax(tk, tl[0] * (1.0 + eps)), tl[nt - 1] * (1.0 - eps));
+    int j = 0;
+    for (int k = 1; k < nt; k++)
+      if (tc - tl[k] >= 0.0) j = k;
+    jt[r] = j;
+    ft[r] = (tc - tl[j]) / (tl[j + 1] - tl[j]);
+  }
+  const double compfp = 1.0 - fp;
+  g[SG_G_F00] = compfp * (1.0 - ft[0]);
+  g[SG_G_F10] = fp * (1.0 - ft[1]);
+  g[SG_G_F01] = compfp * ft[0];
+  g[SG_G_F11] = fp * ft[1];
+  g[SG_G_IDX] = (double)(jp + 256 * jt[0] + 65536 * jt[1]);
+}
+
 // Pade-2 droplet optics (opt_prop_pade_2_mod.F90:100-143): extinction,
 // scattering, first phase-function moment and forward scattering, all
 // weighted by the in-cloud mixing ratio.

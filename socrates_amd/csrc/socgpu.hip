@@ -141,6 +141,7 @@ struct CallDims {
   int raw_range;     // cloud range on the fractions as given (kernel-level cloud_gen)
   int cg_pre;        // overlap parameters precomputed by the pack kernel
   int f32;           // mixed form: McICA c_sub and droplet optics kept in FP32
+  int gx;            // per-layer gas record (look-up-table terms or continua)
 };
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
@@ -460,6 +461,7 @@ constexpr int kKeqMaxChunks = (SG_MAXL + kWave - 1) / kWave;
 template <int kKeqChunks>
 __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__restrict__ spec,
                                           int l_rayleigh, const double *__restrict__ colrec,
+                                          const double *__restrict__ grec, const double *__restrict__ wm,
                                           const double *__restrict__ tlev,
                                           const double *__restrict__ colscal,
                                           double *__restrict__ bandrec,
@@ -507,7 +509,9 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
 #pragma unroll
     for (int c = 0; c < kKeqChunks; c++) {
       const int i = c * kWave + lane;
-      mmr[c] = (c < nch && i < L) ? rec[(size_t)i * SG_NQ + SG_Q_MMR + gas] : 0.0;
+      mmr[c] = (c < nch && i < L) ? (gas < SG_GAS_WM0 ? rec[(size_t)i * SG_NQ + SG_Q_MMR + gas]
+                                                      : wm[gas - SG_GAS_WM0])
+                                  : 0.0;
       num[c] = 0.0; den[c] = 0.0; kmin[c] = DBL_MAX;
     }
     double sflux = 0.0;
@@ -516,7 +520,17 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
       double kk[kKeqChunks];
 #pragma unroll
       for (int c = 0; c < kKeqChunks; c++) {
-        kk[c] = (c < nch && c * kWave + lane < L) ? k_term(t, lnp[c], lnt[c], mmr[c]) : 0.0;
+        const int i = c * kWave + lane;
+        if (c < nch && i < L) {
+          if (t.fnc == SG_FNC_POWER_LAW) kk[c] = k_term(t, lnp[c], lnt[c], mmr[c]);
+          else {
+            const double *r = rec + (size_t)i * SG_NQ;
+            kk[c] = k_scaled(t, spec, lnp[c], r[SG_Q_P], r[SG_Q_T],
+                             grec ? grec + ((size_t)col * L + i) * SG_NGQ : nullptr, mmr[c]);
+          }
+        } else {
+          kk[c] = 0.0;
+        }
         kmin[c] = fmin(kmin[c], kk[c]);
       }
       if (solar) {
@@ -623,8 +637,8 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
   for (int c = 0; c < kKeqChunks; c++) {
     const int i = c * kWave + lane;
     if (c < nch && i < L)
-      prep_layer(d, spec, l_rayleigh, colrec, colscal, bandrec, bandscal, cldopt, krange, col, b, i, keq[c],
-                 adj[c], b0[c], b1[c]);
+      prep_layer(d, spec, l_rayleigh, colrec, grec, colscal, bandrec, bandscal, cldopt, krange, col, b, i,
+                 keq[c], adj[c], b0[c], b1[c]);
   }
 }
 
@@ -637,6 +651,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
 // bandrec[col][band][layer][4].
 __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__restrict__ spec,
                                            int l_rayleigh, const double *__restrict__ colrec,
+                                           const double *__restrict__ grec,
                                            const double *__restrict__ colscal,
                                            double *__restrict__ bandrec,
                                            double *__restrict__ bandscal,
@@ -674,7 +689,13 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
   const double sec0 = colscal[(size_t)col * SG_NS + SG_S_SEC0];
   const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
-  br[0] = (0.0 + ray) + keq;
+  // clear-air grey extinction: continua, then Rayleigh scattering
+  // (grey_opt_prop.F90:801-835), then the equivalent extinction
+  const double kcont = B.n_cont > 0 ? continuum_k(B, r[SG_Q_P], r[SG_Q_T],
+                                                  grec[((size_t)col * L + i) * SG_NGQ + SG_G_RHO],
+                                                  r[SG_Q_MMR + SG_GAS_H2O])
+                                    : 0.0;
+  br[0] = (kcont + ray) + keq;
   if (d.triple) {
     // grey optics of each cloud type (grey_opt_prop.F90:975-1000,
     // 1185-1230: Rayleigh, then the type's droplets and its ice component of
@@ -685,7 +706,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
     const int nct = krange[0] < L ? krange[0] : L;
     double *co = cldopt + (((size_t)col * NB + b) * L + i) * 8;
     for (int k = 0; k < 2; k++) {
-      double kgt = 0.0 + ray, ksc = ray, phf = 0.0, fwd = 0.0;
+      double kgt = kcont + ray, ksc = ray, phf = 0.0, fwd = 0.0;
       if (i + 1 >= nct && r[SG_Q_WCLD] > 0.0) {
         const SgDrop &dr = spec->drop[k == 0 ? d.drop : d.drop2];
         const double mmr = r[k == 0 ? SG_Q_LWC : SG_Q_LWC2], re = r[k == 0 ? SG_Q_RE : SG_Q_RE2];
@@ -747,6 +768,7 @@ struct LaneCtx {
 // that the loads of layer i+1 are in flight while layer i is computed.
 struct LayerIn {
   double lnp, lnt, mmr, mass, frac1;  // column record (wave-uniform but mmr)
+  double kx;                          // the term's k_abs, for scalings other than the power law
   double kgt, q1, q2;                 // band record: grey extinction, adjust or Planck terms
   double cv;                          // the lane's sub-column condensate scaling
   double co[4];                       // droplet optics at unit scaling
@@ -756,6 +778,10 @@ struct LaneSrc {
   const double *rec, *br, *csub, *cop;
   const float *csubf, *copf;  // FP32 forms (mixed form)
   int gas, isub;
+  int fnc;                    // the term's scaling function
+  const SgTerm *term;         // the term (device table), for fnc other than the power law
+  const SgSpec *spec;
+  const double *grec;         // the column's gas record, or null
 };
 
 // MIXED reads the FP32 c_sub and droplet optics (kext, kabs, first moment,
@@ -768,6 +794,10 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
   x.mmr = r[SG_Q_MMR + s.gas];
   x.mass = r[SG_Q_MASS];
   x.frac1 = r[SG_Q_FRAC1];
+  x.kx = s.fnc == SG_FNC_POWER_LAW
+             ? 0.0
+             : k_scaled(*s.term, s.spec, x.lnp, r[SG_Q_P], r[SG_Q_T], s.grec ? s.grec + i * SG_NGQ : nullptr,
+                        x.mmr);
   const double *q = s.br + i * SG_NBQ;
   x.kgt = q[0];
   x.q1 = q[1];
@@ -791,7 +821,7 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
 // first moment, forward scattering), from prep_kernel.
 __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                            Coef &o) {
-  double kgas = k_term(t, x.lnp, x.lnt, x.mmr);
+  double kgas = t.fnc == SG_FNC_POWER_LAW ? k_term(t, x.lnp, x.lnt, x.mmr) : x.kx;
   const double frac1 = x.frac1, kgt = x.kgt, csub = x.cv;
   if (csub * frac1 > 0.0) {
     const double kext = x.co[0], kscat = x.co[1], phf = x.co[2], fwd = x.co[3];
@@ -825,7 +855,8 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
 __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                                 Coef &o) {
 #pragma clang fp contract(on)
-  const float kgas = k_term_f32(t, (float)x.lnp, (float)x.lnt, (float)x.mmr);
+  const float kgas =
+      t.fnc == SG_FNC_POWER_LAW ? k_term_f32(t, (float)x.lnp, (float)x.lnt, (float)x.mmr) : (float)x.kx;
   const double csub = x.cv;
   const double *co = x.co;
   const float frac1 = (float)x.frac1;
@@ -947,6 +978,7 @@ __device__ __forceinline__ void seg_reduce8(const double (&v)[8], int seg, int s
 template <bool MIXED, bool RECOMP>
 __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__restrict__ spec,
                                            int l_rayleigh, const double *__restrict__ colrec,
+                                           const double *__restrict__ grec,
                                            const double *__restrict__ colscal,
                                            const double *__restrict__ bandrec,
                                            const double *__restrict__ bandscal,
@@ -1006,6 +1038,10 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   src.csubf = (const float *)c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
   src.gas = t.gas;
   src.isub = isub;
+  src.fnc = t.fnc;
+  src.term = spec->g + gi;
+  src.spec = spec;
+  src.grec = grec ? grec + (size_t)col * L * SG_NGQ : nullptr;
   const double alb_diff = bs[2], alb_dir = bs[3];
 
   constexpr int kNst = RECOMP ? (SG_MAXL + kSeg - 1) / kSeg : SG_MAXL;
@@ -1309,7 +1345,12 @@ __device__ __forceinline__ void region_mix(double f, Coef &o) {
 }
 
 template <bool MIXED>
-__device__ __forceinline__ double lane_kgas(const SgTerm &t, const double *r) {
+__device__ __forceinline__ double lane_kgas(const SgTerm &t, const SgSpec *spec, const double *r,
+                                            const double *g) {
+  if (t.fnc != SG_FNC_POWER_LAW) {
+    const double k = k_scaled(t, spec, r[SG_Q_LNP], r[SG_Q_P], r[SG_Q_T], g, r[SG_Q_MMR + t.gas]);
+    return MIXED ? (double)(float)k : k;
+  }
   if (MIXED) return (double)k_term_f32(t, (float)r[SG_Q_LNP], (float)r[SG_Q_LNT], (float)r[SG_Q_MMR + t.gas]);
   return k_term(t, r[SG_Q_LNP], r[SG_Q_LNT], r[SG_Q_MMR + t.gas]);
 }
@@ -1349,6 +1390,7 @@ constexpr int kTriQ = 12;
 template <bool MIXED>
 __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *__restrict__ spec,
                                                int l_rayleigh, const double *__restrict__ colrec,
+                                               const double *__restrict__ grec,
                                                const double *__restrict__ colscal,
                                                const double *__restrict__ bandrec,
                                                const double *__restrict__ bandscal,
@@ -1404,7 +1446,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
       const double *r = rec + (size_t)i * SG_NQ;
       const double *bq = br + (size_t)i * SG_NBQ;
       const double adj = bq[1];
-      const double kgas = lane_kgas<MIXED>(t, r);
+      const double kgas = lane_kgas<MIXED>(t, spec, r, grec ? grec + ((size_t)col * L + i) * SG_NGQ : nullptr);
       Coef o[3];
       const bool cloud_rows = i + 1 >= nct;
       tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, cloud_rows, o);
@@ -1456,7 +1498,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   for (int i = L - 1; i >= 0; i--) {
     const double *r = rec + (size_t)i * SG_NQ;
     const double *bq = br + (size_t)i * SG_NBQ;
-    const double kgas = lane_kgas<MIXED>(t, r);
+    const double kgas = lane_kgas<MIXED>(t, spec, r, grec ? grec + ((size_t)col * L + i) * SG_NGQ : nullptr);
     const bool cloud_rows = i + 1 >= nct;
     Coef o[3];
     tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, cloud_rows, o);
@@ -1613,7 +1655,7 @@ struct ReqDesc {
   double dp_cf, dp_cw;
   SocgpuRunesArgs a;            // device input pointers
   double *up, *dn, *dir, *hr;   // device outputs (dir may be null)
-  double *colrec, *tlev, *colscal, *bandrec, *bandscal, *csub, *minor, *cldopt, *partial;
+  double *colrec, *tlev, *colscal, *bandrec, *bandscal, *csub, *grec, *cldopt, *partial;
   double *ovl, *tws;            // triple column: overlap coefficients, solve workspace
   int *krange;
   long long s[kNWork];          // first work item of this request in each space
@@ -1664,7 +1706,16 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
     const int ntc = (P + kPackC - 1) / kPackC;
     const int c0 = (t % ntc) * kPackC, i0 = (t / ntc) * kPackL;
     const int col = c0 + tx, i = i0 + ty;
-    if (col < P && i < L) pack_layer(q.d, q.spec, q.a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ);
+    if (col < P && i < L) {
+      pack_layer(q.d, q.spec, q.a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ);
+      if (q.d.gx) {
+        // gas record: look-up factors and the density (continua)
+        double *g = q.grec + ((size_t)col * L + i) * SG_NGQ;
+        const int idx = i * P + col;
+        if (q.spec->has_lut) lookup_factors(q.spec, q.a.p_layer[idx], q.a.t_layer[idx], g);
+        g[SG_G_RHO] = q.a.density ? q.a.density[idx] : 0.0;
+      }
+    }
     if (col < P && i <= L) shl[tx * kPackL + ty] = q.a.t_level ? q.a.t_level[(size_t)i * P + col] : 0.0;
     if (col < P && i == 0) pack_column(q.d, q.a, col, q.colscal, q.bandscal);
     __syncthreads();
@@ -1786,8 +1837,8 @@ __global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? 3 : 2) void b_band_ker
     const ReqDesc &q = rq[r];
     const int nch = (q.d.L + kWave - 1) / kWave;
     if ((NCH == 2 && nch <= 2) || (NCH > 2 && nch > 2))
-      band_body<NCH>(q.d, q.spec, q.l_rayleigh, q.colrec, q.tlev, q.colscal, q.bandrec, q.bandscal, q.cldopt,
-                     q.krange, (int)(w - q.s[W_MINOR]), lane);
+      band_body<NCH>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.a.gas_mix_ratio, q.tlev, q.colscal,
+                     q.bandrec, q.bandscal, q.cldopt, q.krange, (int)(w - q.s[W_MINOR]), lane);
   }
 }
 
@@ -1800,7 +1851,7 @@ __global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, co
   for (long long w = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; w < tot; w += nw) {
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_SOLVE, w));
     const ReqDesc &q = rq[r];
-    solve_body<MIXED, RECOMP>(q.d, q.spec, q.l_rayleigh, q.colrec, q.colscal, q.bandrec, q.bandscal,
+    solve_body<MIXED, RECOMP>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec, q.bandscal,
                               q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
   }
 }
@@ -1862,7 +1913,7 @@ __global__ __launch_bounds__(256) void b_tsolve_kernel(const BatchHdr *h, const 
   for (long long w = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; w < tot; w += nw) {
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_TSOLVE, w));
     const ReqDesc &q = rq[r];
-    tri_solve_body<MIXED>(q.d, q.spec, q.l_rayleigh, q.colrec, q.colscal, q.bandrec, q.bandscal, q.cldopt,
+    tri_solve_body<MIXED>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec, q.bandscal, q.cldopt,
                           q.ovl, q.krange, q.tws, q.partial, (int)(w - q.s[W_TSOLVE]), lane);
   }
 }
@@ -2243,13 +2294,11 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   if (solar && (!a->cos_zenith_angle || !a->solar_irrad)) return fail(SOCGPU_EINVAL, "solar call needs cos_zenith_angle and solar_irrad");
   if (!solar && (!a->t_level || !a->t_ground)) return fail(SOCGPU_EINVAL, "thermal call needs t_level and t_ground");
   if (!solar && S.n_deg <= 0 && S.band[0].planck[0] == 0.0) return fail(SOCGPU_EUNSUPPORTED, "spectral file has no Planck data");
-  bool gx = S.has_lut || S.has_cont;
-  for (int t = 0; t < S.n_g; t++) gx |= S.g[t].fnc != SG_FNC_POWER_LAW;
-  for (int t = 0; t < S.n_minor; t++) gx |= S.minor[t].fnc != SG_FNC_POWER_LAW || S.minor[t].gas >= SG_GAS_WM0;
-  if (gx) return fail(SOCGPU_EUNSUPPORTED, "look-up-table, power-quadratic and Wenyi k-terms, continua and further gases are not yet on the device path");
+  if (S.has_cont && !a->density) return fail(SOCGPU_EINVAL, "density is required for continuum absorption");
   CallDims d;
   std::memset(&d, 0, sizeof d);
   d.P = P; d.L = L; d.NB = NB; d.isolir = a->i_source;
+  d.gx = S.has_lut || S.has_cont;
   int cloud = a->i_cloud_representation;
   auto check_drop = [&](int t, const char *what) -> int {
     if (t <= 0) return fail(SOCGPU_EUNSUPPORTED, "Prescribed liquid cloud not yet implemented.");
@@ -2313,6 +2362,7 @@ static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
   const size_t P = d.P, L = d.L, NB = d.NB;
   size_t n = P * L * SG_NQ + P * (L + 1) + P * SG_NS + P * NB * L * SG_NBQ + P * NB * SG_NBQ +
              partial_doubles(d) + 2;
+  if (d.gx) n += P * L * SG_NGQ;
   if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * 4;
   if (d.triple) n += P * NB * L * 8 + P * (L + 1) * kOv + P * d.nwpc * L * kTriQ * kWave;
   return n;
@@ -2345,6 +2395,7 @@ static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cur
     return p;
   };
   q.colrec = take(P * L * SG_NQ);
+  q.grec = d.gx ? take(P * L * SG_NGQ) : nullptr;
   q.tlev = take(P * (L + 1));
   q.colscal = take(P * SG_NS);
   q.bandrec = take(P * NB * L * SG_NBQ);

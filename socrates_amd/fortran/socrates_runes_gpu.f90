@@ -50,6 +50,8 @@ module socrates_gpu_c
     type(c_ptr) :: conv_frac = c_null_ptr, liq_conv_mmr = c_null_ptr
     type(c_ptr) :: liq_conv_dim = c_null_ptr
     type(c_ptr) :: rand_seed = c_null_ptr
+    ! n2o, ch4, o2, so2, ocs, cfc11, cfc12, cfc113, hcfc22, hfc134a
+    real(c_double) :: gas_mix_ratio(10) = 0.0_c_double
   end type socgpu_runes_args
 
   ! SocgpuDiag
@@ -235,7 +237,9 @@ contains
   subroutine runes(n_profile, n_layer, diag, spectrum_name, n_subcol_gen, &
       p_layer, t_layer, p_level, t_level, mass, density, h2o, o3, &
       h2o_mix_ratio, co2_mix_ratio, o3_mix_ratio, n2o_mix_ratio, ch4_mix_ratio, &
-      o2_mix_ratio, t_ground, cos_zenith_angle, solar_irrad, &
+      o2_mix_ratio, so2_mix_ratio, ocs_mix_ratio, cfc11_mix_ratio, cfc12_mix_ratio, &
+      cfc113_mix_ratio, hcfc22_mix_ratio, hfc134a_mix_ratio, &
+      t_ground, cos_zenith_angle, solar_irrad, &
       l_grey_albedo, grey_albedo, albedo_diff, albedo_dir, &
       cloud_frac, conv_frac, liq_mmr, liq_conv_mmr, liq_dim, liq_conv_dim, &
       cloud_vertical_decorr, conv_vertical_decorr, cloud_horizontal_rsd, &
@@ -252,6 +256,9 @@ contains
     real(RealExt), intent(in), optional, target, contiguous :: h2o(:, :), o3(:, :)
     real(RealExt), intent(in), optional :: h2o_mix_ratio, co2_mix_ratio, o3_mix_ratio
     real(RealExt), intent(in), optional :: n2o_mix_ratio, ch4_mix_ratio, o2_mix_ratio
+    real(RealExt), intent(in), optional :: so2_mix_ratio, ocs_mix_ratio, cfc11_mix_ratio
+    real(RealExt), intent(in), optional :: cfc12_mix_ratio, cfc113_mix_ratio
+    real(RealExt), intent(in), optional :: hcfc22_mix_ratio, hfc134a_mix_ratio
     real(RealExt), intent(in), optional, target, contiguous :: t_ground(:)
     real(RealExt), intent(in), optional, target, contiguous :: cos_zenith_angle(:)
     real(RealExt), intent(in), optional, target, contiguous :: solar_irrad(:)
@@ -286,10 +293,6 @@ contains
     if (.not. present(spectrum_name)) call fail('spectrum_name is required')
     name = spectrum_name
     call check(socgpu_spectrum_query(c_str(name), nb, nabs, types, 32_c_int), 'RUNES')
-    ! well-mixed gases the binding has no slot for must not be in the file
-    if (present(n2o_mix_ratio)) call absent_gas(4, 'N2O')
-    if (present(ch4_mix_ratio)) call absent_gas(6, 'CH4')
-    if (present(o2_mix_ratio)) call absent_gas(7, 'O2')
     if (present(h2o_mix_ratio) .and. .not. present(h2o)) call fail('h2o_mix_ratio: give h2o')
     if (present(o3_mix_ratio) .and. .not. present(o3)) call fail('o3_mix_ratio: give o3')
 
@@ -305,6 +308,17 @@ contains
     if (present(i_cnv_water)) a%i_cnv_water = i_cnv_water
     if (present(n_subcol_gen)) a%n_subcol_gen = n_subcol_gen
     if (present(co2_mix_ratio)) a%co2_mix_ratio = co2_mix_ratio
+    ! further well-mixed gases (absent = 0, socrates_set_atm.F90:596-617)
+    if (present(n2o_mix_ratio)) a%gas_mix_ratio(1) = n2o_mix_ratio
+    if (present(ch4_mix_ratio)) a%gas_mix_ratio(2) = ch4_mix_ratio
+    if (present(o2_mix_ratio)) a%gas_mix_ratio(3) = o2_mix_ratio
+    if (present(so2_mix_ratio)) a%gas_mix_ratio(4) = so2_mix_ratio
+    if (present(ocs_mix_ratio)) a%gas_mix_ratio(5) = ocs_mix_ratio
+    if (present(cfc11_mix_ratio)) a%gas_mix_ratio(6) = cfc11_mix_ratio
+    if (present(cfc12_mix_ratio)) a%gas_mix_ratio(7) = cfc12_mix_ratio
+    if (present(cfc113_mix_ratio)) a%gas_mix_ratio(8) = cfc113_mix_ratio
+    if (present(hcfc22_mix_ratio)) a%gas_mix_ratio(9) = hcfc22_mix_ratio
+    if (present(hfc134a_mix_ratio)) a%gas_mix_ratio(10) = hfc134a_mix_ratio
     if (present(cloud_vertical_decorr)) a%cloud_vertical_decorr = cloud_vertical_decorr
     if (present(conv_vertical_decorr)) a%conv_vertical_decorr = conv_vertical_decorr
     if (present(cloud_horizontal_rsd)) a%cloud_horizontal_rsd = cloud_horizontal_rsd
@@ -359,13 +373,6 @@ contains
       write(*, '(a, a)') '? Error from routine: RUNES: ', msg
       error stop 1
     end subroutine fail
-
-    subroutine absent_gas(ip, label)
-      integer, intent(in) :: ip
-      character(len=*), intent(in) :: label
-      if (any(types(1:nabs) == ip)) &
-        call fail(label // ' is in the spectral file: only H2O, CO2 and O3 amounts are supported')
-    end subroutine absent_gas
 
     subroutine field(x, n2, p, label)
       real(RealExt), intent(in), optional, target, contiguous :: x(:, :)

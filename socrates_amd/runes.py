@@ -19,7 +19,7 @@ import os
 
 import numpy as np
 
-from .case_io import Case
+from .case_io import WM_GASES, Case
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libsocgpu.so")
@@ -52,7 +52,8 @@ class RunesArgs(ctypes.Structure):
         "p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
         "p_level", "t_level", "t_ground", "cos_zenith_angle", "solar_irrad",
         "albedo_diff", "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac",
-        "liq_conv_mmr", "liq_conv_dim")] + [("rand_seed", _I)]
+        "liq_conv_mmr", "liq_conv_dim")] + [("rand_seed", _I),
+                                             ("gas_mix_ratio", ctypes.c_double * len(WM_GASES))]
 
 
 class Diag(ctypes.Structure):
@@ -134,10 +135,17 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
           liq_conv_dim=None, cloud_vertical_decorr=0.0, conv_vertical_decorr=0.0,
           cloud_horizontal_rsd=0.0, i_cloud_representation=0, i_overlap=0, i_inhom=0,
           i_mcica_sampling=0, i_st_water=5, i_cnv_water=5, n_subcol_gen=0,
-          rand_seed=None, out=None):
+          rand_seed=None, out=None, **mix_ratios):
     """ctypes argument and output structures of one call (and the objects
-    that must stay alive while it runs)."""
+    that must stay alive while it runs). mix_ratios: the well-mixed mixing
+    ratios of further gases, as runes names them (n2o_mix_ratio,
+    ch4_mix_ratio, o2_mix_ratio, so2_mix_ratio, ocs_mix_ratio,
+    cfc11_mix_ratio, cfc12_mix_ratio, cfc113_mix_ratio, hcfc22_mix_ratio,
+    hfc134a_mix_ratio)."""
     init()
+    unknown = set(mix_ratios) - {g + "_mix_ratio" for g in WM_GASES}
+    if unknown:
+        raise TypeError("unexpected arguments: %s" % ", ".join(sorted(unknown)))
     inputs = dict(p_layer=p_layer, t_layer=t_layer, mass=mass, density=density,
                   layer_heat_capacity=layer_heat_capacity, h2o=h2o, o3=o3, p_level=p_level,
                   t_level=t_level, t_ground=t_ground, cos_zenith_angle=cos_zenith_angle,
@@ -154,6 +162,8 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     a.i_st_water, a.i_cnv_water, a.n_subcol_gen = int(i_st_water), int(i_cnv_water), int(n_subcol_gen)
     a.l_inputs_on_device = int(on_dev)
     a.co2_mix_ratio = float(co2_mix_ratio)
+    for j, g in enumerate(WM_GASES):
+        a.gas_mix_ratio[j] = float(mix_ratios.get(g + "_mix_ratio", 0.0) or 0.0)
     a.cloud_vertical_decorr, a.conv_vertical_decorr = float(cloud_vertical_decorr), float(conv_vertical_decorr)
     a.cloud_horizontal_rsd = float(cloud_horizontal_rsd)
     a.l_grey_albedo, a.grey_albedo = int(bool(l_grey_albedo)), float(grey_albedo)
@@ -376,6 +386,10 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         i_inhom=case.i_inhom, i_mcica_sampling=case.i_mcica_sampling,
         i_st_water=case.i_st_water, i_cnv_water=case.i_st_water,
         n_subcol_gen=case.n_subcol_gen, rand_seed=g("rand_seed") if cloudy else None, out=out)
+    if case.wm_mix_ratio is not None:
+        for j, gname in enumerate(WM_GASES):
+            if case.wm_mix_ratio[j] != 0.0:
+                kw[gname + "_mix_ratio"] = float(case.wm_mix_ratio[j])
     return spectrum_name, kw
 
 
