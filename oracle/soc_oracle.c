@@ -781,13 +781,28 @@ static void pade_2(const double *cp, double mmr, double re, double *k_ext_tot,
  * (idx 1-9: v, 10-18: u, as the reference numbers them). */
 void soc_oracle_overlap_triple(int L, int nct, const double *w_cloud, const double *frac_st,
                                const double *frac_cv, double *ov);
+/* ioverlap: ip_max_rand (0, correlation 1 between like regions),
+ * ip_rand (1, correlation 0) or ip_exponential_rand (3, correlation
+ * exp((p(i) - p(i+1)) / dp) at interfaces 1 < i < L, with dp = 1 Pa for the
+ * clear region and the stratiform / convective decorrelation pressures,
+ * overlap_coupled.F90:190-231). p: the profile's layer pressures. */
 static void overlap_triple_profile(int L, int nct, const double *w_cloud, const double *frac_st,
-                                   const double *frac_cv, double *ov) {
+                                   const double *frac_cv, int ioverlap, const double *p, double dps,
+                                   double dpc, double *ov) {
   const double tol_cloud = 1.0e+02 * DBL_EPSILON;
-  double area_upper[4], area_lower[4], ao[4][4], aru[4], arl[4];
+  double area_upper[4], area_lower[4], ao[4][4], aru[4], arl[4], corr[4];
   area_upper[1] = 1.0; area_upper[2] = 0.0; area_upper[3] = 0.0;
   for (int i = nct - 1; i <= L; i++) {
     double *co = ov + (size_t)(i - (nct - 1)) * 18 - 1; /* co[idx] for idx 1..18 */
+    for (int k = 1; k <= 3; k++) {
+      if (ioverlap == 0) corr[k] = 1.0;
+      else if (ioverlap == 1) corr[k] = 0.0;
+      else {
+        const double dp_corr = k == 1 ? 1.0 : k == 2 ? dps : dpc;
+        /* dp_corr = 0: exp(-inf) = 0 for layers of distinct pressure */
+        corr[k] = (i < L && i > 1) ? (dp_corr > 0.0 ? exp((p[i - 1] - p[i]) / dp_corr) : 0.0) : 1.0;
+      }
+    }
     for (int k = 1; k <= 3; k++) {
       if (i < L) {
         if (k == 1) area_lower[k] = 1.0 - w_cloud[i];           /* w_free(i+1) */
@@ -795,7 +810,7 @@ static void overlap_triple_profile(int L, int nct, const double *w_cloud, const 
       } else {
         area_lower[k] = k == 1 ? 1.0 : 0.0;
       }
-      ao[k][k] = 1.0 * fmin(area_lower[k], area_upper[k]);
+      ao[k][k] = corr[k] * fmin(area_lower[k], area_upper[k]);
     }
     for (int k = 1; k <= 3; k++)
       for (int j = 1; j < k; j++) { ao[k][j] = 0.0; ao[j][k] = 0.0; }
@@ -830,7 +845,7 @@ static void overlap_triple_profile(int L, int nct, const double *w_cloud, const 
 /* exported for the tests */
 void soc_oracle_overlap_triple(int L, int nct, const double *w_cloud, const double *frac_st,
                                const double *frac_cv, double *ov) {
-  overlap_triple_profile(L, nct, w_cloud, frac_st, frac_cv, ov);
+  overlap_triple_profile(L, nct, w_cloud, frac_st, frac_cv, 0, NULL, 0.0, 0.0, ov);
 }
 
 typedef struct {
@@ -1104,10 +1119,20 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     }
   /* ip_cloud_conv_strat: triple-column overlap with maximum-random overlap;
    * McICA is switched off for it (socrates_set_control.F90:338-356) */
-  int triple = c->l_cloud && c->i_cloud_repr == 3;
-  int mcica = c->l_cloud && !triple && c->i_inhom == 2;
+  /* ip_cloud_homogen without McICA is the mixed-column representation
+   * (ip_cloud_mix_max / ip_cloud_mix_random / ip_cloud_part_corr with
+   * solver_mix_direct_hogan, socrates_set_control.F90:319-334); its two
+   * regions are the clear and stratiform regions of the triple-column
+   * solve with an empty convective region, which the reference computes to
+   * the same bits. ip_cloud_conv_strat with random overlap is
+   * ip_cloud_mix_random, which the reference rejects (mix_column.F90). */
+  int conv_strat = c->l_cloud && c->i_cloud_repr == 3;
+  int mixcol = c->l_cloud && c->i_cloud_repr == 1 && c->i_inhom != 2;
+  int triple = conv_strat || mixcol;
+  int mcica = c->l_cloud && c->i_cloud_repr == 1 && c->i_inhom == 2;
   if (c->l_cloud && !mcica && !triple) return -10;
-  if (triple && c->i_overlap != 0) return -10;
+  if (triple && c->i_overlap != 0 && c->i_overlap != 1 && c->i_overlap != 3) return -10;
+  if (conv_strat && c->i_overlap == 1) return -10;
   if (mcica) {
     rc = read_mcica(mcica_file, &mc);
     if (rc) return rc;
@@ -1221,9 +1246,9 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   }
   tri_t tri;
   double *tbuf = NULL;
-  double *kcs[2][4], *tau_k[2], *om_k[2], *g_k[2], *frk[2], *wcol = NULL;
+  double *kcs[2][4], *tau_k[2], *om_k[2], *g_k[2], *frk[2], *wcol = NULL, *pcol = NULL;
   if (triple) {
-    tbuf = (double *)malloc(sizeof(double) * (size_t)(60 * (L + 2) + 2 * L + 4));
+    tbuf = (double *)malloc(sizeof(double) * (size_t)(61 * (L + 2) + 2 * L + 4));
     double *qq = tbuf;
     tri.L = L; tri.nct = nct;
     for (int r = 0; r < 3; r++) {
@@ -1237,6 +1262,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       tau_k[k] = qq; qq += L; om_k[k] = qq; qq += L; g_k[k] = qq; qq += L; frk[k] = qq; qq += L;
     }
     wcol = qq; qq += L;
+    pcol = qq; qq += L;
     tri.flux_direct = qq; qq += L + 1;
     tri.flux_total = qq; qq += 2 * L + 2;
   }
@@ -1279,7 +1305,9 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       for (int i = 0; i < L; i++) {
         wcol[i] = LAY(tw, i, l); fs1[i] = LAY(tfs, i, l); fc1[i] = LAY(tfc, i, l);
       }
-      overlap_triple_profile(L, nct, wcol, fs1, fc1, ovl);
+      for (int i = 0; i < L; i++) pcol[i] = LAY(c->p_layer, i, l);
+      overlap_triple_profile(L, nct, wcol, fs1, fc1, c->i_overlap, pcol, c->cloud_vertical_decorr,
+                             c->conv_vertical_decorr, ovl);
     }
     double zen_0 = 1.0, sol = 0.0;
     if (isolir == IP_SOLAR) {

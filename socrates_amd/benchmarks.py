@@ -9,6 +9,8 @@
             p50/p99 submit-to-done latency and throughput per arrival rate.
   triple  - stratiform + convective cloud (triple-column regions) on 4096
             columns x 90 layers, SW and LW in one graph-captured batch.
+  ga9     - the production GA9 spectral files and McICA data on 4096 x 90
+            all-sky columns, SW and LW in one graph-captured batch.
 
 Inputs are synthetic (socrates_amd/synth.py, seeded). Throughput is in
 column.g-point solves/s, and also normalised to 90 layers (x L/90).
@@ -16,6 +18,7 @@ column.g-point solves/s, and also normalised to 90 layers (x L/90).
     python -m socrates_amd.benchmarks ragged [--steps 10]
     python -m socrates_amd.benchmarks serve [--rates 50,200,1000,2000] [--seconds 10] [--inputs host|device]
     python -m socrates_amd.benchmarks triple [--steps 10]
+    python -m socrates_amd.benchmarks ga9 [--steps 10]
 """
 import argparse
 import json
@@ -216,9 +219,52 @@ def triple(steps=10, warmup=2, P=4096, L=90, seed=20261209):
     return line
 
 
+def ga9(steps=10, warmup=2, P=4096, L=90, seed=20261211):
+    """The production GA9 spectral files (41 SW and 81 LW k-terms, P/T
+    look-up tables, water-vapour continuum, ten further gases) with their
+    McICA data on P x L all-sky columns, SW and LW in one graph-captured
+    batch per step, FP64 and mixed; (column, k-term) solves per second."""
+    import torch
+    dev = torch.device("cuda:0")
+    R.init(0)
+    ga9_dir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "ga9")
+    R.set_spectrum("sw_ga9", os.path.join(ga9_dir, "sp_sw_ga9"))
+    R.set_spectrum("lw_ga9", os.path.join(ga9_dir, "sp_lw_ga9"))
+    R.set_mcica(os.path.join(ga9_dir, "mcica_data"), sw_spectrum_name="sw_ga9", lw_spectrum_name="lw_ga9")
+    wm = np.array([4.7255e-7, 9.637e-7, 0.23139, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    calls, keep = [], []
+    n_g = {"sw": 41, "lw": 81}
+    for src in ("sw", "lw"):
+        case = synth.make_case(P, L, src, 6 if src == "sw" else 9, seed=seed + (src == "sw"), cloudy=True)
+        case.wm_mix_ratio = wm.copy()
+        arrs, out = _device_case(torch, case, dev)
+        keep.append((arrs, out))
+        calls.append(R.case_call(case, src + "_ga9", device_arrays=arrs, out=out))
+    reqs = R.Requests(calls)
+    solves = P * (n_g["sw"] + n_g["lw"])
+    res = {}
+    for prec in ("fp64", "mixed"):
+        b = R.Batch(prec)
+        for _ in range(warmup):
+            b.run(reqs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run(reqs)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        b.close()
+        res[prec] = dict(ms_per_step=dt * 1e3, solves_per_s=solves / dt)
+    line = {"benchmark": "GA9 spectral files, all-sky McICA, one graph-captured batch of the SW and LW calls per step",
+            "columns": P, "layers": L, "k_terms": n_g, "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_over_fp64": res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"]}
+    print(json.dumps(line), flush=True)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["ragged", "serve", "triple"])
+    ap.add_argument("what", choices=["ragged", "serve", "triple", "ga9"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--rates", default="50,200,1000,2000")
     ap.add_argument("--seconds", type=float, default=10.0)
@@ -229,6 +275,8 @@ def main():
         ragged(steps=a.steps)
     elif a.what == "triple":
         triple(steps=a.steps)
+    elif a.what == "ga9":
+        ga9(steps=a.steps)
     else:
         serve(rates=[int(x) for x in a.rates.split(",")], seconds=a.seconds, max_requests=a.max_requests,
               inputs=a.inputs)

@@ -1249,8 +1249,8 @@ __device__ __forceinline__ void heating_body(int P, int L, const double *__restr
 constexpr int kOv = 18;
 
 __device__ __forceinline__ void overlap_body(const CallDims &d, const double *__restrict__ colrec,
-                                             const int *__restrict__ krange, double *__restrict__ ovl,
-                                             int idx) {
+                                             const int *__restrict__ krange, double dp_strat,
+                                             double dp_conv, double *__restrict__ ovl, int idx) {
   const int P = d.P, L = d.L;
   if (idx >= P * (L + 1)) return;
   const int col = idx % P, j = idx / P;
@@ -1271,9 +1271,25 @@ __device__ __forceinline__ void overlap_body(const CallDims &d, const double *__
     lo[2] = r[SG_Q_WCLD] * r[SG_Q_FRAC2];
   }
   const double tol = 100.0 * kEps;
+  // correlation of like regions across the interface (overlap_coupled.F90:
+  // 190-231): 1 for maximum-random overlap, 0 for random, and for
+  // exponential-random exp((p_j - p_j+1) / dp) at interfaces 1 < j < L with
+  // dp = 1 Pa for the clear region and the decorrelation pressures of the
+  // stratiform and convective regions
+  double corr[3] = {1.0, 1.0, 1.0};
+  if (d.ioverlap == SOCGPU_OVERLAP_RANDOM) {
+    corr[0] = 0.0; corr[1] = 0.0; corr[2] = 0.0;
+  } else if (d.ioverlap == SOCGPU_OVERLAP_EXP_RAND && j > 1 && j < L) {
+    const double dpl = rec[(size_t)(j - 1) * SG_NQ + SG_Q_P] - rec[(size_t)j * SG_NQ + SG_Q_P];
+    corr[0] = exp(dpl / 1.0);
+    // (a zero decorrelation pressure gives correlation 0, as exp(-inf) in
+    // the reference for layers of distinct pressure)
+    corr[1] = dp_strat > 0.0 ? exp(dpl / dp_strat) : 0.0;
+    corr[2] = dp_conv > 0.0 ? exp(dpl / dp_conv) : 0.0;
+  }
   double ao[3][3];  // ao[k][m]: area of region k above overlapping region m below
   for (int k = 0; k < 3; k++)
-    for (int m = 0; m < 3; m++) ao[k][m] = k == m ? fmin(lo[k], up[k]) : 0.0;
+    for (int m = 0; m < 3; m++) ao[k][m] = k == m ? corr[k] * fmin(lo[k], up[k]) : 0.0;
   double art = 1.0 - ao[0][0];
   art = art - ao[1][1];
   art = art - ao[2][2];
@@ -1901,7 +1917,8 @@ __global__ void b_overlap_kernel(const BatchHdr *h, const ReqDesc *rq) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
        g += (long long)gridDim.x * blockDim.x) {
     const ReqDesc &q = rq[find_req(h, rq, W_TOVL, g)];
-    overlap_body(q.d, q.colrec, q.krange, q.ovl, (int)(g - q.s[W_TOVL]));
+    overlap_body(q.d, q.colrec, q.krange, q.a.cloud_vertical_decorr, q.a.conv_vertical_decorr, q.ovl,
+                 (int)(g - q.s[W_TOVL]));
   }
 }
 
@@ -2308,8 +2325,10 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   };
   if (cloud == SOCGPU_CLOUD_CONV_STRAT) {
     // stratiform and convective cloud: triple-column regions; McICA does not
-    // apply to this representation (socrates_set_control.F90:338-356)
-    if (a->i_overlap != SOCGPU_OVERLAP_MAX_RAND)
+    // apply to this representation (socrates_set_control.F90:338-356).
+    // Random overlap selects ip_cloud_mix_random, which the reference's
+    // mix_column rejects for the triple solver.
+    if (a->i_overlap != SOCGPU_OVERLAP_MAX_RAND && a->i_overlap != SOCGPU_OVERLAP_EXP_RAND)
       return fail(SOCGPU_EUNSUPPORTED, "overlap %d with convective and stratiform cloud is not supported", a->i_overlap);
     if (!a->cloud_frac || !a->conv_frac) return fail(SOCGPU_EINVAL, "Cloud fraction not provided.");
     if (int rc = check_drop(a->i_st_water, "Liquid")) return rc;
@@ -2317,9 +2336,25 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     d.triple = 1;
     d.drop = a->i_st_water - 1;
     d.drop2 = a->i_cnv_water - 1;
+    d.ioverlap = a->i_overlap;
+  } else if (cloud == SOCGPU_CLOUD_HOMOGEN && a->i_inhom != SOCGPU_INHOM_MCICA) {
+    // mixed-column representation (ip_cloud_mix_max / ip_cloud_mix_random /
+    // ip_cloud_part_corr with solver_mix_direct_hogan, socrates_set_control.F90:
+    // 319-334): its clear and cloudy regions are the clear and stratiform
+    // regions of the triple-column solve with no convective cloud, which the
+    // reference computes to the same bits
+    if (a->i_overlap != SOCGPU_OVERLAP_MAX_RAND && a->i_overlap != SOCGPU_OVERLAP_EXP_RAND &&
+        a->i_overlap != SOCGPU_OVERLAP_RANDOM)
+      return fail(SOCGPU_EUNSUPPORTED, "overlap %d is not supported", a->i_overlap);
+    if (!a->cloud_frac || !a->liq_mmr || !a->liq_dim) return fail(SOCGPU_EINVAL, "Cloud fraction not provided.");
+    if (int rc = check_drop(a->i_st_water, "Liquid")) return rc;
+    d.triple = 1;
+    d.drop = a->i_st_water - 1;
+    d.drop2 = a->i_st_water - 1;
+    d.ioverlap = a->i_overlap;
   } else if (cloud != SOCGPU_CLOUD_OFF) {
     if (cloud != SOCGPU_CLOUD_HOMOGEN || a->i_inhom != SOCGPU_INHOM_MCICA)
-      return fail(SOCGPU_EUNSUPPORTED, "cloud representation %d with i_inhom %d is not on the McICA path", cloud, a->i_inhom);
+      return fail(SOCGPU_EUNSUPPORTED, "cloud representation %d with i_inhom %d is not on the path", cloud, a->i_inhom);
     if (!S.has_mcica) return fail(SOCGPU_ENOTFOUND, "Spectrum name not associated with MCICA data.");
     if (a->i_mcica_sampling != SOCGPU_MCICA_SINGLE && a->i_mcica_sampling != SOCGPU_MCICA_OPTIMAL)
       return fail(SOCGPU_EUNSUPPORTED, "McICA sampling %d not supported", a->i_mcica_sampling);

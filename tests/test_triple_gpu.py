@@ -105,7 +105,9 @@ def test_triple_in_a_batch_with_mcica_calls():
 
 
 def test_triple_rejects_unsupported_overlap():
+    """Random overlap with convective and stratiform cloud is
+    ip_cloud_mix_random, which the reference rejects for the triple solver."""
     case = triple_case(4, 30, "lw", 5)
-    case.i_overlap = synth.IP_EXPONENTIAL_RAND
+    case.i_overlap = 1
     with pytest.raises(R.SocgpuError):
         R.runes_case(case, "lw", precision="fp64")
