@@ -1852,8 +1852,13 @@ __global__ __launch_bounds__(64 * kCgWaves) void b_cloudgen_kernel(const BatchHd
   }
 }
 
+// waves per SIMD the band kernel is built for (calls of up to 128 layers)
+#ifndef SG_BAND_OCC
+#define SG_BAND_OCC 3
+#endif
+
 template <int NCH, bool GEN>
-__global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? 3 : 2) void b_band_kernel(const BatchHdr *h, const ReqDesc *rq) {
+__global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? SG_BAND_OCC : 2) void b_band_kernel(const BatchHdr *h, const ReqDesc *rq) {
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const long long tot = h->tot[W_MINOR];
   for (long long w = blockIdx.x * (long long)kMinorWaves + wv; w < tot; w += (long long)gridDim.x * kMinorWaves) {
