@@ -414,26 +414,52 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
 }
 
 // Wave-wide inclusive scans over the 64 lanes (lane order = scan order).
+// A double moved across lanes by DPP (both 32-bit halves with the same
+// control): row_shr:n (CTRL 0x110 + n) within rows of 16, row_bcast:15
+// (0x142) and row_bcast:31 (0x143) across rows. Lanes with no source lane,
+// or in rows ROWMASK leaves out, take `id`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v, double id) {
+  const long long vb = __double_as_longlong(v), ib = __double_as_longlong(id);
+  const int lo = __builtin_amdgcn_update_dpp((int)ib, (int)vb, CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ib >> 32), (int)(vb >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Inclusive product scan over the 64 lanes (lane j ends with x_0 ... x_j):
+// shifts by 1, 2, 4 and 8 within each row of 16, then the last lane of row 0
+// (and of row 2) into rows 1 (and 3), then lane 31 into rows 2 and 3. DPP
+// moves are register operations, unlike the LDS-routed shuffles.
 __device__ __forceinline__ double scan_mul(double x, int lane) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    double y = __shfl_up(x, off, kWave);
-    if (lane >= off) x *= y;
-  }
+  (void)lane;
+  x = x * dpp_f64<0x111, 0xf>(x, 1.0);
+  x = x * dpp_f64<0x112, 0xf>(x, 1.0);
+  x = x * dpp_f64<0x114, 0xf>(x, 1.0);
+  x = x * dpp_f64<0x118, 0xf>(x, 1.0);
+  x = x * dpp_f64<0x142, 0xa>(x, 1.0);
+  x = x * dpp_f64<0x143, 0xc>(x, 1.0);
   return x;
 }
 
+// One step of the affine scan: the maps of lower lanes (A0, C0; identity
+// (1, 0) where there is none) applied before the lane's own.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void affine_step(double &A, double &C) {
+  const double A0 = dpp_f64<CTRL, ROWMASK>(A, 1.0), C0 = dpp_f64<CTRL, ROWMASK>(C, 0.0);
+  C = A * C0 + C;
+  A = A * A0;
+}
+
 // Composition of affine maps F -> A F + C in lane order: lane j ends with
-// the map of lanes 0..j applied in turn.
+// the map of lanes 0..j applied in turn (the steps of scan_mul).
 __device__ __forceinline__ void scan_affine(double &A, double &C, int lane) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    double A0 = __shfl_up(A, off, kWave), C0 = __shfl_up(C, off, kWave);
-    if (lane >= off) {
-      C = A * C0 + C;
-      A = A * A0;
-    }
-  }
+  (void)lane;
+  affine_step<0x111, 0xf>(A, C);
+  affine_step<0x112, 0xf>(A, C);
+  affine_step<0x114, 0xf>(A, C);
+  affine_step<0x118, 0xf>(A, C);
+  affine_step<0x142, 0xa>(A, C);
+  affine_step<0x143, 0xc>(A, C);
 }
 
 // One wave per (column, band), lanes over layers (64 at a time, at most
