@@ -63,70 +63,98 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// Cross-lane exchanges of doubles without the LDS: gfx950's
+// v_permlane32_swap / v_permlane16_swap (swap the upper 32 lanes of one
+// operand with the lower 32 of the other; the odd 16-lane rows of one with
+// the even rows of the other) and DPP row_mirror (lane i <-> 15 - i in each
+// row), row_half_mirror (i <-> 7 - i in each 8 lanes) and quad_perm. Each
+// acts on the two 32-bit halves alike.
+__device__ __forceinline__ unsigned lo32(double v) { return (unsigned)__double_as_longlong(v); }
+__device__ __forceinline__ unsigned hi32(double v) { return (unsigned)(__double_as_longlong(v) >> 32); }
+__device__ __forceinline__ double mk64(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Pairwise sum across the wave halves: lanes 0..31 get a + (a of lane i+32),
+// lanes 32..63 get b + (b of lane i-32).
+__device__ __forceinline__ double swap32_sum(double a, double b) {
+  const auto l = __builtin_amdgcn_permlane32_swap(lo32(a), lo32(b), false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi32(a), hi32(b), false, false);
+  return mk64(l[0], h[0]) + mk64(l[1], h[1]);
+}
+
+// The same across 16-lane rows: even rows get a + (a of the row above),
+// odd rows b + (b of the row below).
+__device__ __forceinline__ double swap16_sum(double a, double b) {
+  const auto l = __builtin_amdgcn_permlane16_swap(lo32(a), lo32(b), false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi32(a), hi32(b), false, false);
+  return mk64(l[0], h[0]) + mk64(l[1], h[1]);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)lo32(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)hi32(v), CTRL, 0xf, 0xf, false);
+  return mk64((unsigned)lo, (unsigned)hi);
+}
+
+// The value of lane 63 in every lane, and the value of the lane below
+// (wave_shr:1; lane 0 gets 0).
+__device__ __forceinline__ double lane63(double v) {
+  return mk64((unsigned)__builtin_amdgcn_readlane((int)lo32(v), 63),
+              (unsigned)__builtin_amdgcn_readlane((int)hi32(v), 63));
+}
+
+// Sum over each group of 8 lanes, in every lane of the group.
+__device__ __forceinline__ double sum8(double x) {
+  x = x + dpp_move<0x141>(x);  // row_half_mirror
+  x = x + dpp_move<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+  x = x + dpp_move<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+  return x;
+}
+
+__device__ __forceinline__ double lane_below(double v) { return dpp_move<0x138>(v); }
+
 // Butterfly reduce-scatter of 16 values per lane over the 64 lanes of a
-// wave: afterwards lane 4j (j = 0..15) holds the wave total of value j in
-// v[0] (lanes 4j+1..4j+3 hold the same total). 17 shuffles instead of the 96
+// wave: afterwards lanes 4j..4j+3 (j = 0..15) hold the wave total of value j
+// in v[0]. Each step halves the values a lane carries, summing them with
+// the matching values of its partner lane: 15 exchanges instead of the 96
 // of sixteen separate all-reductions.
 __device__ __forceinline__ void reduce16(double (&v)[16], int lane) {
-  {
-    const bool hi = lane & 32;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      double keep = hi ? v[k + 8] : v[k], give = hi ? v[k] : v[k + 8];
-      v[k] = keep + __shfl_xor(give, 32, kWave);
-    }
-  }
-  {
-    const bool hi = lane & 16;
+  for (int k = 0; k < 8; k++) v[k] = swap32_sum(v[k], v[k + 8]);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      double keep = hi ? v[k + 4] : v[k], give = hi ? v[k] : v[k + 4];
-      v[k] = keep + __shfl_xor(give, 16, kWave);
-    }
-  }
+  for (int k = 0; k < 4; k++) v[k] = swap16_sum(v[k], v[k + 4]);
   {
     const bool hi = lane & 8;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-      double keep = hi ? v[k + 2] : v[k], give = hi ? v[k] : v[k + 2];
-      v[k] = keep + __shfl_xor(give, 8, kWave);
+      const double keep = hi ? v[k + 2] : v[k], give = hi ? v[k] : v[k + 2];
+      v[k] = keep + dpp_move<0x140>(give);  // row_mirror: a partner in the other 8 lanes
     }
   }
   {
     const bool hi = lane & 4;
-    double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
-    v[0] = keep + __shfl_xor(give, 4, kWave);
+    const double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
+    v[0] = keep + dpp_move<0x141>(give);    // row_half_mirror: the other quad
   }
-  v[0] += __shfl_xor(v[0], 2, kWave);
-  v[0] += __shfl_xor(v[0], 1, kWave);
+  v[0] = v[0] + dpp_move<0x4E>(v[0]);
+  v[0] = v[0] + dpp_move<0xB1>(v[0]);
 }
 
-// Same for 8 values per lane: lane 8j (j = 0..7) holds the total of value j.
+// Same for 8 values per lane: lanes 8j..8j+7 (j = 0..7) hold the total of
+// value j.
 __device__ __forceinline__ void reduce8(double (&v)[8], int lane) {
-  {
-    const bool hi = lane & 32;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      double keep = hi ? v[k + 4] : v[k], give = hi ? v[k] : v[k + 4];
-      v[k] = keep + __shfl_xor(give, 32, kWave);
-    }
-  }
-  {
-    const bool hi = lane & 16;
+  for (int k = 0; k < 4; k++) v[k] = swap32_sum(v[k], v[k + 4]);
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-      double keep = hi ? v[k + 2] : v[k], give = hi ? v[k] : v[k + 2];
-      v[k] = keep + __shfl_xor(give, 16, kWave);
-    }
-  }
+  for (int k = 0; k < 2; k++) v[k] = swap16_sum(v[k], v[k + 2]);
   {
     const bool hi = lane & 8;
-    double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
-    v[0] = keep + __shfl_xor(give, 8, kWave);
+    const double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
+    v[0] = keep + dpp_move<0x140>(give);
   }
-  v[0] += __shfl_xor(v[0], 4, kWave);
-  v[0] += __shfl_xor(v[0], 2, kWave);
-  v[0] += __shfl_xor(v[0], 1, kWave);
+  v[0] = sum8(v[0]);
 }
 
 struct CallDims {
@@ -572,7 +600,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
             const int i = c * kWave + lane;
             const double x = i < L ? exp(-kk[c] * mass[c] * sec0) : 1.0;
             f[c] = fdt * scan_mul(x, lane);
-            fdt = __shfl(f[c], kWave - 1, kWave);
+            fdt = lane63(f[c]);
           }
         }
         // lanes below layer L-1 carry the identity, so fdt is now the
@@ -602,10 +630,10 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
             }
             scan_affine(A, C, lane);
             const double below = A * F + C;
-            double above = __shfl_up(below, 1, kWave);
+            double above = lane_below(below);
             if (lane == 0) above = F;
             fdl[c] = above;
-            F = __shfl(below, kWave - 1, kWave);
+            F = lane63(below);
           }
         }
         // lanes below layer L-1 carry the identity, so F is the downward
@@ -623,7 +651,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
             if (i < L) { A = trr; C = sur; }
             scan_affine(A, C, lane);
             const double above = A * U + C;                  // F up at the top of layer i
-            double belowu = __shfl_up(above, 1, kWave);      // at its bottom
+            double belowu = lane_below(above);                // at its bottom
             if (lane == 0) belowu = U;
             // back to natural lane order: lane l holds layer cc*64 + l
             const double bl = __shfl(belowu, rl, kWave);
@@ -633,7 +661,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
               num[cc] = num[cc] + (kk[cc] * a + kk[cc] * bb);
               den[cc] = den[cc] + (a + bb);
             }
-            U = __shfl(above, kWave - 1, kWave);
+            U = lane63(above);
           }
         }
       }
@@ -648,8 +676,8 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
           kq = sflux > 0.0 ? num[c] / sflux : kmin[c];
           // flux_gas at the top of layer i: the summed transmission one layer
           // up (1 above the first layer)
-          double fgp = __shfl_up(den[c], 1, kWave);
-          const double carry = __shfl(den[c > 0 ? c - 1 : 0], kWave - 1, kWave);
+          double fgp = lane_below(den[c]);
+          const double carry = lane63(den[c > 0 ? c - 1 : 0]);
           if (lane == 0) fgp = c > 0 ? carry : 1.0;
           if (i == 0) fgp = 1.0;
           if (i < L && fgp > 0.0) adj[c] = adj[c] * den[c] / fgp;
