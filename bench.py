@@ -56,7 +56,7 @@ HBM_PEAK = 8000.0                          # GB/s
 # HBM bytes per step moved by the solve kernel (one launch covers both
 # calls), from rocprofv3 FETCH_SIZE (x2, the factor a plain 512 MiB copy
 # reads) and WRITE_SIZE (x1) runs of this script; profiles/c2_solve_traffic.md
-TRAFFIC = {"mixed": 2.138e9, "fp64": 4.317e9}
+TRAFFIC = {"mixed": 2.134e9, "fp64": 4.260e9}
 
 
 def make_inputs(torch, dev):
