@@ -19,6 +19,19 @@
  *                                    coefficients against closed-form limits.
  *   cloud_gen                      - pinned bit-exactly against oracle/_ref
  *                                    c_sub dumps (golden vectors).
+ *   look-up-table, power-quadratic
+ *   and Wenyi k-term scaling,
+ *   continua, further gases        - pinned by the reference's own known-good
+ *                                    output of its runes example on the GA9
+ *                                    files (examples/runes) and by GA9 golden
+ *                                    vectors from oracle/_ref (bit-exact).
+ *   triple column (maximum-random
+ *   and exponential-random) and
+ *   mixed column (all overlaps)    - pinned bit-exactly against oracle/_ref
+ *                                    (golden vectors); the mixed column is
+ *                                    restated as the triple column with no
+ *                                    convective cloud, which the reference
+ *                                    computes to the same bits.
  */
 #ifndef SOC_ORACLE_H
 #define SOC_ORACLE_H
