@@ -22,6 +22,8 @@ program runes_gpu_case
   integer :: i_inhom, i_mcica_sampling, n_subcol_gen, i_solver_kind, prec
   real(RealExt) :: co2_mix_ratio, cloud_vertical_decorr, cloud_horizontal_rsd
   real(RealExt) :: conv_vertical_decorr
+  ! well-mixed n2o, ch4, o2, so2, ocs, cfc11, cfc12, cfc113, hcfc22, hfc134a
+  real(RealExt) :: wm(10)
   real(RealExt), allocatable, dimension(:,:) :: p_layer, t_layer, mass, density, &
     layer_heat_capacity, h2o, o3, p_level, t_level, albedo_diff, albedo_dir, &
     cloud_frac, liq_mmr, liq_dim, conv_frac, liq_conv_mmr, liq_conv_dim
@@ -68,6 +70,8 @@ program runes_gpu_case
   read(iu) cloud_frac, liq_mmr, liq_dim
   read(iu) conv_frac, liq_conv_mmr, liq_conv_dim
   read(iu) rand_seed
+  wm = 0.0_RealExt
+  if (version >= 2) read(iu) wm
   close(iu)
   if (i_solver_kind /= 0) stop 'triple-column cases are not on this binding'
 
@@ -98,6 +102,10 @@ program runes_gpu_case
       i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
       p_level=p_level, t_level=t_level, mass=mass, density=density, &
       h2o=h2o, o3=o3, co2_mix_ratio=co2_mix_ratio, &
+      n2o_mix_ratio=wm(1), ch4_mix_ratio=wm(2), o2_mix_ratio=wm(3), &
+      so2_mix_ratio=wm(4), ocs_mix_ratio=wm(5), cfc11_mix_ratio=wm(6), &
+      cfc12_mix_ratio=wm(7), cfc113_mix_ratio=wm(8), hcfc22_mix_ratio=wm(9), &
+      hfc134a_mix_ratio=wm(10), &
       t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
       albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
       l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
@@ -107,6 +115,10 @@ program runes_gpu_case
       i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
       p_level=p_level, t_level=t_level, mass=mass, density=density, &
       h2o=h2o, o3=o3, co2_mix_ratio=co2_mix_ratio, &
+      n2o_mix_ratio=wm(1), ch4_mix_ratio=wm(2), o2_mix_ratio=wm(3), &
+      so2_mix_ratio=wm(4), ocs_mix_ratio=wm(5), cfc11_mix_ratio=wm(6), &
+      cfc12_mix_ratio=wm(7), cfc113_mix_ratio=wm(8), hcfc22_mix_ratio=wm(9), &
+      hfc134a_mix_ratio=wm(10), &
       t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
       albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
       l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &

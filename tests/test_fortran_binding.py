@@ -45,6 +45,25 @@ def test_fortran_binding_matches_oracle(src, cloudy):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("src", ["lw", "sw"])
+def test_fortran_binding_ga9_with_further_gases(src):
+    """The binding on the GA9 files: the further gases reach the library
+    through runes' *_mix_ratio keywords."""
+    from ga9_cases import MCICA_GA9, SPEC_GA9
+    case = synth.make_case(10, 60, src, NBAND[src], seed=950 + (src == "sw"), cloudy=True)
+    case.wm_mix_ratio = np.array([4.7e-7, 9.6e-7, 0.23, 2e-10, 8e-10, 1.1e-9, 2.4e-9, 3.3e-10, 9e-10, 4.5e-10])
+    ref = oracle.runes(case, SPEC_GA9[src], MCICA_GA9)
+    with tempfile.TemporaryDirectory() as tmp:
+        cf, of = os.path.join(tmp, "case.bin"), os.path.join(tmp, "out.bin")
+        case.write(cf)
+        r = subprocess.run([EXE, SPEC_GA9["sw"], SPEC_GA9["lw"], MCICA_GA9, cf, of, "fp64"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        out = read_output(of, case.n_profile, case.n_layer)
+        check_fp64(out, ref, "fortran ga9 %s" % src, src, case.layer_heat_capacity)
+
+
+@pytest.mark.gpu
 def test_grey_albedo_is_every_band():
     R.register_default_spectra()
     case = synth.make_case(6, 40, "sw", NBAND["sw"], seed=77, cloudy=False)
