@@ -24,6 +24,7 @@ program runes_gpu_case
   real(RealExt) :: conv_vertical_decorr
   ! well-mixed n2o, ch4, o2, so2, ocs, cfc11, cfc12, cfc113, hcfc22, hfc134a
   real(RealExt) :: wm(10)
+  logical :: inv
   real(RealExt), allocatable, dimension(:,:) :: p_layer, t_layer, mass, density, &
     layer_heat_capacity, h2o, o3, p_level, t_level, albedo_diff, albedo_dir, &
     cloud_frac, liq_mmr, liq_dim, conv_frac, liq_conv_mmr, liq_conv_dim
@@ -74,6 +75,22 @@ program runes_gpu_case
   if (version >= 2) read(iu) wm
   close(iu)
   if (i_solver_kind /= 0) stop 'triple-column cases are not on this binding'
+  ! optional 7th argument 'invert': hand runes bottom-up profiles
+  ! (l_invert = .true.) and flip its outputs back for the output file
+  inv = .false.
+  if (command_argument_count() >= 7) then
+    call get_command_argument(7, arg)
+    inv = trim(arg) == 'invert'
+  end if
+  if (inv) then
+    p_layer = p_layer(:, n_layer:1:-1); t_layer = t_layer(:, n_layer:1:-1)
+    mass = mass(:, n_layer:1:-1); density = density(:, n_layer:1:-1)
+    layer_heat_capacity = layer_heat_capacity(:, n_layer:1:-1)
+    h2o = h2o(:, n_layer:1:-1); o3 = o3(:, n_layer:1:-1)
+    p_level = p_level(:, n_layer:0:-1); t_level = t_level(:, n_layer:0:-1)
+    cloud_frac = cloud_frac(:, n_layer:1:-1); liq_mmr = liq_mmr(:, n_layer:1:-1)
+    liq_dim = liq_dim(:, n_layer:1:-1)
+  end if
 
   allocate(flux_up(n_profile, 0:n_layer), flux_down(n_profile, 0:n_layer), &
     flux_direct(n_profile, 0:n_layer), heating_rate(n_profile, n_layer))
@@ -109,7 +126,7 @@ program runes_gpu_case
       t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
       albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
       l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
-      precision=prec)
+      l_invert=inv, precision=prec)
   else
     call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &
       i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
@@ -128,9 +145,13 @@ program runes_gpu_case
       i_cloud_representation=i_cloud_repr, i_overlap=i_overlap, &
       i_inhom=i_inhom, i_mcica_sampling=i_mcica_sampling, &
       i_st_water=i_st_water, n_subcol_gen=n_subcol_gen, rand_seed=rand_seed, &
-      precision=prec)
+      l_invert=inv, precision=prec)
   end if
   write(*, '(a)') 'runes done'
+  if (inv) then
+    flux_up = flux_up(:, n_layer:0:-1); flux_down = flux_down(:, n_layer:0:-1)
+    flux_direct = flux_direct(:, n_layer:0:-1); heating_rate = heating_rate(:, n_layer:1:-1)
+  end if
 
   open(newunit=iu, file=trim(out_file), access='stream', form='unformatted', &
        status='replace')

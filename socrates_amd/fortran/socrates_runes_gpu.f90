@@ -279,14 +279,23 @@ contains
 
     type(socgpu_runes_args) :: a
     type(socgpu_diag) :: d
+    ! l_invert: bottom-up caller profiles, flipped into top-down copies and
+    ! the outputs flipped back (socrates_set_atm.F90:218-225,
+    ! socrates_set_diag.F90:113-151)
+    type :: flip_buf
+      real(RealExt), allocatable :: v(:, :)
+    end type flip_buf
+    type(flip_buf), target :: fin(20), fout(4)
+    integer :: nfin
+    logical :: inv
     integer(c_int) :: prec, nb, nabs, types(32), status
     integer(c_int), allocatable, target :: seeds(:)
     character(len=64) :: name
 
     call ensure_init()
-    if (present(l_invert)) then
-      if (l_invert) call fail('l_invert = .true. is not supported')
-    end if
+    inv = .false.
+    if (present(l_invert)) inv = l_invert
+    nfin = 0
     if (present(l_profile_last)) then
       if (l_profile_last) call fail('l_profile_last = .true. is not supported')
     end if
@@ -329,23 +338,23 @@ contains
       a%l_grey_albedo = 0
     end if
 
-    call field(p_layer, n_layer, a%p_layer, 'p_layer')
-    call field(t_layer, n_layer, a%t_layer, 't_layer')
-    call field(mass, n_layer, a%mass, 'mass')
-    call field(density, n_layer, a%density, 'density')
-    call field(layer_heat_capacity, n_layer, a%layer_heat_capacity, 'layer_heat_capacity')
-    call field(h2o, n_layer, a%h2o, 'h2o')
-    call field(o3, n_layer, a%o3, 'o3')
-    call field(p_level, n_layer + 1, a%p_level, 'p_level')
-    call field(t_level, n_layer + 1, a%t_level, 't_level')
-    call field(albedo_diff, nb, a%albedo_diff, 'albedo_diff')
-    call field(albedo_dir, nb, a%albedo_dir, 'albedo_dir')
-    call field(cloud_frac, n_layer, a%cloud_frac, 'cloud_frac')
-    call field(liq_mmr, n_layer, a%liq_mmr, 'liq_mmr')
-    call field(liq_dim, n_layer, a%liq_dim, 'liq_dim')
-    call field(conv_frac, n_layer, a%conv_frac, 'conv_frac')
-    call field(liq_conv_mmr, n_layer, a%liq_conv_mmr, 'liq_conv_mmr')
-    call field(liq_conv_dim, n_layer, a%liq_conv_dim, 'liq_conv_dim')
+    call field(p_layer, n_layer, a%p_layer, 'p_layer', .true.)
+    call field(t_layer, n_layer, a%t_layer, 't_layer', .true.)
+    call field(mass, n_layer, a%mass, 'mass', .true.)
+    call field(density, n_layer, a%density, 'density', .true.)
+    call field(layer_heat_capacity, n_layer, a%layer_heat_capacity, 'layer_heat_capacity', .true.)
+    call field(h2o, n_layer, a%h2o, 'h2o', .true.)
+    call field(o3, n_layer, a%o3, 'o3', .true.)
+    call field(p_level, n_layer + 1, a%p_level, 'p_level', .true.)
+    call field(t_level, n_layer + 1, a%t_level, 't_level', .true.)
+    call field(albedo_diff, nb, a%albedo_diff, 'albedo_diff', .false.)
+    call field(albedo_dir, nb, a%albedo_dir, 'albedo_dir', .false.)
+    call field(cloud_frac, n_layer, a%cloud_frac, 'cloud_frac', .true.)
+    call field(liq_mmr, n_layer, a%liq_mmr, 'liq_mmr', .true.)
+    call field(liq_dim, n_layer, a%liq_dim, 'liq_dim', .true.)
+    call field(conv_frac, n_layer, a%conv_frac, 'conv_frac', .true.)
+    call field(liq_conv_mmr, n_layer, a%liq_conv_mmr, 'liq_conv_mmr', .true.)
+    call field(liq_conv_dim, n_layer, a%liq_conv_dim, 'liq_conv_dim', .true.)
     call column(t_ground, a%t_ground, 't_ground')
     call column(cos_zenith_angle, a%cos_zenith_angle, 'cos_zenith_angle')
     call column(solar_irrad, a%solar_irrad, 'solar_irrad')
@@ -356,15 +365,21 @@ contains
       a%rand_seed = c_loc(seeds)
     end if
 
-    call output(diag%flux_up, n_layer + 1, d%flux_up, 'flux_up')
-    call output(diag%flux_down, n_layer + 1, d%flux_down, 'flux_down')
-    call output(diag%flux_direct, n_layer + 1, d%flux_direct, 'flux_direct')
-    call output(diag%heating_rate, n_layer, d%heating_rate, 'heating_rate')
+    call output(diag%flux_up, n_layer + 1, d%flux_up, 'flux_up', 1)
+    call output(diag%flux_down, n_layer + 1, d%flux_down, 'flux_down', 2)
+    call output(diag%flux_direct, n_layer + 1, d%flux_direct, 'flux_direct', 3)
+    call output(diag%heating_rate, n_layer, d%heating_rate, 'heating_rate', 4)
 
     prec = ip_precision_fp64
     if (present(precision)) prec = precision
     status = socgpu_runes(c_str(name), a, d, prec)
     call check(status, 'RUNES')
+    if (inv) then
+      if (associated(diag%flux_up)) diag%flux_up = fout(1)%v(:, n_layer + 1:1:-1)
+      if (associated(diag%flux_down)) diag%flux_down = fout(2)%v(:, n_layer + 1:1:-1)
+      if (associated(diag%flux_direct)) diag%flux_direct = fout(3)%v(:, n_layer + 1:1:-1)
+      if (associated(diag%heating_rate)) diag%heating_rate = fout(4)%v(:, n_layer:1:-1)
+    end if
 
   contains
 
@@ -374,16 +389,24 @@ contains
       error stop 1
     end subroutine fail
 
-    subroutine field(x, n2, p, label)
+    subroutine field(x, n2, p, label, vertical)
       real(RealExt), intent(in), optional, target, contiguous :: x(:, :)
       integer, intent(in) :: n2
       type(c_ptr), intent(out) :: p
       character(len=*), intent(in) :: label
+      logical, intent(in) :: vertical
       p = c_null_ptr
       if (.not. present(x)) return
       if (size(x, 1) /= n_profile .or. size(x, 2) /= n2) &
         call fail(label // ': shape must be (n_profile, levels or layers or bands)')
-      p = c_loc(x)
+      if (inv .and. vertical) then
+        nfin = nfin + 1
+        allocate(fin(nfin)%v(n_profile, n2))
+        fin(nfin)%v = x(:, n2:1:-1)
+        p = c_loc(fin(nfin)%v)
+      else
+        p = c_loc(x)
+      end if
     end subroutine field
 
     subroutine column(x, p, label)
@@ -396,9 +419,9 @@ contains
       p = c_loc(x)
     end subroutine column
 
-    subroutine output(x, n2, p, label)
+    subroutine output(x, n2, p, label, k)
       real(RealExt), pointer :: x(:, :)
-      integer, intent(in) :: n2
+      integer, intent(in) :: n2, k
       type(c_ptr), intent(out) :: p
       character(len=*), intent(in) :: label
       p = c_null_ptr
@@ -406,7 +429,12 @@ contains
       if (size(x, 1) /= n_profile .or. size(x, 2) /= n2) &
         call fail(label // ': shape must be (n_profile, 0:n_layer) or (n_profile, n_layer)')
       if (.not. is_contiguous(x)) call fail(label // ' must be contiguous')
-      p = c_loc(x)
+      if (inv) then
+        allocate(fout(k)%v(n_profile, n2))
+        p = c_loc(fout(k)%v)
+      else
+        p = c_loc(x)
+      end if
     end subroutine output
 
   end subroutine runes

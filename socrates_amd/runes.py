@@ -201,11 +201,41 @@ def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass, p
           **kw):
     """One radiation call (socgpu_runes). Returns dict(flux_up, flux_down,
     flux_direct, heating_rate) in the input memory space (numpy or torch)."""
+    inv = bool(kw.pop("l_invert", False))
+    if inv:
+        # bottom-up profiles (socrates_set_atm.F90:218-225): flipped into
+        # top-down copies, the outputs flipped back (socrates_set_diag.F90:113-151)
+        p_layer, t_layer, mass = _flip(p_layer), _flip(t_layer), _flip(mass)
+        for k in _VERTICAL:
+            if kw.get(k) is not None:
+                kw[k] = _flip(kw[k])
+        user_out = kw.pop("out", None)
     a, d, out, keep = _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
                                **kw)
     _check(lib().socgpu_runes(spectrum_name.encode(), ctypes.byref(a), ctypes.byref(d),
                               _PREC[precision]))
+    if inv:
+        flipped = {k: _flip(v) for k, v in out.items()}
+        if user_out:
+            for k, v in user_out.items():
+                if v is not None:
+                    v.copy_(flipped[k]) if _is_torch(v) else np.copyto(v, flipped[k])
+            return user_out
+        return flipped
     return out
+
+
+# vertical fields of runes(), [layer or level][profile]
+_VERTICAL = ("density", "layer_heat_capacity", "h2o", "o3", "p_level", "t_level", "cloud_frac",
+             "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr", "liq_conv_dim")
+
+
+def _flip(a):
+    """A copy of a with its first (vertical) axis reversed."""
+    if _is_torch(a):
+        import torch
+        return torch.flip(a, [0]).contiguous()
+    return np.ascontiguousarray(np.asarray(a)[::-1])
 
 
 def runes_case(case: Case, spectrum_name, precision="fp64", device_arrays=None, out=None):

@@ -64,6 +64,28 @@ def test_fortran_binding_ga9_with_further_gases(src):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("src", ["lw", "sw"])
+def test_fortran_binding_l_invert(src):
+    """Bottom-up profiles with l_invert = .true. (socrates_set_atm.F90:
+    218-225, socrates_set_diag.F90:113-151) give the top-down call's results,
+    flipped: the example program flips its inputs, calls runes with l_invert
+    and flips the outputs back, to the same bits as the plain call."""
+    case = synth.make_case(9, 40, src, NBAND[src], seed=970 + (src == "sw"), cloudy=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        cf = os.path.join(tmp, "case.bin")
+        case.write(cf)
+        outs = []
+        for mode in ("plain", "invert"):
+            of = os.path.join(tmp, "out_%s.bin" % mode)
+            r = subprocess.run([EXE, SPEC["sw"], SPEC["lw"], MCICA, cf, of, "fp64", mode],
+                               capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stdout + r.stderr
+            outs.append(read_output(of, case.n_profile, case.n_layer))
+        for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+            assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.gpu
 def test_grey_albedo_is_every_band():
     R.register_default_spectra()
     case = synth.make_case(6, 40, "sw", NBAND["sw"], seed=77, cloudy=False)
@@ -78,3 +100,20 @@ def test_grey_albedo_is_every_band():
                   cos_zenith_angle=case.cos_zen, solar_irrad=case.solar_irrad,
                   l_grey_albedo=True, grey_albedo=grey, l_rayleigh=case.l_rayleigh)
     check_fp64(out, ref, "grey albedo", "sw", case.layer_heat_capacity)
+
+
+@pytest.mark.gpu
+def test_python_l_invert_matches_top_down():
+    """runes(..., l_invert=True) on bottom-up arrays gives the top-down
+    call's outputs, flipped, to the bit."""
+    from socrates_amd import runes as R
+    R.register_default_spectra()
+    case = synth.make_case(6, 35, "lw", NBAND["lw"], seed=975, cloudy=True)
+    name, kw = R.case_call(case, "lw")
+    P, L, src = kw.pop("n_profile"), kw.pop("n_layer"), kw.pop("i_source")
+    top = R.runes(P, L, name, src, precision="fp64", **kw)
+    flip = {k: (np.ascontiguousarray(v[::-1]) if isinstance(v, np.ndarray) and v.ndim == 2 and
+                k not in ("albedo_diff", "albedo_dir") else v) for k, v in kw.items()}
+    bot = R.runes(P, L, name, src, precision="fp64", l_invert=True, **flip)
+    for k in ("flux_up", "flux_down", "heating_rate"):
+        assert np.array_equal(np.asarray(bot[k])[::-1], np.asarray(top[k])), k
