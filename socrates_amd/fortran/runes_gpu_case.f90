@@ -24,7 +24,7 @@ program runes_gpu_case
   real(RealExt) :: conv_vertical_decorr
   ! well-mixed n2o, ch4, o2, so2, ocs, cfc11, cfc12, cfc113, hcfc22, hfc134a
   real(RealExt) :: wm(10)
-  logical :: inv
+  logical :: inv, last
   real(RealExt), allocatable, dimension(:,:) :: p_layer, t_layer, mass, density, &
     layer_heat_capacity, h2o, o3, p_level, t_level, albedo_diff, albedo_dir, &
     cloud_frac, liq_mmr, liq_dim, conv_frac, liq_conv_mmr, liq_conv_dim
@@ -77,10 +77,24 @@ program runes_gpu_case
   if (i_solver_kind /= 0) stop 'triple-column cases are not on this binding'
   ! optional 7th argument 'invert': hand runes bottom-up profiles
   ! (l_invert = .true.) and flip its outputs back for the output file
+  ! or 'last': fields as (levels or layers or bands, n_profile) with
+  ! l_profile_last = .true., outputs transposed back
   inv = .false.
+  last = .false.
   if (command_argument_count() >= 7) then
     call get_command_argument(7, arg)
     inv = trim(arg) == 'invert'
+    last = trim(arg) == 'last'
+  end if
+  if (last) then
+    p_layer = transpose(p_layer); t_layer = transpose(t_layer)
+    mass = transpose(mass); density = transpose(density)
+    layer_heat_capacity = transpose(layer_heat_capacity)
+    h2o = transpose(h2o); o3 = transpose(o3)
+    p_level = transpose(p_level); t_level = transpose(t_level)
+    albedo_diff = transpose(albedo_diff); albedo_dir = transpose(albedo_dir)
+    cloud_frac = transpose(cloud_frac); liq_mmr = transpose(liq_mmr)
+    liq_dim = transpose(liq_dim)
   end if
   if (inv) then
     p_layer = p_layer(:, n_layer:1:-1); t_layer = t_layer(:, n_layer:1:-1)
@@ -92,8 +106,13 @@ program runes_gpu_case
     liq_dim = liq_dim(:, n_layer:1:-1)
   end if
 
-  allocate(flux_up(n_profile, 0:n_layer), flux_down(n_profile, 0:n_layer), &
-    flux_direct(n_profile, 0:n_layer), heating_rate(n_profile, n_layer))
+  if (last) then
+    allocate(flux_up(0:n_layer, n_profile), flux_down(0:n_layer, n_profile), &
+      flux_direct(0:n_layer, n_profile), heating_rate(n_layer, n_profile))
+  else
+    allocate(flux_up(n_profile, 0:n_layer), flux_down(n_profile, 0:n_layer), &
+      flux_direct(n_profile, 0:n_layer), heating_rate(n_profile, n_layer))
+  end if
   flux_direct = 0.0_RealExt
   diag%flux_up => flux_up
   diag%flux_down => flux_down
@@ -126,7 +145,7 @@ program runes_gpu_case
       t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
       albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
       l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
-      l_invert=inv, precision=prec)
+      l_invert=inv, l_profile_last=last, precision=prec)
   else
     call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &
       i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
@@ -145,9 +164,13 @@ program runes_gpu_case
       i_cloud_representation=i_cloud_repr, i_overlap=i_overlap, &
       i_inhom=i_inhom, i_mcica_sampling=i_mcica_sampling, &
       i_st_water=i_st_water, n_subcol_gen=n_subcol_gen, rand_seed=rand_seed, &
-      l_invert=inv, precision=prec)
+      l_invert=inv, l_profile_last=last, precision=prec)
   end if
   write(*, '(a)') 'runes done'
+  if (last) then
+    flux_up = transpose(flux_up); flux_down = transpose(flux_down)
+    flux_direct = transpose(flux_direct); heating_rate = transpose(heating_rate)
+  end if
   if (inv) then
     flux_up = flux_up(:, n_layer:0:-1); flux_down = flux_down(:, n_layer:0:-1)
     flux_direct = flux_direct(:, n_layer:0:-1); heating_rate = heating_rate(:, n_layer:1:-1)

@@ -287,7 +287,7 @@ contains
     end type flip_buf
     type(flip_buf), target :: fin(20), fout(4)
     integer :: nfin
-    logical :: inv
+    logical :: inv, last
     integer(c_int) :: prec, nb, nabs, types(32), status
     integer(c_int), allocatable, target :: seeds(:)
     character(len=64) :: name
@@ -296,9 +296,11 @@ contains
     inv = .false.
     if (present(l_invert)) inv = l_invert
     nfin = 0
-    if (present(l_profile_last)) then
-      if (l_profile_last) call fail('l_profile_last = .true. is not supported')
-    end if
+    ! l_profile_last: fields (levels or layers or bands, n_profile), handed
+    ! to the library transposed and the outputs transposed back
+    ! (socrates_set_atm.F90:449-470, socrates_set_bound.F90:156-172)
+    last = .false.
+    if (present(l_profile_last)) last = l_profile_last
     if (.not. present(spectrum_name)) call fail('spectrum_name is required')
     name = spectrum_name
     call check(socgpu_spectrum_query(c_str(name), nb, nabs, types, 32_c_int), 'RUNES')
@@ -374,12 +376,10 @@ contains
     if (present(precision)) prec = precision
     status = socgpu_runes(c_str(name), a, d, prec)
     call check(status, 'RUNES')
-    if (inv) then
-      if (associated(diag%flux_up)) diag%flux_up = fout(1)%v(:, n_layer + 1:1:-1)
-      if (associated(diag%flux_down)) diag%flux_down = fout(2)%v(:, n_layer + 1:1:-1)
-      if (associated(diag%flux_direct)) diag%flux_direct = fout(3)%v(:, n_layer + 1:1:-1)
-      if (associated(diag%heating_rate)) diag%heating_rate = fout(4)%v(:, n_layer:1:-1)
-    end if
+    if (associated(diag%flux_up)) call give_back(diag%flux_up, 1, n_layer + 1)
+    if (associated(diag%flux_down)) call give_back(diag%flux_down, 2, n_layer + 1)
+    if (associated(diag%flux_direct)) call give_back(diag%flux_direct, 3, n_layer + 1)
+    if (associated(diag%heating_rate)) call give_back(diag%heating_rate, 4, n_layer)
 
   contains
 
@@ -397,17 +397,40 @@ contains
       logical, intent(in) :: vertical
       p = c_null_ptr
       if (.not. present(x)) return
-      if (size(x, 1) /= n_profile .or. size(x, 2) /= n2) &
+      if (last) then
+        if (size(x, 1) /= n2 .or. size(x, 2) /= n_profile) &
+          call fail(label // ': shape must be (levels or layers or bands, n_profile)')
+      else if (size(x, 1) /= n_profile .or. size(x, 2) /= n2) then
         call fail(label // ': shape must be (n_profile, levels or layers or bands)')
-      if (inv .and. vertical) then
+      end if
+      if (last .or. (inv .and. vertical)) then
         nfin = nfin + 1
         allocate(fin(nfin)%v(n_profile, n2))
-        fin(nfin)%v = x(:, n2:1:-1)
+        if (last) then
+          fin(nfin)%v = transpose(x)
+        else
+          fin(nfin)%v = x
+        end if
+        if (inv .and. vertical) fin(nfin)%v = fin(nfin)%v(:, n2:1:-1)
         p = c_loc(fin(nfin)%v)
       else
         p = c_loc(x)
       end if
     end subroutine field
+
+    ! an output written to a library buffer, flipped and / or transposed
+    ! into the caller's array
+    subroutine give_back(x, k, n2)
+      real(RealExt), pointer :: x(:, :)
+      integer, intent(in) :: k, n2
+      if (.not. (inv .or. last)) return
+      if (inv) fout(k)%v = fout(k)%v(:, n2:1:-1)
+      if (last) then
+        x = transpose(fout(k)%v)
+      else
+        x = fout(k)%v
+      end if
+    end subroutine give_back
 
     subroutine column(x, p, label)
       real(RealExt), intent(in), optional, target, contiguous :: x(:)
@@ -426,10 +449,14 @@ contains
       character(len=*), intent(in) :: label
       p = c_null_ptr
       if (.not. associated(x)) return
-      if (size(x, 1) /= n_profile .or. size(x, 2) /= n2) &
+      if (last) then
+        if (size(x, 1) /= n2 .or. size(x, 2) /= n_profile) &
+          call fail(label // ': shape must be (0:n_layer, n_profile) or (n_layer, n_profile)')
+      else if (size(x, 1) /= n_profile .or. size(x, 2) /= n2) then
         call fail(label // ': shape must be (n_profile, 0:n_layer) or (n_profile, n_layer)')
+      end if
       if (.not. is_contiguous(x)) call fail(label // ' must be contiguous')
-      if (inv) then
+      if (inv .or. last) then
         allocate(fout(k)%v(n_profile, n2))
         p = c_loc(fout(k)%v)
       else

@@ -67,15 +67,16 @@ def test_fortran_binding_ga9_with_further_gases(src):
 @pytest.mark.parametrize("src", ["lw", "sw"])
 def test_fortran_binding_l_invert(src):
     """Bottom-up profiles with l_invert = .true. (socrates_set_atm.F90:
-    218-225, socrates_set_diag.F90:113-151) give the top-down call's results,
-    flipped: the example program flips its inputs, calls runes with l_invert
-    and flips the outputs back, to the same bits as the plain call."""
+    218-225, socrates_set_diag.F90:113-151) and profile-last fields with
+    l_profile_last = .true. give the plain call's results, flipped or
+    transposed: the example program rearranges its inputs, calls runes with
+    the flag and rearranges the outputs back, to the same bits."""
     case = synth.make_case(9, 40, src, NBAND[src], seed=970 + (src == "sw"), cloudy=True)
     with tempfile.TemporaryDirectory() as tmp:
         cf = os.path.join(tmp, "case.bin")
         case.write(cf)
         outs = []
-        for mode in ("plain", "invert"):
+        for mode in ("plain", "invert", "last"):
             of = os.path.join(tmp, "out_%s.bin" % mode)
             r = subprocess.run([EXE, SPEC["sw"], SPEC["lw"], MCICA, cf, of, "fp64", mode],
                                capture_output=True, text=True, timeout=120)
@@ -83,6 +84,7 @@ def test_fortran_binding_l_invert(src):
             outs.append(read_output(of, case.n_profile, case.n_layer))
         for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
             assert np.array_equal(outs[0][k], outs[1][k]), k
+            assert np.array_equal(outs[0][k], outs[2][k]), k
 
 
 @pytest.mark.gpu
