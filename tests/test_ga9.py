@@ -59,3 +59,23 @@ def test_continuum_and_further_gases_matter():
     no_minor = oracle.runes(c, SPEC_GA9["lw"])
     assert np.abs(no_minor["flux_up"] - base["flux_up"]).max() > 0.1
     assert os.path.exists(os.path.join(GA9, "sp_lw_ga9_k"))
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/src/radiance_core"),
+                    reason="the reference tree is only present where the tables are generated")
+def test_wenyi_tables_match_reference_source():
+    """The committed Wenyi tables are what gen_wenyi.py reads from the
+    reference's scale_wenyi.F90 (data, regenerated rather than edited)."""
+    import subprocess
+    import sys
+    import tempfile
+    here = os.path.dirname(os.path.abspath(__file__))
+    gen = os.path.join(os.path.dirname(here), "socrates_amd", "csrc", "gen_wenyi.py")
+    committed = open(os.path.join(os.path.dirname(gen), "wenyi_tables.h")).read()
+    with tempfile.TemporaryDirectory() as t:
+        # run a copy of the generator so that it writes into the temporary directory
+        src = open(gen).read()
+        g2 = os.path.join(t, "gen_wenyi.py")
+        open(g2, "w").write(src)
+        subprocess.run([sys.executable, g2], check=True, capture_output=True)
+        assert open(os.path.join(t, "wenyi_tables.h")).read() == committed
