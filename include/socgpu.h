@@ -254,6 +254,15 @@ int socgpu_serve_stats(SocgpuServer *server, SocgpuServeStats *stats);
 int socgpu_serve_reset_stats(SocgpuServer *server);
 /* Drains the queue, stops the server thread and frees its buffers. */
 int socgpu_serve_stop(SocgpuServer *server);
+/* Load driver for measuring the serving loop: submits pool[pick[i]] at
+ * arrival_s[i] seconds after the call starts (a schedule the caller draws,
+ * for example Poisson arrivals), from the calling thread on a steady clock,
+ * then waits for every call (timeout_s each); *wall_s gets the time from the
+ * start to the last completion. A call picked several times writes the same
+ * outputs each time. No reference counterpart. */
+int socgpu_serve_replay(SocgpuServer *server, int n_pool, const SocgpuRequest *pool,
+                        int n_arrival, const double *arrival_s, const int *pick,
+                        double timeout_s, double *wall_s);
 
 /* Launch shape of the column-solve kernel, per precision form (no reference
  * counterpart; socgpu_autotune picks it):

@@ -1217,9 +1217,13 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     const int its = c->i_st_water, itc = c->i_st_water;  /* the case carries one droplet type for both */
     for (int i = 0; i < L; i++)
       for (int l = 0; l < P; l++) {
-        /* type fractions and condensate (socrates_set_cld.F90:567-581, 284-330) */
-        double fs = fmax(LAY(c->cloud_frac, i, l), 0.0), fcv = fmax(LAY(c->conv_frac, i, l), 0.0);
-        double ms = fmax(LAY(c->liq_mmr, i, l), 0.0), mcv = fmax(LAY(c->liq_conv_mmr, i, l), 0.0);
+        /* type fractions and condensate (socrates_set_cld.F90:567-581, 284-330);
+         * the mixed column (ip_cloud_homogen) reads no convective field
+         * (socrates_set_cld.F90:477-495, l_combine false) */
+        double fs = fmax(LAY(c->cloud_frac, i, l), 0.0);
+        double fcv = conv_strat ? fmax(LAY(c->conv_frac, i, l), 0.0) : 0.0;
+        double ms = fmax(LAY(c->liq_mmr, i, l), 0.0);
+        double mcv = conv_strat ? fmax(LAY(c->liq_conv_mmr, i, l), 0.0) : 0.0;
         /* in-cloud mixing ratios (:817-826) */
         LAY(tms, i, l) = fs > eps ? ms / fs : 0.0;
         LAY(tmc, i, l) = fcv > eps ? mcv / fcv : 0.0;
@@ -1232,7 +1236,8 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         LAY(tw, i, l) = wc; LAY(tfs, i, l) = fs; LAY(tfc, i, l) = fcv;
         /* radius clamps (socrates_set_cld_dim.F90:214-220) */
         LAY(trs, i, l) = fmin(fmax(fmax(LAY(c->liq_dim, i, l), 0.0), sp.drop_min[its]), sp.drop_max[its]);
-        LAY(trc, i, l) = fmin(fmax(fmax(LAY(c->liq_conv_dim, i, l), 0.0), sp.drop_min[itc]), sp.drop_max[itc]);
+        LAY(trc, i, l) = fmin(fmax(conv_strat ? fmax(LAY(c->liq_conv_dim, i, l), 0.0) : 0.0,
+                                   sp.drop_min[itc]), sp.drop_max[itc]);
       }
     /* cloud top over the whole call (set_cloud_geometry.F90:52-72) */
     nct = 1;
@@ -1618,10 +1623,12 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
               wc.trans_0[i] = w.trans_0[i]; wc.sc1[i] = w.sc1[i]; wc.sc2[i] = w.sc2[i];
               double cs = c_sub[((size_t)(isub - 1) * L + i) * P + l];
               double frac1 = LAY(w_cloud, i, l) > 0.0 ? 1.0 : 0.0;
-              /* frac_cloud(l,i,1) = frac/w_cloud = 1 where cloudy */
+              /* frac_cloud(l,i,1) = frac/w_cloud = 1 where cloudy, normalised
+               * by the fraction before the clamp of w_cloud to 1
+               * (socrates_set_cld.F90:828-853) */
               if (LAY(w_cloud, i, l) > 0.0) {
                 double fr = fmax(LAY(c->cloud_frac, i, l), 0.0);
-                frac1 = fr / LAY(w_cloud, i, l);
+                frac1 = fr / fr;
               }
               if (!(cs * frac1 > 0.0)) continue;
               /* cloudy single-scattering (mcica_sample.F90:614-700) */

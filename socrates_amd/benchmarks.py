@@ -52,7 +52,7 @@ def _device_case(torch, case, dev):
     return arrs, out
 
 
-def ragged(steps=10, warmup=2, n_columns=10000, seed=20261206):
+def ragged(steps=10, warmup=2, n_columns=10000, seed=20261206, quiet=False):
     import torch
     dev = torch.device("cuda:0")
     R.init(0)
@@ -96,7 +96,8 @@ def ragged(steps=10, warmup=2, n_columns=10000, seed=20261206):
             "split": "50% LW (9 bands, 50 g) / 50% SW (6 bands, 40 g), all-sky McICA",
             "fp64": res["fp64"], "mixed": res["mixed"],
             "mixed_over_fp64": res["mixed"]["solves_per_s_l90"] / res["fp64"]["solves_per_s_l90"]}
-    print(json.dumps(line), flush=True)
+    if not quiet:
+        print(json.dumps(line), flush=True)
     return line
 
 

@@ -14,7 +14,8 @@ import dataclasses
 import numpy as np
 
 MAGIC = 1397703491
-VERSION = 2     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds
+VERSION = 3     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
+                # 3 adds l_flux_ground and flux_ground (n_band, n_profile) after them
 
 # further gases runes takes as well-mixed mixing ratios (the *_mix_ratio
 # arguments, socrates_runes.F90:69-83), in this order
@@ -75,6 +76,8 @@ class Case:
     liq_conv_dim: np.ndarray = None
     rand_seed: np.ndarray = None
     wm_mix_ratio: np.ndarray = None   # by WM_GASES, 0 = absent
+    l_flux_ground: int = 0            # surface emission per band given (flux_ground)
+    flux_ground: np.ndarray = None    # (n_band, n_profile) W m-2
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile
@@ -86,6 +89,10 @@ class Case:
             self.rand_seed = np.arange(P, dtype=np.int32)
         if self.wm_mix_ratio is None:
             self.wm_mix_ratio = np.zeros(len(WM_GASES))
+        if self.flux_ground is None:
+            self.flux_ground = np.zeros((self.n_band, P))
+        if self.t_ground is None:
+            self.t_ground = np.zeros(P)
         return self
 
     def subset(self, cols):
@@ -108,6 +115,8 @@ class Case:
                 np.ascontiguousarray(getattr(self, k), dtype=np.float64).tofile(f)
             np.ascontiguousarray(self.rand_seed, dtype=np.int32).tofile(f)
             np.ascontiguousarray(self.wm_mix_ratio, dtype=np.float64).tofile(f)
+            np.array([int(self.l_flux_ground)], dtype=np.int32).tofile(f)
+            np.ascontiguousarray(self.flux_ground, dtype=np.float64).tofile(f)
 
     @classmethod
     def read(cls, path):
@@ -135,12 +144,19 @@ class Case:
         off += 4 * P
         if hdr[1] >= 2:
             c.wm_mix_ratio = np.frombuffer(raw, dtype=np.float64, count=len(WM_GASES), offset=off).copy()
+            off += 8 * len(WM_GASES)
+        if hdr[1] >= 3:
+            c.l_flux_ground = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
+            off += 4
+            c.flux_ground = np.frombuffer(raw, dtype=np.float64, count=B * P, offset=off).reshape(B, P).copy()
         return c
 
 
-def read_output(path, n_profile, n_layer):
+def read_output(path, n_profile, n_layer, clear=False):
     """Outputs of the reference harness: flux_up, flux_down, flux_direct
-    (n_layer+1, n_profile) and heating_rate (n_layer, n_profile) [K/s]."""
+    (n_layer+1, n_profile) and heating_rate (n_layer, n_profile) [K/s]; with
+    clear, also the clear-sky flux_up_clear, flux_down_clear and
+    flux_direct_clear (n_layer+1, n_profile) the harness writes after them."""
     a = np.fromfile(path, dtype=np.float64)
     P, L = n_profile, n_layer
     n1 = P * (L + 1)
@@ -148,7 +164,13 @@ def read_output(path, n_profile, n_layer):
     down = a[n1:2 * n1].reshape(L + 1, P)
     direct = a[2 * n1:3 * n1].reshape(L + 1, P)
     hr = a[3 * n1:3 * n1 + P * L].reshape(L, P)
-    return dict(flux_up=up, flux_down=down, flux_direct=direct, heating_rate=hr)
+    out = dict(flux_up=up, flux_down=down, flux_direct=direct, heating_rate=hr)
+    if clear:
+        o = 3 * n1 + P * L
+        for k in ("flux_up_clear", "flux_down_clear", "flux_direct_clear"):
+            out[k] = a[o:o + n1].reshape(L + 1, P)
+            o += n1
+    return out
 
 
 def case_to_arrays(case):

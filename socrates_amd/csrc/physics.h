@@ -86,7 +86,11 @@ SG_HD double planck_band(const SgBand &b, int n_deg, double t_ref, double temp) 
 }
 
 // k_abs of one term at one layer: k * max(0, (p/p_ref)^a (T/T_ref)^b * mmr)
-// (radiance_calc.F90:1457-1491 with scale_absorb.F90:137-152)
+// (radiance_calc.F90:1457-1491 with scale_absorb.F90:137-152), the two
+// powers as one exponential of the layer's logarithms. (The reference's two
+// pow calls, rtor_v in vectlib_mod.F90:80-103, were measured on the device
+// too: the FP64 fluxes moved by under 1e-11 relative, and the solve took 26 %
+// longer.) lnp, lnt: logarithms of the layer pressure and temperature.
 SG_HD double k_term(const SgTerm &t, double lnp, double lnt, double mmr) {
   double s = exp(t.a * (lnp - t.lnpref) + t.b * (lnt - t.lntref));
   return t.k * fmax(0.0, s * mmr);

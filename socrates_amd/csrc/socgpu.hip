@@ -172,6 +172,9 @@ struct CallDims {
   int gx;            // per-layer gas record (look-up-table terms or continua)
   int gen;           // general gas optics (scalings other than the power law,
                      // continua, further gases)
+  int conv;          // convective cloud fields are read (ip_cloud_conv_strat
+                     // only: the reference ignores them for ip_cloud_homogen,
+                     // socrates_set_cld.F90:477-495)
 };
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
@@ -181,9 +184,13 @@ struct CallDims {
 // set_cld_mcica (socrates_set_cld_mcica.F90:168-180).
 // Record of layer i of column col (caller index idx = i P + col) into r;
 // dp_cf, dp_cw: the call's decorrelation pressure scales (McICA).
-__device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__restrict__ spec,
+// Returns true where the layer's total cloud fraction exceeds 1 by more than
+// the 0.001 threshold, which the reference treats as a fatal error
+// (socrates_set_cld.F90:848-853); fractions up to that are clamped to 1.
+__device__ __forceinline__ bool pack_layer(const CallDims &d, const SgSpec *__restrict__ spec,
                                            const SocgpuRunesArgs &a, int idx, double dp_cf, double dp_cw,
                                            double *r) {
+  bool bad = false;
   {
     double p = a.p_layer[idx], t = a.t_layer[idx];
     r[SG_Q_LNP] = log(p);
@@ -204,18 +211,19 @@ __device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__re
       // 0.001 threshold (:817-852)
       const double eps = kEps;
       double fs = fmax(a.cloud_frac[idx], 0.0);
-      double fc = a.conv_frac ? fmax(a.conv_frac[idx], 0.0) : 0.0;
+      double fc = (d.conv && a.conv_frac) ? fmax(a.conv_frac[idx], 0.0) : 0.0;
       double ms = a.liq_mmr ? fmax(a.liq_mmr[idx], 0.0) : 0.0;
-      double mc = a.liq_conv_mmr ? fmax(a.liq_conv_mmr[idx], 0.0) : 0.0;
+      double mc = (d.conv && a.liq_conv_mmr) ? fmax(a.liq_conv_mmr[idx], 0.0) : 0.0;
       lwc = fs > eps ? ms / fs : 0.0;
       lwc2 = fc > eps ? mc / fc : 0.0;
       w = fs + fc;
       if (w > 0.001) { f1 = fs / w; f2 = fc / w; }
       else { w = 0.0; f1 = 0.0; f2 = 0.0; }
+      bad = w > 1.0 + 0.001;
       if (w > 1.0) w = 1.0;
       const SgDrop &ds = spec->drop[d.drop], &dc = spec->drop[d.drop2];
       re = fmin(fmax(a.liq_dim ? fmax(a.liq_dim[idx], 0.0) : 0.0, ds.dim_min), ds.dim_max);
-      re2 = fmin(fmax(a.liq_conv_dim ? fmax(a.liq_conv_dim[idx], 0.0) : 0.0, dc.dim_min), dc.dim_max);
+      re2 = fmin(fmax((d.conv && a.liq_conv_dim) ? fmax(a.liq_conv_dim[idx], 0.0) : 0.0, dc.dim_min), dc.dim_max);
     } else if (a.cloud_frac) {
       const double eps = kEps;
       double frac = fmax(a.cloud_frac[idx], 0.0);
@@ -225,6 +233,7 @@ __device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__re
       f1 = frac;
       if (w > 0.001) f1 = f1 / w;
       else { w = 0.0; f1 = 0.0; }
+      bad = w > 1.0 + 0.001;
       if (w > 1.0) w = 1.0;
       const SgDrop &dr = spec->drop[d.drop];
       double r0 = a.liq_dim ? fmax(a.liq_dim[idx], 0.0) : 0.0;
@@ -261,6 +270,7 @@ __device__ __forceinline__ void pack_layer(const CallDims &d, const SgSpec *__re
       r[SG_Q_RCORR] = rcorr;
     }
   }
+  return bad;
 }
 
 // Per-column scalars and per-(column, band) albedos.
@@ -958,6 +968,32 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   if (cloudy) { o.T *= frac1; o.R *= frac1; o.T0 *= frac1; o.s1 *= frac1; o.s2 *= frac1; }
 }
 
+// The elimination update of one layer: with the layer's transmission and
+// reflection (o.T, o.R) and its upward and downward sources (su, sd), the
+// reflectance of the atmosphere above (rho) and the diffuse flux it sends
+// down (dn) move to the layer's bottom; returns the back-substitution pair
+// fu(top) = a fu(bottom) + bb. The mixed form takes 1 / (1 - R rho) from the
+// hardware reciprocal estimate and two Newton steps (a few ulp of FP64).
+template <bool MIXED>
+__device__ __forceinline__ void elim_step(const Coef &o, double su, double sd, double &rho, double &dn,
+                                          double &a, double &bb) {
+#pragma clang fp contract(on)
+  double bp;
+  if (MIXED) {
+    const double x = 1.0 - o.R * rho;
+    bp = __builtin_amdgcn_rcp(x);
+    bp = fma(bp, fma(-x, bp, 1.0), bp);
+    bp = fma(bp, fma(-x, bp, 1.0), bp);
+  } else {
+    bp = 1.0 / (1.0 - o.R * rho);
+  }
+  a = o.T * bp;
+  bb = (o.R * dn + su) * bp;
+  double rho_n = o.R + o.T * rho * a;
+  dn = o.T * (rho * bb + dn) + sd;
+  rho = rho_n;
+}
+
 // One layer of the top-down elimination for one lane: the layer's
 // coefficients (FP64 layer_coef or the mixed layer_coef_mixed), its sources
 // (solar_source.F90:104-162 with the k_eqv direct-beam adjustment, or
@@ -982,21 +1018,7 @@ __device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, co
     su = o.s1 * x.q2 + o.s2 * x.q1;
     sd = -o.s1 * x.q2 + o.s2 * x.q1;
   }
-  double bp;
-  if (MIXED) {
-    // reciprocal from the hardware estimate and two Newton steps (a few ulp)
-    const double x = 1.0 - o.R * rho;
-    bp = __builtin_amdgcn_rcp(x);
-    bp = fma(bp, fma(-x, bp, 1.0), bp);
-    bp = fma(bp, fma(-x, bp, 1.0), bp);
-  } else {
-    bp = 1.0 / (1.0 - o.R * rho);
-  }
-  a = o.T * bp;
-  bb = (o.R * dn + su) * bp;
-  double rho_n = o.R + o.T * rho * a;
-  dn = o.T * (rho * bb + dn) + sd;
-  rho = rho_n;
+  elim_step<MIXED>(o, su, sd, rho, dn, a, bb);
 }
 
 // The column solve, one lane per item: a clear-sky k-term (weight
@@ -1739,8 +1761,11 @@ struct ReqDesc {
   long long s[kNWork];          // first work item of this request in each space
 };
 
+// err: 0, or 1 + the (descriptor-order) index of a request whose input the
+// kernels found invalid (a cloud fraction above 1.001); the host writes 0
+// with every descriptor upload and reads it back after the launch.
 struct BatchHdr {
-  int R, pad;
+  int R, err;
   long long tot[kNWork];
 };
 
@@ -1785,7 +1810,8 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
     const int c0 = (t % ntc) * kPackC, i0 = (t / ntc) * kPackL;
     const int col = c0 + tx, i = i0 + ty;
     if (col < P && i < L) {
-      pack_layer(q.d, q.spec, q.a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ);
+      if (pack_layer(q.d, q.spec, q.a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ))
+        atomicMax(&((BatchHdr *)h)->err, r + 1);
       if (q.d.gx) {
         // gas record: look-up factors and the density (continua)
         double *g = q.grec + ((size_t)col * L + i) * SG_NGQ;
@@ -1825,7 +1851,7 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
 constexpr int kRangeChunk = 8;
 __device__ __forceinline__ bool layer_cloudy(const CallDims &d, const SocgpuRunesArgs &a, size_t idx) {
   double w = fmax(a.cloud_frac[idx], 0.0);
-  if (d.triple) w = w + (a.conv_frac ? fmax(a.conv_frac[idx], 0.0) : 0.0);
+  if (d.conv) w = w + (a.conv_frac ? fmax(a.conv_frac[idx], 0.0) : 0.0);
   return w > 0.001;
 }
 
@@ -2013,7 +2039,9 @@ __global__ void b_heating_kernel(const BatchHdr *h, const ReqDesc *rq) {
 
 // Kernel-level two-stream solve on prescribed optical properties, one lane
 // per (term, profile) (profile fastest, so every per-layer read and
-// per-level write is coalesced): two_coeff, the infra-red (ir_source.F90:
+// per-level write is coalesced), with the packed solve's per-layer
+// functions (two_coeff or two_coeff_f32 with coef_from_f32, elim_step):
+// two_coeff, the infra-red (ir_source.F90:
 // 93-120) or solar (solar_source.F90:104-162) sources, the surface source
 // (column_solver.F90:160-185) and the column solve (solver_homogen_direct.F90
 // :93-127 as a top-down elimination and bottom-up back-substitution).
@@ -2073,13 +2101,10 @@ __global__ __launch_bounds__(256) void two_stream_kernel(TwoStreamArgs a) {
       su = o.s1 * a.dpl[kl];
       sd = -su;
     }
-    double bp = 1.0 / (1.0 - o.R * rho);
-    double aa = o.T * bp;
-    double bb = (o.R * dn + su) * bp;
-    st[i][0] = aa; st[i][1] = bb; st[i][2] = rho; st[i][3] = dn;
-    double rho_n = o.R + o.T * rho * aa;
-    dn = o.T * (rho * bb + dn) + sd;
-    rho = rho_n;
+    double aa, bb;
+    const double rho0 = rho, dn0 = dn;
+    elim_step<MIXED>(o, su, sd, rho, dn, aa, bb);
+    st[i][0] = aa; st[i][1] = bb; st[i][2] = rho0; st[i][3] = dn0;
   }
   const double ad = a.alb_diff[l];
   const double sg = solar ? (a.alb_dir[l] - ad) * fd : (1.0 - ad) * a.dsurf[l];
@@ -2404,6 +2429,7 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     if (int rc = check_drop(a->i_st_water, "Liquid")) return rc;
     if (int rc = check_drop(a->i_cnv_water, "Convective liquid")) return rc;
     d.triple = 1;
+    d.conv = 1;
     d.drop = a->i_st_water - 1;
     d.drop2 = a->i_cnv_water - 1;
     d.ioverlap = a->i_overlap;
@@ -2634,6 +2660,8 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   const ReqDesc *dq = (const ReqDesc *)((char *)g_ctx.desc.p + offsetof(decltype(host), q));
   if (int rc = launch_batch(dh, (ReqDesc *)dq, host.h.tot, 1, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
+  int herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, &dh->err, sizeof herr, hipMemcpyDeviceToHost, s));
   const size_t nout = (size_t)P * (L + 1) * 3 + (size_t)P * L;
   const size_t nl = (size_t)P * (L + 1);
   char *oh = diag->l_outputs_on_device ? nullptr : g_ctx.out_host.use(nout * 8);
@@ -2647,6 +2675,7 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
     if (diag->heating_rate) HIPCHK(hipMemcpyAsync(diag->heating_rate, ohr, (size_t)P * L * 8, hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
+  if (herr) return fail(SOCGPU_EINVAL, "Cloud fraction greater than 1.");
   if (oh) {
     const double *h = (const double *)oh;
     if (diag->flux_up) std::memcpy(diag->flux_up, h, nl * 8);
@@ -2675,8 +2704,17 @@ struct Plan {
   size_t ws_doubles = 0, in_bytes = 0, out_doubles = 0, out_used = 0, in_used = 0;
   struct Back { double *dst; const double *src; size_t n; };
   std::vector<Back> back;
+  std::vector<int> order;  // caller index of each descriptor
   long long need[kNWork];
 };
+
+// The failure a batch's error word (BatchHdr::err) reports, if any.
+static int batch_err(const Plan &pl, int err) {
+  if (err <= 0) return 0;
+  const int j = err - 1;
+  const int i = j < (int)pl.order.size() ? pl.order[j] : j;
+  return fail(SOCGPU_EINVAL, "request %d: Cloud fraction greater than 1.", i);
+}
 
 struct Resolved {
   SpecEntry *e;
@@ -2711,6 +2749,7 @@ static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolve
   std::vector<int> order(n);
   for (int i = 0; i < n; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return res[x].d.L > res[y].d.L; });
+  pl.order = order;
   pl.rq.assign(n, ReqDesc());
   pl.back.clear();
   std::memset(&pl.h, 0, sizeof pl.h);
@@ -2829,8 +2868,11 @@ int socgpu_runes_batch(int n_request, const SocgpuRequest *requests, int precisi
   ReqDesc *dq = (ReqDesc *)((char *)g_ctx.desc.p + sizeof(BatchHdr));
   if (int rc = launch_batch(dh, dq, pl.h.tot, n_request, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
+  int herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, &dh->err, sizeof herr, hipMemcpyDeviceToHost, s));
   if (int rc = copy_back_issue(pl, g_ctx.out_stage.as<double>(), oh, s)) return rc;
   HIPCHK(hipStreamSynchronize(s));
+  if (int rc = batch_err(pl, herr)) return rc;
   copy_back_finish(pl, g_ctx.out_stage.as<double>(), oh);
   float t_all = 0.f, t_solve = 0.f;
   HIPCHK(hipEventElapsedTime(&t_all, g_ctx.ev0, g_ctx.ev3));
@@ -2918,8 +2960,11 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
   HIPCHK(hipEventRecord(b->ev0, s));
   HIPCHK(hipGraphLaunch(b->exec, s));
   HIPCHK(hipEventRecord(b->ev1, s));
+  // the error word back into the pinned descriptor block (outside the graph)
+  HIPCHK(hipMemcpyAsync(&((BatchHdr *)b->pinned)->err, &dh->err, sizeof(int), hipMemcpyDeviceToHost, s));
   if (int rc = copy_back_issue(pl, b->out_stage.as<double>(), oh, s)) return rc;
   HIPCHK(hipStreamSynchronize(s));
+  if (int rc = batch_err(pl, ((const BatchHdr *)b->pinned)->err)) return rc;
   copy_back_finish(pl, b->out_stage.as<double>(), oh);
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -3016,11 +3061,19 @@ static size_t footprint_bytes(const SocgpuRequest &r, const CallDims &d, const S
 }
 
 static void serve_finish_slot(SocgpuServer *sv, ServeSlot &sl, int status) {
+  // an invalid input fails the call it belongs to; the batch's other calls
+  // complete
+  int bad = -1;
+  if (status == 0 && sl.pinned) {
+    const int err = ((const BatchHdr *)sl.pinned)->err;
+    if (err > 0 && err - 1 < (int)sl.plan.order.size()) bad = sl.plan.order[err - 1];
+  }
   if (status == 0) copy_back_finish(sl.plan, sl.out_stage.as<double>(), sl.out_host_used);
   const auto now = std::chrono::steady_clock::now();
   std::lock_guard<std::mutex> lk(sv->mu);
-  for (auto &it : sl.items) {
-    it->status = status;
+  for (size_t k = 0; k < sl.items.size(); k++) {
+    auto &it = sl.items[k];
+    it->status = (int)k == bad ? SOCGPU_EINVAL : status;
     sv->used -= it->footprint;
     if (status == 0) {
       sv->lat_ms.push_back(std::chrono::duration<double, std::milli>(now - it->t_submit).count());
@@ -3064,6 +3117,7 @@ static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
   const BatchHdr *dh = (const BatchHdr *)sl.desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)sl.desc.p + sizeof(BatchHdr));
   if (int rc = launch_batch(dh, dq, sl.plan.h.tot, n, sv->cfg.precision, s, nullptr, nullptr)) return rc;
+  HIPCHK(hipMemcpyAsync(&((BatchHdr *)sl.pinned)->err, &dh->err, sizeof(int), hipMemcpyDeviceToHost, s));
   if (int rc = copy_back_issue(sl.plan, sl.out_stage.as<double>(), sl.out_host_used, s)) return rc;
   HIPCHK(hipEventRecord(sl.done, s));
   long long sol = 0;
@@ -3243,6 +3297,34 @@ int socgpu_serve_stop(SocgpuServer *sv) {
   }
   delete sv;
   return 0;
+}
+
+int socgpu_serve_replay(SocgpuServer *sv, int n_pool, const SocgpuRequest *pool, int n_arrival,
+                        const double *arrival_s, const int *pick, double timeout_s, double *wall_s) {
+  if (!sv || n_pool <= 0 || !pool || n_arrival < 0 || (n_arrival > 0 && (!arrival_s || !pick)))
+    return fail(SOCGPU_EINVAL, "null or empty argument");
+  for (int i = 0; i < n_arrival; i++)
+    if (pick[i] < 0 || pick[i] >= n_pool) return fail(SOCGPU_EINVAL, "pick %d out of range", i);
+  std::vector<long long> tickets((size_t)n_arrival);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < n_arrival; i++) {
+    // sleep to within 200 us of the arrival, then spin
+    const auto due = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                              std::chrono::duration<double>(arrival_s[i]));
+    for (;;) {
+      const auto now = std::chrono::steady_clock::now();
+      if (now >= due) break;
+      if (due - now > std::chrono::microseconds(200)) std::this_thread::sleep_for(due - now - std::chrono::microseconds(150));
+    }
+    if (int rc = socgpu_serve_submit(sv, &pool[pick[i]], &tickets[i])) return rc;
+  }
+  int first_rc = 0;
+  for (int i = 0; i < n_arrival; i++) {
+    const int rc = socgpu_serve_wait(sv, tickets[i], timeout_s);
+    if (rc && !first_rc) first_rc = rc;
+  }
+  if (wall_s) *wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return first_rc;
 }
 
 int socgpu_autotune(int n_request, const SocgpuRequest *requests, int precision, int repeats,

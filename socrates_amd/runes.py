@@ -71,6 +71,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise SocgpuError("socgpu library not built: %s (run socrates_amd/build.py)" % LIB_PATH)
+        # torch ships its own HIP runtime: loaded first, it is the one the
+        # library binds to, so that torch tensors and the library share one
+        # runtime (loaded after the library, torch finds no device)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.socgpu_last_error.restype = ctypes.c_char_p
         for fn in ("socgpu_init", "socgpu_spectrum_register", "socgpu_mcica_register",

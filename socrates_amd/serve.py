@@ -52,6 +52,34 @@ class Server:
                                           ctypes.c_double(timeout_s)))
         return self._live.pop(ticket)[0]
 
+    def replay(self, calls, arrival_s, picks, timeout_s=600.0):
+        """Submit calls[picks[i]] at arrival_s[i] seconds after the start,
+        from the library's own submitter thread (socgpu_serve_replay), and
+        wait for all of them. calls: list of (spectrum_name, kwargs) as for
+        submit; a call picked several times writes the same output arrays
+        each time. Returns the wall time from the start to the last
+        completion (s)."""
+        import numpy as np
+        n = len(calls)
+        keep, reqs = [], (R.Request * n)()
+        for k, (name, kw) in enumerate(calls):
+            kw = dict(kw)
+            P, L, src = kw.pop("n_profile"), kw.pop("n_layer"), kw.pop("i_source")
+            a, d, out, kp = R._prepare(P, L, name, src, **kw)
+            nm = name.encode()
+            keep += [a, d, kp, nm]
+            reqs[k] = R.Request(nm, ctypes.pointer(a), ctypes.pointer(d))
+        t = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        p = np.ascontiguousarray(picks, dtype=np.int32)
+        if p.size and (p.min() < 0 or p.max() >= n):
+            raise ValueError("picks out of range")
+        wall = ctypes.c_double()
+        R._check(R.lib().socgpu_serve_replay(
+            self._h, n, reqs, int(t.size), t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            p.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_double(timeout_s),
+            ctypes.byref(wall)))
+        return wall.value
+
     def stats(self):
         st = ServeStats()
         R._check(R.lib().socgpu_serve_stats(self._h, ctypes.byref(st)))

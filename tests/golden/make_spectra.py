@@ -242,6 +242,96 @@ def write_mcica(path, rng, lw_bands, sw_bands):
         f.write("\n".join(out) + "\n")
 
 
+# Precision-stress spectral files (sp_sw_stress, sp_lw_stress): the band
+# structure of the synthetic files (same k-terms per band, so the same McICA
+# data apply), one well-mixed major absorber (CO2, whose mixing ratio is the
+# call's constant co2_mix_ratio) with no pressure or temperature scaling
+# (a = b = 0), no minor gases:
+#  * SW: Rayleigh coefficients 10^U(-4, 0) m2/kg per band, so layer optical
+#    depths over a 60-layer column (layer masses 0.02..1800 kg m-2) span
+#    1e-6..1e3; k-terms chosen so that k co2 / k_rayleigh = 10^U(-8, -1):
+#    the clear-sky single-scattering albedo of each g-point is
+#    1 - 10^U(-8, -1);
+#  * LW: k-terms 10^U(-6, -2) m2/kg (gas optical depths down to 1e-11 at the
+#    top, into the thin-layer branch of the infra-red source);
+#  * both: Pade-2 droplets with the dev file's extinction fit, a co-albedo
+#    10^U(-8, -1) per band and an asymmetry rising linearly from 0 at
+#    re = 5 um to 0.95 at re = 20 um, so cloudy McICA sub-columns cover
+#    omega -> 1 with asymmetry U(0, 0.95).
+STRESS_CO2 = 6.0e-4
+
+
+def stress_block5(rng, bands, ray):
+    out = ["*BLOCK: TYPE =    5: SUBTYPE =    0: VERSION =    1",
+           "Exponential sum fiting coefficients: (exponents: m2/kg)",
+           "Band        Gas, Number of k-terms, Scaling type and scaling function,",
+           "             followed by reference pressure and temperature,",
+           "                   k-terms, weights and scaling parameters."]
+    for ib, (_, n, _) in enumerate(bands):
+        out.append("%5d       %5d       %5d       %5d       %5d" % (ib + 1, 1, n, 2, 1))
+        out.append("      %s      %s" % (e16(1.0e5), e16(250.0)))
+        if ray is not None:
+            k = ray[ib] * 10.0 ** rng.uniform(-8.0, -1.0, n) / STRESS_CO2
+        else:
+            k = 10.0 ** rng.uniform(-6.0, -2.0, n)
+        k.sort()
+        w = rng.uniform(0.05, 1.0, n)
+        w /= w.sum()
+        w = np.array([float("%16.9E" % x) for x in w])
+        w[-1] = 1.0 - w[:-1].sum()
+        for i in range(n):
+            out.append("   %s   %s%3d%s   %s" % (e16(k[i]), e16(w[i]), 0, e16(0.0), e16(0.0)))
+    out.append("*END")
+    return out
+
+
+def stress_block10(rng, dev_lines, n_band):
+    """Pade-2 droplets (type 5): the dev extinction fit, co-albedo
+    10^U(-8, -1) per band, asymmetry linear in re (0 at 5 um, 0.95 at 20 um)."""
+    head = dev_lines[:6]
+    out = list(head)
+    vals = []
+    i = 6
+    while not dev_lines[i].startswith("*END"):
+        if dev_lines[i].startswith("Band"):
+            i += 1
+            continue
+        vals += [float(x) for x in dev_lines[i].split()]
+        i += 1
+    slope = 0.95 / 15.0e-6
+    for b in range(n_band):
+        ext = vals[16 * b:16 * b + 6]
+        p = ext + [10.0 ** rng.uniform(-8.0, -1.0), 0.0, 0.0, 0.0, 0.0,
+                   -5.0e-6 * slope, slope, 0.0, 0.0, 0.0]
+        out.append("Band = %5d     Fitting parameters:" % (b + 1))
+        for j in range(0, 16, 4):
+            out.append("".join("    %12.5E" % x for x in p[j:j + 4]))
+    out.append("*END")
+    return out
+
+
+def write_stress_spectrum(path, dev, bands, rng, solar):
+    n_band = len(bands)
+    lines = block0(n_band, [2])
+    lines += read_block(dev, 1)
+    ray = None
+    if solar:
+        lines += read_block(dev, 2)
+        ray = 10.0 ** rng.uniform(-4.0, 0.0, n_band)
+        b3 = read_block(dev, 3)
+        lines += b3[:4] + ["%5d        %s" % (b + 1, "%15.9E" % ray[b]) for b in range(n_band)] + ["*END"]
+        ray = np.array([float("%15.9E" % x) for x in ray])
+    lines += block4([(2, n, []) for _, n, _ in bands], {2: 1})
+    lines += stress_block5(rng, bands, ray)
+    lines += block8(n_band)
+    if not solar:
+        lines += block6_from_dev(read_block(dev, 6))
+    lines += stress_block10(rng, read_block(dev, 10), n_band)
+    lines += read_block(dev, 12)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -254,6 +344,11 @@ def main():
     write_spectrum(os.path.join(OUT, "sp_sw_synth"), os.path.join(dev, "sp_sw_6_jm2"),
                    SW_BANDS, rng, solar=True)
     write_mcica(os.path.join(OUT, "mcica_synth"), rng, LW_BANDS, SW_BANDS)
+    rs = np.random.default_rng(20261203)
+    write_stress_spectrum(os.path.join(OUT, "sp_lw_stress"), os.path.join(dev, "sp_lw_9_jm3"),
+                          LW_BANDS, rs, solar=False)
+    write_stress_spectrum(os.path.join(OUT, "sp_sw_stress"), os.path.join(dev, "sp_sw_6_jm2"),
+                          SW_BANDS, rs, solar=True)
     print("wrote", sorted(os.listdir(OUT)))
 
 

@@ -1,17 +1,23 @@
 """Parity of the HIP path (through the C ABI) with the C restatement oracle.
 
-FP64 form, fluxes: infra-red within max(1e-10 |F|, 1e-9 W m-2) of the
-oracle. Solar within max(2e-8 |F|, 1e-9 W m-2): the reference's PIFM80 source
-coefficients cancel near the resonance lambda = sec(theta_0) (guarded only
-within 32 eps, solar_coefficient_basic.F90:124-146), and a 1-ulp change of
-exp alone moves its solar fluxes by up to ~3.4e-9 relative (measured on the
-restatement, tests/test_oracle.py::test_solar_exp_conditioning); the device
-exp is within 1-2 ulp of the host one. Heating rates: within the tolerance the
-flux tolerance implies through the layer heat capacity, or 1e-10 relative.
-The absolute floor is the larger of 1e-9 W m-2 and 1e-11 of the column's
-largest flux: thin-layer infra-red sources of the reference,
+FP64 form, fluxes: infra-red and solar within max(1e-10 |F|, 1e-9 W m-2) of
+the oracle. The absolute floor is the larger of 1e-9 W m-2 and 1e-11 of the
+column's largest flux: thin-layer infra-red sources of the reference,
 (1 - T + R + sqrt(eps)) / (sqrt(eps) + tau Sigma), carry last-bit exp noise
-of that size on the small downward fluxes high in the column.
+of that size on the small downward fluxes high in the column. Heating rates:
+within the tolerance the flux tolerance implies through the layer heat
+capacity, or 1e-10 relative. Measured on the 10 000 ragged columns and the
+4096 x 90 workload: largest relative flux error 4e-12 (solar) and 3e-11
+(infra-red upward flux). The reference's own formulas are ill-conditioned in
+some regimes (the PIFM80 source terms near lambda = sec(theta_0), guarded
+only within 32 eps by solar_coefficient_basic.F90:124-146; near-conservative
+scattering, 1 - e^2 gamma^2 -> 0; thin infra-red layers): a one-ulp change of
+exp there moves the reference's own fluxes by more than 1e-10 relative, up to
+~2e-8 (tests/test_oracle.py::test_solar_exp_conditioning). Tests over
+thousands of columns and of the precision-stress files therefore add that
+measured movement, element by element (spread=, oracle_spread: the largest
+change of two runs of the oracle with every exp perturbed by one ulp; 8x for
+the stress files, 1x elsewhere).
 McICA sub-column masks and condensate scalings: bit-identical.
 
 Mixed form (FP32 optical properties and two-stream coefficients, FP64
@@ -24,6 +30,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
+from socrates_amd import errstats
 from socrates_amd import runes as R
 from socrates_amd import synth
 
@@ -40,13 +47,14 @@ def spectra():
     R.register_default_spectra()
 
 
-REL_TOL = {"lw": 1e-10, "sw": 2e-8}
+REL_TOL = {"lw": 1e-10, "sw": 1e-10}
 
 
-def check_fp64(gpu, ref, label, src, heat_capacity, spread=None):
+def check_fp64(gpu, ref, label, src, heat_capacity, spread=None, factor=1.0):
     """spread: optional per-element sensitivity of the reference's own
     arithmetic (how far the oracle moves when its exp is perturbed by one
-    ulp, oracle_spread); 8x of it is added to the flux tolerance."""
+    ulp, oracle_spread); factor x spread is added to the flux tolerance."""
+    print(errstats.line(label + " fp64", errstats.stats(gpu, ref)))
     rel = REL_TOL[src]
     ftol = {}
     scale = np.maximum(np.abs(ref["flux_up"]).max(axis=0), np.abs(ref["flux_down"]).max(axis=0))
@@ -54,7 +62,7 @@ def check_fp64(gpu, ref, label, src, heat_capacity, spread=None):
         g, r = np.asarray(gpu[k]), ref[k]
         tol = np.maximum(np.maximum(rel * np.abs(r), 1e-9), 1e-11 * scale[None, :])
         if spread is not None:
-            tol = tol + 8.0 * spread[k]
+            tol = tol + factor * spread[k]
         ftol[k] = tol
         bad = np.abs(g - r) > tol
         assert not bad.any(), "%s %s: max err %.3e at %s" % (
@@ -115,6 +123,7 @@ MIXED_HR_TOL = 0.01 / 86400.0    # K s-1
 
 
 def check_mixed(gpu, ref, label):
+    print(errstats.line(label, errstats.stats(gpu, ref)))
     for k in ("flux_up", "flux_down", "flux_direct"):
         err = np.abs(np.asarray(gpu[k]) - ref[k]).max()
         assert err <= MIXED_FLUX_TOL, "%s %s: max err %.4f W m-2" % (label, k, err)
@@ -136,8 +145,7 @@ def test_runes_mixed_within_bounds(src, cloudy):
 @pytest.mark.parametrize("src", ["lw", "sw"])
 def test_runes_full_size(src):
     """The benchmark's own workload: 4096 columns x 90 layers, all-sky McICA,
-    both forms against the oracle on the same inputs. Over 372k levels the
-    FP64 tolerance includes the reference's own one-ulp sensitivity."""
+    both forms against the oracle on the same inputs."""
     P, L = 4096, 90
     case = synth.make_case(P, L, src, NBAND[src], seed=20261202 + (2 if src == "sw" else 1),
                            cloudy=True)

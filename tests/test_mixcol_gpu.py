@@ -104,3 +104,64 @@ def test_conv_strat_random_overlap_rejected():
     case = mixcol_case("lw", 1, 4, 30, 5, conv=True)
     with pytest.raises(R.SocgpuError):
         R.runes_case(case, "lw", precision="fp64")
+
+
+@pytest.mark.parametrize("src", ["sw", "lw"])
+def test_mixcol_ignores_convective_fields(src):
+    """ip_cloud_homogen reads no convective field (socrates_set_cld.F90:
+    477-495 add them only for the combined representations): a host that
+    always passes conv_frac, liq_conv_mmr and liq_conv_dim, as LFRic-style
+    callers do, gets the results of the call without them, bit for bit, and
+    the reference's own mixed-column results."""
+    path = os.path.join(GOLD, "runes_%s_mixcol_ov3.npz" % src)
+    d = np.load(path)
+    case = case_from_arrays(d)
+    ref = {k: d["ref_" + k] for k in ("flux_up", "flux_down", "flux_direct", "heating_rate")}
+    name, kw = R.case_call(case, src)
+    plain = R.runes(kw.pop("n_profile"), kw.pop("n_layer"), name, kw.pop("i_source"),
+                    precision="fp64", **dict(kw))
+    rng = np.random.default_rng(4)
+    L, P = case.n_layer, case.n_profile
+    name, kw = R.case_call(case, src)
+    kw["conv_frac"] = rng.uniform(0.0, 0.4, (L, P)) * (np.asarray(case.cloud_frac) < 0.5)
+    kw["liq_conv_mmr"] = rng.uniform(0.0, 1e-3, (L, P))
+    kw["liq_conv_dim"] = rng.uniform(5e-6, 20e-6, (L, P))
+    for prec in ("fp64", "mixed"):
+        with_conv = R.runes(kw["n_profile"], kw["n_layer"], name, kw["i_source"], precision=prec,
+                            **{k: v for k, v in kw.items() if k not in ("n_profile", "n_layer", "i_source")})
+        base = plain if prec == "fp64" else R.runes_case(case, src, precision="mixed")
+        for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+            assert np.array_equal(np.asarray(with_conv[k]), np.asarray(base[k])), (prec, k)
+    check_fp64(plain, ref, "mixcol with conv fields %s" % src, src, case.layer_heat_capacity)
+
+
+def test_cloud_fraction_above_one_is_an_error():
+    """A total cloud fraction above 1 + 0.001 is a fatal error in the
+    reference (socrates_set_cld.F90:848-853): an error here, for the
+    McICA, triple-column and mixed-column paths and inside a batch."""
+    bad = synth.make_case(6, 40, "lw", 9, seed=9, cloudy=True)
+    bad.cloud_frac = bad.cloud_frac.copy()
+    bad.cloud_frac[20, 3] = 1.01
+    with pytest.raises(R.SocgpuError, match="greater than 1"):
+        R.runes_case(bad, "lw", precision="fp64")
+    tri = mixcol_case("sw", 0, 6, 40, 11, conv=True)
+    tri.cloud_frac = tri.cloud_frac.copy()
+    tri.conv_frac = tri.conv_frac.copy()
+    tri.cloud_frac[25, 2], tri.conv_frac[25, 2] = 0.7, 0.5
+    with pytest.raises(R.SocgpuError, match="greater than 1"):
+        R.runes_case(tri, "sw", precision="mixed")
+    good = synth.make_case(5, 30, "sw", 6, seed=10, cloudy=True)
+    with pytest.raises(R.SocgpuError, match="request 1"):
+        R.runes_batch([R.case_call(good, "sw"), R.case_call(bad, "lw")], precision="fp64")
+    # a fraction within the threshold above 1 is clamped, as in the reference
+    ok = synth.make_case(6, 40, "lw", 9, seed=9, cloudy=True)
+    ok.cloud_frac = ok.cloud_frac.copy()
+    ok.cloud_frac[20, 3] = 1.0005
+    ref = oracle.runes(ok, SPEC["lw"], synth_mcica())
+    gpu = R.runes_case(ok, "lw", precision="fp64")
+    check_fp64(gpu, ref, "fraction 1.0005", "lw", ok.layer_heat_capacity)
+
+
+def synth_mcica():
+    from test_gpu_parity import MCICA
+    return MCICA

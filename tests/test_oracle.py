@@ -210,3 +210,105 @@ def test_triple_overlap_coefficients_conserve_area():
         assert np.allclose(v.sum(axis=1), 1.0, atol=1e-12)
         assert np.allclose(u.sum(axis=1), 1.0, atol=1e-12)
         assert (ov >= -1e-15).all() and (ov <= 1 + 1e-12).all()
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference not built here (oracle/_ref)")
+@pytest.mark.parametrize("src", ["lw", "sw"])
+def test_mixcol_with_convective_fields_matches_compiled_reference(src):
+    """The mixed column (ip_cloud_homogen without McICA) with non-zero
+    convective fields passed, as the reference harness passes them: the
+    reference ignores them (socrates_set_cld.F90:477-495), and so does the
+    restatement."""
+    case = synth.make_case(6, 40, src, 9 if src == "lw" else 6, seed=4040, cloudy=True,
+                           overlap="triple")
+    case.i_cloud_repr, case.i_inhom = 1, 0
+    assert (case.conv_frac > 0).any() and (case.liq_conv_mmr > 0).any()
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "c.bin"), os.path.join(tmp, "c.out")
+        case.write(fin)
+        subprocess.run([REF, os.path.join(DATA, "sp_sw_synth"), os.path.join(DATA, "sp_lw_synth"),
+                        "-", fin, fout], check=True, capture_output=True)
+        ref = read_output(fout, case.n_profile, case.n_layer)
+    out = oracle.runes(case, spec_for(case), None)
+    for k in ref:
+        assert np.array_equal(out[k], ref[k]), (k, np.abs(out[k] - ref[k]).max())
+    # and the convective fields change nothing
+    no_conv = synth.make_case(6, 40, src, 9 if src == "lw" else 6, seed=4040, cloudy=True,
+                              overlap="triple")
+    no_conv.i_cloud_repr, no_conv.i_inhom = 1, 0
+    no_conv.conv_frac = np.zeros_like(no_conv.conv_frac)
+    no_conv.liq_conv_mmr = np.zeros_like(no_conv.liq_conv_mmr)
+    out2 = oracle.runes(no_conv, spec_for(no_conv), None)
+    for k in ref:
+        assert np.array_equal(out[k], out2[k]), k
+
+
+def test_cloud_fraction_above_one_rejected_by_oracle():
+    """The restatement returns an error where the reference stops
+    (socrates_set_cld.F90:848-853) and clamps fractions within 0.001 of 1."""
+    case = synth.make_case(4, 30, "lw", 9, seed=5, cloudy=True)
+    case.cloud_frac = case.cloud_frac.copy()
+    case.cloud_frac[10, 1] = 1.01
+    with pytest.raises(RuntimeError):
+        oracle.runes(case, os.path.join(DATA, "sp_lw_synth"), os.path.join(DATA, "mcica_synth"))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "stress_runes_*.npz"))))
+def test_oracle_matches_reference_golden_stress(path):
+    """Precision-stress spectral files (near-conservative scattering, optical
+    depths 1e-6..1e3): the restatement equals the compiled reference."""
+    d = np.load(path)
+    case = case_from_arrays(d)
+    src = "sw" if case.i_source == 1 else "lw"
+    mcica = os.path.join(DATA, "mcica_synth") if case.l_cloud else None
+    out = oracle.runes(case, os.path.join(DATA, "sp_%s_stress" % src), mcica)
+    for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+        assert np.array_equal(out[k], d["ref_" + k]), (k, np.abs(out[k] - d["ref_" + k]).max())
+
+
+def test_stress_spectra_cover_the_regimes():
+    """The stress files reach what the precision stress names: clear-sky
+    single-scattering co-albedo 10^(-8..-1) (SW, Rayleigh over CO2 terms),
+    layer optical depths from below 1e-5 (the thin-layer branch of the
+    infra-red source, tau < 2^-13) to above 1e2 on the 60-layer columns."""
+    import re
+    txt = open(os.path.join(DATA, "sp_sw_stress")).read()
+    ray = [float(x) for x in re.findall(r"^\s+\d+\s+([0-9.E+-]+)\s*$",
+                                          txt.split("TYPE =    3")[1].split("*END")[0], re.M)]
+    blk5 = txt.split("TYPE =    5")[1].split("*END")[0].splitlines()
+    st = synth.profiles(64, 60, 1)
+    mass = st["mass"][:, 0]
+    omc, tau = [], []
+    band = 0
+    for ln in blk5[5:]:
+        f = ln.split()
+        if len(f) == 5 and "E" not in ln:
+            band = int(f[0]) - 1
+        elif len(f) == 5:
+            k = float(f[0])
+            kg = k * 6.0e-4
+            omc.append(kg / (ray[band] + kg))
+            tau += list((ray[band] + kg) * mass)
+    omc, tau = np.array(omc), np.array(tau)
+    assert omc.min() < 1e-7 and omc.max() > 1e-2
+    assert tau.min() < 1e-5 and tau.max() > 1e2
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference not built here (oracle/_ref)")
+@pytest.mark.parametrize("frac", [1.0005, 1.0])
+def test_cloud_fraction_clamp_matches_compiled_reference(frac):
+    """A cloud fraction within 0.001 above 1 is clamped after the type
+    fractions are normalised by it (socrates_set_cld.F90:828-853)."""
+    case = synth.make_case(6, 40, "lw", 9, seed=9, cloudy=True)
+    case.cloud_frac = case.cloud_frac.copy()
+    case.cloud_frac[20, 3] = frac
+    case.cloud_frac[30, 1] = frac
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "c.bin"), os.path.join(tmp, "c.out")
+        case.write(fin)
+        subprocess.run([REF, os.path.join(DATA, "sp_sw_synth"), os.path.join(DATA, "sp_lw_synth"),
+                        os.path.join(DATA, "mcica_synth"), fin, fout], check=True, capture_output=True)
+        ref = read_output(fout, case.n_profile, case.n_layer)
+    out = oracle.runes(case, spec_for(case), os.path.join(DATA, "mcica_synth"))
+    for k in ref:
+        assert np.array_equal(out[k], ref[k]), (k, np.abs(out[k] - ref[k]).max())
