@@ -146,6 +146,25 @@ int socgpu_mcica_register(const char *path, const char *sw_name,
 int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *args,
                  SocgpuDiag *diag, int precision);
 
+/* Allocation-free form of socgpu_runes. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory while radiation runs
+ * (code under HIP graph capture of their own, tight serving loops, harnesses
+ * that forbid hipMalloc). The caller passes device inputs and outputs
+ * (l_inputs_on_device and l_outputs_on_device set; flux_up and flux_down
+ * required). The call's descriptor and workspace live in a fixed device
+ * array of SOCGPU_NOALLOC_MAX_BYTES inside the library, mapped once per GPU
+ * at socgpu_init and held until the process ends; a call whose workspace
+ * does not fit (above about 500 all-sky McICA columns of 90 layers, or 4000
+ * clear-sky ones) fails with SOCGPU_ENOMEM and nothing runs. Nothing here
+ * calls hipMalloc or hipFree. Every call on a GPU reuses the same array,
+ * which keeps the last call's contents: two calls using it at the same time
+ * would race, so the entry point holds the library's lock for the whole call
+ * and concurrent callers in one process run one after the other. Results
+ * are those of socgpu_runes, bit for bit. Synchronous. */
+#define SOCGPU_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+int socgpu_runes_noalloc(const char *spectrum_name, const SocgpuRunesArgs *args,
+                         SocgpuDiag *diag, int precision);
+
 /* Kernel-level entry: the two-stream solve on prescribed layer optical
  * properties, n_term independent solves per profile (for example the
  * g-points of a band). Replaces two_stream (two_stream.F90:160-205):

@@ -2183,6 +2183,15 @@ struct SpecEntry {
   bool solar_hint = false;
 };
 
+// Fixed device workspace of the allocation-free entry point
+// (socgpu_runes_noalloc): one call's descriptor followed by its workspace.
+// Every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` only names the symbol: its device address comes
+// from hipGetSymbolAddress at socgpu_init (Context::scratch_dev), and that
+// address is what kernels, copies and fills take.
+constexpr size_t kScratchBytes = SOCGPU_NOALLOC_MAX_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
@@ -2192,6 +2201,8 @@ struct Context {
   DevBuf in_stage, out_stage;
   HostBuf in_host, out_host;
   double last_total_ms = 0.0, last_solve_ms = 0.0;
+  unsigned char *scratch_dev = nullptr;  // device address of `scratch` on `device`
+  int *err_host = nullptr;               // pinned word for the noalloc call's error flag
   std::mutex mu;
 };
 static Context g_ctx;
@@ -2225,6 +2236,13 @@ int socgpu_init(int device) {
   HIPCHK(hipEventCreate(&g_ctx.ev1));
   HIPCHK(hipEventCreate(&g_ctx.ev2));
   HIPCHK(hipEventCreate(&g_ctx.ev3));
+  // the allocation-free entry point's workspace on this device, and a pinned
+  // word its error flag comes back in (allocated here, once, so that the
+  // entry point itself allocates nothing)
+  void *sp = nullptr;
+  HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_ctx.scratch_dev = (unsigned char *)sp;
+  HIPCHK(hipHostMalloc((void **)&g_ctx.err_host, 256, hipHostMallocDefault));
   g_ctx.device = device;
   return 0;
 }
@@ -2683,6 +2701,75 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
     if (diag->flux_direct) std::memcpy(diag->flux_direct, h + 2 * nl, nl * 8);
     if (diag->heating_rate) std::memcpy(diag->heating_rate, h + 3 * nl, (size_t)P * L * 8);
   }
+  float t_all = 0.f, t_solve = 0.f;
+  HIPCHK(hipEventElapsedTime(&t_all, g_ctx.ev0, g_ctx.ev3));
+  HIPCHK(hipEventElapsedTime(&t_solve, g_ctx.ev1, g_ctx.ev2));
+  g_ctx.last_total_ms = t_all;
+  g_ctx.last_solve_ms = t_solve;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace sg {
+
+// A one-call descriptor block written into device memory by a kernel that
+// takes it by value (the allocation-free entry point copies nothing across
+// with the runtime's copy engines: no staging buffer is involved).
+struct DescBlob {
+  BatchHdr h;
+  ReqDesc q;
+};
+static_assert(sizeof(DescBlob) % 8 == 0, "descriptor block is copied in 8-byte words");
+
+__global__ void b_desc_store_kernel(DescBlob v, DescBlob *dst) {
+  const unsigned long long *src = (const unsigned long long *)&v;
+  unsigned long long *d = (unsigned long long *)dst;
+  for (int i = threadIdx.x; i < (int)(sizeof(DescBlob) / 8); i += blockDim.x) d[i] = src[i];
+}
+
+}  // namespace sg
+
+extern "C" {
+
+int socgpu_runes_noalloc(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag *diag,
+                         int precision) {
+  if (int rc = ensure_init()) return rc;
+  if (!spectrum_name || !a || !diag) return fail(SOCGPU_EINVAL, "null argument");
+  if (!a->l_inputs_on_device || !diag->l_outputs_on_device)
+    return fail(SOCGPU_EINVAL, "socgpu_runes_noalloc takes device inputs and outputs "
+                               "(l_inputs_on_device and l_outputs_on_device set)");
+  if (!diag->flux_up || !diag->flux_down)
+    return fail(SOCGPU_EINVAL, "socgpu_runes_noalloc needs flux_up and flux_down");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  if (!g_ctx.scratch_dev || !g_ctx.err_host) return fail(SOCGPU_EDEVICE, "fixed workspace not mapped");
+  SpecEntry *e = nullptr;
+  CallDims d;
+  if (int rc = resolve_request(spectrum_name, a, precision, &e, &d)) return rc;
+  const SgSpec &S = e->host;
+  const size_t ws_off = (sizeof(DescBlob) + 255) & ~(size_t)255;
+  const size_t need = ws_off + padded_ws_doubles(d, S) * 8;
+  if (need > kScratchBytes)
+    return fail(SOCGPU_ENOMEM, "call needs %zu bytes of workspace, above the %zu-byte fixed workspace of "
+                               "socgpu_runes_noalloc (use socgpu_runes)", need, kScratchBytes);
+  DescBlob host;
+  std::memset(&host, 0, sizeof host);
+  fill_desc(host.q, d, *e, *a, diag->flux_up, diag->flux_down, diag->flux_direct, diag->heating_rate);
+  size_t cursor = 0;
+  place_request(host.q, S, (double *)(g_ctx.scratch_dev + ws_off), &cursor);
+  long long w[kNWork];
+  request_work(d, S, w);
+  host.h.R = 1;
+  for (int k = 0; k < kNWork; k++) { host.q.s[k] = 0; host.h.tot[k] = w[k]; }
+  hipStream_t s = g_ctx.stream;
+  DescBlob *dd = (DescBlob *)g_ctx.scratch_dev;
+  HIPCHK(hipEventRecord(g_ctx.ev0, s));
+  b_desc_store_kernel<<<1, 64, 0, s>>>(host, dd);
+  if (int rc = launch_batch(&dd->h, &dd->q, host.h.tot, 1, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
+  HIPCHK(hipEventRecord(g_ctx.ev3, s));
+  HIPCHK(hipMemcpyAsync(g_ctx.err_host, &dd->h.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (*g_ctx.err_host) return fail(SOCGPU_EINVAL, "Cloud fraction greater than 1.");
   float t_all = 0.f, t_solve = 0.f;
   HIPCHK(hipEventElapsedTime(&t_all, g_ctx.ev0, g_ctx.ev3));
   HIPCHK(hipEventElapsedTime(&t_solve, g_ctx.ev1, g_ctx.ev2));
@@ -3557,6 +3644,9 @@ int socgpu_finalize(void) {
     b->p = nullptr;
     b->n = 0;
   }
+  if (g_ctx.err_host) (void)hipHostFree(g_ctx.err_host);
+  g_ctx.err_host = nullptr;
+  g_ctx.scratch_dev = nullptr;
   (void)hipEventDestroy(g_ctx.ev0); (void)hipEventDestroy(g_ctx.ev1);
   (void)hipEventDestroy(g_ctx.ev2); (void)hipEventDestroy(g_ctx.ev3);
   (void)hipStreamDestroy(g_ctx.stream);

@@ -232,6 +232,21 @@ def runes(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass, p
     return out
 
 
+def runes_noalloc(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
+                  precision="fp64", **kw):
+    """runes() through the allocation-free entry point
+    (socgpu_runes_noalloc): inputs and outputs must be torch tensors on the
+    GPU; the call's workspace is the library's fixed device array, so calls
+    above its size fail with SocgpuError."""
+    if not _is_torch(p_layer):
+        raise SocgpuError("runes_noalloc takes device (torch) arrays")
+    a, d, out, keep = _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass,
+                               **kw)
+    _check(lib().socgpu_runes_noalloc(spectrum_name.encode(), ctypes.byref(a), ctypes.byref(d),
+                                      _PREC[precision]))
+    return out
+
+
 # vertical fields of runes(), [layer or level][profile]
 _VERTICAL = ("density", "layer_heat_capacity", "h2o", "o3", "p_level", "t_level", "cloud_frac",
              "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr", "liq_conv_dim")
