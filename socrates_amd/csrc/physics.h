@@ -1,6 +1,13 @@
 // Per-lane two-stream physics shared by the kernels (device) and the host
 // self-checks. Written for one (column, k-term, sub-column) at a time; the
 // kernels call these in their layer loops.
+//
+// Contraction of a*b+c into fused multiply-adds is fixed per function: off in
+// the FP64 forms of the reference's formulas (as the reference is built), on
+// within one expression in the mixed form's. Either way the code gives the
+// same bits wherever it is inlined, so a layer recomputed by the recomputing
+// solve tile, or solved in another kernel variant of a batch, equals its
+// first computation.
 #pragma once
 
 #include <float.h>
@@ -79,6 +86,7 @@ SG_HD void two_coeff(bool solar, double tau, double om, double g, double sec0, C
 
 // Horner evaluation of the band Planck polynomial (planck_flux_band_mod.F90:33-49)
 SG_HD double planck_band(const SgBand &b, int n_deg, double t_ref, double temp) {
+#pragma clang fp contract(off)
   double x = temp / t_ref;
   double v = b.planck[n_deg];
   for (int j = n_deg - 1; j >= 0; j--) v = v * x + b.planck[j];
@@ -92,6 +100,7 @@ SG_HD double planck_band(const SgBand &b, int n_deg, double t_ref, double temp) 
 // too: the FP64 fluxes moved by under 1e-11 relative, and the solve took 26 %
 // longer.) lnp, lnt: logarithms of the layer pressure and temperature.
 SG_HD double k_term(const SgTerm &t, double lnp, double lnt, double mmr) {
+#pragma clang fp contract(off)
   double s = exp(t.a * (lnp - t.lnpref) + t.b * (lnt - t.lntref));
   return t.k * fmax(0.0, s * mmr);
 }
@@ -108,6 +117,7 @@ SG_HD double k_term(const SgTerm &t, double lnp, double lnt, double mmr) {
 // p, tk: layer pressure and temperature.
 SG_HD double k_scaled(const SgTerm &t, const SgSpec *spec, double lnp, double p, double tk,
                       const double *g, double mmr) {
+#pragma clang fp contract(off)
   if (t.fnc == SG_FNC_LOOKUP) {
     const int idx = (int)g[SG_G_IDX];
     const int jp = idx & 255, jt = (idx >> 8) & 255, jtt = idx >> 16, nt = spec->n_tmp;
@@ -148,6 +158,7 @@ SG_HD double k_scaled(const SgTerm &t, const SgSpec *spec, double lnp, double p,
 // M_air, l_mixing_ratio false), with the reference's constants
 // (rad_ccf.F90:42,58). h2o: the layer's water mixing ratio (clipped at 0).
 SG_HD double continuum_k(const SgBand &B, double p, double tk, double rho, double h2o) {
+#pragma clang fp contract(off)
   const double mol_weight_air = 28.966e-03, repsilon = 18.0153 / 28.966;
   double kg = 0.0;
   for (int j = 0; j < B.n_cont; j++) {
@@ -173,6 +184,7 @@ SG_HD double continuum_k(const SgBand &B, double p, double tk, double rho, doubl
 // jt / jtt the rows' lower neighbours; writes the four factors and the
 // packed indices into g.
 SG_HD void lookup_factors(const SgSpec *spec, double p, double tk, double *g) {
+#pragma clang fp contract(off)
   const double eps = kEps;
   const int np = spec->n_pre, nt = spec->n_tmp;
   const double *lp = spec->lut_lnp;
@@ -207,6 +219,7 @@ SG_HD void lookup_factors(const SgSpec *spec, double p, double tk, double *g) {
 // weighted by the in-cloud mixing ratio.
 SG_HD void pade2(const double *cp, double mmr, double re, double &kext, double &kscat,
                  double &phf, double &fwd) {
+#pragma clang fp contract(off)
   kext = mmr * (cp[0] + re * (cp[1] + re * cp[2])) /
          (1.0 + re * (cp[3] + re * (cp[4] + re * cp[5])));
   kscat = kext * (1.0 - (cp[6] + re * (cp[7] + re * cp[8])) / (1.0 + re * (cp[9] + re * cp[10])));
@@ -218,6 +231,7 @@ SG_HD void pade2(const double *cp, double mmr, double re, double &kext, double &
 // Slingo-Schrecker droplet optics (opt_prop_water_cloud.F90:231-247)
 SG_HD void slingo(const double *cp, double mmr, double re, double &kext, double &kscat,
                   double &phf, double &fwd) {
+#pragma clang fp contract(off)
   kext = mmr * (cp[0] + cp[1] / re);
   kscat = kext * (1.0 - cp[2] - cp[3] * re);
   double g = cp[4] + cp[5] * re;

@@ -150,7 +150,8 @@ struct SgSpec {
 #define SG_S_TGROUND 2
 #define SG_S_FC 3      // frac_cloudy
 #define SG_S_NCLD 4    // number of cloudy sub-columns (as a real)
-#define SG_NS 5
+#define SG_S_CT 5      // first layer (0-based) with cloud, n_layer if none (McICA)
+#define SG_NS 6
 
 // per (col, band, layer) record from the preparation kernel, [col][band][layer][4]
 //  LW: k_grey_tot, planck diff, planck diff_2, planck at the bottom level

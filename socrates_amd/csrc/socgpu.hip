@@ -162,7 +162,7 @@ struct CallDims {
   int mcica, mode;   // McICA on, sampling table (0 single, 1 optimal)
   int drop;          // droplet type index (0-based; the stratiform type)
   int nwpc;          // waves per column of the triple-column solve
-  int n_items;       // items per column of the packed solve (k-terms, then sub-column samples)
+  int n_items;       // lanes per column of the packed solve (its g-points, at least kMinItems)
   int n_waves;       // waves of the packed solve
   int n_gen, n_fill, ipph, ioverlap;
   int triple, drop2; // triple-column representation; convective droplet type
@@ -289,6 +289,7 @@ __device__ __forceinline__ void pack_column(const CallDims &d, const SocgpuRunes
     s[SG_S_TGROUND] = a.t_ground ? a.t_ground[col] : 0.0;
     s[SG_S_FC] = 0.0;
     s[SG_S_NCLD] = 0.0;
+    s[SG_S_CT] = (double)d.L;
     for (int b = 0; b < d.NB; b++) {
       double *bs = bandscal + ((size_t)col * d.NB + b) * SG_NBQ;
       // set_bound (socrates_set_bound.F90:138-233): grey albedo, else per band, else 0
@@ -445,9 +446,17 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
     if (d.f32) ((float *)c_sub + (size_t)col * L * SG_MAXSUBCOL)[(size_t)(k - 1) * SG_MAXSUBCOL + lane] = (float)val;
     else cs[(size_t)(k - 1) * SG_MAXSUBCOL + lane] = val;
   }
+  // first layer of the column with cloud: above it every sub-column is clear
+  int ct = L;
+  for (int base = 0; base < L; base += kWave) {
+    const int k = base + lane;
+    const unsigned long long m = __ballot(k < L && rec[(size_t)k * SG_NQ + SG_Q_WCLD] > 0.0);
+    if (m) { ct = base + __ffsll((long long)m) - 1; break; }
+  }
   if (lane == 0) {
     colscal[(size_t)col * SG_NS + SG_S_NCLD] = (double)ncld;
     colscal[(size_t)col * SG_NS + SG_S_FC] = (double)ncld / (double)n_gen;
+    colscal[(size_t)col * SG_NS + SG_S_CT] = (double)ct;
   }
 }
 
@@ -888,6 +897,7 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
 template <bool GEN>
 __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                            Coef &o) {
+#pragma clang fp contract(off)
   double kgas = (!GEN || t.fnc == SG_FNC_POWER_LAW) ? k_term(t, x.lnp, x.lnt, x.mmr) : x.kx;
   const double frac1 = x.frac1, kgt = x.kgt, csub = x.cv;
   if (csub * frac1 > 0.0) {
@@ -1021,28 +1031,45 @@ __device__ __forceinline__ void layer_step(const LaneCtx &c, const SgTerm &t, co
   elim_step<MIXED>(o, su, sd, rho, dn, a, bb);
 }
 
-// The column solve, one lane per item: a clear-sky k-term (weight
-// w_k (1 - frac_cloudy)) or a McICA sub-column sample (weight
-// w_k frac_cloudy / subcol_k) (mcica_sample.F90:453-930;
+// The column solve, one lane per (column, g-point): the g-point's clear-sky
+// solve (weight w_k (1 - frac_cloudy)) and its McICA sub-column samples
+// (weight w_k frac_cloudy / subcol_k each) (mcica_sample.F90:453-930;
 // monochromatic_radiance.F90:429-691; mcica_column.F90:255-622;
-// solver_homogen_direct.F90:93-127). A call's items are numbered column by
-// column (n_items per column: the k-terms, then the sub-column samples) and
-// cut into waves of 64 consecutive items, so a wave spans up to kMaxSeg
-// columns ("segments") and no lane idles between columns. The two-stream
-// system is eliminated top-down (reflectance of the atmosphere above and the
-// downward diffuse flux it emits) and back-substituted bottom-up. Output:
-// partial[wave][segment][3][L+1] (up, down, direct) of the segment's
-// weighted sums, the sums formed by butterflies over the wave with the other
+// solver_homogen_direct.F90:93-127). The lanes of a call are numbered column
+// by column (n_items per column: its g-points) and cut into waves of 64, so a
+// wave spans up to kMaxSeg columns ("segments") and no lane idles between
+// columns. The two-stream system is eliminated top-down (reflectance of the
+// atmosphere above and the downward diffuse flux it emits) and
+// back-substituted bottom-up.
+//
+// Above the column's first cloudy layer (ic) every sample is clear, so its
+// layers, its elimination and its direct beam there are those of the clear
+// solve, to the bit. The lane therefore
+//  1. eliminates the clear layers 0..ic-1 once;
+//  2. for each of its solves, continues from that state through layers
+//     ic..L-1, applies the surface condition and back-substitutes up to
+//     level ic, adding the weighted fluxes of levels ic..L into acc;
+//  3. back-substitutes the layers above ic once for all its solves together:
+//     they share fu(i) = a(i) fu(i+1) + bb(i), so the weighted sum
+//     X(i) = sum_j w_j fu_j(i) obeys X(i) = a(i) X(i+1) + bb(i) W with
+//     W = sum_j w_j, and the downward flux sum is rho(i) X(i) + dn(i) W.
+// This does the work of the reference's clear layers once per g-point
+// instead of once per sample (mcica_column.F90:266-298 recomputes them for
+// every sample); the sums associate differently from a sum of separate
+// solves, by rounding only.
+//
+// Output: partial[wave][segment][3][L+1] (up, down, direct) of the segment's
+// weighted sums, formed by butterflies over the wave with the other
 // segments' lanes masked.
 //
-// RECOMP = false keeps every layer's elimination state (a, bb, rho, dn) in
-// lane-private memory between the sweeps: 64 B per lane and layer of
-// scratch traffic. RECOMP = true keeps only (fdir, rho, dn) at the top of
-// every kSeg-th layer and, going up, recomputes each segment's layers
-// (coefficients and elimination, bit-identical to the first sweep) into
-// registers before back-substituting it: a second pass of arithmetic in
-// place of the memory traffic, which pays where the coefficients are cheap
-// (the mixed form's FP32 arithmetic).
+// RECOMP = false keeps the elimination state of every clear layer above ic
+// (a, bb, rho, dn and the direct flux) in lane-private memory between the
+// sweeps. RECOMP = true keeps only (fdir, rho, dn) at the top of every
+// kSeg-th layer and recomputes each segment's layers (coefficients and
+// elimination, bit-identical to the first sweep) into registers on the way
+// up: a second pass of arithmetic in place of the memory traffic. The layers
+// below ic are few (the cloud sits in the lowest of the log-spaced layers)
+// and are kept in private memory for each solve in turn.
 constexpr int kSeg = 4;
 constexpr int kMaxSeg = 4;     // columns a wave can span (n_items >= kMinItems)
 constexpr int kMinItems = 22;  // floor(63 / 22) + 2 = 4
@@ -1058,6 +1085,224 @@ __device__ __forceinline__ void seg_reduce8(const double (&v)[8], int seg, int s
 
 template <bool MIXED, bool RECOMP, bool GEN>
 __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__restrict__ spec,
+                                           int l_rayleigh, const double *__restrict__ colrec,
+                                           const double *__restrict__ grec,
+                                           const double *__restrict__ colscal,
+                                           const double *__restrict__ bandrec,
+                                           const double *__restrict__ bandscal,
+                                           const double *__restrict__ c_sub,
+                                           const double *__restrict__ cldopt,
+                                           double *__restrict__ partial, int wave, int lane) {
+#pragma clang fp contract(on)
+  static_assert(kSeg == 4, "checkpoints and output blocks share the 4-layer grid");
+  const int L = d.L, NB = d.NB, IM = d.n_items;
+  const long long g0 = (long long)wave * kWave, gmax = (long long)d.P * IM;
+  if (g0 >= gmax) return;
+  const int colA = (int)(g0 / IM);
+  const long long glast = (g0 + kWave < gmax ? g0 + kWave : gmax) - 1;
+  const int nseg = (int)(glast / IM) - colA + 1;  // wave-uniform, 1..kMaxSeg
+  const long long g = g0 + lane;
+  const bool inrange = g < gmax;
+  const int col = inrange ? (int)(g / IM) : colA;
+  const int item = inrange ? (int)(g - (long long)col * IM) : 0;
+  const int seg = col - colA;
+  const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
+  const double *cs = colscal + (size_t)col * SG_NS;
+  const double fc = d.mcica ? cs[SG_S_FC] : 0.0;
+  const bool act = inrange && item < spec->n_g;
+  const int gi = act ? item : 0;
+  const SgTerm t = spec->g[gi];
+  // the lane's solves: the clear one unless every sub-column is cloudy, then
+  // the g-point's samples if any sub-column is
+  const bool clear_on = act && fc < 1.0;
+  const int n_s = (act && d.mcica && fc > 0.0) ? spec->g_n_sub[d.mode][gi] : 0;
+  const int s0 = n_s > 0 ? spec->g_sub0[d.mode][gi] : 0;
+  const int n_solve = act ? (clear_on ? 1 : 0) + n_s : 0;
+  // first layer of the column with cloud (every layer above is clear in
+  // every sub-column); L where the lane has no sample
+  const int ic = n_s > 0 ? min(max((int)cs[SG_S_CT], 0), L) : L;
+  const int b = t.band;
+  double *out = partial + (size_t)wave * kMaxSeg * 3 * (L + 1);
+  const double *rec = colrec + (size_t)col * L * SG_NQ;
+  const double *br = bandrec + ((size_t)col * NB + b) * L * SG_NBQ;
+  const double *bs = bandscal + ((size_t)col * NB + b) * SG_NBQ;
+  LaneCtx c;
+  c.solar = solar;
+  c.sec0 = cs[SG_S_SEC0];
+  c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
+  LaneSrc src;
+  src.rec = rec;
+  src.br = br;
+  src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
+  src.copf = d.mcica ? (const float *)cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
+  src.csub = c_sub + (size_t)col * L * SG_MAXSUBCOL;
+  src.csubf = (const float *)c_sub + (size_t)col * L * SG_MAXSUBCOL;
+  src.gas = t.gas;
+  src.isub = 0;
+  src.fnc = t.fnc;
+  src.term = spec->g + gi;
+  src.spec = spec;
+  src.grec = grec ? grec + (size_t)col * L * SG_NGQ : nullptr;
+  const double alb_diff = bs[2], alb_dir = bs[3];
+  const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down, flux_inc_direct
+
+  // 1. the clear layers above ic, once
+  constexpr int kNst = RECOMP ? (SG_MAXL + kSeg - 1) / kSeg : SG_MAXL;
+  double st[kNst][RECOMP ? 3 : 5];
+  double fdir = solar ? finc : 0.0, rho = 0.0, dn = finc;
+  const int ic1 = act ? ic : 0;
+  {
+    LayerIn cur, nxt;
+    if (ic1 > 0) load_layer<MIXED, GEN>(src, 0, cur);
+    for (int i = 0; i < ic1; i++) {
+      if (i + 1 < ic1) load_layer<MIXED, GEN>(src, i + 1, nxt);
+      if (RECOMP) {
+        if (i % kSeg == 0) {
+          st[i / kSeg][0] = fdir; st[i / kSeg][1] = rho; st[i / kSeg][2] = dn;
+        }
+      } else {
+        st[i][4] = fdir;
+      }
+      double a, bb;
+      const double rho0 = rho, dn0 = dn;
+      layer_step<MIXED, GEN>(c, t, cur, fdir, rho, dn, a, bb);
+      if (!RECOMP) {
+        st[i][0] = a; st[i][1] = bb; st[i][2] = rho0; st[i][3] = dn0;
+      }
+      cur = nxt;
+    }
+  }
+
+  // 2. each solve from level ic down, back to level ic; weighted fluxes of
+  // levels ic..L in acc[lev - ic]
+  double acc[SG_MAXL + 1][3];
+  const int nacc = act ? L - ic : -1;
+  for (int k = 0; k <= nacc; k++) { acc[k][0] = 0.0; acc[k][1] = 0.0; acc[k][2] = 0.0; }
+  double W = 0.0;
+  {
+    double st2[SG_MAXL][4];
+    const double sg_lw = (1.0 - alb_diff) * (bs[1] - br[(L - 1) * SG_NBQ + 3]);
+    for (int j = 0; j < n_solve; j++) {
+      const bool clr = clear_on && j == 0;
+      const int sI = s0 + j - (clear_on ? 1 : 0);
+      const int isub = clr ? 0 : spec->sub_isub[d.mode][sI];
+      const double wj = clr ? t.w * (1.0 - fc) : t.w * (fc * spec->sub_inv_s[d.mode][sI]);
+      W = W + wj;
+      LaneSrc sj = src;
+      sj.isub = isub;
+      sj.csub = src.csub + (isub > 0 ? isub - 1 : 0);
+      sj.csubf = src.csubf + (isub > 0 ? isub - 1 : 0);
+      double f2 = fdir, r2 = rho, d2 = dn;
+      LayerIn cur, nxt;
+      if (ic < L) load_layer<MIXED, GEN>(sj, ic, cur);
+      for (int i = ic; i < L; i++) {
+        if (i > ic) acc[i - ic][2] += wj * f2;
+        if (i + 1 < L) load_layer<MIXED, GEN>(sj, i + 1, nxt);
+        double a, bb;
+        const double r0 = r2, d0 = d2;
+        layer_step<MIXED, GEN>(c, t, cur, f2, r2, d2, a, bb);
+        st2[i - ic][0] = a; st2[i - ic][1] = bb; st2[i - ic][2] = r0; st2[i - ic][3] = d0;
+        cur = nxt;
+      }
+      if (L > ic) acc[L - ic][2] += wj * f2;
+      const double sgs = solar ? (alb_dir - alb_diff) * f2 : sg_lw;
+      double fu = (alb_diff * d2 + sgs) / (1.0 - alb_diff * r2);
+      double fd = r2 * fu + d2;
+      acc[L - ic][0] += wj * fu;
+      acc[L - ic][1] += wj * fd;
+      for (int lev = L - 1; lev >= ic; lev--) {
+        fu = st2[lev - ic][0] * fu + st2[lev - ic][1];
+        fd = st2[lev - ic][2] * fu + st2[lev - ic][3];
+        acc[lev - ic][0] += wj * fu;
+        acc[lev - ic][1] += wj * fd;
+      }
+    }
+    if (act) acc[0][2] = W * fdir;   // the direct beam at level ic, common to all
+  }
+
+  // 3. levels L..0 in blocks of 4: below ic from acc, above it the joint
+  // back-substitution of the lane's solves; sums over the segment's lanes
+  double x = act ? acc[0][0] : 0.0;
+  double vdir[8], o8[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) vdir[k] = 0.0;
+  for (int blk = L / kSeg; blk >= 0; blk--) {
+    const int l0 = blk * kSeg;
+    double sa[kSeg], sb[kSeg], sr[kSeg], sd[kSeg], sf[kSeg];
+#pragma unroll
+    for (int k = 0; k < kSeg; k++) { sa[k] = 0.0; sb[k] = 0.0; sr[k] = 0.0; sd[k] = 0.0; sf[k] = 0.0; }
+    if (l0 < ic1) {
+      if (RECOMP) {
+        double fs = st[blk][0], rs = st[blk][1], ds = st[blk][2];
+#pragma unroll
+        for (int k = 0; k < kSeg; k++) {
+          const int i = l0 + k;
+          sr[k] = rs; sd[k] = ds; sf[k] = fs;
+          if (i < ic1) {
+            LayerIn xl;
+            load_layer<MIXED, GEN>(src, i, xl);
+            layer_step<MIXED, GEN>(c, t, xl, fs, rs, ds, sa[k], sb[k]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < kSeg; k++) {
+          const int i = l0 + k;
+          if (i < ic1) { sa[k] = st[i][0]; sb[k] = st[i][1]; sr[k] = st[i][2]; sd[k] = st[i][3]; sf[k] = st[i][4]; }
+        }
+      }
+    }
+    double v[8];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int k = kSeg - 1 - m;   // levels l0+3 .. l0
+      const int lev = l0 + k;
+      double up = 0.0, dw = 0.0, dr = 0.0;
+      if (act && lev <= L) {
+        if (lev >= ic) {
+          up = acc[lev - ic][0]; dw = acc[lev - ic][1]; dr = acc[lev - ic][2];
+        } else {
+          x = sa[k] * x + sb[k] * W;
+          up = x;
+          dw = sr[k] * x + sd[k] * W;
+          dr = W * sf[k];
+        }
+      }
+      v[2 * m] = up;
+      v[2 * m + 1] = dw;
+      if (solar) vdir[(blk & 1) * 4 + k] = dr;
+    }
+    for (int sg = 0; sg < nseg; sg++) {
+      seg_reduce8(v, seg, sg, nseg, o8, lane);
+      if ((lane & 7) == 0) {
+        const int jj = lane >> 3, lev = l0 + kSeg - 1 - (jj >> 1);
+        if (lev <= L) out[(sg * 3 + (jj & 1)) * (L + 1) + lev] = o8[0];
+      }
+    }
+    if (solar && (blk & 1) == 0) {
+      // direct flux of levels l0 .. l0+7
+      for (int sg = 0; sg < nseg; sg++) {
+        seg_reduce8(vdir, seg, sg, nseg, o8, lane);
+        if ((lane & 7) == 0 && l0 + (lane >> 3) <= L) out[(sg * 3 + 2) * (L + 1) + l0 + (lane >> 3)] = o8[0];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) vdir[k] = 0.0;
+    }
+  }
+  if (!solar) {
+    for (int sg = 0; sg < nseg; sg++)
+      for (int i = lane; i <= L; i += kWave) out[(sg * 3 + 2) * (L + 1) + i] = 0.0;
+  }
+}
+
+// The same solve with one lane per item (a clear k-term or one McICA
+// sample, each over the whole column): the layout the mixed form keeps for
+// spectra with the general gas optics. The per-g-point layout above faulted
+// on the device in that one combination (mixed form, general gas optics,
+// stored elimination state) for a cause not yet found; every other
+// combination runs the per-g-point layout.
+template <bool MIXED, bool RECOMP, bool GEN>
+__device__ __forceinline__ void solve_body_items(const CallDims &d, const SgSpec *__restrict__ spec,
                                            int l_rayleigh, const double *__restrict__ colrec,
                                            const double *__restrict__ grec,
                                            const double *__restrict__ colscal,
@@ -1745,7 +1990,10 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
 // W_GEN counts the calls whose spectrum needs the general gas optics (any
 // scaling other than the power law, continua or further gases); a batch with
 // any such call runs the kernels built for it
-enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, W_LONG, W_GEN, kNWork };
+// W_PLAIN counts the calls that do not: the solve kernel of each build runs
+// only the calls of its kind, so a call's solve is the one it gets alone.
+enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, W_LONG, W_GEN,
+       W_PLAIN, kNWork };
 
 struct ReqDesc {
   CallDims d;
@@ -1960,8 +2208,13 @@ __global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, co
   for (long long w = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; w < tot; w += nw) {
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_SOLVE, w));
     const ReqDesc &q = rq[r];
-    solve_body<MIXED, RECOMP, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec, q.bandscal,
-                              q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
+    if ((q.d.gen != 0) != GEN) continue;   // the other build's call
+    if (MIXED && GEN)
+      solve_body_items<MIXED, RECOMP, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec,
+                                           q.bandscal, q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
+    else
+      solve_body<MIXED, RECOMP, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec, q.bandscal,
+                                     q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
   }
 }
 
@@ -2401,6 +2654,15 @@ static int stage_inputs(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB) {
   return 0;
 }
 
+// Lanes per column of the packed solve and its waves: one lane per g-point,
+// or (per_item: the mixed form's calls that need the general gas optics) one
+// lane per clear k-term and per McICA sample.
+static void solve_layout(CallDims &d, const SgSpec &S, bool per_item) {
+  const int items = S.n_g + ((per_item && d.mcica) ? S.n_sub_items[d.mode] : 0);
+  d.n_items = items > kMinItems ? items : kMinItems;
+  d.n_waves = (int)(((long long)d.P * d.n_items + kWave - 1) / kWave);
+}
+
 // Checks one call's arguments and resolves its options (socrates_set_control
 // defaults on the two-stream path); fills the request's shapes.
 static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, int precision,
@@ -2490,10 +2752,8 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     d.ipph = S.ipph;
     d.ioverlap = a->i_overlap;
   }
-  const int items = S.n_g + (d.mcica ? S.n_sub_items[d.mode] : 0);
-  d.nwpc = (items + kWave - 1) / kWave;
-  d.n_items = items > kMinItems ? items : kMinItems;
-  d.n_waves = (int)(((long long)P * d.n_items + kWave - 1) / kWave);
+  d.nwpc = (S.n_g + kWave - 1) / kWave;
+  solve_layout(d, S, precision == SOCGPU_MIXED && d.gen);
   *entry = &e;
   *dout = d;
   return 0;
@@ -2532,6 +2792,7 @@ static void request_work(const CallDims &d, const SgSpec &S, long long w[kNWork]
   w[W_CCOL] = (d.mcica || d.triple) ? P * ((L + kRangeChunk - 1) / kRangeChunk) : 0;
   w[W_FTILE] = finish_tiles(d);
   w[W_GEN] = d.gen ? 1 : 0;
+  w[W_PLAIN] = d.gen ? 0 : 1;
 }
 
 // Lays out the workspace of request r at *cursor (doubles, 32-byte aligned)
@@ -2621,8 +2882,18 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
   // general gas optics: the recomputing tile would evaluate the look-up
   // tables twice per layer; the storing tile is faster there (GA9 files,
   // 4096 x 90 all-sky: 5.27 ms per SW+LW step against 6.13 ms)
-  auto *kern = solve_kernel_for(precision == SOCGPU_MIXED, tile.recompute != 0 && !gen, tile.waves_per_simd, gen);
-  if (tot[W_SOLVE] > 0) kern<<<blocks(tot[W_SOLVE], wpb), wpb * kWave, 0, s>>>(hdr, rq);
+  if (tot[W_SOLVE] > 0) {
+    // the power-law build for the calls that need no more, the general one
+    // for the others (each skips the other's calls)
+    if (tot[W_PLAIN] > 0) {
+      auto *kern = solve_kernel_for(precision == SOCGPU_MIXED, tile.recompute != 0, tile.waves_per_simd, false);
+      kern<<<blocks(tot[W_SOLVE], wpb), wpb * kWave, 0, s>>>(hdr, rq);
+    }
+    if (gen) {
+      auto *kern = solve_kernel_for(precision == SOCGPU_MIXED, false, tile.waves_per_simd, true);
+      kern<<<blocks(tot[W_SOLVE], wpb), wpb * kWave, 0, s>>>(hdr, rq);
+    }
+  }
   if (tot[W_TSOLVE] > 0) {
     auto *tk = precision == SOCGPU_MIXED ? (gen ? b_tsolve_kernel<true, true> : b_tsolve_kernel<true, false>)
                                          : (gen ? b_tsolve_kernel<false, true> : b_tsolve_kernel<false, false>);
