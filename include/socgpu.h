@@ -93,6 +93,29 @@ typedef struct SocgpuRunesArgs {
    * absent gas, as in set_atm (socrates_set_atm.F90:535-622). Read only for
    * gases the spectral file lists. */
   double gas_mix_ratio[10];
+  /* Surface emission per band (runes' l_flux_ground and flux_ground,
+   * socrates_runes.F90:103; socrates_set_bound.F90:251-290): when
+   * l_flux_ground is set, a thermal call's ground source in band b is
+   * flux_ground[b][profile] / (1 - albedo_diff) in place of the Planck flux
+   * of t_ground (diff_planck_source_mod.F90:123-136); flux_ground NULL is
+   * zero emission; t_ground is then not needed. */
+  int l_flux_ground;
+  /* CO2 as a full field (runes' co2, socrates_runes.F90:51): [layer][profile]
+   * mass mixing ratio, negative values taken as 0; used unless NULL or
+   * l_co2_well_mixed is set (runes' l_co2_well_mixed), when co2_mix_ratio
+   * applies (set_gas_mix_ratio, socrates_set_atm.F90:535-622). */
+  int l_co2_well_mixed;
+  const double *flux_ground;   /* [band][profile] W m-2 */
+  const double *co2;           /* [layer][profile] */
+  /* Well-mixed H2O and O3 (runes' h2o_mix_ratio, o3_mix_ratio), used where
+   * the h2o / o3 field is NULL (set_gas_mix_ratio, socrates_set_atm.F90:
+   * 535-622: a field takes precedence unless l_*_well_mixed, which a
+   * binding expresses by passing NULL). */
+  double h2o_mix_ratio, o3_mix_ratio;
+  /* Fields of the further gases, by SOCGPU_GAS_* index ([layer][profile]
+   * mass mixing ratio, negative values taken as 0; runes' n2o, ch4, o2, ...
+   * and their _1d forms, socrates_runes.F90:51-64); NULL: gas_mix_ratio. */
+  const double *gas_field[10];
 } SocgpuRunesArgs;
 
 #define SOCGPU_GAS_N2O 0
@@ -108,10 +131,18 @@ typedef struct SocgpuRunesArgs {
 
 /* Outputs (StrDiag, socrates_def_diag.F90:31-76): caller-owned, NULL = not
  * requested. Fluxes [level][profile] in W m-2, heating rate [layer][profile]
- * in K s-1 (set_diag, socrates_set_diag.F90:140-210). */
+ * in K s-1 (set_diag, socrates_set_diag.F90:140-210). The clear-sky fluxes
+ * (flux_up_clear, flux_down_clear, flux_direct_clear,
+ * socrates_def_diag.F90:46-52) are, as in the reference, the fluxes of the
+ * same call with cloud off: asking for any of them runs a cloud-free
+ * companion of the call in the same launch (socrates_set_control.F90:146-160
+ * turn on the reference's clear-sky pass the same way). With
+ * l_outputs_on_device, clear-sky output needs flux_up_clear and
+ * flux_down_clear both. */
 typedef struct SocgpuDiag {
   double *flux_up, *flux_down, *flux_direct, *heating_rate;
   int l_outputs_on_device;
+  double *flux_up_clear, *flux_down_clear, *flux_direct_clear;
 } SocgpuDiag;
 
 /* Select the device and create the library's stream. Replaces nothing in the
@@ -122,6 +153,15 @@ int socgpu_init(int device);
  * `name`. Replaces set_spectrum (socrates_set_spectrum.F90:29-149), which
  * reads into the module-saved spectrum_array (:20-23). */
 int socgpu_spectrum_register(const char *name, const char *path);
+
+/* socgpu_spectrum_register keeping only the gaseous absorbers whose
+ * identifiers (rad_pcf.F90 ip_h2o = 1, ...) are listed: set_spectrum's
+ * l_h2o, l_co2, ... flags without l_all_gases (compress_spectrum,
+ * socrates_set_spectrum.F90:152-321: each band keeps the listed absorbers of
+ * its list, in order, and the first one kept is its major gas). n_type < 0
+ * keeps every absorber (l_all_gases). */
+int socgpu_spectrum_register_gases(const char *name, const char *path, int n_type,
+                                   const int *type_absorb);
 
 /* Shape of a registered spectrum: band count and the identifiers of its
  * gaseous absorbers (type_absorb of the file's block 0, rad_pcf.F90 ip_h2o = 1,

@@ -30,6 +30,9 @@ FC = os.environ.get("SOC_FC", "/opt/rocm/bin/amdflang")
 FFLAGS = ["-O2", "-fPIC", "-ffp-contract=off"]
 # reference programs built as-is: the runes test driver behind examples/runes
 REF_PROGRAMS = ["aux/runes_driver.f90"]
+# host programs of the Fortran binding that also build against the
+# reference, so that both read the same caller arrays
+HOST_PROGRAMS = [os.path.join(os.path.dirname(HERE), "socrates_amd", "fortran", "runes_lfric_case.f90")]
 
 _mod_re = re.compile(r"^\s*module\s+(\w+)\s*(?:!.*)?$", re.I | re.M)
 _use_re = re.compile(r"^\s*use\s*(?:,\s*intrinsic\s*::)?\s*(\w+)", re.I | re.M)
@@ -97,7 +100,7 @@ def main():
     # the reference's own driver program (compiled where it lies) and our
     # harness programs, linked against the reference library
     progs = [os.path.join(src_root, p) for p in REF_PROGRAMS]
-    progs += sorted(glob.glob(os.path.join(HERE, "harness", "*.f90")))
+    progs += sorted(glob.glob(os.path.join(HERE, "harness", "*.f90"))) + HOST_PROGRAMS
     for h in progs:
         exe = os.path.join(OUT, os.path.basename(h)[:-4])
         cmd = [FC] + FFLAGS + ["-module-dir", objdir, "-I", moddir, h, "-o", exe,

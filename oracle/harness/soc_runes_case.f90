@@ -7,7 +7,9 @@
 ! checker (golden vectors) or as the CPU baseline in the benchmark.
 !
 ! Usage: soc_runes_case <sw_spectral_file> <lw_spectral_file> <mcica_file|-> \
-!                       <case.bin> <out.bin> [n_repeat]
+!                       <case.bin> <out.bin> [n_repeat] [clear]
+! With "clear" the clear-sky diagnostics flux_up_clear, flux_down_clear and
+! flux_direct_clear are requested too and written after the heating rate.
 program soc_runes_case
 
   use socrates_set_spectrum, only: set_spectrum, set_mcica
@@ -22,7 +24,8 @@ program soc_runes_case
   integer :: iu, n_profile, n_layer, i_source, l_cloud, i_overlap, l_rayleigh
   integer :: magic, version, n_band, i_st_water, i_st_ice, i_cloud_repr
   integer :: i_inhom, i_mcica_sampling, n_repeat, irep, n_subcol_gen
-  integer :: i_solver_kind, i
+  integer :: i_solver_kind, i, lfg, lco2
+  logical :: l_clear_diag
   real(RealExt) :: co2_mix_ratio, cloud_vertical_decorr, cloud_horizontal_rsd
   real(RealExt) :: conv_vertical_decorr
   ! well-mixed n2o, ch4, o2, so2, ocs, cfc11, cfc12, cfc113, hcfc22, hfc134a
@@ -32,9 +35,18 @@ program soc_runes_case
     cloud_frac, liq_mmr, liq_dim, ice_mmr, ice_dim, conv_frac, liq_conv_mmr, &
     liq_conv_dim, ice_conv_mmr, ice_conv_dim
   real(RealExt), allocatable, dimension(:) :: t_ground, cos_zen, solar_irrad
+  ! surface emission per band and a CO2 field: left unallocated (and so
+  ! not present in the runes calls) when the case file does not give them
+  real(RealExt), allocatable, dimension(:,:) :: flux_ground, co2
+  ! fields of n2o, ch4, o2, so2, ocs, cfc11, cfc12, cfc113, hcfc22, hfc134a
+  ! (version 5), unallocated when not given
+  real(RealExt), allocatable, dimension(:,:) :: f_n2o, f_ch4, f_o2, f_so2, &
+    f_ocs, f_cfc11, f_cfc12, f_cfc113, f_hcfc22, f_hfc134a
+  integer :: lwf(10)
   integer, allocatable :: rand_seed(:)
   real(RealExt), allocatable, target, dimension(:,:) :: flux_up, flux_down, &
-    flux_direct, heating_rate
+    flux_direct, heating_rate, flux_up_clear, flux_down_clear, &
+    flux_direct_clear
   type(StrDiag) :: diag
   integer(8) :: t0, t1, rate
 
@@ -47,6 +59,11 @@ program soc_runes_case
   if (command_argument_count() >= 6) then
     call get_command_argument(6, arg)
     read(arg, *) n_repeat
+  end if
+  l_clear_diag = .false.
+  if (command_argument_count() >= 7) then
+    call get_command_argument(7, arg)
+    l_clear_diag = trim(arg) == 'clear'
   end if
 
   open(newunit=iu, file=trim(case_file), access='stream', form='unformatted', &
@@ -80,6 +97,35 @@ program soc_runes_case
   read(iu) rand_seed
   wm = 0.0_RealExt
   if (version >= 2) read(iu) wm
+  lfg = 0
+  lco2 = 0
+  if (version >= 3) then
+    read(iu) lfg
+    allocate(flux_ground(n_profile, n_band))
+    read(iu) flux_ground
+    if (lfg == 0) deallocate(flux_ground)
+  end if
+  if (version >= 4) then
+    read(iu) lco2
+    if (lco2 /= 0) then
+      allocate(co2(n_profile, n_layer))
+      read(iu) co2
+    end if
+  end if
+  lwf = 0
+  if (version >= 5) then
+    read(iu) lwf
+    call get_field(lwf(1), f_n2o)
+    call get_field(lwf(2), f_ch4)
+    call get_field(lwf(3), f_o2)
+    call get_field(lwf(4), f_so2)
+    call get_field(lwf(5), f_ocs)
+    call get_field(lwf(6), f_cfc11)
+    call get_field(lwf(7), f_cfc12)
+    call get_field(lwf(8), f_cfc113)
+    call get_field(lwf(9), f_hcfc22)
+    call get_field(lwf(10), f_hfc134a)
+  end if
   close(iu)
   ice_mmr = 0.0_RealExt
   ice_dim = 3.0e-5_RealExt
@@ -93,6 +139,17 @@ program soc_runes_case
   diag%flux_down => flux_down
   diag%heating_rate => heating_rate
   if (i_source == 1) diag%flux_direct => flux_direct
+  allocate(flux_up_clear(n_profile, 0:n_layer), &
+    flux_down_clear(n_profile, 0:n_layer), &
+    flux_direct_clear(n_profile, 0:n_layer))
+  flux_up_clear = 0.0_RealExt
+  flux_down_clear = 0.0_RealExt
+  flux_direct_clear = 0.0_RealExt
+  if (l_clear_diag) then
+    diag%flux_up_clear => flux_up_clear
+    diag%flux_down_clear => flux_down_clear
+    if (i_source == 1) diag%flux_direct_clear => flux_direct_clear
+  end if
 
   ! Both spectra are registered, as a host model does at start-up; McICA
   ! data are associated with the pair (a single LW name is looked up in
@@ -117,12 +174,15 @@ program soc_runes_case
       call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &
         i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
         p_level=p_level, t_level=t_level, mass=mass, density=density, &
-        h2o=h2o, o3=o3, co2_mix_ratio=co2_mix_ratio, &
+        h2o=h2o, o3=o3, co2=co2, co2_mix_ratio=co2_mix_ratio, &
+        n2o=f_n2o, ch4=f_ch4, o2=f_o2, so2=f_so2, ocs=f_ocs, cfc11=f_cfc11, &
+        cfc12=f_cfc12, cfc113=f_cfc113, hcfc22=f_hcfc22, hfc134a=f_hfc134a, &
         n2o_mix_ratio=wm(1), ch4_mix_ratio=wm(2), o2_mix_ratio=wm(3), &
         so2_mix_ratio=wm(4), ocs_mix_ratio=wm(5), cfc11_mix_ratio=wm(6), &
         cfc12_mix_ratio=wm(7), cfc113_mix_ratio=wm(8), hcfc22_mix_ratio=wm(9), &
         hfc134a_mix_ratio=wm(10), &
-        t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
+        t_ground=t_ground, l_flux_ground=(lfg /= 0), flux_ground=flux_ground, &
+        cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
         albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
         l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity)
     else if (i_solver_kind == 0) then
@@ -130,12 +190,15 @@ program soc_runes_case
       call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &
         i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
         p_level=p_level, t_level=t_level, mass=mass, density=density, &
-        h2o=h2o, o3=o3, co2_mix_ratio=co2_mix_ratio, &
+        h2o=h2o, o3=o3, co2=co2, co2_mix_ratio=co2_mix_ratio, &
+        n2o=f_n2o, ch4=f_ch4, o2=f_o2, so2=f_so2, ocs=f_ocs, cfc11=f_cfc11, &
+        cfc12=f_cfc12, cfc113=f_cfc113, hcfc22=f_hcfc22, hfc134a=f_hfc134a, &
         n2o_mix_ratio=wm(1), ch4_mix_ratio=wm(2), o2_mix_ratio=wm(3), &
         so2_mix_ratio=wm(4), ocs_mix_ratio=wm(5), cfc11_mix_ratio=wm(6), &
         cfc12_mix_ratio=wm(7), cfc113_mix_ratio=wm(8), hcfc22_mix_ratio=wm(9), &
         hfc134a_mix_ratio=wm(10), &
-        t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
+        t_ground=t_ground, l_flux_ground=(lfg /= 0), flux_ground=flux_ground, &
+        cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
         albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
         l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
         cloud_frac=cloud_frac, liq_mmr=liq_mmr, liq_dim=liq_dim, &
@@ -151,12 +214,15 @@ program soc_runes_case
       call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &
         i_source=i_source, p_layer=p_layer, t_layer=t_layer, &
         p_level=p_level, t_level=t_level, mass=mass, density=density, &
-        h2o=h2o, o3=o3, co2_mix_ratio=co2_mix_ratio, &
+        h2o=h2o, o3=o3, co2=co2, co2_mix_ratio=co2_mix_ratio, &
+        n2o=f_n2o, ch4=f_ch4, o2=f_o2, so2=f_so2, ocs=f_ocs, cfc11=f_cfc11, &
+        cfc12=f_cfc12, cfc113=f_cfc113, hcfc22=f_hcfc22, hfc134a=f_hfc134a, &
         n2o_mix_ratio=wm(1), ch4_mix_ratio=wm(2), o2_mix_ratio=wm(3), &
         so2_mix_ratio=wm(4), ocs_mix_ratio=wm(5), cfc11_mix_ratio=wm(6), &
         cfc12_mix_ratio=wm(7), cfc113_mix_ratio=wm(8), hcfc22_mix_ratio=wm(9), &
         hfc134a_mix_ratio=wm(10), &
-        t_ground=t_ground, cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
+        t_ground=t_ground, l_flux_ground=(lfg /= 0), flux_ground=flux_ground, &
+        cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
         albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
         l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
         cloud_frac=cloud_frac, conv_frac=conv_frac, &
@@ -177,6 +243,17 @@ program soc_runes_case
   open(newunit=iu, file=trim(out_file), access='stream', form='unformatted', &
        status='replace')
   write(iu) flux_up, flux_down, flux_direct, heating_rate
+  if (l_clear_diag) write(iu) flux_up_clear, flux_down_clear, flux_direct_clear
   close(iu)
+
+contains
+
+  subroutine get_field(flag, f)
+    integer, intent(in) :: flag
+    real(RealExt), allocatable, intent(inout) :: f(:,:)
+    if (flag == 0) return
+    allocate(f(n_profile, n_layer))
+    read(iu) f
+  end subroutine get_field
 
 end program soc_runes_case

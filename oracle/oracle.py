@@ -27,7 +27,9 @@ class _Case(ctypes.Structure):
         "p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3",
         "p_level", "t_level", "t_ground", "cos_zen", "solar_irrad", "albedo_diff",
         "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr",
-        "liq_conv_dim")] + [("rand_seed", _I), ("wm_mix_ratio", ctypes.c_double * 10)]
+        "liq_conv_dim")] + [("rand_seed", _I), ("wm_mix_ratio", ctypes.c_double * 10),
+                            ("l_flux_ground", ctypes.c_int), ("flux_ground", _D), ("co2", _D),
+                            ("wm_field", _D * 10)]
 
 
 def build():
@@ -70,8 +72,15 @@ def runes(case, spectral_file, mcica_file=None, variant=None):
             setattr(cs, f, _ptr(v, keep, np.int32))
         elif f == "wm_mix_ratio":
             setattr(cs, f, (ctypes.c_double * 10)(*[float(x) for x in v]))
+        elif f == "wm_field":
+            ptrs = (_D * 10)()
+            for j, x in (v or {}).items():
+                ptrs[j] = _ptr(x, keep)
+            setattr(cs, f, ptrs)
         elif isinstance(v, np.ndarray):
             setattr(cs, f, _ptr(v, keep))
+        elif v is None:
+            setattr(cs, f, None)
         else:
             setattr(cs, f, v)
     L, P = case.n_layer, case.n_profile

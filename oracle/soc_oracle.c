@@ -1039,22 +1039,29 @@ static void solve_triple(int isolir, tri_t *t, const double *tau_c, const double
 
 /* atm%gas_mix_ratio of one absorber (socrates_set_atm.F90:227-415, 535-622):
  * full fields clipped at zero, well-mixed values as given, absent gases 0 */
+/* gas amounts as set_gas_mix_ratio gives them (socrates_set_atm.F90:535-622):
+ * a field clamped at 0, else the well-mixed ratio */
+static double wm_amount(const soc_case *c, int slot, int i, int l) {
+  const double *f = c->wm_field[slot];
+  return f ? fmax(f[(size_t)i * c->n_profile + l], 0.0) : c->wm_mix_ratio[slot];
+}
+
 static double gas_mix_ratio(const soc_case *c, int type, int i, int l) {
   const int P = c->n_profile;
   switch (type) {
     case IP_H2O: return fmax(c->h2o[(size_t)i * P + l], 0.0);
     case IP_O3: return fmax(c->o3[(size_t)i * P + l], 0.0);
-    case IP_CO2: return c->co2_mix_ratio;
-    case IP_N2O: return c->wm_mix_ratio[SOC_WM_N2O];
-    case IP_CH4: return c->wm_mix_ratio[SOC_WM_CH4];
-    case IP_O2: return c->wm_mix_ratio[SOC_WM_O2];
-    case IP_SO2: return c->wm_mix_ratio[SOC_WM_SO2];
-    case IP_OCS: return c->wm_mix_ratio[SOC_WM_OCS];
-    case IP_CFC11: return c->wm_mix_ratio[SOC_WM_CFC11];
-    case IP_CFC12: return c->wm_mix_ratio[SOC_WM_CFC12];
-    case IP_CFC113: return c->wm_mix_ratio[SOC_WM_CFC113];
-    case IP_HCFC22: return c->wm_mix_ratio[SOC_WM_HCFC22];
-    case IP_HFC134A: return c->wm_mix_ratio[SOC_WM_HFC134A];
+    case IP_CO2: return c->co2 ? fmax(c->co2[(size_t)i * P + l], 0.0) : c->co2_mix_ratio;
+    case IP_N2O: return wm_amount(c, SOC_WM_N2O, i, l);
+    case IP_CH4: return wm_amount(c, SOC_WM_CH4, i, l);
+    case IP_O2: return wm_amount(c, SOC_WM_O2, i, l);
+    case IP_SO2: return wm_amount(c, SOC_WM_SO2, i, l);
+    case IP_OCS: return wm_amount(c, SOC_WM_OCS, i, l);
+    case IP_CFC11: return wm_amount(c, SOC_WM_CFC11, i, l);
+    case IP_CFC12: return wm_amount(c, SOC_WM_CFC12, i, l);
+    case IP_CFC113: return wm_amount(c, SOC_WM_CFC113, i, l);
+    case IP_HCFC22: return wm_amount(c, SOC_WM_HCFC22, i, l);
+    case IP_HFC134A: return wm_amount(c, SOC_WM_HFC134A, i, l);
     default: return 0.0;
   }
 }
@@ -1425,7 +1432,13 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           double bt = planck_flux_band(&sp, b, LAY(c->t_layer, i - 1, l));
           dpl2[i - 1] = 2.0 * (planck[i] + planck[i - 1] - 2.0 * bt);
         }
-        flux_ground = planck_flux_band(&sp, b, c->t_ground[l]);
+        /* surface emission per band when given (socrates_set_bound.F90:
+         * 251-290), else the Planck flux of the surface temperature
+         * (diff_planck_source_mod.F90:123-136) */
+        if (c->l_flux_ground)
+          flux_ground = (c->flux_ground ? c->flux_ground[(size_t)b * P + l] : 0.0) / (1.0 - alb_diff);
+        else
+          flux_ground = planck_flux_band(&sp, b, c->t_ground ? c->t_ground[l] : 0.0);
       }
       /* equivalent extinction of minor gases (solve_band_k_eqv_scl.F90:
        * 572-1047) */

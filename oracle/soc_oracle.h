@@ -64,6 +64,15 @@ typedef struct {
   /* well-mixed mass mixing ratios of the further gases (the *_mix_ratio
    * arguments of runes, socrates_runes.F90:69-83), by SOC_WM_* slot */
   double wm_mix_ratio[10];
+  /* surface emission per band (runes' l_flux_ground / flux_ground,
+   * [band][profile]) and CO2 as a full field ([layer][profile], NULL: the
+   * well-mixed co2_mix_ratio) */
+  int l_flux_ground;
+  const double *flux_ground;
+  const double *co2;
+  /* fields of the further gases by SOC_WM_* slot ([layer][profile], NULL:
+   * the well-mixed wm_mix_ratio) */
+  const double *wm_field[10];
 } soc_case;
 
 enum { SOC_WM_N2O, SOC_WM_CH4, SOC_WM_O2, SOC_WM_SO2, SOC_WM_OCS, SOC_WM_CFC11, SOC_WM_CFC12,

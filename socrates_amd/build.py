@@ -1,7 +1,8 @@
 """Build the HIP library socrates_amd/lib/libsocgpu.so for gfx950 with hipcc,
 and the Fortran binding (socrates_amd/fortran) with amdflang: its module
-files and object in socrates_amd/lib/fortran, and the example host program
-socrates_amd/lib/runes_gpu_case linked against libsocgpu.so.
+files and object in socrates_amd/lib/fortran, and the example host programs
+socrates_amd/lib/runes_gpu_case and runes_lfric_case linked against
+libsocgpu.so.
 
 All HIP code is compiled with packed FP32 instructions disabled
 (-target-feature -packed-fp32-ops).
@@ -51,6 +52,9 @@ def build_fortran(verbose=False):
         [FC, "-O2", "-I", FOUT, os.path.join(FDIR, "runes_gpu_case.f90"), obj,
          "-L" + os.path.dirname(OUT), "-lsocgpu", "-Wl,-rpath,$ORIGIN",
          "-o", os.path.join(HERE, "lib", "runes_gpu_case")],
+        [FC, "-O2", "-I", FOUT, os.path.join(FDIR, "runes_lfric_case.f90"), obj,
+         "-L" + os.path.dirname(OUT), "-lsocgpu", "-Wl,-rpath,$ORIGIN",
+         "-o", os.path.join(HERE, "lib", "runes_lfric_case")],
     ]
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=FOUT)

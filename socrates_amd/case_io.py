@@ -14,8 +14,10 @@ import dataclasses
 import numpy as np
 
 MAGIC = 1397703491
-VERSION = 3     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
-                # 3 adds l_flux_ground and flux_ground (n_band, n_profile) after them
+VERSION = 5     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
+                # 3 adds l_flux_ground and flux_ground (n_band, n_profile) after them;
+                # 4 adds a flag and, when it is set, a CO2 field (n_layer, n_profile);
+                # 5 adds a flag per WM_GASES gas and, for each one set, its field
 
 # further gases runes takes as well-mixed mixing ratios (the *_mix_ratio
 # arguments, socrates_runes.F90:69-83), in this order
@@ -78,6 +80,8 @@ class Case:
     wm_mix_ratio: np.ndarray = None   # by WM_GASES, 0 = absent
     l_flux_ground: int = 0            # surface emission per band given (flux_ground)
     flux_ground: np.ndarray = None    # (n_band, n_profile) W m-2
+    co2: np.ndarray = None            # (n_layer, n_profile) CO2 mmr; None: co2_mix_ratio
+    wm_field: dict = None             # {WM_GASES index: (n_layer, n_profile) field}
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile
@@ -103,6 +107,8 @@ class Case:
             v = getattr(self, f.name)
             if isinstance(v, np.ndarray) and f.name != "wm_mix_ratio":
                 setattr(out, f.name, np.ascontiguousarray(v[..., cols]))
+            elif isinstance(v, dict):
+                setattr(out, f.name, {j: np.ascontiguousarray(x[..., cols]) for j, x in v.items()})
         return out
 
     def write(self, path):
@@ -117,6 +123,14 @@ class Case:
             np.ascontiguousarray(self.wm_mix_ratio, dtype=np.float64).tofile(f)
             np.array([int(self.l_flux_ground)], dtype=np.int32).tofile(f)
             np.ascontiguousarray(self.flux_ground, dtype=np.float64).tofile(f)
+            np.array([int(self.co2 is not None)], dtype=np.int32).tofile(f)
+            if self.co2 is not None:
+                np.ascontiguousarray(self.co2, dtype=np.float64).tofile(f)
+            fields = self.wm_field or {}
+            np.array([int(j in fields) for j in range(len(WM_GASES))], dtype=np.int32).tofile(f)
+            for j in range(len(WM_GASES)):
+                if j in fields:
+                    np.ascontiguousarray(fields[j], dtype=np.float64).tofile(f)
 
     @classmethod
     def read(cls, path):
@@ -149,6 +163,22 @@ class Case:
             c.l_flux_ground = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
             off += 4
             c.flux_ground = np.frombuffer(raw, dtype=np.float64, count=B * P, offset=off).reshape(B, P).copy()
+            off += 8 * B * P
+        if hdr[1] >= 4:
+            has_co2 = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
+            off += 4
+            if has_co2:
+                c.co2 = np.frombuffer(raw, dtype=np.float64, count=L * P, offset=off).reshape(L, P).copy()
+                off += 8 * L * P
+        if hdr[1] >= 5:
+            flags = np.frombuffer(raw, dtype=np.int32, count=len(WM_GASES), offset=off)
+            off += 4 * len(WM_GASES)
+            c.wm_field = {}
+            for j in range(len(WM_GASES)):
+                if flags[j]:
+                    c.wm_field[j] = np.frombuffer(raw, dtype=np.float64, count=L * P,
+                                                  offset=off).reshape(L, P).copy()
+                    off += 8 * L * P
         return c
 
 
@@ -179,6 +209,12 @@ def case_to_arrays(case):
     out = {}
     for f in dataclasses.fields(case):
         v = getattr(case, f.name)
+        if v is None:       # optional field not given (co2)
+            continue
+        if isinstance(v, dict):
+            for j, x in v.items():
+                out["case_%s_%d" % (f.name, j)] = np.asarray(x)
+            continue
         out["case_" + f.name] = np.asarray(v)
     return out
 
@@ -186,6 +222,11 @@ def case_to_arrays(case):
 def case_from_arrays(d):
     kw = {}
     for f in dataclasses.fields(Case):
+        if f.name == "wm_field":
+            fields = {j: d["case_wm_field_%d" % j] for j in range(len(WM_GASES))
+                      if "case_wm_field_%d" % j in d}
+            kw[f.name] = fields or None
+            continue
         if "case_" + f.name not in d:   # fields added after the arrays were written
             continue
         v = d["case_" + f.name]

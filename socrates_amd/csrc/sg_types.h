@@ -151,12 +151,14 @@ struct SgSpec {
 #define SG_S_FC 3      // frac_cloudy
 #define SG_S_NCLD 4    // number of cloudy sub-columns (as a real)
 #define SG_S_CT 5      // first layer (0-based) with cloud, n_layer if none (McICA)
-#define SG_NS 6
+#define SG_S_LFG 6     // 1: the ground source of each band is given (band scalar 1)
+#define SG_NS 7
 
 // per (col, band, layer) record from the preparation kernel, [col][band][layer][4]
 //  LW: k_grey_tot, planck diff, planck diff_2, planck at the bottom level
 //  SW: k_grey_tot, adjust_solar_ke, 0, 0
 // per (col, band) scalars [col][band][4]
-//  LW: planck at level 0, planck at the ground, albedo diff, albedo dir
+//  LW: planck at level 0, planck at the ground (or the given surface emission
+//      over 1 - albedo diff), albedo diff, albedo dir
 //  SW: band solar irradiance, 0, albedo diff, albedo dir
 #define SG_NBQ 4

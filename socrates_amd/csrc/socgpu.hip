@@ -196,9 +196,11 @@ __device__ __forceinline__ bool pack_layer(const CallDims &d, const SgSpec *__re
     r[SG_Q_LNP] = log(p);
     r[SG_Q_LNT] = log(t);
     r[SG_Q_MASS] = a.mass[idx];
-    r[SG_Q_MMR + SG_GAS_H2O] = a.h2o ? fmax(a.h2o[idx], 0.0) : 0.0;
-    r[SG_Q_MMR + SG_GAS_CO2] = fmax(a.co2_mix_ratio, 0.0);
-    r[SG_Q_MMR + SG_GAS_O3] = a.o3 ? fmax(a.o3[idx], 0.0) : 0.0;
+    r[SG_Q_MMR + SG_GAS_H2O] = a.h2o ? fmax(a.h2o[idx], 0.0) : a.h2o_mix_ratio;
+    // CO2: the field, clamped at 0, unless absent or declared well mixed
+    // (set_gas_mix_ratio, socrates_set_atm.F90:560-612)
+    r[SG_Q_MMR + SG_GAS_CO2] = (a.co2 && !a.l_co2_well_mixed) ? fmax(a.co2[idx], 0.0) : a.co2_mix_ratio;
+    r[SG_Q_MMR + SG_GAS_O3] = a.o3 ? fmax(a.o3[idx], 0.0) : a.o3_mix_ratio;
     r[SG_Q_MMR + SG_GAS_ZERO] = 0.0;
     r[SG_Q_P] = p;
     r[SG_Q_T] = t;
@@ -290,6 +292,8 @@ __device__ __forceinline__ void pack_column(const CallDims &d, const SocgpuRunes
     s[SG_S_FC] = 0.0;
     s[SG_S_NCLD] = 0.0;
     s[SG_S_CT] = (double)d.L;
+    const bool lfg = a.l_flux_ground && d.isolir == SOCGPU_SOURCE_THERMAL;
+    s[SG_S_LFG] = lfg ? 1.0 : 0.0;
     for (int b = 0; b < d.NB; b++) {
       double *bs = bandscal + ((size_t)col * d.NB + b) * SG_NBQ;
       // set_bound (socrates_set_bound.F90:138-233): grey albedo, else per band, else 0
@@ -300,6 +304,10 @@ __device__ __forceinline__ void pack_column(const CallDims &d, const SocgpuRunes
         bs[2] = a.albedo_diff ? a.albedo_diff[(size_t)b * P + col] : 0.0;
         bs[3] = a.albedo_dir ? a.albedo_dir[(size_t)b * P + col] : 0.0;
       }
+      // given surface emission: the ground's Planck flux is the emission
+      // over the diffuse co-albedo (diff_planck_source_mod.F90:123-129;
+      // socrates_set_bound.F90:251-290, absent = 0)
+      if (lfg) bs[1] = (a.flux_ground ? a.flux_ground[(size_t)b * P + col] : 0.0) / (1.0 - bs[2]);
     }
   }
 }
@@ -537,7 +545,7 @@ template <int kKeqChunks, bool GEN>
 __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__restrict__ spec,
                                           int l_rayleigh, const double *__restrict__ colrec,
                                           const double *__restrict__ grec, const double *__restrict__ wm,
-                                          const double *__restrict__ tlev,
+                                          const double *const *gfield, const double *__restrict__ tlev,
                                           const double *__restrict__ colscal,
                                           double *__restrict__ bandrec,
                                           double *__restrict__ bandscal,
@@ -573,7 +581,9 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
   if (!solar) {
     b_top = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[0]);
     b_bot = planck_band(B, spec->n_deg, spec->t_ref_planck, tl[L]);
-    bg = planck_band(B, spec->n_deg, spec->t_ref_planck, colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
+    bg = colscal[(size_t)col * SG_NS + SG_S_LFG] != 0.0
+             ? bandscal[((size_t)col * d.NB + b) * SG_NBQ + 1]
+             : planck_band(B, spec->n_deg, spec->t_ref_planck, colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
   }
   const double sqe = 1.4901161193847656e-08;
   int t0 = B.minor0;
@@ -584,8 +594,12 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
 #pragma unroll
     for (int c = 0; c < kKeqChunks; c++) {
       const int i = c * kWave + lane;
+      // further gases (general gas optics): the call's field, clamped at
+      // 0, else its well-mixed ratio (socrates_set_atm.F90:535-622)
+      const double *fld = (GEN && gas >= SG_GAS_WM0) ? gfield[gas - SG_GAS_WM0] : nullptr;
       mmr[c] = (c < nch && i < L) ? ((!GEN || gas < SG_GAS_WM0) ? rec[(size_t)i * SG_NQ + SG_Q_MMR + gas]
-                                                                : wm[gas - SG_GAS_WM0])
+                                     : fld ? fmax(fld[(size_t)i * d.P + col], 0.0)
+                                           : wm[gas - SG_GAS_WM0])
                                   : 0.0;
       num[c] = 0.0; den[c] = 0.0; kmin[c] = DBL_MAX;
     }
@@ -820,8 +834,10 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
     br[3] = b1;                          // planck at the bottom of the layer
     if (i == 0) {
       bs[0] = b0;
-      bs[1] = planck_band(B, spec->n_deg, spec->t_ref_planck,
-                          colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
+      // (the given surface emission was stored by the pack kernel)
+      if (colscal[(size_t)col * SG_NS + SG_S_LFG] == 0.0)
+        bs[1] = planck_band(B, spec->n_deg, spec->t_ref_planck,
+                            colscal[(size_t)col * SG_NS + SG_S_TGROUND]);
     }
   }
 }
@@ -2194,7 +2210,8 @@ __global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? SG_BAND_OCC : 2) void 
     const ReqDesc &q = rq[r];
     const int nch = (q.d.L + kWave - 1) / kWave;
     if ((NCH == 2 && nch <= 2) || (NCH > 2 && nch > 2))
-      band_body<NCH, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.a.gas_mix_ratio, q.tlev, q.colscal,
+      band_body<NCH, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.a.gas_mix_ratio, q.a.gas_field, q.tlev,
+                          q.colscal,
                      q.bandrec, q.bandscal, q.cldopt, q.krange, (int)(w - q.s[W_MINOR]), lane);
   }
 }
@@ -2500,14 +2517,16 @@ int socgpu_init(int device) {
   return 0;
 }
 
-int socgpu_spectrum_register(const char *name, const char *path) {
+}  // extern "C"
+
+static int register_spectrum(const char *name, const char *path, const std::vector<int> *retain) {
   if (int rc = ensure_init()) return rc;
   if (!name || !path) return fail(SOCGPU_EINVAL, "spectrum name and path are required");
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   auto e = std::make_unique<SpecEntry>();
   std::string err;
   SpecTables tab;
-  int rc = read_spectrum(path, &e->host, &tab, &err);
+  int rc = read_spectrum(path, &e->host, &tab, &err, retain);
   if (rc) return fail(rc == -3 ? SOCGPU_EUNSUPPORTED : SOCGPU_EIO, "%s: %s", path, err.c_str());
   if (!tab.data.empty()) {
     HIPCHK(hipMalloc(&e->tables, tab.data.size() * sizeof(double)));
@@ -2531,6 +2550,19 @@ int socgpu_spectrum_register(const char *name, const char *path) {
   }
   g_ctx.spectra[name] = std::move(e);
   return 0;
+}
+
+extern "C" {
+
+int socgpu_spectrum_register(const char *name, const char *path) {
+  return register_spectrum(name, path, nullptr);
+}
+
+int socgpu_spectrum_register_gases(const char *name, const char *path, int n_type, const int *type_absorb) {
+  if (n_type < 0) return register_spectrum(name, path, nullptr);
+  if (n_type > 0 && !type_absorb) return fail(SOCGPU_EINVAL, "null absorber list");
+  const std::vector<int> keep(type_absorb, type_absorb + n_type);
+  return register_spectrum(name, path, &keep);
 }
 
 int socgpu_spectrum_query(const char *name, int *n_band, int *n_absorb, int *type_absorb,
@@ -2582,8 +2614,14 @@ int socgpu_mcica_register(const char *path, const char *sw_name, const char *lw_
   return 0;
 }
 
+// A call that asks for any clear-sky flux.
+static bool wants_clear(const SocgpuDiag *g) {
+  return g && (g->flux_up_clear || g->flux_down_clear || g->flux_direct_clear);
+}
+
 // Input arrays of one call in the order they are staged; n = element counts.
 struct InField { const double **dst; const double *src; size_t n; };
+constexpr int kMaxInFields = 48;
 
 static size_t input_fields(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB, InField *f) {
   const size_t P = a->n_profile, L = a->n_layer;
@@ -2599,8 +2637,16 @@ static size_t input_fields(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
       {&dv->cloud_frac, a->cloud_frac, L * P}, {&dv->liq_mmr, a->liq_mmr, L * P},
       {&dv->liq_dim, a->liq_dim, L * P}, {&dv->conv_frac, a->conv_frac, L * P},
       {&dv->liq_conv_mmr, a->liq_conv_mmr, L * P}, {&dv->liq_conv_dim, a->liq_conv_dim, L * P},
+      {&dv->flux_ground, a->l_flux_ground ? a->flux_ground : nullptr, NB * P},
+      {&dv->co2, a->l_co2_well_mixed ? nullptr : a->co2, L * P},
+      {&dv->gas_field[0], a->gas_field[0], L * P}, {&dv->gas_field[1], a->gas_field[1], L * P},
+      {&dv->gas_field[2], a->gas_field[2], L * P}, {&dv->gas_field[3], a->gas_field[3], L * P},
+      {&dv->gas_field[4], a->gas_field[4], L * P}, {&dv->gas_field[5], a->gas_field[5], L * P},
+      {&dv->gas_field[6], a->gas_field[6], L * P}, {&dv->gas_field[7], a->gas_field[7], L * P},
+      {&dv->gas_field[8], a->gas_field[8], L * P}, {&dv->gas_field[9], a->gas_field[9], L * P},
   };
   const size_t n = sizeof g / sizeof g[0];
+  static_assert(sizeof g / sizeof g[0] <= kMaxInFields, "input field table too small");
   for (size_t k = 0; k < n; k++) f[k] = g[k];
   return n;
 }
@@ -2609,7 +2655,7 @@ static size_t input_fields(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
 static size_t stage_bytes(const SocgpuRunesArgs *a, int NB) {
   if (a->l_inputs_on_device) return 0;
   SocgpuRunesArgs dv = *a;
-  InField f[32];
+  InField f[kMaxInFields];
   size_t n = input_fields(a, &dv, NB, f), total = 0;
   for (size_t k = 0; k < n; k++) if (f[k].src) total += (f[k].n * 8 + 255) & ~(size_t)255;
   if (a->rand_seed) total += ((size_t)a->n_profile * 4 + 255) & ~(size_t)255;
@@ -2624,7 +2670,7 @@ static int stage_inputs_at(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
                            size_t *off, hipStream_t s) {
   *dv = *a;
   if (a->l_inputs_on_device) return 0;
-  InField f[32];
+  InField f[kMaxInFields];
   size_t n = input_fields(a, dv, NB, f);
   for (size_t k = 0; k < n; k++) {
     InField &x = f[k];
@@ -2681,7 +2727,8 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   if (!a->p_layer || !a->t_layer || !a->mass) return fail(SOCGPU_EINVAL, "p_layer, t_layer and mass are required");
   const bool solar = a->i_source == SOCGPU_SOURCE_SOLAR;
   if (solar && (!a->cos_zenith_angle || !a->solar_irrad)) return fail(SOCGPU_EINVAL, "solar call needs cos_zenith_angle and solar_irrad");
-  if (!solar && (!a->t_level || !a->t_ground)) return fail(SOCGPU_EINVAL, "thermal call needs t_level and t_ground");
+  if (!solar && (!a->t_level || (!a->t_ground && !a->l_flux_ground)))
+    return fail(SOCGPU_EINVAL, "thermal call needs t_level and t_ground (or l_flux_ground)");
   if (!solar && S.n_deg <= 0 && S.band[0].planck[0] == 0.0) return fail(SOCGPU_EUNSUPPORTED, "spectral file has no Planck data");
   if (S.has_cont && !a->density) return fail(SOCGPU_EINVAL, "density is required for continuum absorption");
   CallDims d;
@@ -2909,6 +2956,11 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
 int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag *diag, int precision) {
   if (int rc = ensure_init()) return rc;
   if (!spectrum_name || !a || !diag) return fail(SOCGPU_EINVAL, "null argument");
+  if (wants_clear(diag)) {
+    // the call and its cloud-free companion, packed as one batch
+    const SocgpuRequest r{spectrum_name, a, diag};
+    return socgpu_runes_batch(1, &r, precision);
+  }
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   SpecEntry *e = nullptr;
   CallDims d;
@@ -3012,6 +3064,8 @@ int socgpu_runes_noalloc(const char *spectrum_name, const SocgpuRunesArgs *a, So
                                "(l_inputs_on_device and l_outputs_on_device set)");
   if (!diag->flux_up || !diag->flux_down)
     return fail(SOCGPU_EINVAL, "socgpu_runes_noalloc needs flux_up and flux_down");
+  if (diag->flux_up_clear || diag->flux_down_clear || diag->flux_direct_clear)
+    return fail(SOCGPU_EUNSUPPORTED, "socgpu_runes_noalloc gives no clear-sky fluxes (use socgpu_runes)");
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   if (!g_ctx.scratch_dev || !g_ctx.err_host) return fail(SOCGPU_EDEVICE, "fixed workspace not mapped");
   SpecEntry *e = nullptr;
@@ -3056,7 +3110,54 @@ namespace sg {
 // One packed batch on the host: descriptors (ordered by decreasing layer
 // count, so that the long columns start first), totals and the outputs to
 // copy back to host memory after the launch.
+// Clear-sky diagnostics (flux_up_clear, flux_down_clear, flux_direct_clear):
+// the reference's clear-sky pass of a cloudy call gives, bit for bit, the
+// fluxes of the same call with cloud off (its i_solver_clear is the
+// homogeneous solver of the cloud-free call, socrates_set_control.F90:
+// 310-375), so a call that asks for them gets a cloud-free companion request
+// in the same batch that writes them.
+struct Expanded {
+  std::vector<SocgpuRequest> req;
+  std::vector<int> caller;           // the caller's index of each request
+  std::deque<SocgpuRunesArgs> args;  // companions' arguments (stable addresses)
+  std::deque<SocgpuDiag> diag;
+};
+
+static int expand_clear(int n, const SocgpuRequest *rq, Expanded &x) {
+  x.req.clear();
+  x.caller.clear();
+  x.args.clear();
+  x.diag.clear();
+  for (int i = 0; i < n; i++) {
+    x.req.push_back(rq[i]);
+    x.caller.push_back(i);
+    const SocgpuDiag *g = rq[i].diag;
+    if (!rq[i].args || !wants_clear(g)) continue;
+    if (g->l_outputs_on_device && (!g->flux_up_clear || !g->flux_down_clear))
+      return fail(SOCGPU_EINVAL, "request %d: device clear-sky outputs need flux_up_clear and flux_down_clear", i);
+    SocgpuRunesArgs c = *rq[i].args;
+    c.i_cloud_representation = SOCGPU_CLOUD_OFF;
+    c.i_inhom = SOCGPU_INHOM_HOMOGENEOUS;
+    c.cloud_frac = c.liq_mmr = c.liq_dim = nullptr;
+    c.conv_frac = c.liq_conv_mmr = c.liq_conv_dim = nullptr;
+    c.rand_seed = nullptr;
+    x.args.push_back(c);
+    SocgpuDiag cd;
+    std::memset(&cd, 0, sizeof cd);
+    cd.flux_up = g->flux_up_clear;
+    cd.flux_down = g->flux_down_clear;
+    cd.flux_direct = g->flux_direct_clear;
+    cd.l_outputs_on_device = g->l_outputs_on_device;
+    x.diag.push_back(cd);
+    x.req.push_back(SocgpuRequest{rq[i].spectrum_name, &x.args.back(), &x.diag.back()});
+    x.caller.push_back(i);
+  }
+  return 0;
+}
+
 struct Plan {
+  Expanded x;              // the requests planned (the caller's and companions)
+  int n() const { return (int)x.req.size(); }
   std::vector<ReqDesc> rq;
   BatchHdr h;
   size_t ws_doubles = 0, in_bytes = 0, out_doubles = 0, out_used = 0, in_used = 0;
@@ -3070,7 +3171,8 @@ struct Plan {
 static int batch_err(const Plan &pl, int err) {
   if (err <= 0) return 0;
   const int j = err - 1;
-  const int i = j < (int)pl.order.size() ? pl.order[j] : j;
+  int i = j < (int)pl.order.size() ? pl.order[j] : j;
+  if (i < (int)pl.x.caller.size()) i = pl.x.caller[i];
   return fail(SOCGPU_EINVAL, "request %d: Cloud fraction greater than 1.", i);
 }
 
@@ -3079,13 +3181,19 @@ struct Resolved {
   CallDims d;
 };
 
-// Sizes of a batch (workspace, staging, outputs) without placing it.
-static int plan_sizes(int n, const SocgpuRequest *req, int precision, std::vector<Resolved> &res, Plan &pl) {
+// Sizes of a batch (workspace, staging, outputs) without placing it; the
+// plan keeps the request list (with the clear-sky companions).
+static int plan_sizes(int n_caller, const SocgpuRequest *caller_req, int precision, std::vector<Resolved> &res,
+                      Plan &pl) {
+  for (int i = 0; i < n_caller; i++)
+    if (!caller_req[i].args || !caller_req[i].diag) return fail(SOCGPU_EINVAL, "request %d: null args or diag", i);
+  if (int rc = expand_clear(n_caller, caller_req, pl.x)) return rc;
+  const int n = pl.n();
+  const SocgpuRequest *req = pl.x.req.data();
   res.resize(n);
   pl.ws_doubles = pl.in_bytes = pl.out_doubles = 0;
   for (int k = 0; k < kNWork; k++) pl.need[k] = 0;
   for (int i = 0; i < n; i++) {
-    if (!req[i].args || !req[i].diag) return fail(SOCGPU_EINVAL, "request %d: null args or diag", i);
     if (int rc = resolve_request(req[i].spectrum_name, req[i].args, precision, &res[i].e, &res[i].d))
       return rc;
     const CallDims &d = res[i].d;
@@ -3102,8 +3210,10 @@ static int plan_sizes(int n, const SocgpuRequest *req, int precision, std::vecto
 // Places a sized batch into the given buffers; host inputs are gathered into
 // the pinned block in_host and copied across in one copy on s, or (in_host
 // null) copied array by array.
-static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolved> &res, Plan &pl,
-                      double *ws, char *in_stage, char *in_host, double *out_stage, hipStream_t s) {
+static int plan_place(const std::vector<Resolved> &res, Plan &pl, double *ws, char *in_stage, char *in_host,
+                      double *out_stage, hipStream_t s) {
+  const int n = pl.n();
+  const SocgpuRequest *req = pl.x.req.data();
   std::vector<int> order(n);
   for (int i = 0; i < n; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return res[x].d.L > res[y].d.L; });
@@ -3125,7 +3235,8 @@ static int plan_place(int n, const SocgpuRequest *req, const std::vector<Resolve
     double *up, *dn, *dir, *hr;
     if (g->l_outputs_on_device) {
       up = g->flux_up; dn = g->flux_down; dir = g->flux_direct; hr = g->heating_rate;
-      if (!up || !dn) return fail(SOCGPU_EINVAL, "request %d: device outputs need flux_up and flux_down", i);
+      if (!up || !dn)
+        return fail(SOCGPU_EINVAL, "request %d: device outputs need flux_up and flux_down", pl.x.caller[i]);
     } else {
       double *b = out_stage + out_cur;
       up = b; dn = b + nl; dir = b + 2 * nl; hr = b + 3 * nl;
@@ -3211,20 +3322,20 @@ int socgpu_runes_batch(int n_request, const SocgpuRequest *requests, int precisi
   Plan pl;
   if (int rc = plan_sizes(n_request, requests, precision, res, pl)) return rc;
   if (g_ctx.ws.ensure(pl.ws_doubles * 8) || g_ctx.in_stage.ensure(pl.in_bytes + 256) ||
-      g_ctx.out_stage.ensure(pl.out_doubles * 8 + 64) || g_ctx.desc.ensure(desc_bytes(n_request)))
+      g_ctx.out_stage.ensure(pl.out_doubles * 8 + 64) || g_ctx.desc.ensure(desc_bytes(pl.n())))
     return fail(SOCGPU_ENOMEM, "device buffer allocation failed");
   char *ih = g_ctx.in_host.use(pl.in_bytes), *oh = g_ctx.out_host.use(pl.out_doubles * 8);
   hipStream_t s = g_ctx.stream;
   HIPCHK(hipEventRecord(g_ctx.ev0, s));
-  if (int rc = plan_place(n_request, requests, res, pl, g_ctx.ws.as<double>(), g_ctx.in_stage.as<char>(),
-                          ih, g_ctx.out_stage.as<double>(), s))
+  if (int rc = plan_place(res, pl, g_ctx.ws.as<double>(), g_ctx.in_stage.as<char>(), ih,
+                          g_ctx.out_stage.as<double>(), s))
     return rc;
-  std::vector<char> blob(desc_bytes(n_request));
+  std::vector<char> blob(desc_bytes(pl.n()));
   pack_desc(pl, blob.data());
   HIPCHK(hipMemcpyAsync(g_ctx.desc.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)g_ctx.desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)g_ctx.desc.p + sizeof(BatchHdr));
-  if (int rc = launch_batch(dh, dq, pl.h.tot, n_request, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
+  if (int rc = launch_batch(dh, dq, pl.h.tot, pl.n(), precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
   int herr = 0;
   HIPCHK(hipMemcpyAsync(&herr, &dh->err, sizeof herr, hipMemcpyDeviceToHost, s));
@@ -3278,7 +3389,7 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
     return true;
   };
   bool g_ws = grow(pl.ws_doubles, b->cap_ws), g_in = grow(pl.in_bytes + 256, b->cap_in);
-  bool g_out = grow(pl.out_doubles + 64, b->cap_out), g_desc = grow(desc_bytes(n_request), b->cap_desc);
+  bool g_out = grow(pl.out_doubles + 64, b->cap_out), g_desc = grow(desc_bytes(pl.n()), b->cap_desc);
   for (int k = 0; k < kNWork; k++)
     if (pl.need[k] > b->cap[k]) { b->cap[k] = pl.need[k] + pl.need[k] / 4 + 64; recapture = true; }
   if (g_ws || g_desc) recapture = true;
@@ -3294,11 +3405,10 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
   hipStream_t s = b->stream;
   // the previous replay has finished (every run ends synchronised), so the
   // pinned descriptor block can be rewritten
-  if (int rc = plan_place(n_request, requests, res, pl, b->ws.as<double>(), b->in_stage.as<char>(),
-                          ih, b->out_stage.as<double>(), s))
+  if (int rc = plan_place(res, pl, b->ws.as<double>(), b->in_stage.as<char>(), ih, b->out_stage.as<double>(), s))
     return rc;
   pack_desc(pl, b->pinned);
-  HIPCHK(hipMemcpyAsync(b->desc.p, b->pinned, desc_bytes(n_request), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->desc.p, b->pinned, desc_bytes(pl.n()), hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)b->desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)b->desc.p + sizeof(BatchHdr));
   if (recapture) {
@@ -3424,7 +3534,7 @@ static void serve_finish_slot(SocgpuServer *sv, ServeSlot &sl, int status) {
   int bad = -1;
   if (status == 0 && sl.pinned) {
     const int err = ((const BatchHdr *)sl.pinned)->err;
-    if (err > 0 && err - 1 < (int)sl.plan.order.size()) bad = sl.plan.order[err - 1];
+    if (err > 0 && err - 1 < (int)sl.plan.order.size()) bad = sl.plan.x.caller[sl.plan.order[err - 1]];
   }
   if (status == 0) copy_back_finish(sl.plan, sl.out_stage.as<double>(), sl.out_host_used);
   const auto now = std::chrono::steady_clock::now();
@@ -3451,8 +3561,9 @@ static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
   for (int i = 0; i < n; i++) req[i] = sl.items[i]->req;
   std::vector<Resolved> res;
   if (int rc = plan_sizes(n, req.data(), sv->cfg.precision, res, sl.plan)) return rc;
+  const int nq = sl.plan.n();
   if (sl.ws.ensure(sl.plan.ws_doubles * 8) || sl.in_stage.ensure(sl.plan.in_bytes + 256) ||
-      sl.out_stage.ensure(sl.plan.out_doubles * 8 + 64) || sl.desc.ensure(desc_bytes(n)))
+      sl.out_stage.ensure(sl.plan.out_doubles * 8 + 64) || sl.desc.ensure(desc_bytes(nq)))
     return fail(SOCGPU_ENOMEM, "serving slot allocation failed");
   // host arrays cross PCIe array by array: gathering them into pinned
   // staging here would put the copies on this thread, which also launches
@@ -3460,26 +3571,25 @@ static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
   // load)
   char *ih = nullptr;
   sl.out_host_used = nullptr;
-  if (sl.pinned_bytes < desc_bytes(n)) {
+  if (sl.pinned_bytes < desc_bytes(nq)) {
     if (sl.pinned) (void)hipHostFree(sl.pinned);
     sl.pinned = nullptr;
-    sl.pinned_bytes = desc_bytes(n) * 2;
+    sl.pinned_bytes = desc_bytes(nq) * 2;
     HIPCHK(hipHostMalloc((void **)&sl.pinned, sl.pinned_bytes, hipHostMallocDefault));
   }
   hipStream_t s = sl.stream;
-  if (int rc = plan_place(n, req.data(), res, sl.plan, sl.ws.as<double>(), sl.in_stage.as<char>(),
-                          ih, sl.out_stage.as<double>(), s))
+  if (int rc = plan_place(res, sl.plan, sl.ws.as<double>(), sl.in_stage.as<char>(), ih, sl.out_stage.as<double>(), s))
     return rc;
   pack_desc(sl.plan, sl.pinned);
-  HIPCHK(hipMemcpyAsync(sl.desc.p, sl.pinned, desc_bytes(n), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(sl.desc.p, sl.pinned, desc_bytes(nq), hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)sl.desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)sl.desc.p + sizeof(BatchHdr));
-  if (int rc = launch_batch(dh, dq, sl.plan.h.tot, n, sv->cfg.precision, s, nullptr, nullptr)) return rc;
+  if (int rc = launch_batch(dh, dq, sl.plan.h.tot, nq, sv->cfg.precision, s, nullptr, nullptr)) return rc;
   HIPCHK(hipMemcpyAsync(&((BatchHdr *)sl.pinned)->err, &dh->err, sizeof(int), hipMemcpyDeviceToHost, s));
   if (int rc = copy_back_issue(sl.plan, sl.out_stage.as<double>(), sl.out_host_used, s)) return rc;
   HIPCHK(hipEventRecord(sl.done, s));
   long long sol = 0;
-  for (int i = 0; i < n; i++) sol += (long long)res[i].d.P * res[i].e->host.n_g;
+  for (int i = 0; i < nq; i++) sol += (long long)res[i].d.P * res[i].e->host.n_g;
   std::lock_guard<std::mutex> lk(sv->mu);
   sv->batches++;
   sv->batched_requests += n;
@@ -3575,7 +3685,13 @@ int socgpu_serve_submit(SocgpuServer *sv, const SocgpuRequest *request, long lon
     CallDims d;
     if (int rc = resolve_request(it->name.c_str(), &it->args, sv->cfg.precision, &e, &d)) return rc;
     SocgpuRequest r{it->name.c_str(), &it->args, &it->diag};
-    it->footprint = footprint_bytes(r, d, e->host);
+    // with its clear-sky companion, if it asks for clear-sky fluxes
+    Expanded x;
+    if (int rc = expand_clear(1, &r, x)) return rc;
+    for (const SocgpuRequest &q : x.req) {
+      if (int rc = resolve_request(q.spectrum_name, q.args, sv->cfg.precision, &e, &d)) return rc;
+      it->footprint += footprint_bytes(q, d, e->host);
+    }
   }
   it->req = SocgpuRequest{it->name.c_str(), &it->args, &it->diag};
   if (it->footprint > sv->cfg.hbm_budget_bytes)

@@ -10,6 +10,7 @@
 // path - SES2 and temperature-only look-up scaling, self-broadened look-up
 // tables, generalised continua, aerosol-only data - are rejected with a clear
 // error, never silently dropped.
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -22,6 +23,8 @@
 
 #include "sg_types.h"
 #include "spectrum.h"
+
+#include <algorithm>
 #include "wenyi_tables.h"
 
 namespace sg {
@@ -122,7 +125,7 @@ bool e13_values(Lines &L, int n, double *out) {
 
 }  // namespace
 
-int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err) {
+int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err, const std::vector<int> *retain) {
   bool ok = false;
   Lines L = read_lines(path, &ok);
   if (!ok) { *err = std::string("cannot open spectral file ") + path; return -1; }
@@ -142,6 +145,8 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
   struct Cont { int fnc = 0; double k = 0, pref = 0, tref = 0, sc[3] = {0, 0, 0}; bool set = false; };
   std::vector<std::vector<Cont>> cont;
   bool have5 = false, have6 = false, lookup = false;
+  // spectral sub-bands (block 17): band and k-term of each, in file order
+  std::vector<std::array<int, 2>> sub_band;
   while (L.more()) {
     const std::string ln = L.next();
     if (ln.compare(0, 6, "*BLOCK") != 0) continue;
@@ -323,6 +328,17 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
           c.sc[1] = fnum(d, 50, 16);
           if (n_scale_variable(fnc) >= 3) c.sc[2] = fnum(L.next(), 28, 16);
         }
+    } else if (type == 17) {
+      L.next(); L.next();
+      const std::string h = L.next();
+      if (h.find('=') == std::string::npos) { *err = "bad block 17"; return -1; }
+      const int n = after_eq_int(h);
+      L.next();
+      for (int q = 0; q < n; q++) {
+        std::vector<double> x = numbers(L.next());
+        if (x.size() < 3) { *err = "bad sub-band line in block 17"; return -1; }
+        sub_band.push_back({(int)x[1], (int)x[2]});
+      }
     } else if (type == 10) {
       if (ver != 2) { *err = "droplet block version not supported"; return -3; }
       L.next();
@@ -386,13 +402,40 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
       }
     s->has_lut = 1;
   }
+  if (retain) {
+    // gas selection (compress_spectrum, socrates_set_spectrum.F90:185-303):
+    // a band keeps the retained absorbers of its list in order, and its
+    // first absorber also where its first sub-band (scanned on from the
+    // previous band's, as the reference does) names a k-term
+    size_t i_sub = 0;
+    for (int b = 0; b < s->n_band; b++) {
+      bool keep_major = false;
+      for (;;) {
+        i_sub++;
+        if (i_sub > sub_band.size()) break;
+        if (sub_band[i_sub - 1][0] == b + 1) {
+          keep_major = sub_band[i_sub - 1][1] > 0;
+          break;
+        }
+      }
+      std::vector<int> kept;
+      for (int j = 0; j < n_band_absorb[b]; j++) {
+        const int gi = index_absorb[b][j] - 1;
+        const bool listed = gi >= 0 && gi < (int)type_absorb.size() &&
+                            std::find(retain->begin(), retain->end(), type_absorb[gi]) != retain->end();
+        if (listed || (j == 0 && keep_major)) kept.push_back(index_absorb[b][j]);
+      }
+      index_absorb[b] = kept;
+      n_band_absorb[b] = (int)kept.size();
+    }
+  }
   // assemble the term tables: major gas is the first absorber listed for the
   // band (radiance_calc.F90: index_abs(1) is the major absorber)
   int ng = 0, nm = 0;
   bool wenyi = false;
   for (int b = 0; b < s->n_band; b++) {
     SgBand &B = s->band[b];
-    if (n_band_absorb[b] < 1) { *err = "band without gaseous absorption"; return -3; }
+    if (n_band_absorb[b] < 1) { *err = "a band without gaseous absorption is not on this path"; return -3; }
     for (int j = 0; j < n_band_absorb[b]; j++) {
       int gi = index_absorb[b][j] - 1;
       if (gi < 0 || gi >= (int)type_absorb.size()) { *err = "bad absorber index in block 4"; return -1; }

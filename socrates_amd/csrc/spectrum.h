@@ -24,7 +24,10 @@ struct SpecTables {
   long wenyi_off = -1;
 };
 
-int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err);
+// retain: absorber identifiers to keep (set_spectrum's gas flags), or null
+// for every absorber of the file.
+int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err,
+                  const std::vector<int> *retain = nullptr);
 int read_mcica(const char *path, McicaData *m, std::string *err);
 int attach_mcica(SgSpec *s, const McicaData &m, bool solar, std::string *err);
 

@@ -53,12 +53,25 @@ class RunesArgs(ctypes.Structure):
         "p_level", "t_level", "t_ground", "cos_zenith_angle", "solar_irrad",
         "albedo_diff", "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac",
         "liq_conv_mmr", "liq_conv_dim")] + [("rand_seed", _I),
-                                             ("gas_mix_ratio", ctypes.c_double * len(WM_GASES))]
+                                             ("gas_mix_ratio", ctypes.c_double * len(WM_GASES)),
+                                             ("l_flux_ground", ctypes.c_int),
+                                             ("l_co2_well_mixed", ctypes.c_int),
+                                             ("flux_ground", _D), ("co2", _D),
+                                             ("h2o_mix_ratio", ctypes.c_double),
+                                             ("o3_mix_ratio", ctypes.c_double),
+                                             ("gas_field", _D * len(WM_GASES))]
 
 
 class Diag(ctypes.Structure):
     _fields_ = [("flux_up", _D), ("flux_down", _D), ("flux_direct", _D),
-                ("heating_rate", _D), ("l_outputs_on_device", ctypes.c_int)]
+                ("heating_rate", _D), ("l_outputs_on_device", ctypes.c_int),
+                ("flux_up_clear", _D), ("flux_down_clear", _D), ("flux_direct_clear", _D)]
+
+
+# outputs of a call: the StrDiag fields of the path (socrates_def_diag.F90:
+# 34-52); the clear-sky ones only when asked for (l_clear)
+_OUTPUTS = ("flux_up", "flux_down", "flux_direct", "heating_rate")
+_CLEAR_OUTPUTS = ("flux_up_clear", "flux_down_clear", "flux_direct_clear")
 
 
 _lib = None
@@ -142,15 +155,23 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
           liq_conv_dim=None, cloud_vertical_decorr=0.0, conv_vertical_decorr=0.0,
           cloud_horizontal_rsd=0.0, i_cloud_representation=0, i_overlap=0, i_inhom=0,
           i_mcica_sampling=0, i_st_water=5, i_cnv_water=5, n_subcol_gen=0,
-          rand_seed=None, out=None, **mix_ratios):
+          rand_seed=None, l_flux_ground=False, flux_ground=None, co2=None,
+          l_co2_well_mixed=False, h2o_mix_ratio=0.0, o3_mix_ratio=0.0, l_clear=False, out=None,
+          **gas_amounts):
     """ctypes argument and output structures of one call (and the objects
     that must stay alive while it runs). mix_ratios: the well-mixed mixing
     ratios of further gases, as runes names them (n2o_mix_ratio,
     ch4_mix_ratio, o2_mix_ratio, so2_mix_ratio, ocs_mix_ratio,
     cfc11_mix_ratio, cfc12_mix_ratio, cfc113_mix_ratio, hcfc22_mix_ratio,
-    hfc134a_mix_ratio)."""
+    hfc134a_mix_ratio) or, under the gas's own name (n2o, ch4, ...), its
+    [layer][profile] field. l_flux_ground / flux_ground [band][profile]:
+    surface emission per band; co2 [layer][profile]: CO2 field (co2_mix_ratio
+    applies when it is None or l_co2_well_mixed); h2o_mix_ratio /
+    o3_mix_ratio: well-mixed H2O / O3 where h2o / o3 is None; l_clear: also the
+    clear-sky fluxes flux_up_clear, flux_down_clear, flux_direct_clear (an
+    out dict holding any of these keys asks for them as well)."""
     init()
-    unknown = set(mix_ratios) - {g + "_mix_ratio" for g in WM_GASES}
+    unknown = set(gas_amounts) - {g + "_mix_ratio" for g in WM_GASES} - set(WM_GASES)
     if unknown:
         raise TypeError("unexpected arguments: %s" % ", ".join(sorted(unknown)))
     inputs = dict(p_layer=p_layer, t_layer=t_layer, mass=mass, density=density,
@@ -158,7 +179,8 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
                   t_level=t_level, t_ground=t_ground, cos_zenith_angle=cos_zenith_angle,
                   solar_irrad=solar_irrad, albedo_diff=albedo_diff, albedo_dir=albedo_dir,
                   cloud_frac=cloud_frac, liq_mmr=liq_mmr, liq_dim=liq_dim, conv_frac=conv_frac,
-                  liq_conv_mmr=liq_conv_mmr, liq_conv_dim=liq_conv_dim)
+                  liq_conv_mmr=liq_conv_mmr, liq_conv_dim=liq_conv_dim,
+                  flux_ground=flux_ground if l_flux_ground else None, co2=co2)
     on_dev = _is_torch(p_layer)
     keep = []
     a = RunesArgs()
@@ -170,10 +192,17 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     a.l_inputs_on_device = int(on_dev)
     a.co2_mix_ratio = float(co2_mix_ratio)
     for j, g in enumerate(WM_GASES):
-        a.gas_mix_ratio[j] = float(mix_ratios.get(g + "_mix_ratio", 0.0) or 0.0)
+        a.gas_mix_ratio[j] = float(gas_amounts.get(g + "_mix_ratio", 0.0) or 0.0)
+        f = gas_amounts.get(g)
+        if f is not None:
+            if _is_torch(f) != on_dev:
+                raise SocgpuError("all inputs must be in the same memory space (%s)" % g)
+            a.gas_field[j] = _ptr(f, keep)
+    a.h2o_mix_ratio, a.o3_mix_ratio = float(h2o_mix_ratio), float(o3_mix_ratio)
     a.cloud_vertical_decorr, a.conv_vertical_decorr = float(cloud_vertical_decorr), float(conv_vertical_decorr)
     a.cloud_horizontal_rsd = float(cloud_horizontal_rsd)
     a.l_grey_albedo, a.grey_albedo = int(bool(l_grey_albedo)), float(grey_albedo)
+    a.l_flux_ground, a.l_co2_well_mixed = int(bool(l_flux_ground)), int(bool(l_co2_well_mixed))
     for k, v in inputs.items():
         if v is not None and _is_torch(v) != on_dev:
             raise SocgpuError("all inputs must be in the same memory space (%s)" % k)
@@ -181,25 +210,21 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     a.rand_seed = _ptr(rand_seed, keep, np.int32)
     L, P = int(n_layer), int(n_profile)
     if out is None:
+        names = _OUTPUTS + (_CLEAR_OUTPUTS if l_clear else ())
         if on_dev:
             import torch
             dev = p_layer.device
-            out = dict(flux_up=torch.empty((L + 1, P), dtype=torch.float64, device=dev),
-                       flux_down=torch.empty((L + 1, P), dtype=torch.float64, device=dev),
-                       flux_direct=torch.empty((L + 1, P), dtype=torch.float64, device=dev),
-                       heating_rate=torch.empty((L, P), dtype=torch.float64, device=dev))
+            out = {k: torch.empty((L if k == "heating_rate" else L + 1, P), dtype=torch.float64,
+                                  device=dev) for k in names}
         else:
-            out = dict(flux_up=np.zeros((L + 1, P)), flux_down=np.zeros((L + 1, P)),
-                       flux_direct=np.zeros((L + 1, P)), heating_rate=np.zeros((L, P)))
+            out = {k: np.zeros((L if k == "heating_rate" else L + 1, P)) for k in names}
     d = Diag()
-    d.flux_up = _ptr(out["flux_up"], keep)
-    d.flux_down = _ptr(out["flux_down"], keep)
-    d.flux_direct = _ptr(out["flux_direct"], keep)
-    d.heating_rate = _ptr(out["heating_rate"], keep)
+    for k in _OUTPUTS + _CLEAR_OUTPUTS:
+        setattr(d, k, _ptr(out.get(k), keep))
     d.l_outputs_on_device = int(on_dev)
     if not on_dev:
         for k in out:
-            if not out[k].flags["C_CONTIGUOUS"]:
+            if out[k] is not None and not out[k].flags["C_CONTIGUOUS"]:
                 raise SocgpuError("output %s must be contiguous" % k)
     return a, d, out, keep
 
@@ -249,7 +274,7 @@ def runes_noalloc(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer,
 
 # vertical fields of runes(), [layer or level][profile]
 _VERTICAL = ("density", "layer_heat_capacity", "h2o", "o3", "p_level", "t_level", "cloud_frac",
-             "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr", "liq_conv_dim")
+             "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr", "liq_conv_dim", "co2") + WM_GASES
 
 
 def _flip(a):
@@ -438,6 +463,12 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         i_inhom=case.i_inhom, i_mcica_sampling=case.i_mcica_sampling,
         i_st_water=case.i_st_water, i_cnv_water=case.i_st_water,
         n_subcol_gen=case.n_subcol_gen, rand_seed=g("rand_seed") if cloudy else None, out=out)
+    if case.l_flux_ground:
+        kw.update(l_flux_ground=True, flux_ground=g("flux_ground"))
+    if case.co2 is not None:
+        kw["co2"] = g("co2")
+    for j, x in (case.wm_field or {}).items():
+        kw[WM_GASES[j]] = src.get(WM_GASES[j], x)
     if case.wm_mix_ratio is not None:
         for j, gname in enumerate(WM_GASES):
             if case.wm_mix_ratio[j] != 0.0:

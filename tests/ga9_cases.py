@@ -25,8 +25,12 @@ def _f32(x):
 
 
 def runes_example_case(src):
-    """The runes example's SW call (clear, Rayleigh, grey albedo 0.06) or its
-    first LW call (clear, surface temperature, grey albedo 0)."""
+    """The runes example's SW call (clear, Rayleigh, grey albedo 0.06), its
+    first LW call (clear, surface temperature, grey albedo 0) or ("lw2") its
+    second LW call (the surface emission per band, flux_ground, in place of
+    the surface temperature)."""
+    second = src == "lw2"
+    src = "lw" if second else src
     d = np.load(KGO)
     L = 32
     g, r_gas, cp = _f32(9.80665), _f32(287.026), 1.005e3
@@ -59,6 +63,10 @@ def runes_example_case(src):
     if src == "sw":
         wm[2] = _f32(d["o2_mix_ratio"])
     c.wm_mix_ratio = wm
+    if second:
+        c.l_flux_ground = 1
+        c.flux_ground = _f32(d["flux_ground"]).reshape(nb, 1)
+        c.t_ground = None
     return c
 
 

@@ -3,8 +3,10 @@
 test driver (src/aux/runes_driver.f90:23-143: one mid-latitude-summer
 column, 32 layers, GA9 spectral files) and its known-good outputs
 (examples/runes/{gfortran_8_1_0,ifort_19_0_0,ifx_2025_0_4}.txt: the clear SW
-call and the first clear LW call, flux down, flux up and heating rate at
-levels/layers 1..32, 8 decimals). Data only: the arrays are read from the
+call, the clear LW call with the surface temperature and the clear LW call
+with the surface emission per band (flux_ground, runes_driver.f90:117-119,
+232-252), flux down, flux up and heating rate at levels/layers 1..32, 8
+decimals). Data only: the arrays are read from the
 driver's source text and the KGO files.
 
     python3 tests/golden/make_ga9_kgo.py [--ref /root/reference]
@@ -45,7 +47,8 @@ def main():
     ap.add_argument("--ref", default="/root/reference")
     a = ap.parse_args()
     src = open(os.path.join(a.ref, "src/aux/runes_driver.f90")).read()
-    d = {k: f90_array(src, k) for k in ("p_layer", "t_layer", "p_level", "t_level", "h2o", "o3")}
+    d = {k: f90_array(src, k) for k in ("p_layer", "t_layer", "p_level", "t_level", "h2o", "o3",
+                                         "flux_ground")}
     for k in ("t_ground", "co2_mix_ratio", "ch4_mix_ratio", "n2o_mix_ratio", "o2_mix_ratio",
               "solar_irrad", "grey_albedo_sw", "grey_albedo_lw"):
         d[k] = np.array(f90_scalar(src, k))
@@ -53,6 +56,7 @@ def main():
         secs = kgo_sections(os.path.join(a.ref, "examples/runes", tag + ".txt"))
         d["kgo_sw_" + tag] = secs[0]
         d["kgo_lw_" + tag] = secs[1]
+        d["kgo_lw2_" + tag] = secs[2]
     np.savez(os.path.join(HERE, "kgo_runes_ga9.npz"), **d)
 
 

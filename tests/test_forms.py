@@ -1,0 +1,189 @@
+"""Further forms of runes' inputs and outputs, pinned on the CPU:
+
+* the oracle's CO2 field, per-layer further-gas fields and surface emission
+  per band (l_flux_ground) against the reference's golden vectors
+  (tests/golden/forms_ga9_*.npz, make_golden_forms.py), and its cloud-free
+  call against the reference's clear-sky diagnostics of the cloudy call;
+* the host-model calling form (profile list, strided bottom-up fields with
+  profiles last, gases by field or well-mixed switch): the reference's own
+  outputs in that form equal its plain call's (tests/golden/lfric_*.npz), so
+  those fixtures pin the gathering the binding has to reproduce; leaving out
+  an absorber with set_spectrum's gas flags changes them;
+* the case file's round trip and the C ABI's struct layout as the ctypes
+  and Fortran mirrors declare it.
+"""
+import ctypes
+import dataclasses
+import glob
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from socrates_amd import case_io
+from socrates_amd.case_io import case_from_arrays
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+GA9 = os.path.join(ROOT, "socrates_amd", "data", "ga9")
+SPEC_GA9 = {"sw": os.path.join(GA9, "sp_sw_ga9"), "lw": os.path.join(GA9, "sp_lw_ga9")}
+MCICA_GA9 = os.path.join(GA9, "mcica_data")
+OUTS = ("flux_up", "flux_down", "flux_direct", "heating_rate")
+CLEAR = ("flux_up_clear", "flux_down_clear", "flux_direct_clear")
+
+
+def lfric_parse(a, P, L):
+    """Outputs of runes_lfric_case (socrates_amd/fortran/runes_lfric_case.f90)."""
+    n1, o, out = P * (L + 1), 0, {}
+    for k in OUTS[:3]:
+        out[k] = a[o:o + n1].reshape(L + 1, P)
+        o += n1
+    out["heating_rate"] = a[o:o + P * L].reshape(L, P)
+    o += P * L
+    for k in CLEAR:
+        out[k] = a[o:o + n1].reshape(L + 1, P)
+        o += n1
+    for k in ("up_surf", "down_surf", "direct_surf", "up_toa", "down_toa", "direct_toa",
+              "hr_above", "hr_below"):
+        out[k] = a[o:o + P]
+        o += P
+    out["unused_columns"] = a[o:]
+    return out
+
+
+@pytest.mark.parametrize("src", ["sw", "lw"])
+def test_oracle_forms_match_reference(src):
+    d = np.load(os.path.join(GOLD, "forms_ga9_%s.npz" % src))
+    case = case_from_arrays(d)
+    assert case.co2 is not None and len(case.wm_field) >= 2
+    assert (src == "lw") == bool(case.l_flux_ground)
+    out = oracle.runes(case, SPEC_GA9[src], MCICA_GA9)
+    for k in OUTS:
+        r = d["ref_" + k]
+        assert (np.abs(out[k] - r) <= 1e-13 * np.maximum(np.abs(r), 1.0)).all(), k
+    # clear-sky diagnostics: the same call with cloud off
+    clear = oracle.runes(dataclasses.replace(case, l_cloud=0, i_inhom=0), SPEC_GA9[src])
+    for k in CLEAR:
+        r = d["ref_" + k]
+        assert (np.abs(clear[k[:-6]] - r) <= 1e-13 * np.maximum(np.abs(r), 1.0)).all(), k
+
+
+@pytest.mark.parametrize("src", ["sw", "lw"])
+def test_forms_change_the_fluxes(src):
+    """Each field differs from what its absence would give, so the goldens
+    pin it: the CO2 field, each further-gas field, the surface emission."""
+    case = case_from_arrays(np.load(os.path.join(GOLD, "forms_ga9_%s.npz" % src)))
+    base = oracle.runes(case, SPEC_GA9[src], MCICA_GA9)
+    variants = [dataclasses.replace(case, co2=None)]
+    for j in case.wm_field:
+        variants.append(dataclasses.replace(case, wm_field={i: f for i, f in case.wm_field.items() if i != j}))
+    if case.l_flux_ground:
+        variants.append(dataclasses.replace(case, l_flux_ground=0))
+    for v in variants:
+        out = oracle.runes(v, SPEC_GA9[src], MCICA_GA9)
+        assert np.abs(out["flux_up"] - base["flux_up"]).max() > 1e-6
+
+
+def test_clear_sky_differs_from_cloudy():
+    d = np.load(os.path.join(GOLD, "forms_ga9_sw.npz"))
+    assert np.abs(d["ref_flux_up"] - d["ref_flux_up_clear"]).max() > 1.0
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "lfric_*.npz"))))
+def test_lfric_form_fixtures_equal_plain_call(path):
+    d = np.load(path)
+    case = case_from_arrays(d)
+    P, L = case.n_profile, case.n_layer
+    lf = lfric_parse(d["lfric_raw"], P, L)
+    assert (lf["hr_above"] == 0).all() and (lf["hr_below"] == 0).all()
+    assert (lf["unused_columns"] == -7.5e3).all()
+    assert np.array_equal(lf["up_surf"], lf["flux_up"][-1]) and np.array_equal(lf["down_toa"], lf["flux_down"][0])
+    if int(d["drop"]):
+        plain = np.load(path.replace("_drop%d" % int(d["drop"]), ""))
+        other = lfric_parse(plain["lfric_raw"], P, L)
+        assert np.abs(other["flux_up"] - lf["flux_up"]).max() > 1e-3
+        return
+    for k in OUTS + CLEAR:
+        assert np.array_equal(lf[k], d["ref_" + k]), k
+
+
+def test_case_file_round_trip_of_new_fields():
+    from socrates_amd import synth
+    c = synth.make_case(3, 7, "lw", 9, seed=5, cloudy=True)
+    rng = np.random.default_rng(1)
+    c.co2 = rng.uniform(size=(7, 3))
+    c.wm_field = {1: rng.uniform(size=(7, 3)), 8: rng.uniform(size=(7, 3))}
+    c.l_flux_ground = 1
+    c.flux_ground = rng.uniform(size=(9, 3))
+    with tempfile.TemporaryDirectory() as t:
+        c.write(os.path.join(t, "c"))
+        r = case_io.Case.read(os.path.join(t, "c"))
+    assert np.array_equal(r.co2, c.co2) and np.array_equal(r.flux_ground, c.flux_ground)
+    assert r.l_flux_ground == 1 and sorted(r.wm_field) == [1, 8]
+    assert np.array_equal(r.wm_field[8], c.wm_field[8])
+    arr = case_io.case_to_arrays(c)
+    back = case_from_arrays(arr)
+    assert np.array_equal(back.wm_field[1], c.wm_field[1])
+
+
+_LAYOUT_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "socgpu.h"
+#define F(T, m) printf("%s %s %zu\n", #T, #m, offsetof(T, m));
+int main(void) {
+  printf("SocgpuRunesArgs size %zu\n", sizeof(SocgpuRunesArgs));
+  printf("SocgpuDiag size %zu\n", sizeof(SocgpuDiag));
+  F(SocgpuRunesArgs, co2_mix_ratio) F(SocgpuRunesArgs, p_layer) F(SocgpuRunesArgs, rand_seed)
+  F(SocgpuRunesArgs, gas_mix_ratio) F(SocgpuRunesArgs, l_flux_ground) F(SocgpuRunesArgs, flux_ground)
+  F(SocgpuRunesArgs, co2) F(SocgpuRunesArgs, h2o_mix_ratio) F(SocgpuRunesArgs, gas_field)
+  F(SocgpuDiag, l_outputs_on_device) F(SocgpuDiag, flux_up_clear) F(SocgpuDiag, flux_direct_clear)
+  return 0;
+}
+"""
+
+
+def test_ctypes_structs_match_c_header():
+    from socrates_amd import runes as R
+    with tempfile.TemporaryDirectory() as t:
+        src = os.path.join(t, "l.c")
+        open(src, "w").write(_LAYOUT_C)
+        exe = os.path.join(t, "l")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        lines = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    types = {"SocgpuRunesArgs": R.RunesArgs, "SocgpuDiag": R.Diag}
+    for ln in filter(None, lines):
+        t, m, v = ln.split()
+        if m == "size":
+            assert ctypes.sizeof(types[t]) == int(v), t
+        else:
+            assert getattr(types[t], m).offset == int(v), (t, m)
+
+
+_LAYOUT_F = """
+program layout
+  use, intrinsic :: iso_c_binding
+  use socrates_gpu_c, only: socgpu_runes_args, socgpu_diag
+  type(socgpu_runes_args) :: a
+  type(socgpu_diag) :: d
+  print '(i0, 1x, i0)', c_sizeof(a), c_sizeof(d)
+end program layout
+"""
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/bin/amdflang"), reason="amdflang not present")
+def test_fortran_bind_c_types_match_c_header():
+    from socrates_amd import runes as R
+    mod = os.path.join(ROOT, "socrates_amd", "lib", "fortran")
+    if not os.path.exists(os.path.join(mod, "socrates_gpu_c.mod")):
+        pytest.skip("Fortran binding not built")
+    with tempfile.TemporaryDirectory() as t:
+        src = os.path.join(t, "l.f90")
+        open(src, "w").write(_LAYOUT_F)
+        exe = os.path.join(t, "l")
+        subprocess.run(["/opt/rocm/bin/amdflang", "-I", mod, src, "-o", exe], check=True, capture_output=True)
+        a, d = (int(x) for x in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split())
+    assert a == ctypes.sizeof(R.RunesArgs) and d == ctypes.sizeof(R.Diag)

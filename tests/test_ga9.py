@@ -22,9 +22,11 @@ from socrates_amd.case_io import case_from_arrays
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("src", ["sw", "lw"])
+@pytest.mark.parametrize("src", ["sw", "lw", "lw2"])
 def test_oracle_reproduces_runes_example(src):
-    out = oracle.runes(runes_example_case(src), SPEC_GA9[src])
+    """All three sections of the example: SW, LW with the surface
+    temperature, LW with the surface emission per band (flux_ground)."""
+    out = oracle.runes(runes_example_case(src), SPEC_GA9["sw" if src == "sw" else "lw"])
     err = np.abs(example_table(out) - kgo(src))
     # printed to 8 decimals: half a unit of the last place, plus rounding
     assert err.max() <= 6e-9, err.max(axis=0)
@@ -48,6 +50,15 @@ def test_golden_cases_carry_every_further_gas():
     non-zero amount, so a dropped or misrouted gas changes the result."""
     for path in glob.glob(os.path.join(GOLD, "ga9_runes_*.npz")):
         assert (np.load(path)["case_wm_mix_ratio"] > 0).all()
+
+
+def test_flux_ground_section_differs_from_skin_temperature():
+    """The example's surface emission per band is close to the Planck flux
+    of its surface temperature, but the two LW KGO sections still differ by
+    some 400 units of their last printed place, so reproducing the second
+    to 6e-9 pins the flux_ground path (its division by the diffuse
+    co-albedo is pinned by the forms goldens, the example's albedo being 0)."""
+    assert np.abs(kgo("lw2")[:, 1] - kgo("lw")[:, 1]).max() > 1e-6
 
 
 def test_continuum_and_further_gases_matter():

@@ -44,9 +44,11 @@ def ga9_case(src, kind, P, L, seed):
     return c
 
 
-@pytest.mark.parametrize("src", ["sw", "lw"])
+@pytest.mark.parametrize("src", ["sw", "lw", "lw2"])
 def test_runes_example_on_gpu(src):
-    out = R.runes_case(runes_example_case(src), NAME[src], precision="fp64")
+    """The example's three sections, the third with the surface emission
+    per band (flux_ground)."""
+    out = R.runes_case(runes_example_case(src), NAME["sw" if src == "sw" else "lw"], precision="fp64")
     err = np.abs(example_table(out) - kgo(src))
     assert err.max() <= 6e-9, err.max(axis=0)
 
