@@ -116,6 +116,16 @@ typedef struct SocgpuRunesArgs {
    * mass mixing ratio, negative values taken as 0; runes' n2o, ch4, o2, ...
    * and their _1d forms, socrates_runes.F90:51-64); NULL: gas_mix_ratio. */
   const double *gas_field[10];
+  /* Ice (socrates_runes.F90:118-121, set_cld and set_cld_dim): the ice types
+   * of the stratiform (or homogeneous) and convective cloud (i_st_ice,
+   * i_cnv_ice; the spectral file's types of Fu's phase-function fit, Baran's
+   * fit or Pade-2; i_cnv_ice <= 0 takes i_st_ice) and their gridbox-mean
+   * mass mixing ratios and dimensions ([layer][profile], NULL: zero).
+   * i_st_ice <= 0: no ice component (the reference needs an ice type; with
+   * one whose fit is zero at zero ice, Fu's or Pade-2, the results are the
+   * same; Baran's fit adds a small extinction even at zero ice). */
+  int i_st_ice, i_cnv_ice;
+  const double *ice_mmr, *ice_dim, *ice_conv_mmr, *ice_conv_dim;
 } SocgpuRunesArgs;
 
 #define SOCGPU_GAS_N2O 0

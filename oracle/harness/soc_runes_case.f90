@@ -42,7 +42,7 @@ program soc_runes_case
   ! (version 5), unallocated when not given
   real(RealExt), allocatable, dimension(:,:) :: f_n2o, f_ch4, f_o2, f_so2, &
     f_ocs, f_cfc11, f_cfc12, f_cfc113, f_hcfc22, f_hfc134a
-  integer :: lwf(10)
+  integer :: lwf(10), lice
   integer, allocatable :: rand_seed(:)
   real(RealExt), allocatable, target, dimension(:,:) :: flux_up, flux_down, &
     flux_direct, heating_rate, flux_up_clear, flux_down_clear, &
@@ -113,6 +113,10 @@ program soc_runes_case
     end if
   end if
   lwf = 0
+  ice_mmr = 0.0_RealExt
+  ice_dim = 3.0e-5_RealExt
+  ice_conv_mmr = 0.0_RealExt
+  ice_conv_dim = 3.0e-5_RealExt
   if (version >= 5) then
     read(iu) lwf
     call get_field(lwf(1), f_n2o)
@@ -126,11 +130,11 @@ program soc_runes_case
     call get_field(lwf(9), f_hcfc22)
     call get_field(lwf(10), f_hfc134a)
   end if
+  if (version >= 6) then
+    read(iu) lice
+    if (lice /= 0) read(iu) ice_mmr, ice_dim, ice_conv_mmr, ice_conv_dim
+  end if
   close(iu)
-  ice_mmr = 0.0_RealExt
-  ice_dim = 3.0e-5_RealExt
-  ice_conv_mmr = 0.0_RealExt
-  ice_conv_dim = 3.0e-5_RealExt
 
   allocate(flux_up(n_profile, 0:n_layer), flux_down(n_profile, 0:n_layer), &
     flux_direct(n_profile, 0:n_layer), heating_rate(n_profile, n_layer))

@@ -29,7 +29,9 @@ class _Case(ctypes.Structure):
         "albedo_dir", "cloud_frac", "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr",
         "liq_conv_dim")] + [("rand_seed", _I), ("wm_mix_ratio", ctypes.c_double * 10),
                             ("l_flux_ground", ctypes.c_int), ("flux_ground", _D), ("co2", _D),
-                            ("wm_field", _D * 10)]
+                            ("wm_field", _D * 10), ("i_st_ice", ctypes.c_int),
+                            ("ice_mmr", _D), ("ice_dim", _D), ("ice_conv_mmr", _D),
+                            ("ice_conv_dim", _D)]
 
 
 def build():

@@ -39,6 +39,7 @@ static double jit_exp(double x) {
 #define NTMP 8     /* temperatures of the k-term look-up table */
 #define NDROP 16   /* droplet types */
 #define NPARM 20   /* parameters per droplet fit */
+#define NICE 16    /* ice types */
 
 /* option codes, rad_pcf.F90 */
 enum { IP_SOLAR = 1, IP_INFRA_RED = 2 };
@@ -55,6 +56,7 @@ enum { IP_SELF_CONTINUUM = 1, IP_FRN_CONTINUUM = 2 };
 static const double MOL_WEIGHT_AIR = 28.966e-03;
 static const double REPSILON = 18.0153 / 28.966;
 enum { IP_DROP_PADE_2 = 5, IP_SLINGO_SCHRECKER = 1 };
+enum { IP_ICE_FU_PHF = 12, IP_ICE_BARAN = 19, IP_ICE_PADE_2_PHF = 20 };
 
 static const double ELSASSER = 1.66;  /* diffusivity_factor.F90 */
 static const double DIFF_MINOR = 1.66;
@@ -90,6 +92,10 @@ typedef struct {
   int l_drop_type[NDROP], i_drop_parm[NDROP], n_drop_parm[NDROP];
   double drop_min[NDROP], drop_max[NDROP];
   double drop_parm[NDROP][NB][NPARM];
+  /* ice crystals (block 12, read_spectrum.F90:1175-1262) */
+  int l_ice_type[NICE], i_ice_parm[NICE], n_ice_parm[NICE];
+  double ice_min[NICE], ice_max[NICE];
+  double ice_parm[NICE][NB][NPARM];
 } spec_t;
 
 typedef struct {
@@ -341,6 +347,40 @@ static int read_spectrum(const char *path, spec_t *s) {
           }
         }
         for (int k = 0; k <= s->n_deg_fit; k++) s->thermal_coeff[b][k] = vals[k + 1];
+      }
+    } else if (t == 12) {
+      /* ice: the same layout as the droplet block, the type number after
+       * its '=' */
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln); int iice = atoi(strchr(ln, '=') + 1);
+      next_line(f, ln, sizeof ln);
+      int iparm = (int)fld(ln, 34, 5), npar = (int)fld(ln, 66, 5);
+      next_line(f, ln, sizeof ln);
+      next_line(f, ln, sizeof ln);
+      /* a type whose fit is larger than the restatement keeps is read past
+       * and left unset (a call that names it is refused) */
+      const int keep = iice >= 1 && iice < NICE && npar <= NPARM;
+      if (keep) {
+        s->ice_min[iice] = fld(ln, 39, 12);
+        s->ice_max[iice] = fld(ln, 55, 12);
+        s->l_ice_type[iice] = 1;
+        s->i_ice_parm[iice] = iparm;
+        s->n_ice_parm[iice] = npar;
+      }
+      for (int b = 0; b < s->n_band; b++) {
+        next_line(f, ln, sizeof ln);
+        int got = 0;
+        while (got < npar) {
+          next_line(f, ln, sizeof ln);
+          char *p = ln, *e;
+          for (;;) {
+            double x = strtod(p, &e);
+            if (e == p) break;
+            if (keep) s->ice_parm[iice][b][got] = x;
+            got++;
+            p = e;
+          }
+        }
       }
     } else if (t == 10) {
       next_line(f, ln, sizeof ln);
@@ -769,6 +809,62 @@ static void pade_2(const double *cp, double mmr, double re, double *k_ext_tot,
   *fwd = *phf * asym;
 }
 
+/* Droplet optics by the type's scheme: Pade-2 (above), or Slingo-Schrecker
+ * (opt_prop_water_cloud.F90:231-247, forward scattering k_scat g^2 with the
+ * power formed first, :292-298) */
+static void drop_optics(const spec_t *sp, int type, int b, double mmr, double re, double *k_ext_tot,
+                        double *k_ext_scat, double *phf, double *fwd) {
+  const double *cp = sp->drop_parm[type][b];
+  if (sp->i_drop_parm[type] == IP_SLINGO_SCHRECKER) {
+    *k_ext_tot = mmr * (cp[0] + cp[1] / re);
+    *k_ext_scat = *k_ext_tot * (1.0 - cp[2] - cp[3] * re);
+    double asym = cp[4] + cp[5] * re;
+    *phf = *k_ext_scat * asym;
+    *fwd = *k_ext_scat * (asym * asym);
+  } else {
+    pade_2(cp, mmr, re, k_ext_tot, k_ext_scat, phf, fwd);
+  }
+}
+
+/* Ice optics by the type's scheme, for the two-stream case of
+ * opt_prop_ice_cloud.F90:227-256 (one phase-function moment, rescaled,
+ * forward scattering of order 2): Fu's phase-function fit
+ * (opt_prop_fu_phf_mod.F90:93-130), Baran's temperature fit
+ * (opt_prop_baran_mod.F90:93-125, t**4 formed left to right as the compiled
+ * reference forms it) or Pade-2 (the droplet form). t: the layer
+ * temperature. */
+static void ice_optics(const spec_t *sp, int type, int b, double iwc, double de, double t,
+                       double *k_ext_tot, double *k_ext_scat, double *phf, double *fwd) {
+  const double *cp = sp->ice_parm[type][b];
+  switch (sp->i_ice_parm[type]) {
+    case IP_ICE_FU_PHF: {
+      double x = cp[3] / de;
+      *k_ext_tot = iwc * ((cp[2] * x + cp[1]) * x + cp[0]);
+      x = de / cp[8];
+      *k_ext_scat = *k_ext_tot * (1.0 - (cp[4] + x * (cp[5] + x * (cp[6] + x * cp[7]))));
+      x = de / cp[13];
+      double asym = cp[9] + x * (cp[10] + x * (cp[11] + x * cp[12]));
+      *phf = *k_ext_scat * asym;
+      *fwd = *phf * asym;
+      break;
+    }
+    case IP_ICE_BARAN: {
+      double x = fmin(iwc + 1.0e-12, 1.0e-01);
+      *k_ext_tot = cp[0] * x / (t * t * t * t);
+      x = fmin(iwc + 1.0e-12, 1.0e-03 - DBL_EPSILON);
+      double om = fmin(cp[1] + (cp[2] * x * t), 1.0 - DBL_EPSILON);
+      *k_ext_scat = *k_ext_tot * om;
+      double asym = fmin(cp[3] + (cp[4] * x * t), 1.0 - DBL_EPSILON);
+      *phf = *k_ext_scat * asym;
+      *fwd = *phf * asym;
+      break;
+    }
+    default:
+      pade_2(cp, iwc, de, k_ext_tot, k_ext_scat, phf, fwd);
+      break;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Triple-column cloud overlap: clear, stratiform and convective regions     */
 /* (ip_cloud_conv_strat with maximum-random overlap, ip_cloud_triple, and    */
@@ -1147,6 +1243,9 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
 #define LAY(a, i, l) (a)[(size_t)(i) * P + (l)]     /* i: 0-based layer */
   /* --- set_cld / set_cld_dim / set_cld_mcica inputs (McICA) ----------- */
   double *w_cloud = NULL, *cond_mmr = NULL, *re = NULL, *c_sub = NULL, *frac_cloudy = NULL;
+  double *ice_in = NULL, *de = NULL;   /* in-cloud ice and its clamped dimension */
+  const int ice = c->l_cloud && c->i_st_ice > 0;
+  if (ice && (c->i_st_ice >= NICE || !sp.l_ice_type[c->i_st_ice])) return -12;
   int *n_subcol_cld = NULL, n_req = 1, n_gen = 1, *subcol_reorder = NULL;
   int first_subcol_k[NB][NK + 1];
   if (mcica) {
@@ -1154,6 +1253,8 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     w_cloud = (double *)calloc((size_t)L * P, sizeof(double));
     cond_mmr = (double *)calloc((size_t)L * P, sizeof(double));
     re = (double *)calloc((size_t)L * P, sizeof(double));
+    ice_in = (double *)calloc((size_t)L * P, sizeof(double));
+    de = (double *)calloc((size_t)L * P, sizeof(double));
     double *rsd = (double *)calloc((size_t)L * P, sizeof(double));
     double *dp = (double *)calloc((size_t)L * P, sizeof(double));
     double *dpc = (double *)calloc((size_t)L * P, sizeof(double));
@@ -1175,10 +1276,19 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         /* effective radius clamp (socrates_set_cld_dim.F90:214-220) */
         double r = fmax(LAY(c->liq_dim, i, l), 0.0);
         LAY(re, i, l) = fmin(fmax(r, sp.drop_min[it]), sp.drop_max[it]);
-        /* cond_rsd (socrates_set_cld_mcica.F90:168-180): liquid, then the ice
-         * component whose mixing ratio is zero; both carry the horizontal rsd */
+        /* the ice component likewise (in-cloud mixing ratio over the same
+         * type fraction, dimension clamped to the ice type's range) */
+        double mi = (ice && c->ice_mmr) ? fmax(LAY(c->ice_mmr, i, l), 0.0) : 0.0;
+        double i_in = frac > eps ? mi / frac : 0.0;
+        LAY(ice_in, i, l) = i_in;
+        if (ice) {
+          double d = c->ice_dim ? fmax(LAY(c->ice_dim, i, l), 0.0) : 0.0;
+          LAY(de, i, l) = fmin(fmax(d, sp.ice_min[c->i_st_ice]), sp.ice_max[c->i_st_ice]);
+        }
+        /* cond_rsd (socrates_set_cld_mcica.F90:168-180): liquid, then ice;
+         * both carry the horizontal rsd */
         double wl = fmax(eps, m_in * fr1);
-        double wi = fmax(eps, 0.0 * fr1);
+        double wi = fmax(eps, i_in * fr1);
         double sum_r = 0.0 + c->cloud_horizontal_rsd * wl;
         sum_r = sum_r + c->cloud_horizontal_rsd * wi;
         double sum_w = 0.0 + wl;
@@ -1214,6 +1324,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
 
   /* --- set_cld / set_cld_dim for the triple-column representation ----- */
   double *tw = NULL, *tfs = NULL, *tfc = NULL, *tms = NULL, *tmc = NULL, *trs = NULL, *trc = NULL;
+  double *tis = NULL, *tic = NULL, *tds = NULL, *tdc = NULL;   /* ice of each type */
   double *ovl = NULL;
   int nct = L;
   if (triple) {
@@ -1221,6 +1332,8 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     tw = (double *)calloc((size_t)7 * L * P, sizeof(double));
     tfs = tw + (size_t)L * P; tfc = tfs + (size_t)L * P; tms = tfc + (size_t)L * P;
     tmc = tms + (size_t)L * P; trs = tmc + (size_t)L * P; trc = trs + (size_t)L * P;
+    tis = (double *)calloc((size_t)4 * L * P, sizeof(double));
+    tic = tis + (size_t)L * P; tds = tic + (size_t)L * P; tdc = tds + (size_t)L * P;
     const int its = c->i_st_water, itc = c->i_st_water;  /* the case carries one droplet type for both */
     for (int i = 0; i < L; i++)
       for (int l = 0; l < P; l++) {
@@ -1234,6 +1347,16 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         /* in-cloud mixing ratios (:817-826) */
         LAY(tms, i, l) = fs > eps ? ms / fs : 0.0;
         LAY(tmc, i, l) = fcv > eps ? mcv / fcv : 0.0;
+        if (ice) {
+          double is = c->ice_mmr ? fmax(LAY(c->ice_mmr, i, l), 0.0) : 0.0;
+          double icv = (conv_strat && c->ice_conv_mmr) ? fmax(LAY(c->ice_conv_mmr, i, l), 0.0) : 0.0;
+          LAY(tis, i, l) = fs > eps ? is / fs : 0.0;
+          LAY(tic, i, l) = fcv > eps ? icv / fcv : 0.0;
+          double ds = c->ice_dim ? fmax(LAY(c->ice_dim, i, l), 0.0) : 0.0;
+          double dc = (conv_strat && c->ice_conv_dim) ? fmax(LAY(c->ice_conv_dim, i, l), 0.0) : 0.0;
+          LAY(tds, i, l) = fmin(fmax(ds, sp.ice_min[c->i_st_ice]), sp.ice_max[c->i_st_ice]);
+          LAY(tdc, i, l) = fmin(fmax(dc, sp.ice_min[c->i_st_ice]), sp.ice_max[c->i_st_ice]);
+        }
         /* w_cloud and normalised type fractions (:828-852) */
         double wc = fs + fcv;
         if (wc > min_cloud_fraction) { fs = fs / wc; fcv = fcv / wc; }
@@ -1295,6 +1418,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   double *adjust = TAKE(L), *kmin = TAKE(L), *tau_clr = TAKE(L), *om_clr = TAKE(L);
   double *planck = TAKE(L + 1), *dpl = TAKE(L), *dpl2 = TAKE(L), *flux_gas = TAKE(L + 1);
   double *kc_tot = TAKE(L), *kc_scat = TAKE(L), *kc_phf = TAKE(L), *kc_fwd = TAKE(L);
+  double *ki_tot = TAKE(L), *ki_scat = TAKE(L), *ki_phf = TAKE(L), *ki_fwd = TAKE(L);
   double *sum_flux = TAKE(2 * L + 3), *sum_k_flux = TAKE(2 * L + 3);
   double *ft = TAKE(2 * L + 2), *tg = TAKE(L), *src = TAKE(L);
   double *kg0 = TAKE(L);  /* clear-air grey extinction before k_eqv */
@@ -1513,14 +1637,19 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       for (int i = 0; i < L; i++) k_grey_tot[i] = k_grey_tot[i] + keqv[i] + kgrey[i];
 
       /* cloud component optics (opt_prop_water_cloud.F90:212-300 with the
-       * Padé-2 branch opt_prop_pade_2_mod.F90:100-143) */
+       * Padé-2 branch opt_prop_pade_2_mod.F90:100-143; opt_prop_ice_cloud.F90
+       * :227-256 for the ice) */
       if (mcica) {
         const int it = c->i_st_water;
         for (int i = 0; i < L; i++) {
           if (LAY(w_cloud, i, l) > 0.0)
-            pade_2(sp.drop_parm[it][b], LAY(cond_mmr, i, l), LAY(re, i, l), &kc_tot[i],
-                   &kc_scat[i], &kc_phf[i], &kc_fwd[i]);
+            drop_optics(&sp, it, b, LAY(cond_mmr, i, l), LAY(re, i, l), &kc_tot[i],
+                        &kc_scat[i], &kc_phf[i], &kc_fwd[i]);
           else kc_tot[i] = kc_scat[i] = kc_phf[i] = kc_fwd[i] = 0.0;
+          if (ice && LAY(w_cloud, i, l) > 0.0)
+            ice_optics(&sp, c->i_st_ice, b, LAY(ice_in, i, l), LAY(de, i, l), LAY(c->t_layer, i, l),
+                       &ki_tot[i], &ki_scat[i], &ki_phf[i], &ki_fwd[i]);
+          else ki_tot[i] = ki_scat[i] = ki_phf[i] = ki_fwd[i] = 0.0;
         }
       }
 
@@ -1533,6 +1662,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         for (int k = 0; k < 2; k++) {
           const int it = k == 0 ? c->i_st_water : c->i_st_water;
           const double *mm = k == 0 ? tms : tmc, *rr = k == 0 ? trs : trc;
+          const double *im = k == 0 ? tis : tic, *id = k == 0 ? tds : tdc;
           for (int i = 0; i < L; i++) {
             const double ray = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
             /* every type starts from the clear-air totals (grey_opt_prop.F90:
@@ -1540,15 +1670,20 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
             double kgt = kg0[i], ksc = ray, phf = 0.0, fwd = 0.0;
             if (i + 1 >= nct && LAY(tw, i, l) > 0.0) {
               double ct, cs, cp, cf;
-              pade_2(sp.drop_parm[it][b], LAY(mm, i, l), LAY(rr, i, l), &ct, &cs, &cp, &cf);
+              drop_optics(&sp, it, b, LAY(mm, i, l), LAY(rr, i, l), &ct, &cs, &cp, &cf);
               kgt = kgt + ct;
               ksc = ksc + cs;
               phf = phf + cp;
               fwd = fwd + cf;
-              kgt = kgt + 0.0;   /* the type's ice component, zero mixing ratio */
-              ksc = ksc + 0.0;
-              phf = phf + 0.0;
-              fwd = fwd + 0.0;
+              /* the type's ice component */
+              ct = cs = cp = cf = 0.0;
+              if (ice)
+                ice_optics(&sp, c->i_st_ice, b, LAY(im, i, l), LAY(id, i, l), LAY(c->t_layer, i, l),
+                           &ct, &cs, &cp, &cf);
+              kgt = kgt + ct;
+              ksc = ksc + cs;
+              phf = phf + cp;
+              fwd = fwd + cf;
             }
             kgt = kgt + keqv[i] + kgrey[i];
             if (ksc > tiny) {
@@ -1646,13 +1781,13 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
               if (!(cs * frac1 > 0.0)) continue;
               /* cloudy single-scattering (mcica_sample.F90:614-700) */
               double kgt = k_grey_tot[i] + kc_tot[i] * cs;
-              kgt = kgt + 0.0 * cs;
+              kgt = kgt + ki_tot[i] * cs;
               double ksc = k_ext_scat[i] + kc_scat[i] * cs;
-              ksc = ksc + 0.0 * cs;
+              ksc = ksc + ki_scat[i] * cs;
               double fwd = 0.0 + kc_fwd[i] * cs;
-              fwd = fwd + 0.0 * cs;
+              fwd = fwd + ki_fwd[i] * cs;
               double phf = 0.0 + kc_phf[i] * cs;
-              phf = phf + 0.0 * cs;
+              phf = phf + ki_phf[i] * cs;
               phf = (phf - fwd) / fmax(ksc - fwd, eps);
               fwd = fwd / fmax(ksc, eps);
               double k_total = kgt + KABS(0, iex, i);
@@ -1725,9 +1860,10 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
     }
   }
   free(kabs); free(buf); free(lu_jp); free(lu_fac);
-  if (triple) { free(tw); free(ovl); free(tbuf); }
+  if (triple) { free(tw); free(tis); free(ovl); free(tbuf); }
   if (mcica) {
     free(w_cloud); free(cond_mmr); free(re); free(c_sub); free(n_subcol_cld);
+    free(ice_in); free(de);
     free(frac_cloudy); free(subcol_reorder); free(mc.xcw); mc.xcw = NULL;
   }
 #undef LAY

@@ -73,6 +73,12 @@ typedef struct {
   /* fields of the further gases by SOC_WM_* slot ([layer][profile], NULL:
    * the well-mixed wm_mix_ratio) */
   const double *wm_field[10];
+  /* ice (socrates_runes.F90:118-121): in-cloud-type stratiform and
+   * convective ice mass mixing ratios and dimensions ([layer][profile];
+   * NULL: zero) and the ice type of both (i_st_ice = i_cnv_ice); type 0:
+   * no ice component */
+  int i_st_ice;
+  const double *ice_mmr, *ice_dim, *ice_conv_mmr, *ice_conv_dim;
 } soc_case;
 
 enum { SOC_WM_N2O, SOC_WM_CH4, SOC_WM_O2, SOC_WM_SO2, SOC_WM_OCS, SOC_WM_CFC11, SOC_WM_CFC12,

@@ -14,10 +14,12 @@ import dataclasses
 import numpy as np
 
 MAGIC = 1397703491
-VERSION = 5     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
+VERSION = 6     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
                 # 3 adds l_flux_ground and flux_ground (n_band, n_profile) after them;
                 # 4 adds a flag and, when it is set, a CO2 field (n_layer, n_profile);
-                # 5 adds a flag per WM_GASES gas and, for each one set, its field
+                # 5 adds a flag per WM_GASES gas and, for each one set, its field;
+                # 6 adds a flag and, when it is set, ice_mmr, ice_dim, ice_conv_mmr,
+                #   ice_conv_dim (n_layer, n_profile)
 
 # further gases runes takes as well-mixed mixing ratios (the *_mix_ratio
 # arguments, socrates_runes.F90:69-83), in this order
@@ -34,6 +36,7 @@ _COLUMN = ("t_ground", "cos_zen", "solar_irrad")
 _BAND = ("albedo_diff", "albedo_dir")
 _CLOUD = ("cloud_frac", "liq_mmr", "liq_dim")
 _CONV = ("conv_frac", "liq_conv_mmr", "liq_conv_dim")
+_ICE = ("ice_mmr", "ice_dim", "ice_conv_mmr", "ice_conv_dim")
 
 
 @dataclasses.dataclass
@@ -82,6 +85,10 @@ class Case:
     flux_ground: np.ndarray = None    # (n_band, n_profile) W m-2
     co2: np.ndarray = None            # (n_layer, n_profile) CO2 mmr; None: co2_mix_ratio
     wm_field: dict = None             # {WM_GASES index: (n_layer, n_profile) field}
+    ice_mmr: np.ndarray = None        # (n_layer, n_profile); None: no ice condensate
+    ice_dim: np.ndarray = None
+    ice_conv_mmr: np.ndarray = None
+    ice_conv_dim: np.ndarray = None
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile
@@ -131,6 +138,14 @@ class Case:
             for j in range(len(WM_GASES)):
                 if j in fields:
                     np.ascontiguousarray(fields[j], dtype=np.float64).tofile(f)
+            has_ice = self.ice_mmr is not None
+            np.array([int(has_ice)], dtype=np.int32).tofile(f)
+            if has_ice:
+                L, P = self.n_layer, self.n_profile
+                for k in _ICE:
+                    v = getattr(self, k)
+                    v = np.zeros((L, P)) if v is None else v
+                    np.ascontiguousarray(v, dtype=np.float64).tofile(f)
 
     @classmethod
     def read(cls, path):
@@ -178,6 +193,14 @@ class Case:
                 if flags[j]:
                     c.wm_field[j] = np.frombuffer(raw, dtype=np.float64, count=L * P,
                                                   offset=off).reshape(L, P).copy()
+                    off += 8 * L * P
+        if hdr[1] >= 6:
+            has_ice = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
+            off += 4
+            if has_ice:
+                for k in _ICE:
+                    setattr(c, k, np.frombuffer(raw, dtype=np.float64, count=L * P,
+                                                offset=off).reshape(L, P).copy())
                     off += 8 * L * P
         return c
 

@@ -228,7 +228,8 @@ SG_HD void pade2(const double *cp, double mmr, double re, double &kext, double &
   fwd = phf * g;
 }
 
-// Slingo-Schrecker droplet optics (opt_prop_water_cloud.F90:231-247)
+// Slingo-Schrecker droplet optics (opt_prop_water_cloud.F90:231-247); the
+// forward scattering k_scat g**2 forms the power first (:292-298)
 SG_HD void slingo(const double *cp, double mmr, double re, double &kext, double &kscat,
                   double &phf, double &fwd) {
 #pragma clang fp contract(off)
@@ -236,7 +237,47 @@ SG_HD void slingo(const double *cp, double mmr, double re, double &kext, double 
   kscat = kext * (1.0 - cp[2] - cp[3] * re);
   double g = cp[4] + cp[5] * re;
   phf = kscat * g;
-  fwd = kscat * g * g;
+  fwd = kscat * (g * g);
+}
+
+// Droplet optics by the type's scheme (5 Pade-2, 1 Slingo-Schrecker).
+SG_HD void drop_optics(const SgDrop &dr, int b, double mmr, double re, double &kext, double &kscat,
+                       double &phf, double &fwd) {
+  if (dr.scheme == 5) pade2(dr.parm[b], mmr, re, kext, kscat, phf, fwd);
+  else slingo(dr.parm[b], mmr, re, kext, kscat, phf, fwd);
+}
+
+// Ice optics of the two-stream case of opt_prop_ice_cloud.F90:227-256 (one
+// phase-function moment, rescaled, forward scattering of order 2): Fu's
+// phase-function fit (scheme 12, opt_prop_fu_phf_mod.F90:93-130), Baran's
+// temperature fit (scheme 19, opt_prop_baran_mod.F90:93-125, t**4 formed
+// left to right) or Pade-2 (scheme 20, the droplet form). t: the layer
+// temperature.
+SG_HD void ice_optics(const SgDrop &ic, int b, double iwc, double de, double t, double &kext,
+                      double &kscat, double &phf, double &fwd) {
+#pragma clang fp contract(off)
+  const double *cp = ic.parm[b];
+  if (ic.scheme == 12) {
+    double x = cp[3] / de;
+    kext = iwc * ((cp[2] * x + cp[1]) * x + cp[0]);
+    x = de / cp[8];
+    kscat = kext * (1.0 - (cp[4] + x * (cp[5] + x * (cp[6] + x * cp[7]))));
+    x = de / cp[13];
+    const double g = cp[9] + x * (cp[10] + x * (cp[11] + x * cp[12]));
+    phf = kscat * g;
+    fwd = phf * g;
+  } else if (ic.scheme == 19) {
+    double x = fmin(iwc + 1.0e-12, 1.0e-01);
+    kext = cp[0] * x / (t * t * t * t);
+    x = fmin(iwc + 1.0e-12, 1.0e-03 - kEps);
+    const double om = fmin(cp[1] + (cp[2] * x * t), 1.0 - kEps);
+    kscat = kext * om;
+    const double g = fmin(cp[3] + (cp[4] * x * t), 1.0 - kEps);
+    phf = kscat * g;
+    fwd = phf * g;
+  } else {
+    pade2(cp, iwc, de, kext, kscat, phf, fwd);
+  }
 }
 
 // McICA cloud-generator linear congruential step (socrates_cloud_gen.F90:48-50

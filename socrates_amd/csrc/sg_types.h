@@ -17,6 +17,7 @@
 #define SG_MAXSUB 192       // McICA sub-column items per file (sum of subcol_k)
 #define SG_NPL 8            // Planck polynomial coefficients
 #define SG_NDROP 8          // droplet types kept
+#define SG_NICE 16          // ice types kept
 #define SG_NDPARM 16        // droplet fit parameters
 #define SG_MAXL 160         // layers per column
 #define SG_MAXSUBCOL 64     // generated McICA sub-columns per column
@@ -97,6 +98,8 @@ struct SgSpec {
   SgTerm g[SG_MAXG];
   SgTerm minor[SG_MAXMINOR];
   SgDrop drop[SG_NDROP];
+  SgDrop ice[SG_NICE];           // ice crystals (block 12): the same record,
+                                 // scheme = the ice parametrisation
   // McICA sampling (set_cld_mcica, socrates_set_cld_mcica.F90:103-151) for
   // [0] single and [1] optimal sampling
   int has_mcica, n_gen, ipph, n1, n2;
@@ -130,7 +133,13 @@ struct SgSpec {
 // parameters of the layer and the one above (in the triple-column slots)
 #define SG_Q_ALPHA 14
 #define SG_Q_RCORR 15
-#define SG_NQ 17
+// ice of the stratiform (or homogeneous) and the convective type: in-cloud
+// mass mixing ratio and the dimension clamped to the ice type's range
+#define SG_Q_IWC 17
+#define SG_Q_DE 18
+#define SG_Q_IWC2 19
+#define SG_Q_DE2 20
+#define SG_NQ 21
 
 // per-layer gas record of calls whose spectrum has look-up-table terms or
 // continua, [col][layer][SG_NGQ]: the look-up interpolation factors

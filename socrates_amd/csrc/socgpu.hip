@@ -175,6 +175,10 @@ struct CallDims {
   int conv;          // convective cloud fields are read (ip_cloud_conv_strat
                      // only: the reference ignores them for ip_cloud_homogen,
                      // socrates_set_cld.F90:477-495)
+  int ice;           // an ice component (i_st_ice given); its types, 0-based:
+  int ice_t, ice_t2; // stratiform (or homogeneous) and convective
+  int co_stride;     // doubles per layer of the McICA cloud optics (4, or 8
+                     // with the ice beside the droplets in the FP64 form)
 };
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
@@ -226,6 +230,19 @@ __device__ __forceinline__ bool pack_layer(const CallDims &d, const SgSpec *__re
       const SgDrop &ds = spec->drop[d.drop], &dc = spec->drop[d.drop2];
       re = fmin(fmax(a.liq_dim ? fmax(a.liq_dim[idx], 0.0) : 0.0, ds.dim_min), ds.dim_max);
       re2 = fmin(fmax((d.conv && a.liq_conv_dim) ? fmax(a.liq_conv_dim[idx], 0.0) : 0.0, dc.dim_min), dc.dim_max);
+      if (d.ice) {
+        // each type's ice over its own fraction, dimensions clamped to the
+        // ice types' ranges (socrates_set_cld.F90:341-415, 817-826;
+        // socrates_set_cld_dim.F90)
+        const double is = a.ice_mmr ? fmax(a.ice_mmr[idx], 0.0) : 0.0;
+        const double ic = (d.conv && a.ice_conv_mmr) ? fmax(a.ice_conv_mmr[idx], 0.0) : 0.0;
+        const SgDrop &es = spec->ice[d.ice_t], &ec = spec->ice[d.ice_t2];
+        r[SG_Q_IWC] = fs > eps ? is / fs : 0.0;
+        r[SG_Q_IWC2] = fc > eps ? ic / fc : 0.0;
+        r[SG_Q_DE] = fmin(fmax(a.ice_dim ? fmax(a.ice_dim[idx], 0.0) : 0.0, es.dim_min), es.dim_max);
+        r[SG_Q_DE2] = fmin(fmax((d.conv && a.ice_conv_dim) ? fmax(a.ice_conv_dim[idx], 0.0) : 0.0, ec.dim_min),
+                           ec.dim_max);
+      }
     } else if (a.cloud_frac) {
       const double eps = kEps;
       double frac = fmax(a.cloud_frac[idx], 0.0);
@@ -240,9 +257,18 @@ __device__ __forceinline__ bool pack_layer(const CallDims &d, const SgSpec *__re
       const SgDrop &dr = spec->drop[d.drop];
       double r0 = a.liq_dim ? fmax(a.liq_dim[idx], 0.0) : 0.0;
       re = fmin(fmax(r0, dr.dim_min), dr.dim_max);
-      // liquid component and the zero-mixing-ratio ice component, both with
-      // the call's horizontal relative standard deviation
-      double wl = fmax(eps, lwc * f1), wi = fmax(eps, 0.0 * f1);
+      // ice of the same type (socrates_set_cld.F90:341-381, 817-826)
+      double iwc = 0.0;
+      if (d.ice) {
+        const double mi = a.ice_mmr ? fmax(a.ice_mmr[idx], 0.0) : 0.0;
+        iwc = frac > eps ? mi / frac : 0.0;
+        const SgDrop &e = spec->ice[d.ice_t];
+        r[SG_Q_IWC] = iwc;
+        r[SG_Q_DE] = fmin(fmax(a.ice_dim ? fmax(a.ice_dim[idx], 0.0) : 0.0, e.dim_min), e.dim_max);
+      }
+      // liquid and ice components, both with the call's horizontal relative
+      // standard deviation (socrates_set_cld_mcica.F90:168-178)
+      double wl = fmax(eps, lwc * f1), wi = fmax(eps, iwc * f1);
       double sr = 0.0 + a.cloud_horizontal_rsd * wl;
       sr = sr + a.cloud_horizontal_rsd * wi;
       double sw = 0.0 + wl;
@@ -758,18 +784,20 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
     // (written only where the layer has cloud: the solve reads them only
     // for cloudy layers of a sub-column)
     if (r[SG_Q_FRAC1] > 0.0) {
-      double *co = cldopt + (((size_t)col * NB + b) * L + i) * 4;
+      double *co = cldopt + (((size_t)col * NB + b) * L + i) * d.co_stride;
       double kext, kscat, phf, fwd;
-      const SgDrop &dr = spec->drop[d.drop];
-      if (dr.scheme == 5) pade2(dr.parm[b], r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
-      else slingo(dr.parm[b], r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
+      drop_optics(spec->drop[d.drop], b, r[SG_Q_LWC], r[SG_Q_RE], kext, kscat, phf, fwd);
+      double ie = 0.0, is = 0.0, ip = 0.0, iff = 0.0;
+      if (d.ice) ice_optics(spec->ice[d.ice_t], b, r[SG_Q_IWC], r[SG_Q_DE], r[SG_Q_T], ie, is, ip, iff);
       if (d.f32) {
-        // mixed form: (kext, kext - kscat, phf, fwd) in FP32, the absorption
-        // formed here in FP64
+        // mixed form: droplets and ice together, (kext, kext - kscat, phf,
+        // fwd) in FP32, the absorption formed here in FP64
         float4 *cf = (float4 *)((float *)cldopt + (((size_t)col * NB + b) * L + i) * 4);
+        kext = kext + ie; kscat = kscat + is; phf = phf + ip; fwd = fwd + iff;
         *cf = make_float4((float)kext, (float)(kext - kscat), (float)phf, (float)fwd);
       } else {
         co[0] = kext; co[1] = kscat; co[2] = phf; co[3] = fwd;
+        if (d.ice) { co[4] = ie; co[5] = is; co[6] = ip; co[7] = iff; }
       }
     }
   }
@@ -787,9 +815,9 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
   br[0] = (kcont + ray) + keq;
   if (d.triple) {
     // grey optics of each cloud type (grey_opt_prop.F90:975-1000,
-    // 1185-1230: Rayleigh, then the type's droplets and its ice component of
-    // zero mixing ratio, at layers from the call's cloud top where the column
-    // has cloud), the equivalent extinction (solve_band_k_eqv_scl.F90:
+    // 1185-1230: Rayleigh, then the type's droplets and its ice component
+    // (zero without ice), at layers from the call's cloud top where the
+    // column has cloud), the equivalent extinction (solve_band_k_eqv_scl.F90:
     // 1038-1045), moments normalised by the scattering (grey_opt_prop.F90:
     // 1474-1500) and the delta-rescaled phase function (rescale_phase_fnc.F90)
     const int nct = krange[0] < L ? krange[0] : L;
@@ -800,8 +828,16 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
         const SgDrop &dr = spec->drop[k == 0 ? d.drop : d.drop2];
         const double mmr = r[k == 0 ? SG_Q_LWC : SG_Q_LWC2], re = r[k == 0 ? SG_Q_RE : SG_Q_RE2];
         double ct, cs, cp, cf;
-        if (dr.scheme == 5) pade2(dr.parm[b], mmr, re, ct, cs, cp, cf);
-        else slingo(dr.parm[b], mmr, re, ct, cs, cp, cf);
+        drop_optics(dr, b, mmr, re, ct, cs, cp, cf);
+        kgt = kgt + ct;
+        ksc = ksc + cs;
+        phf = phf + cp;
+        fwd = fwd + cf;
+        // the type's ice component
+        ct = 0.0; cs = 0.0; cp = 0.0; cf = 0.0;
+        if (d.ice)
+          ice_optics(spec->ice[k == 0 ? d.ice_t : d.ice_t2], b, r[k == 0 ? SG_Q_IWC : SG_Q_IWC2],
+                     r[k == 0 ? SG_Q_DE : SG_Q_DE2], r[SG_Q_T], ct, cs, cp, cf);
         kgt = kgt + ct;
         ksc = ksc + cs;
         phf = phf + cp;
@@ -862,12 +898,17 @@ struct LayerIn {
   double kx;                          // the term's k_abs, for scalings other than the power law
   double kgt, q1, q2;                 // band record: grey extinction, adjust or Planck terms
   double cv;                          // the lane's sub-column condensate scaling
-  double co[4];                       // droplet optics at unit scaling
+  double co[4];                       // cloud optics: FP64 form, scaled by cv and
+                                      // added to the clear totals (k_ext, k_scat,
+                                      // first moment, forward scattering); mixed
+                                      // form, at unit scaling (k_ext, k_abs, ...)
 };
 
 struct LaneSrc {
   const double *rec, *br, *csub, *cop;
   const float *csubf, *copf;  // FP32 forms (mixed form)
+  double ksc_clear;           // clear-air scattering (Rayleigh)
+  int co_stride, ice;         // doubles per layer of cop; ice optics beside the droplets'
   int gas, isub;
   int fnc;                    // the term's scaling function
   const SgTerm *term;         // the term (device table), for fnc other than the power law
@@ -900,16 +941,27 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
       const float4 co = *(const float4 *)(s.copf + i * 4);
       x.co[0] = co.x; x.co[1] = co.y; x.co[2] = co.z; x.co[3] = co.w;
     } else {
-      const double *co = s.cop + i * 4;
-      x.co[0] = co[0]; x.co[1] = co[1]; x.co[2] = co[2]; x.co[3] = co[3];
+      // the sub-column's cloudy totals (mcica_sample.F90:614-640): each
+      // component, droplets then ice, scaled by c_sub and added in turn
+#pragma clang fp contract(off)
+      const double *co = s.cop + i * s.co_stride;
+      const double cs = x.cv;
+      double kg = x.kgt + co[0] * cs, ks = s.ksc_clear + co[1] * cs;
+      double pf = co[2] * cs, fw = co[3] * cs;
+      if (s.ice) {
+        kg = kg + co[4] * cs;
+        ks = ks + co[5] * cs;
+        pf = pf + co[6] * cs;
+        fw = fw + co[7] * cs;
+      }
+      x.co[0] = kg; x.co[1] = ks; x.co[2] = pf; x.co[3] = fw;
     }
   } else {
     x.co[0] = 0.0; x.co[1] = 0.0; x.co[2] = 0.0; x.co[3] = 0.0;
   }
 }
 
-// co: the layer's droplet optics at unit condensate scaling (kext, kscat,
-// first moment, forward scattering), from prep_kernel.
+// x.co: the layer's cloudy totals of the lane's sub-column (load_layer).
 template <bool GEN>
 __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, const LayerIn &x,
                                            Coef &o) {
@@ -917,11 +969,8 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
   double kgas = (!GEN || t.fnc == SG_FNC_POWER_LAW) ? k_term(t, x.lnp, x.lnt, x.mmr) : x.kx;
   const double frac1 = x.frac1, kgt = x.kgt, csub = x.cv;
   if (csub * frac1 > 0.0) {
-    const double kext = x.co[0], kscat = x.co[1], phf = x.co[2], fwd = x.co[3];
-    double kg = kgt + kext * csub;
-    double ks = c.ksc_clear + kscat * csub;
-    double fw = fwd * csub;
-    double pf = phf * csub;
+    const double kg = x.co[0], ks = x.co[1];
+    double pf = x.co[2], fw = x.co[3];
     pf = (pf - fw) / fmax(ks - fw, kEps);
     fw = fw / fmax(ks, kEps);
     double ktot = kg + kgas;
@@ -1149,7 +1198,10 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   LaneSrc src;
   src.rec = rec;
   src.br = br;
-  src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
+  src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * d.co_stride : nullptr;
+  src.co_stride = d.co_stride;
+  src.ice = d.ice && !MIXED;
+  src.ksc_clear = c.ksc_clear;
   src.copf = d.mcica ? (const float *)cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
   src.csub = c_sub + (size_t)col * L * SG_MAXSUBCOL;
   src.csubf = (const float *)c_sub + (size_t)col * L * SG_MAXSUBCOL;
@@ -1374,7 +1426,10 @@ __device__ __forceinline__ void solve_body_items(const CallDims &d, const SgSpec
   LaneSrc src;
   src.rec = rec;
   src.br = br;
-  src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
+  src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * d.co_stride : nullptr;
+  src.co_stride = d.co_stride;
+  src.ice = d.ice && !MIXED;
+  src.ksc_clear = c.ksc_clear;
   src.csub = c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
   src.copf = d.mcica ? (const float *)cldopt + ((size_t)col * NB + b) * L * 4 : nullptr;
   src.csubf = (const float *)c_sub + (size_t)col * L * SG_MAXSUBCOL + (isub > 0 ? isub - 1 : 0);
@@ -2644,6 +2699,8 @@ static size_t input_fields(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
       {&dv->gas_field[4], a->gas_field[4], L * P}, {&dv->gas_field[5], a->gas_field[5], L * P},
       {&dv->gas_field[6], a->gas_field[6], L * P}, {&dv->gas_field[7], a->gas_field[7], L * P},
       {&dv->gas_field[8], a->gas_field[8], L * P}, {&dv->gas_field[9], a->gas_field[9], L * P},
+      {&dv->ice_mmr, a->ice_mmr, L * P}, {&dv->ice_dim, a->ice_dim, L * P},
+      {&dv->ice_conv_mmr, a->ice_conv_mmr, L * P}, {&dv->ice_conv_dim, a->ice_conv_dim, L * P},
   };
   const size_t n = sizeof g / sizeof g[0];
   static_assert(sizeof g / sizeof g[0] <= kMaxInFields, "input field table too small");
@@ -2799,6 +2856,31 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     d.ipph = S.ipph;
     d.ioverlap = a->i_overlap;
   }
+  // the ice component of the cloud types (socrates_set_cld.F90:341-381):
+  // the given ice types, i_cnv_ice defaulting to i_st_ice; without i_st_ice
+  // there is no ice component (the reference's zero-ice call with an ice
+  // fit that is zero at zero ice)
+  d.co_stride = 4;
+  if ((d.mcica || d.triple) && a->i_st_ice > 0) {
+    auto check_ice = [&](int t) -> int {
+      if (t > SG_NICE || !S.ice[t - 1].present) return fail(SOCGPU_EINVAL, "Ice cloud type not in spectral file.");
+      const int sc = S.ice[t - 1].scheme;
+      if (sc != 12 && sc != 19 && sc != 20)
+        return fail(SOCGPU_EUNSUPPORTED, "ice parametrisation %d is not on this path", sc);
+      return 0;
+    };
+    const int tc = a->i_cnv_ice > 0 ? a->i_cnv_ice : a->i_st_ice;
+    if (int rc = check_ice(a->i_st_ice)) return rc;
+    if (d.conv) {
+      if (int rc = check_ice(tc)) return rc;
+    }
+    d.ice = 1;
+    d.ice_t = a->i_st_ice - 1;
+    d.ice_t2 = d.conv ? tc - 1 : d.ice_t;
+    if (d.mcica && !d.f32) d.co_stride = 8;
+  } else if (a->ice_mmr || a->ice_conv_mmr) {
+    if (d.mcica || d.triple) return fail(SOCGPU_EINVAL, "ice condensate needs i_st_ice");
+  }
   d.nwpc = (S.n_g + kWave - 1) / kWave;
   solve_layout(d, S, precision == SOCGPU_MIXED && d.gen);
   *entry = &e;
@@ -2819,7 +2901,7 @@ static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
   size_t n = P * L * SG_NQ + P * (L + 1) + P * SG_NS + P * NB * L * SG_NBQ + P * NB * SG_NBQ +
              partial_doubles(d) + 2;
   if (d.gx) n += P * L * SG_NGQ;
-  if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * 4;
+  if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * d.co_stride;
   if (d.triple) n += P * NB * L * 8 + P * (L + 1) * kOv + P * d.nwpc * L * kTriQ * kWave;
   return n;
 }
@@ -2861,7 +2943,7 @@ static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cur
   q.partial = take(partial_doubles(d));
   q.krange = (int *)take(2);
   q.csub = d.mcica ? take(P * L * SG_MAXSUBCOL) : nullptr;
-  q.cldopt = d.mcica ? take(P * NB * L * 4) : (d.triple ? take(P * NB * L * 8) : nullptr);
+  q.cldopt = d.mcica ? take(P * NB * L * d.co_stride) : (d.triple ? take(P * NB * L * 8) : nullptr);
   q.ovl = d.triple ? take(P * (L + 1) * kOv) : nullptr;
   q.tws = d.triple ? take((size_t)P * d.nwpc * L * kTriQ * kWave) : nullptr;
 }
@@ -3140,6 +3222,8 @@ static int expand_clear(int n, const SocgpuRequest *rq, Expanded &x) {
     c.i_inhom = SOCGPU_INHOM_HOMOGENEOUS;
     c.cloud_frac = c.liq_mmr = c.liq_dim = nullptr;
     c.conv_frac = c.liq_conv_mmr = c.liq_conv_dim = nullptr;
+    c.ice_mmr = c.ice_dim = c.ice_conv_mmr = c.ice_conv_dim = nullptr;
+    c.i_st_ice = c.i_cnv_ice = 0;
     c.rand_seed = nullptr;
     x.args.push_back(c);
     SocgpuDiag cd;

@@ -339,6 +339,38 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
         if (x.size() < 3) { *err = "bad sub-band line in block 17"; return -1; }
         sub_band.push_back({(int)x[1], (int)x[2]});
       }
+    } else if (type == 12) {
+      // ice crystals, laid out as the droplet block (read_spectrum.F90:
+      // 1175-1262); a fit longer than the record keeps is read past and the
+      // type left absent (a call that names it is refused)
+      if (ver != 2) { *err = "ice block version not supported"; return -3; }
+      L.next();
+      const std::string t = L.next();
+      const size_t eq = t.find('=');
+      if (eq == std::string::npos) { *err = "bad ice block"; return -1; }
+      const int iice = std::atoi(t.c_str() + eq + 1);
+      const std::string h = L.next();
+      int scheme = inum(h, 34, 5), npar = inum(h, 66, 5);
+      L.next();
+      const std::string r = L.next();
+      const bool keep = iice >= 1 && iice <= SG_NICE && npar <= SG_NDPARM;
+      SgDrop *d = keep ? &s->ice[iice - 1] : nullptr;
+      if (d) {
+        d->present = 1; d->scheme = scheme; d->n_parm = npar;
+        d->dim_min = fnum(r, 39, 12); d->dim_max = fnum(r, 55, 12);
+      }
+      for (int b = 0; b < s->n_band; b++) {
+        L.next();
+        int got = 0;
+        while (got < npar) {
+          std::vector<double> x = numbers(L.next());
+          if (x.empty()) { *err = "truncated ice block"; return -1; }
+          for (double v : x) {
+            if (got < npar && d) d->parm[b][got] = v;
+            got++;
+          }
+        }
+      }
     } else if (type == 10) {
       if (ver != 2) { *err = "droplet block version not supported"; return -3; }
       L.next();

@@ -19,11 +19,12 @@
 ! socrates_set_bound.F90:112-290, socrates_set_cld.F90:861-910,
 ! socrates_set_cld_mcica.F90:159-167) and scatters the outputs as set_diag
 ! does (socrates_set_diag.F90:104-256, 549-623).
-! Arguments of runes outside the path (aerosols, tiles, orography, ice
-! optics, cloud sub-fractions, photolysis, the clean-air diagnostics, ...) are
-! not declared: a call naming one does not compile, rather than being
-! silently ignored. Ice fields are declared for calls that pass them with no
-! ice: any ice condensate stops the call. A gas the spectral file lists is
+! Arguments of runes outside the path (aerosols, tiles, orography, cloud
+! sub-fractions, photolysis, the clean-air diagnostics, ...) are not
+! declared: a call naming one does not compile, rather than being silently
+! ignored. Ice takes the spectral file's ice types of Fu's phase-function
+! fit, Baran's fit and Pade-2; without i_st_ice a cloudy call has no ice
+! component (the reference requires one). A gas the spectral file lists is
 ! used as the reference uses it; one the library cannot take (other than
 ! H2O, CO2, O3, N2O, CH4, O2, SO2, OCS, CFC-11, CFC-12, CFC-113, HCFC-22 and
 ! HFC-134a) stops the call if it is given a non-zero amount.
@@ -70,6 +71,9 @@ module socrates_gpu_c
     real(c_double) :: h2o_mix_ratio = 0.0_c_double, o3_mix_ratio = 0.0_c_double
     ! fields of the further gases, in gas_mix_ratio's order
     type(c_ptr) :: gas_field(10) = c_null_ptr
+    integer(c_int) :: i_st_ice = 0, i_cnv_ice = 0
+    type(c_ptr) :: ice_mmr = c_null_ptr, ice_dim = c_null_ptr
+    type(c_ptr) :: ice_conv_mmr = c_null_ptr, ice_conv_dim = c_null_ptr
   end type socgpu_runes_args
 
   ! SocgpuDiag
@@ -790,14 +794,19 @@ contains
       id_ct = n_layer + 1 - max(1, n_cloud_layer)
     end if
     if (cloud) then
-      if (nonzero(ice_mmr, ice_mmr_1d) .or. nonzero(ice_conv_mmr, ice_conv_mmr_1d)) &
-        call fail('ice condensate is not on this path')
       a%cloud_frac = lay(cloud_frac, cloud_frac_1d, id_ct)
       a%liq_mmr = lay(liq_mmr, liq_mmr_1d, id_ct)
       a%liq_dim = lay(liq_dim, liq_dim_1d, id_ct)
       a%conv_frac = lay(conv_frac, conv_frac_1d, id_ct)
       a%liq_conv_mmr = lay(liq_conv_mmr, liq_conv_mmr_1d, id_ct)
       a%liq_conv_dim = lay(liq_conv_dim, liq_conv_dim_1d, id_ct)
+      ! ice (socrates_set_cld.F90:341-415)
+      if (present(i_st_ice)) a%i_st_ice = i_st_ice
+      if (present(i_cnv_ice)) a%i_cnv_ice = i_cnv_ice
+      a%ice_mmr = lay(ice_mmr, ice_mmr_1d, id_ct)
+      a%ice_dim = lay(ice_dim, ice_dim_1d, id_ct)
+      a%ice_conv_mmr = lay(ice_conv_mmr, ice_conv_mmr_1d, id_ct)
+      a%ice_conv_dim = lay(ice_conv_dim, ice_conv_dim_1d, id_ct)
       ! McICA seeds (socrates_set_cld_mcica.F90:159-167)
       allocate(seeds(n_profile))
       if (present(rand_seed)) then

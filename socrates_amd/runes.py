@@ -59,7 +59,10 @@ class RunesArgs(ctypes.Structure):
                                              ("flux_ground", _D), ("co2", _D),
                                              ("h2o_mix_ratio", ctypes.c_double),
                                              ("o3_mix_ratio", ctypes.c_double),
-                                             ("gas_field", _D * len(WM_GASES))]
+                                             ("gas_field", _D * len(WM_GASES)),
+                                             ("i_st_ice", ctypes.c_int), ("i_cnv_ice", ctypes.c_int),
+                                             ("ice_mmr", _D), ("ice_dim", _D), ("ice_conv_mmr", _D),
+                                             ("ice_conv_dim", _D)]
 
 
 class Diag(ctypes.Structure):
@@ -156,7 +159,8 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
           cloud_horizontal_rsd=0.0, i_cloud_representation=0, i_overlap=0, i_inhom=0,
           i_mcica_sampling=0, i_st_water=5, i_cnv_water=5, n_subcol_gen=0,
           rand_seed=None, l_flux_ground=False, flux_ground=None, co2=None,
-          l_co2_well_mixed=False, h2o_mix_ratio=0.0, o3_mix_ratio=0.0, l_clear=False, out=None,
+          l_co2_well_mixed=False, h2o_mix_ratio=0.0, o3_mix_ratio=0.0, i_st_ice=0, i_cnv_ice=0,
+          ice_mmr=None, ice_dim=None, ice_conv_mmr=None, ice_conv_dim=None, l_clear=False, out=None,
           **gas_amounts):
     """ctypes argument and output structures of one call (and the objects
     that must stay alive while it runs). mix_ratios: the well-mixed mixing
@@ -167,7 +171,9 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     [layer][profile] field. l_flux_ground / flux_ground [band][profile]:
     surface emission per band; co2 [layer][profile]: CO2 field (co2_mix_ratio
     applies when it is None or l_co2_well_mixed); h2o_mix_ratio /
-    o3_mix_ratio: well-mixed H2O / O3 where h2o / o3 is None; l_clear: also the
+    o3_mix_ratio: well-mixed H2O / O3 where h2o / o3 is None; i_st_ice,
+    i_cnv_ice, ice_mmr, ice_dim, ice_conv_mmr, ice_conv_dim: the ice
+    component (none when i_st_ice is 0); l_clear: also the
     clear-sky fluxes flux_up_clear, flux_down_clear, flux_direct_clear (an
     out dict holding any of these keys asks for them as well)."""
     init()
@@ -180,7 +186,8 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
                   solar_irrad=solar_irrad, albedo_diff=albedo_diff, albedo_dir=albedo_dir,
                   cloud_frac=cloud_frac, liq_mmr=liq_mmr, liq_dim=liq_dim, conv_frac=conv_frac,
                   liq_conv_mmr=liq_conv_mmr, liq_conv_dim=liq_conv_dim,
-                  flux_ground=flux_ground if l_flux_ground else None, co2=co2)
+                  flux_ground=flux_ground if l_flux_ground else None, co2=co2, ice_mmr=ice_mmr,
+                  ice_dim=ice_dim, ice_conv_mmr=ice_conv_mmr, ice_conv_dim=ice_conv_dim)
     on_dev = _is_torch(p_layer)
     keep = []
     a = RunesArgs()
@@ -203,6 +210,7 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     a.cloud_horizontal_rsd = float(cloud_horizontal_rsd)
     a.l_grey_albedo, a.grey_albedo = int(bool(l_grey_albedo)), float(grey_albedo)
     a.l_flux_ground, a.l_co2_well_mixed = int(bool(l_flux_ground)), int(bool(l_co2_well_mixed))
+    a.i_st_ice, a.i_cnv_ice = int(i_st_ice), int(i_cnv_ice)
     for k, v in inputs.items():
         if v is not None and _is_torch(v) != on_dev:
             raise SocgpuError("all inputs must be in the same memory space (%s)" % k)
@@ -274,7 +282,8 @@ def runes_noalloc(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer,
 
 # vertical fields of runes(), [layer or level][profile]
 _VERTICAL = ("density", "layer_heat_capacity", "h2o", "o3", "p_level", "t_level", "cloud_frac",
-             "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr", "liq_conv_dim", "co2") + WM_GASES
+             "liq_mmr", "liq_dim", "conv_frac", "liq_conv_mmr", "liq_conv_dim", "co2", "ice_mmr",
+             "ice_dim", "ice_conv_mmr", "ice_conv_dim") + WM_GASES
 
 
 def _flip(a):
@@ -467,6 +476,11 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         kw.update(l_flux_ground=True, flux_ground=g("flux_ground"))
     if case.co2 is not None:
         kw["co2"] = g("co2")
+    if cloudy and case.ice_mmr is not None:
+        kw.update(i_st_ice=case.i_st_ice, i_cnv_ice=case.i_st_ice, ice_mmr=g("ice_mmr"),
+                  ice_dim=g("ice_dim"))
+        if conv:
+            kw.update(ice_conv_mmr=g("ice_conv_mmr"), ice_conv_dim=g("ice_conv_dim"))
     for j, x in (case.wm_field or {}).items():
         kw[WM_GASES[j]] = src.get(WM_GASES[j], x)
     if case.wm_mix_ratio is not None:
