@@ -241,6 +241,9 @@ def main():
     ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 512 MiB device copy, the known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the path shards by columns with no collective, "
@@ -253,6 +256,12 @@ def main():
     R.init(0)
     R.register_default_spectra()
     cases = make_inputs(torch, dev)
+    if a.copy_calibration:
+        src_buf = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst_buf = torch.empty_like(src_buf)
+        dst_buf.copy_(src_buf)
+        torch.cuda.synchronize()
+        del src_buf, dst_buf
     outs = {}
     for prec in ("fp64", "mixed"):
         outs[prec] = {}

@@ -49,6 +49,7 @@ extern "C" {
                                         overlap only) */
 #define SOCGPU_OVERLAP_MAX_RAND 0    /* ip_max_rand */
 #define SOCGPU_OVERLAP_RANDOM 1      /* ip_rand */
+#define SOCGPU_OVERLAP_EXP 2         /* ip_exponential (McICA) */
 #define SOCGPU_OVERLAP_EXP_RAND 3    /* ip_exponential_rand */
 #define SOCGPU_INHOM_HOMOGENEOUS 0   /* ip_homogeneous */
 #define SOCGPU_INHOM_MCICA 2         /* ip_mcica */
@@ -126,6 +127,12 @@ typedef struct SocgpuRunesArgs {
    * same; Baran's fit adds a small extinction even at zero ice). */
   int i_st_ice, i_cnv_ice;
   const double *ice_mmr, *ice_dim, *ice_conv_mmr, *ice_conv_dim;
+  /* Options of the reference's general driver (l_run_cdl.F90) that runes
+   * fixes: an infra-red source linear in optical depth (l_ir_source_quad
+   * false, ir_source.F90:52-59; runes takes it quadratic) and no continuum
+   * absorption (l_continuum false; runes always includes the file's). */
+  int l_ir_source_linear;
+  int l_no_continuum;
 } SocgpuRunesArgs;
 
 #define SOCGPU_GAS_N2O 0

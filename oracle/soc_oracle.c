@@ -1519,7 +1519,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         k_ext_scat[i] = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
         k_grey_tot[i] = 0.0;
       }
-      for (int jc = 1; jc <= sp.n_band_continuum[b]; jc++) {
+      for (int jc = 1; jc <= (c->l_no_continuum ? 0 : sp.n_band_continuum[b]); jc++) {
         const int icont = sp.index_continuum[b][jc - 1];
         const int fnc = sp.i_scale_fnc_cont[b][jc];
         const double *sc = sp.scale_cont[b][jc];
@@ -1552,9 +1552,12 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           planck[i] = planck_flux_band(&sp, b, LAY(c->t_level, i, l));
           dpl[i - 1] = planck[i] - planck[i - 1];
         }
+        /* the quadratic term (diff_planck_source_mod.F90:112-121); a linear
+         * source has none, which the sources below take as a zero term
+         * (c2 * 0 added: ir_source.F90:40-59 gives the same) */
         for (int i = 1; i <= L; i++) {
           double bt = planck_flux_band(&sp, b, LAY(c->t_layer, i - 1, l));
-          dpl2[i - 1] = 2.0 * (planck[i] + planck[i - 1] - 2.0 * bt);
+          dpl2[i - 1] = c->l_ir_source_linear ? 0.0 : 2.0 * (planck[i] + planck[i - 1] - 2.0 * bt);
         }
         /* surface emission per band when given (socrates_set_bound.F90:
          * 251-290), else the Planck flux of the surface temperature

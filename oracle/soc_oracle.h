@@ -79,6 +79,12 @@ typedef struct {
    * no ice component */
   int i_st_ice;
   const double *ice_mmr, *ice_dim, *ice_conv_mmr, *ice_conv_dim;
+  /* infra-red source linear in optical depth (control%l_ir_source_quad
+   * false, ir_source.F90:52-59) in place of runes' quadratic one */
+  int l_ir_source_linear;
+  /* continuum absorption left out (control%l_continuum false; runes always
+   * includes the file's continua) */
+  int l_no_continuum;
 } soc_case;
 
 enum { SOC_WM_N2O, SOC_WM_CH4, SOC_WM_O2, SOC_WM_SO2, SOC_WM_OCS, SOC_WM_CFC11, SOC_WM_CFC12,

@@ -89,6 +89,8 @@ class Case:
     ice_dim: np.ndarray = None
     ice_conv_mmr: np.ndarray = None
     ice_conv_dim: np.ndarray = None
+    l_ir_source_linear: int = 0       # infra-red source linear in tau (not written to files)
+    l_no_continuum: int = 0           # continuum absorption left out (not written to files)
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile

@@ -179,6 +179,8 @@ struct CallDims {
   int ice_t, ice_t2; // stratiform (or homogeneous) and convective
   int co_stride;     // doubles per layer of the McICA cloud optics (4, or 8
                      // with the ice beside the droplets in the FP64 form)
+  int ir_lin;        // infra-red source linear in optical depth
+  int no_cont;       // continuum absorption left out
 };
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
@@ -390,11 +392,13 @@ __device__ __forceinline__ void cg_layer(int ioverlap, bool pre, int k, const do
       }
     }
   } else {
-    // random / exponential-random (cloud_gen:339-405)
+    // random / exponential / exponential-random (cloud_gen:339-405)
     double alpha = 0.0, rcorr = 0.0;
     if (ioverlap != 1 && k - 1 >= 1) {
       double cfa = rec[(k - 2) * SG_NQ + SG_Q_WCLD];
-      if (cfa > 0.0) {
+      // exponential overlap (2) correlates with the layer above even where
+      // that layer is clear (cloud_gen:355-364)
+      if (cfa > 0.0 || ioverlap == 2) {
         if (pre) {
           alpha = rec[(k - 1) * SG_NQ + SG_Q_ALPHA];
           if (Y) rcorr = rec[(k - 1) * SG_NQ + SG_Q_RCORR];
@@ -443,11 +447,34 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
   }
   const bool any = k_top <= L;
   const int k0 = any ? k_top : 1, k1 = any ? k_base : 0;
+  // The draws x, y start a sub-column where the previous one left them
+  // (cloud_gen keeps them across its sub-column loop). Only exponential
+  // overlap can keep a value at a sub-column's first cloudy layer (the
+  // others correlate only with a cloudy layer above); for it, the start of
+  // every sub-column comes from running them in order, one lane at a time.
+  double x0 = 0.0, y0 = 0.0;
+  if (d.ioverlap == 2) {
+    double xc = 0.0, yc = 0.0;
+    for (int j = 0; j < n_gen; j++) {
+      double xf = 0.0, yf = 0.0;
+      if (lane == j) {
+        x0 = xc; y0 = yc;
+        int ix = sx, iy = sy;
+        double x = xc, y = yc;
+        for (int k = k0; k <= k1; k++)
+          if (rec[(k - 1) * SG_NQ + SG_Q_WCLD] > 0.0)
+            cg_layer<true>(d.ioverlap, d.cg_pre, k, rec, dp_cf, dp_cw, ix, iy, x, y);
+        xf = x; yf = y;
+      }
+      xc = __shfl(xf, j, kWave);
+      yc = __shfl(yf, j, kWave);
+    }
+  }
   // pass 1: cloud presence of this lane's sub-column
   bool cloudy = false;
   {
     int ix = sx, iy = 0;
-    double x = 0.0, y = 0.0;
+    double x = x0, y = 0.0;
     for (int k = k0; k <= k1; k++) {
       const double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
       if (valid && cf > 0.0) {
@@ -468,7 +495,7 @@ __device__ __forceinline__ void cloudgen_body(const CallDims &d, const double *_
   }
   // pass 2: the sub-column of slot `lane` (sub-column src), both streams
   int ix = __shfl(sx, src >= 0 ? src : 0, kWave), iy = __shfl(sy, src >= 0 ? src : 0, kWave);
-  double x = 0.0, y = 0.0;
+  double x = __shfl(x0, src >= 0 ? src : 0, kWave), y = __shfl(y0, src >= 0 ? src : 0, kWave);
   for (int k = 1; k <= L; k++) {
     double val = 0.0;
     const double cf = rec[(k - 1) * SG_NQ + SG_Q_WCLD];
@@ -808,7 +835,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
   const double ray = (solar && l_rayleigh) ? B.rayleigh : 0.0;
   // clear-air grey extinction: continua, then Rayleigh scattering
   // (grey_opt_prop.F90:801-835), then the equivalent extinction
-  const double kcont = (gen && B.n_cont > 0) ? continuum_k(B, r[SG_Q_P], r[SG_Q_T],
+  const double kcont = (gen && B.n_cont > 0 && !d.no_cont) ? continuum_k(B, r[SG_Q_P], r[SG_Q_T],
                                                   grec[((size_t)col * L + i) * SG_NGQ + SG_G_RHO],
                                                   r[SG_Q_MMR + SG_GAS_H2O])
                                     : 0.0;
@@ -865,7 +892,9 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
   } else {
     // b0, b1: the band's Planck fluxes at the top and bottom of the layer
     double bt = planck_band(B, spec->n_deg, spec->t_ref_planck, r[SG_Q_T]);
-    br[1] = 2.0 * (b1 + b0 - 2.0 * bt);  // diff_2
+    // diff_2 (diff_planck_source_mod.F90:112-121); zero for a linear source,
+    // whose sources (ir_source.F90:52-59) are the quadratic ones without it
+    br[1] = d.ir_lin ? 0.0 : 2.0 * (b1 + b0 - 2.0 * bt);
     br[2] = b1 - b0;                     // diff
     br[3] = b1;                          // planck at the bottom of the layer
     if (i == 0) {
@@ -2838,7 +2867,8 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     if (!S.has_mcica) return fail(SOCGPU_ENOTFOUND, "Spectrum name not associated with MCICA data.");
     if (a->i_mcica_sampling != SOCGPU_MCICA_SINGLE && a->i_mcica_sampling != SOCGPU_MCICA_OPTIMAL)
       return fail(SOCGPU_EUNSUPPORTED, "McICA sampling %d not supported", a->i_mcica_sampling);
-    if (a->i_overlap != SOCGPU_OVERLAP_EXP_RAND && a->i_overlap != SOCGPU_OVERLAP_RANDOM && a->i_overlap != SOCGPU_OVERLAP_MAX_RAND)
+    if (a->i_overlap != SOCGPU_OVERLAP_EXP_RAND && a->i_overlap != SOCGPU_OVERLAP_RANDOM &&
+        a->i_overlap != SOCGPU_OVERLAP_MAX_RAND && a->i_overlap != SOCGPU_OVERLAP_EXP)
       return fail(SOCGPU_EUNSUPPORTED, "overlap %d not supported with McICA", a->i_overlap);
     if (!a->cloud_frac || !a->liq_mmr || !a->liq_dim) return fail(SOCGPU_EINVAL, "Cloud fraction not provided.");
     if (!a->rand_seed) return fail(SOCGPU_EINVAL, "rand_seed is required: default seeds depend on the position in the call");
@@ -2881,6 +2911,8 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   } else if (a->ice_mmr || a->ice_conv_mmr) {
     if (d.mcica || d.triple) return fail(SOCGPU_EINVAL, "ice condensate needs i_st_ice");
   }
+  d.ir_lin = !solar && a->l_ir_source_linear;
+  d.no_cont = a->l_no_continuum;
   d.nwpc = (S.n_g + kWave - 1) / kWave;
   solve_layout(d, S, precision == SOCGPU_MIXED && d.gen);
   *entry = &e;
@@ -4030,7 +4062,6 @@ int socgpu_cloud_gen(int n_profile, int n_layer, int n_subcol, int n_req, int ip
   const int P = n_profile, L = n_layer;
   if (P <= 0 || L <= 0 || L > SG_MAXL || n_subcol <= 0 || n_subcol > SG_MAXSUBCOL || n1 < 2 || n2 < 2)
     return fail(SOCGPU_EINVAL, "bad cloud_gen sizes");
-  if (ioverlap == 2) return fail(SOCGPU_EUNSUPPORTED, "ioverlap 2 carries draws across sub-columns");
   std::vector<double> rec((size_t)P * L * SG_NQ, 0.0);
   for (int i = 0; i < L; i++)
     for (int l = 0; l < P; l++) {

@@ -74,6 +74,7 @@ module socrates_gpu_c
     integer(c_int) :: i_st_ice = 0, i_cnv_ice = 0
     type(c_ptr) :: ice_mmr = c_null_ptr, ice_dim = c_null_ptr
     type(c_ptr) :: ice_conv_mmr = c_null_ptr, ice_conv_dim = c_null_ptr
+    integer(c_int) :: l_ir_source_linear = 0, l_no_continuum = 0
   end type socgpu_runes_args
 
   ! SocgpuDiag

@@ -109,7 +109,7 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         for name, (P, L, src, cloudy, seed) in CASES.items():
             run_case(name, P, L, src, cloudy, seed, tmp)
-        for ov in (0, 1, 3):
+        for ov in (0, 1, 2, 3):
             run_cloud_gen(ov, tmp)
         for name, args in TWO_STREAM.items():
             run_two_stream(name, *args, tmp)

@@ -108,7 +108,7 @@ def test_cloud_gen_bit_exact():
     dp = np.full((L, P), synth.DECORR_PA)
     seeds = rng.integers(0, 100000, P).astype(np.int32)
     xcw = np.abs(rng.normal(1.0, 0.5, (100, 40)))
-    for ioverlap in (0, 1, 3):
+    for ioverlap in (0, 1, 2, 3):
         c1, n1 = oracle.cloud_gen(seeds, dp, dp * 0.5, rsd, frac, st["p_layer"], xcw,
                                   n_subcol=64, n_req=52, ioverlap=ioverlap)
         c2, n2 = R.cloud_gen(seeds, dp, dp * 0.5, rsd, frac, st["p_layer"], xcw,

@@ -43,7 +43,7 @@ def test_oracle_matches_reference_golden(path):
         assert (err <= 1e-13 * np.maximum(np.abs(r), 1.0)).all(), (k, err.max())
 
 
-@pytest.mark.parametrize("ioverlap", [0, 1, 3])
+@pytest.mark.parametrize("ioverlap", [0, 1, 2, 3])
 def test_oracle_cloud_gen_bit_exact(ioverlap):
     d = np.load(os.path.join(GOLD, "cloud_gen_ov%d.npz" % ioverlap))
     c_sub, ncld = oracle.cloud_gen(d["rand_seed"], d["rlc_cf"], d["rlc_cw"], d["sigma_qcw"],
