@@ -67,7 +67,7 @@ HBM_PEAK = 8000.0                          # GB/s
 # FETCH_SIZE and WRITE_SIZE runs of this script on the same build, each scaled
 # by what the counter reads for a plain copy of known size; the summary, its
 # commands and the scale factors are in TRAFFIC_SOURCE
-TRAFFIC = {"mixed": 2.134e9, "fp64": 4.260e9}
+TRAFFIC = {"mixed": 2.775e9, "fp64": 4.483e9}
 TRAFFIC_SOURCE = "profiles/c2_solve_traffic.md"
 
 
