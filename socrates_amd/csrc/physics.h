@@ -19,6 +19,22 @@
 
 namespace sg {
 
+// The library's device pointers address global memory (HBM, or host memory
+// mapped for the device), never LDS or private memory. A pointer read from a
+// descriptor or table in memory is generic to the compiler: loads through it
+// become FLAT instructions, and a FLAT load can only be waited for with
+// lgkmcnt(0), which also waits for every other load in flight (a prefetch of
+// the next layer included). Seen through G, the pointer is a global one, so
+// its loads are global loads, waited for one by one with vmcnt.
+template <class T>
+SG_HD T *G(T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (T *)(__attribute__((address_space(1))) T *)(unsigned long long)p;
+#else
+  return p;
+#endif
+}
+
 constexpr double kElsasser = 1.66;       // diffusivity_factor.F90: elsasser_factor
 constexpr double kDiffMinor = 1.66;      // diffusivity_factor_minor
 constexpr double kEps = DBL_EPSILON;
@@ -121,7 +137,7 @@ SG_HD double k_scaled(const SgTerm &t, const SgSpec *spec, double lnp, double p,
   if (t.fnc == SG_FNC_LOOKUP) {
     const int idx = (int)g[SG_G_IDX];
     const int jp = idx & 255, jt = (idx >> 8) & 255, jtt = idx >> 16, nt = spec->n_tmp;
-    const double *kl = spec->klut + (size_t)t.lut * spec->n_pre * nt;
+    const double *kl = G(spec->klut) + (size_t)t.lut * spec->n_pre * nt;
     const double ke = fmax(0.0, g[SG_G_F00] * kl[jp * nt + jt] + g[SG_G_F10] * kl[(jp + 1) * nt + jtt] +
                                     g[SG_G_F01] * kl[jp * nt + jt + 1] +
                                     g[SG_G_F11] * kl[(jp + 1) * nt + jtt + 1]);
@@ -132,7 +148,7 @@ SG_HD double k_scaled(const SgTerm &t, const SgSpec *spec, double lnp, double p,
     const double x = tk * t.tinv - 1.0;
     s = exp(t.a * (lnp - t.lnpref)) * (1.0 + x * t.b + t.c * x * x);
   } else {  // SG_FNC_WENYI
-    const double *gk4 = spec->wenyi, *gk250b = gk4 + 3640, *plg = gk250b + 14, *ttb = plg + 26;
+    const double *gk4 = G(spec->wenyi), *gk250b = gk4 + 3640, *plg = gk250b + 14, *ttb = plg + 26;
     const double cgp = fmax(-5.5, log(p / 100.0));
     int jp = (int)((5.5 + cgp) * 2.0) + 1;
     jp = jp < 1 ? 1 : (jp > 26 ? 26 : jp);

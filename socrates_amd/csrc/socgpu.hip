@@ -649,7 +649,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
       const int i = c * kWave + lane;
       // further gases (general gas optics): the call's field, clamped at
       // 0, else its well-mixed ratio (socrates_set_atm.F90:535-622)
-      const double *fld = (GEN && gas >= SG_GAS_WM0) ? gfield[gas - SG_GAS_WM0] : nullptr;
+      const double *fld = (GEN && gas >= SG_GAS_WM0) ? G(gfield[gas - SG_GAS_WM0]) : nullptr;
       mmr[c] = (c < nch && i < L) ? ((!GEN || gas < SG_GAS_WM0) ? rec[(size_t)i * SG_NQ + SG_Q_MMR + gas]
                                      : fld ? fmax(fld[(size_t)i * d.P + col], 0.0)
                                            : wm[gas - SG_GAS_WM0])
@@ -917,6 +917,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
 // 360-520.
 struct LaneCtx {
   bool solar;
+  bool ice;                 // FP64 form: ice optics beside the droplets'
   double sec0, ksc_clear;
 };
 
@@ -927,10 +928,11 @@ struct LayerIn {
   double kx;                          // the term's k_abs, for scalings other than the power law
   double kgt, q1, q2;                 // band record: grey extinction, adjust or Planck terms
   double cv;                          // the lane's sub-column condensate scaling
-  double co[4];                       // cloud optics: FP64 form, scaled by cv and
-                                      // added to the clear totals (k_ext, k_scat,
+  double co[4];                       // cloud optics at unit condensate scaling:
+                                      // FP64 form, the droplets' (k_ext, k_scat,
                                       // first moment, forward scattering); mixed
-                                      // form, at unit scaling (k_ext, k_abs, ...)
+                                      // form, droplets and ice (k_ext, k_abs, ...)
+  double ci[4];                       // FP64 form: the ice component's
 };
 
 struct LaneSrc {
@@ -946,15 +948,17 @@ struct LaneSrc {
 };
 
 // MIXED reads the FP32 c_sub and droplet optics (kext, kabs, first moment,
-// forward scattering), the FP64 form (kext, kscat, ...) in FP64.
-template <bool MIXED, bool GEN>
+// forward scattering), the FP64 form (kext, kscat, ...) in FP64. CLR: a layer
+// the lane knows to be clear (above the column's first cloudy layer, or its
+// clear solve): no cloud fraction, scaling or cloud optics are read, so a
+// prefetch of the next layer issues its loads and waits for none of them.
+template <bool MIXED, bool GEN, bool CLR = false>
 __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) {
   const double *r = s.rec + i * SG_NQ;
   x.lnp = r[SG_Q_LNP];
   x.lnt = r[SG_Q_LNT];
   x.mmr = r[SG_Q_MMR + s.gas];
   x.mass = r[SG_Q_MASS];
-  x.frac1 = r[SG_Q_FRAC1];
   x.kx = (!GEN || s.fnc == SG_FNC_POWER_LAW)
              ? 0.0
              : k_scaled(*s.term, s.spec, x.lnp, r[SG_Q_P], r[SG_Q_T], s.grec ? s.grec + i * SG_NGQ : nullptr,
@@ -963,30 +967,29 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
   x.kgt = q[0];
   x.q1 = q[1];
   x.q2 = q[2];
+  if (CLR) {
+    x.frac1 = 0.0;
+    x.cv = 0.0;
+    x.co[0] = 0.0; x.co[1] = 0.0; x.co[2] = 0.0; x.co[3] = 0.0;
+    return;
+  }
+  x.frac1 = r[SG_Q_FRAC1];
   if (MIXED) x.cv = s.isub > 0 ? (double)s.csubf[(size_t)i * SG_MAXSUBCOL] : 0.0;
   else x.cv = s.isub > 0 ? s.csub[(size_t)i * SG_MAXSUBCOL] : 0.0;
-  if (x.cv * x.frac1 > 0.0) {
+  // A sampled sub-column's cloud optics are read whether or not the layer is
+  // cloudy in it (the test is made where they are used, and the table spans
+  // every layer), so that a prefetch waits for none of its loads.
+  x.co[0] = 0.0; x.co[1] = 0.0; x.co[2] = 0.0; x.co[3] = 0.0;
+  x.ci[0] = 0.0; x.ci[1] = 0.0; x.ci[2] = 0.0; x.ci[3] = 0.0;
+  if (s.isub > 0) {
     if (MIXED) {
       const float4 co = *(const float4 *)(s.copf + i * 4);
       x.co[0] = co.x; x.co[1] = co.y; x.co[2] = co.z; x.co[3] = co.w;
     } else {
-      // the sub-column's cloudy totals (mcica_sample.F90:614-640): each
-      // component, droplets then ice, scaled by c_sub and added in turn
-#pragma clang fp contract(off)
       const double *co = s.cop + i * s.co_stride;
-      const double cs = x.cv;
-      double kg = x.kgt + co[0] * cs, ks = s.ksc_clear + co[1] * cs;
-      double pf = co[2] * cs, fw = co[3] * cs;
-      if (s.ice) {
-        kg = kg + co[4] * cs;
-        ks = ks + co[5] * cs;
-        pf = pf + co[6] * cs;
-        fw = fw + co[7] * cs;
-      }
-      x.co[0] = kg; x.co[1] = ks; x.co[2] = pf; x.co[3] = fw;
+      x.co[0] = co[0]; x.co[1] = co[1]; x.co[2] = co[2]; x.co[3] = co[3];
+      if (s.ice) { x.ci[0] = co[4]; x.ci[1] = co[5]; x.ci[2] = co[6]; x.ci[3] = co[7]; }
     }
-  } else {
-    x.co[0] = 0.0; x.co[1] = 0.0; x.co[2] = 0.0; x.co[3] = 0.0;
   }
 }
 
@@ -998,8 +1001,17 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
   double kgas = (!GEN || t.fnc == SG_FNC_POWER_LAW) ? k_term(t, x.lnp, x.lnt, x.mmr) : x.kx;
   const double frac1 = x.frac1, kgt = x.kgt, csub = x.cv;
   if (csub * frac1 > 0.0) {
-    const double kg = x.co[0], ks = x.co[1];
-    double pf = x.co[2], fw = x.co[3];
+    // the sub-column's cloudy totals (mcica_sample.F90:614-640): each
+    // component, droplets then ice, scaled by c_sub and added in turn
+    const double cs = csub;
+    double kg = kgt + x.co[0] * cs, ks = c.ksc_clear + x.co[1] * cs;
+    double pf = x.co[2] * cs, fw = x.co[3] * cs;
+    if (c.ice) {
+      kg = kg + x.ci[0] * cs;
+      ks = ks + x.ci[1] * cs;
+      pf = pf + x.ci[2] * cs;
+      fw = fw + x.ci[3] * cs;
+    }
     pf = (pf - fw) / fmax(ks - fw, kEps);
     fw = fw / fmax(ks, kEps);
     double ktot = kg + kgas;
@@ -1224,6 +1236,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   c.solar = solar;
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
+  c.ice = d.ice && !MIXED;
   LaneSrc src;
   src.rec = rec;
   src.br = br;
@@ -1243,20 +1256,23 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   const double alb_diff = bs[2], alb_dir = bs[3];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down, flux_inc_direct
 
-  // 1. the clear layers above ic, once
+  // 1. the clear layers above ic, once. Every load and store of the loop is
+  // unconditional (the prefetch index clamped, the checkpoint of a layer
+  // that starts no segment written to a spare row), so the compiler's wait
+  // counts see one path and the prefetch of layer i+1 stays in flight while
+  // layer i is computed.
   constexpr int kNst = RECOMP ? (SG_MAXL + kSeg - 1) / kSeg : SG_MAXL;
-  double st[kNst][RECOMP ? 3 : 5];
+  double st[kNst + 1][RECOMP ? 3 : 5];
   double fdir = solar ? finc : 0.0, rho = 0.0, dn = finc;
   const int ic1 = act ? ic : 0;
   {
     LayerIn cur, nxt;
-    if (ic1 > 0) load_layer<MIXED, GEN>(src, 0, cur);
+    load_layer<MIXED, GEN, true>(src, 0, cur);
     for (int i = 0; i < ic1; i++) {
-      if (i + 1 < ic1) load_layer<MIXED, GEN>(src, i + 1, nxt);
+      load_layer<MIXED, GEN, true>(src, min(i + 1, ic1 - 1), nxt);
       if (RECOMP) {
-        if (i % kSeg == 0) {
-          st[i / kSeg][0] = fdir; st[i / kSeg][1] = rho; st[i / kSeg][2] = dn;
-        }
+        const int row = i % kSeg == 0 ? i / kSeg : kNst;
+        st[row][0] = fdir; st[row][1] = rho; st[row][2] = dn;
       } else {
         st[i][4] = fdir;
       }
@@ -1271,13 +1287,15 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   }
 
   // 2. each solve from level ic down, back to level ic; weighted fluxes of
-  // levels ic..L in acc[lev - ic]
+  // levels ic..L in acc[lev - ic]. The downward sweep's loop loads and
+  // stores unconditionally (as step 1); the direct flux at each level is kept
+  // with the layer's elimination pair and added to acc on the way back up.
   double acc[SG_MAXL + 1][3];
   const int nacc = act ? L - ic : -1;
   for (int k = 0; k <= nacc; k++) { acc[k][0] = 0.0; acc[k][1] = 0.0; acc[k][2] = 0.0; }
   double W = 0.0;
   {
-    double st2[SG_MAXL][4];
+    double st2[SG_MAXL][5];
     const double sg_lw = (1.0 - alb_diff) * (bs[1] - br[(L - 1) * SG_NBQ + 3]);
     for (int j = 0; j < n_solve; j++) {
       const bool clr = clear_on && j == 0;
@@ -1291,14 +1309,14 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
       sj.csubf = src.csubf + (isub > 0 ? isub - 1 : 0);
       double f2 = fdir, r2 = rho, d2 = dn;
       LayerIn cur, nxt;
-      if (ic < L) load_layer<MIXED, GEN>(sj, ic, cur);
+      load_layer<MIXED, GEN>(sj, min(ic, L - 1), cur);
       for (int i = ic; i < L; i++) {
-        if (i > ic) acc[i - ic][2] += wj * f2;
-        if (i + 1 < L) load_layer<MIXED, GEN>(sj, i + 1, nxt);
+        load_layer<MIXED, GEN>(sj, min(i + 1, L - 1), nxt);
         double a, bb;
-        const double r0 = r2, d0 = d2;
+        const double r0 = r2, d0 = d2, f0 = f2;
         layer_step<MIXED, GEN>(c, t, cur, f2, r2, d2, a, bb);
         st2[i - ic][0] = a; st2[i - ic][1] = bb; st2[i - ic][2] = r0; st2[i - ic][3] = d0;
+        st2[i - ic][4] = f0;
         cur = nxt;
       }
       if (L > ic) acc[L - ic][2] += wj * f2;
@@ -1312,6 +1330,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
         fd = st2[lev - ic][2] * fu + st2[lev - ic][3];
         acc[lev - ic][0] += wj * fu;
         acc[lev - ic][1] += wj * fd;
+        if (lev > ic) acc[lev - ic][2] += wj * st2[lev - ic][4];
       }
     }
     if (act) acc[0][2] = W * fdir;   // the direct beam at level ic, common to all
@@ -1330,16 +1349,19 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
     for (int k = 0; k < kSeg; k++) { sa[k] = 0.0; sb[k] = 0.0; sr[k] = 0.0; sd[k] = 0.0; sf[k] = 0.0; }
     if (l0 < ic1) {
       if (RECOMP) {
+        // the segment's layers again from its checkpoint, each layer's
+        // inputs loaded (unconditionally, index clamped) while the one above
+        // is computed
         double fs = st[blk][0], rs = st[blk][1], ds = st[blk][2];
+        LayerIn xa, xb;
+        load_layer<MIXED, GEN, true>(src, l0, xa);
 #pragma unroll
         for (int k = 0; k < kSeg; k++) {
           const int i = l0 + k;
+          if (k + 1 < kSeg) load_layer<MIXED, GEN, true>(src, min(i + 1, ic1 - 1), xb);
           sr[k] = rs; sd[k] = ds; sf[k] = fs;
-          if (i < ic1) {
-            LayerIn xl;
-            load_layer<MIXED, GEN>(src, i, xl);
-            layer_step<MIXED, GEN>(c, t, xl, fs, rs, ds, sa[k], sb[k]);
-          }
+          if (i < ic1) layer_step<MIXED, GEN>(c, t, xa, fs, rs, ds, sa[k], sb[k]);
+          xa = xb;
         }
       } else {
 #pragma unroll
@@ -1452,6 +1474,7 @@ __device__ __forceinline__ void solve_body_items(const CallDims &d, const SgSpec
   c.solar = solar;
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
+  c.ice = d.ice && !MIXED;
   LaneSrc src;
   src.rec = rec;
   src.br = br;
@@ -1863,6 +1886,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   c.solar = solar;
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
+  c.ice = false;
   const double alb_diff = bs[2], alb_dir = bs[3];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down (= flux_inc_direct)
 #define TWS(i, q) ws[((size_t)(i) * kTriQ + (q)) * kWave]
@@ -2117,6 +2141,22 @@ struct BatchHdr {
   long long tot[kNWork];
 };
 
+// A call's arguments with every input pointer seen as global (G).
+__device__ __forceinline__ SocgpuRunesArgs gargs(const SocgpuRunesArgs &a) {
+  SocgpuRunesArgs g = a;
+  g.p_layer = G(a.p_layer); g.t_layer = G(a.t_layer); g.mass = G(a.mass); g.density = G(a.density);
+  g.layer_heat_capacity = G(a.layer_heat_capacity); g.h2o = G(a.h2o); g.o3 = G(a.o3);
+  g.p_level = G(a.p_level); g.t_level = G(a.t_level); g.t_ground = G(a.t_ground);
+  g.cos_zenith_angle = G(a.cos_zenith_angle); g.solar_irrad = G(a.solar_irrad);
+  g.albedo_diff = G(a.albedo_diff); g.albedo_dir = G(a.albedo_dir);
+  g.cloud_frac = G(a.cloud_frac); g.liq_mmr = G(a.liq_mmr); g.liq_dim = G(a.liq_dim);
+  g.conv_frac = G(a.conv_frac); g.liq_conv_mmr = G(a.liq_conv_mmr); g.liq_conv_dim = G(a.liq_conv_dim);
+  g.rand_seed = G(a.rand_seed); g.flux_ground = G(a.flux_ground); g.co2 = G(a.co2);
+  g.ice_mmr = G(a.ice_mmr); g.ice_dim = G(a.ice_dim); g.ice_conv_mmr = G(a.ice_conv_mmr);
+  g.ice_conv_dim = G(a.ice_conv_dim);
+  return g;
+}
+
 __device__ __forceinline__ int find_req(const BatchHdr *h, const ReqDesc *rq, int w, long long g) {
   int lo = 0, hi = h->R - 1;
   while (lo < hi) {
@@ -2152,35 +2192,38 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
   for (long long tile = blockIdx.x; tile < tot; tile += gridDim.x) {
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_PTILE, tile));
     const ReqDesc &q = rq[r];
+    const SocgpuRunesArgs a = gargs(q.a);
+    const SgSpec *spec = G(q.spec);
+    double *colrec = G(q.colrec), *grec = G(q.grec), *tlev = G(q.tlev);
     const int P = q.d.P, L = q.d.L;
     const int t = (int)(tile - q.s[W_PTILE]);
     const int ntc = (P + kPackC - 1) / kPackC;
     const int c0 = (t % ntc) * kPackC, i0 = (t / ntc) * kPackL;
     const int col = c0 + tx, i = i0 + ty;
     if (col < P && i < L) {
-      if (pack_layer(q.d, q.spec, q.a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ))
+      if (pack_layer(q.d, spec, a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ))
         atomicMax(&((BatchHdr *)h)->err, r + 1);
       if (q.d.gx) {
         // gas record: look-up factors and the density (continua)
-        double *g = q.grec + ((size_t)col * L + i) * SG_NGQ;
+        double *g = grec + ((size_t)col * L + i) * SG_NGQ;
         const int idx = i * P + col;
-        if (q.spec->has_lut) lookup_factors(q.spec, q.a.p_layer[idx], q.a.t_layer[idx], g);
-        g[SG_G_RHO] = q.a.density ? q.a.density[idx] : 0.0;
+        if (spec->has_lut) lookup_factors(spec, a.p_layer[idx], a.t_layer[idx], g);
+        g[SG_G_RHO] = a.density ? a.density[idx] : 0.0;
       }
     }
-    if (col < P && i <= L) shl[tx * kPackL + ty] = q.a.t_level ? q.a.t_level[(size_t)i * P + col] : 0.0;
-    if (col < P && i == 0) pack_column(q.d, q.a, col, q.colscal, q.bandscal);
+    if (col < P && i <= L) shl[tx * kPackL + ty] = a.t_level ? a.t_level[(size_t)i * P + col] : 0.0;
+    if (col < P && i == 0) pack_column(q.d, a, col, G(q.colscal), G(q.bandscal));
     __syncthreads();
     const int nl = L - i0 < kPackL ? L - i0 : kPackL;
     if (nl > 0)
       for (int e = threadIdx.x; e < kPackC * kPackL * SG_NQ; e += kPackC * kPackL) {
         const int cc = e / (kPackL * SG_NQ), off = e - cc * (kPackL * SG_NQ);
         if (c0 + cc < P && off < nl * SG_NQ)
-          q.colrec[((size_t)(c0 + cc) * L + i0) * SG_NQ + off] = sh[cc * kPackL * SG_NQ + off];
+          colrec[((size_t)(c0 + cc) * L + i0) * SG_NQ + off] = sh[cc * kPackL * SG_NQ + off];
       }
     {
       const int e = threadIdx.x, cc = e / kPackL, off = e % kPackL;
-      if (c0 + cc < P && i0 + off <= L) q.tlev[(size_t)(c0 + cc) * (L + 1) + i0 + off] = shl[e];
+      if (c0 + cc < P && i0 + off <= L) tlev[(size_t)(c0 + cc) * (L + 1) + i0 + off] = shl[e];
     }
     __syncthreads();
   }
@@ -2214,18 +2257,19 @@ __global__ void b_cldrange_kernel(const BatchHdr *h, const ReqDesc *rq) {
     if (g < tot) {
       r = find_req(h, rq, W_CCOL, g);
       const ReqDesc &q = rq[r];
+      const SocgpuRunesArgs a = gargs(q.a);
       const int idx = (int)(g - q.s[W_CCOL]), P = q.d.P, L = q.d.L;
       const int col = idx % P, i0 = (idx / P) * kRangeChunk;
       const int i1 = i0 + kRangeChunk < L ? i0 + kRangeChunk : L;
       if (q.d.raw_range) {
         for (int i = i0; i < i1; i++) {
-          const double w = q.a.cloud_frac[(size_t)i * P + col];
+          const double w = a.cloud_frac[(size_t)i * P + col];
           if (w > kEps && kmin == INT32_MAX) kmin = i + 1;
           if (w >= kEps) kmax = i + 1;
         }
       } else {
         for (int i = i0; i < i1; i++) {
-          if (layer_cloudy(q.d, q.a, (size_t)i * P + col)) {
+          if (layer_cloudy(q.d, a, (size_t)i * P + col)) {
             kmin = kmin < i + 1 ? kmin : i + 1;
             kmax = i + 1;
           }
@@ -2265,7 +2309,8 @@ __global__ __launch_bounds__(64 * kCgWaves) void b_cloudgen_kernel(const BatchHd
     const ReqDesc &q0 = rq[__builtin_amdgcn_readfirstlane(find_req(h, rq, W_CG, base))];
     const int nx0 = q0.n1 * q0.n2;
     if (q0.xcw != loaded && nx0 <= kXcwMax) {
-      for (int e = threadIdx.x; e < nx0; e += blockDim.x) sx[e] = q0.xcw[e];
+      const double *x0 = G(q0.xcw);
+      for (int e = threadIdx.x; e < nx0; e += blockDim.x) sx[e] = x0[e];
       loaded = q0.xcw;
     }
     __syncthreads();
@@ -2273,8 +2318,8 @@ __global__ __launch_bounds__(64 * kCgWaves) void b_cloudgen_kernel(const BatchHd
     if (w < tot) {
       const ReqDesc &q = rq[__builtin_amdgcn_readfirstlane(find_req(h, rq, W_CG, w))];
       const double *tab = (q.xcw == loaded && q.n1 * q.n2 <= kXcwMax) ? sx : q.xcw;
-      cloudgen_body(q.d, q.colrec, q.a.rand_seed, q.dp_cf, q.dp_cw, q.krange, tab, q.n1, q.n2, q.csub,
-                    q.colscal, (int)(w - q.s[W_CG]), lane);
+      cloudgen_body(q.d, G(q.colrec), G(q.a.rand_seed), q.dp_cf, q.dp_cw, G(q.krange), tab, q.n1, q.n2,
+                    G(q.csub), G(q.colscal), (int)(w - q.s[W_CG]), lane);
     }
     __syncthreads();
   }
@@ -2294,9 +2339,9 @@ __global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? SG_BAND_OCC : 2) void 
     const ReqDesc &q = rq[r];
     const int nch = (q.d.L + kWave - 1) / kWave;
     if ((NCH == 2 && nch <= 2) || (NCH > 2 && nch > 2))
-      band_body<NCH, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.a.gas_mix_ratio, q.a.gas_field, q.tlev,
-                          q.colscal,
-                     q.bandrec, q.bandscal, q.cldopt, q.krange, (int)(w - q.s[W_MINOR]), lane);
+      band_body<NCH, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), q.a.gas_mix_ratio, q.a.gas_field,
+                          G(q.tlev), G(q.colscal), G(q.bandrec), G(q.bandscal), G(q.cldopt), G(q.krange),
+                          (int)(w - q.s[W_MINOR]), lane);
   }
 }
 
@@ -2311,11 +2356,13 @@ __global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, co
     const ReqDesc &q = rq[r];
     if ((q.d.gen != 0) != GEN) continue;   // the other build's call
     if (MIXED && GEN)
-      solve_body_items<MIXED, RECOMP, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec,
-                                           q.bandscal, q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
+      solve_body_items<MIXED, RECOMP, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), G(q.colscal),
+                                           G(q.bandrec), G(q.bandscal), G(q.csub), G(q.cldopt), G(q.partial),
+                                           (int)(w - q.s[W_SOLVE]), lane);
     else
-      solve_body<MIXED, RECOMP, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec, q.bandscal,
-                                     q.csub, q.cldopt, q.partial, (int)(w - q.s[W_SOLVE]), lane);
+      solve_body<MIXED, RECOMP, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), G(q.colscal),
+                                     G(q.bandrec), G(q.bandscal), G(q.csub), G(q.cldopt), G(q.partial),
+                                     (int)(w - q.s[W_SOLVE]), lane);
   }
 }
 
@@ -2342,7 +2389,7 @@ __global__ __launch_bounds__(kFinC * kFinL) void b_finish_kernel(const BatchHdr 
     {
       const int col = c0 + tb, lev = l0 + ta;   // levels fastest
       if (col < P && lev <= L)
-        finish_body(q.d, q.partial, q.bandrec, q.bandscal, col, lev, sh[0][tb][ta], sh[1][tb][ta],
+        finish_body(q.d, G(q.partial), G(q.bandrec), G(q.bandscal), col, lev, sh[0][tb][ta], sh[1][tb][ta],
                     sh[2][tb][ta]);
     }
     __syncthreads();
@@ -2350,9 +2397,9 @@ __global__ __launch_bounds__(kFinC * kFinL) void b_finish_kernel(const BatchHdr 
       const int col = c0 + ta, lev = l0 + tb;   // profiles fastest
       if (col < P && lev <= L) {
         const size_t idx = (size_t)lev * P + col;
-        q.up[idx] = sh[0][ta][tb];
-        q.dn[idx] = sh[1][ta][tb];
-        if (q.dir) q.dir[idx] = sh[2][ta][tb];
+        G(q.up)[idx] = sh[0][ta][tb];
+        G(q.dn)[idx] = sh[1][ta][tb];
+        if (q.dir) G(q.dir)[idx] = sh[2][ta][tb];
       }
     }
     __syncthreads();
@@ -2364,7 +2411,7 @@ __global__ void b_overlap_kernel(const BatchHdr *h, const ReqDesc *rq) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
        g += (long long)gridDim.x * blockDim.x) {
     const ReqDesc &q = rq[find_req(h, rq, W_TOVL, g)];
-    overlap_body(q.d, q.colrec, q.krange, q.a.cloud_vertical_decorr, q.a.conv_vertical_decorr, q.ovl,
+    overlap_body(q.d, G(q.colrec), G(q.krange), q.a.cloud_vertical_decorr, q.a.conv_vertical_decorr, G(q.ovl),
                  (int)(g - q.s[W_TOVL]));
   }
 }
@@ -2377,8 +2424,9 @@ __global__ __launch_bounds__(256) void b_tsolve_kernel(const BatchHdr *h, const 
   for (long long w = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; w < tot; w += nw) {
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_TSOLVE, w));
     const ReqDesc &q = rq[r];
-    tri_solve_body<MIXED, GEN>(q.d, q.spec, q.l_rayleigh, q.colrec, q.grec, q.colscal, q.bandrec, q.bandscal, q.cldopt,
-                          q.ovl, q.krange, q.tws, q.partial, (int)(w - q.s[W_TSOLVE]), lane);
+    tri_solve_body<MIXED, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), G(q.colscal), G(q.bandrec),
+                               G(q.bandscal), G(q.cldopt), G(q.ovl), G(q.krange), G(q.tws), G(q.partial),
+                               (int)(w - q.s[W_TSOLVE]), lane);
   }
 }
 
@@ -2387,7 +2435,8 @@ __global__ void b_heating_kernel(const BatchHdr *h, const ReqDesc *rq) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < tot;
        g += (long long)gridDim.x * blockDim.x) {
     const ReqDesc &q = rq[find_req(h, rq, W_LAY, g)];
-    if (q.hr) heating_body(q.d.P, q.d.L, q.up, q.dn, q.a.layer_heat_capacity, q.hr, (int)(g - q.s[W_LAY]));
+    if (q.hr)
+      heating_body(q.d.P, q.d.L, G(q.up), G(q.dn), G(q.a.layer_heat_capacity), G(q.hr), (int)(g - q.s[W_LAY]));
   }
 }
 
@@ -2546,9 +2595,33 @@ struct SpecEntry {
 constexpr size_t kScratchBytes = SOCGPU_NOALLOC_MAX_BYTES;
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
+// A second stream beside a launch stream, and the events that fork the cloud
+// generator onto it after the cloud range is known and join it back before
+// the solve: the generator (one wave per column) and the band kernel (one
+// wave per column and band) depend on the same inputs but not on each
+// other, and both spend most of their time waiting on memory, so they share
+// the device. Streams are created with the default priority.
+struct Fork {
+  hipStream_t side = nullptr;
+  hipEvent_t f = nullptr, j = nullptr;
+  bool make() {
+    return hipStreamCreateWithFlags(&side, hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess &&
+           hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+  }
+  void drop() {
+    if (f) (void)hipEventDestroy(f);
+    if (j) (void)hipEventDestroy(j);
+    if (side) (void)hipStreamDestroy(side);
+    f = j = nullptr;
+    side = nullptr;
+  }
+};
+
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
+  Fork fork;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::map<std::string, std::unique_ptr<SpecEntry>> spectra;
   DevBuf ws, desc;
@@ -2590,6 +2663,7 @@ int socgpu_init(int device) {
   HIPCHK(hipEventCreate(&g_ctx.ev1));
   HIPCHK(hipEventCreate(&g_ctx.ev2));
   HIPCHK(hipEventCreate(&g_ctx.ev3));
+  if (!g_ctx.fork.make()) return fail(SOCGPU_EDEVICE, "stream or event creation failed");
   // the allocation-free entry point's workspace on this device, and a pinned
   // word its error flag comes back in (allocated here, once, so that the
   // entry point itself allocates nothing)
@@ -3015,7 +3089,7 @@ static SolveKernel solve_kernel_for(bool mixed, bool recomp, int occ, bool gen) 
 }
 
 static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long tot[kNWork], int R,
-                        int precision, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b) {
+                        int precision, hipStream_t s, const Fork *fk, hipEvent_t ev_a, hipEvent_t ev_b) {
   const int T = 256;
   const bool gen = tot[W_GEN] > 0;
   auto blocks = [](long long n, int per) {
@@ -3025,7 +3099,17 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
   b_init_kernel<<<(R + 63) / 64, 64, 0, s>>>(hdr, rq);
   b_pack_kernel<<<blocks(tot[W_PTILE], 1), kPackC * kPackL, 0, s>>>(hdr, rq);
   if (tot[W_CCOL] > 0) b_cldrange_kernel<<<blocks(tot[W_CCOL], T), T, 0, s>>>(hdr, rq);
-  if (tot[W_CG] > 0) b_cloudgen_kernel<<<blocks(tot[W_CG], kCgWaves), 64 * kCgWaves, 0, s>>>(hdr, rq);
+  // the cloud generator on the side stream when there is one, beside the
+  // band kernel; joined before the solve
+  const bool fork = fk && fk->side && tot[W_CG] > 0;
+  if (fork) {
+    HIPCHK(hipEventRecord(fk->f, s));
+    HIPCHK(hipStreamWaitEvent(fk->side, fk->f, 0));
+    b_cloudgen_kernel<<<blocks(tot[W_CG], kCgWaves), 64 * kCgWaves, 0, fk->side>>>(hdr, rq);
+    HIPCHK(hipEventRecord(fk->j, fk->side));
+  } else if (tot[W_CG] > 0) {
+    b_cloudgen_kernel<<<blocks(tot[W_CG], kCgWaves), 64 * kCgWaves, 0, s>>>(hdr, rq);
+  }
   {
     // calls of up to 128 layers, then the longer ones (each kernel skips the
     // other's waves)
@@ -3037,6 +3121,7 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
     }
   }
   if (tot[W_TOVL] > 0) b_overlap_kernel<<<blocks(tot[W_TOVL], T), T, 0, s>>>(hdr, rq);
+  if (fork) HIPCHK(hipStreamWaitEvent(s, fk->j, 0));
   if (ev_a) HIPCHK(hipEventRecord(ev_a, s));
   const SocgpuTile &tile = g_tile[precision == SOCGPU_MIXED ? 1 : 0];
   const int wpb = tile.waves_per_block;
@@ -3113,7 +3198,7 @@ int socgpu_runes(const char *spectrum_name, const SocgpuRunesArgs *a, SocgpuDiag
   HIPCHK(hipMemcpyAsync(g_ctx.desc.p, &host, sizeof host, hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)g_ctx.desc.p;
   const ReqDesc *dq = (const ReqDesc *)((char *)g_ctx.desc.p + offsetof(decltype(host), q));
-  if (int rc = launch_batch(dh, (ReqDesc *)dq, host.h.tot, 1, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
+  if (int rc = launch_batch(dh, (ReqDesc *)dq, host.h.tot, 1, precision, s, &g_ctx.fork, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
   int herr = 0;
   HIPCHK(hipMemcpyAsync(&herr, &dh->err, sizeof herr, hipMemcpyDeviceToHost, s));
@@ -3204,7 +3289,7 @@ int socgpu_runes_noalloc(const char *spectrum_name, const SocgpuRunesArgs *a, So
   DescBlob *dd = (DescBlob *)g_ctx.scratch_dev;
   HIPCHK(hipEventRecord(g_ctx.ev0, s));
   b_desc_store_kernel<<<1, 64, 0, s>>>(host, dd);
-  if (int rc = launch_batch(&dd->h, &dd->q, host.h.tot, 1, precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
+  if (int rc = launch_batch(&dd->h, &dd->q, host.h.tot, 1, precision, s, &g_ctx.fork, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
   HIPCHK(hipMemcpyAsync(g_ctx.err_host, &dd->h.err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -3421,6 +3506,7 @@ struct SocgpuBatch {
   sg::HostBuf in_host, out_host;
   char *pinned = nullptr;
   hipStream_t stream = nullptr;
+  Fork fork;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -3451,7 +3537,7 @@ int socgpu_runes_batch(int n_request, const SocgpuRequest *requests, int precisi
   HIPCHK(hipMemcpyAsync(g_ctx.desc.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)g_ctx.desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)g_ctx.desc.p + sizeof(BatchHdr));
-  if (int rc = launch_batch(dh, dq, pl.h.tot, pl.n(), precision, s, g_ctx.ev1, g_ctx.ev2)) return rc;
+  if (int rc = launch_batch(dh, dq, pl.h.tot, pl.n(), precision, s, &g_ctx.fork, g_ctx.ev1, g_ctx.ev2)) return rc;
   HIPCHK(hipEventRecord(g_ctx.ev3, s));
   int herr = 0;
   HIPCHK(hipMemcpyAsync(&herr, &dh->err, sizeof herr, hipMemcpyDeviceToHost, s));
@@ -3474,7 +3560,7 @@ int socgpu_batch_create(SocgpuBatch **batch, int precision) {
   auto *b = new SocgpuBatch();
   b->precision = precision;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess || !b->fork.make()) {
     delete b;
     return fail(SOCGPU_EDEVICE, "stream or event creation failed");
   }
@@ -3531,7 +3617,7 @@ int socgpu_batch_run(SocgpuBatch *b, int n_request, const SocgpuRequest *request
     batch_drop_graph(b);
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = launch_batch(dh, dq, b->cap, b->cap_desc / sizeof(ReqDesc) + 1, b->precision, s, nullptr, nullptr);
+    int rc = launch_batch(dh, dq, b->cap, b->cap_desc / sizeof(ReqDesc) + 1, b->precision, s, &b->fork, nullptr, nullptr);
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(s, &g);
     if (rc) return rc;
@@ -3578,6 +3664,7 @@ int socgpu_batch_destroy(SocgpuBatch *b) {
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->fork.drop();
   delete b;
   return 0;
 }
@@ -3606,6 +3693,7 @@ struct ServeItem {
 
 struct ServeSlot {
   hipStream_t stream = nullptr;
+  Fork fork;
   hipEvent_t done = nullptr;
   DevBuf ws, in_stage, out_stage, desc;
   char *out_host_used = nullptr;  // the pinned output block of the batch in flight, or null
@@ -3700,7 +3788,7 @@ static int serve_launch(SocgpuServer *sv, ServeSlot &sl) {
   HIPCHK(hipMemcpyAsync(sl.desc.p, sl.pinned, desc_bytes(nq), hipMemcpyHostToDevice, s));
   const BatchHdr *dh = (const BatchHdr *)sl.desc.p;
   ReqDesc *dq = (ReqDesc *)((char *)sl.desc.p + sizeof(BatchHdr));
-  if (int rc = launch_batch(dh, dq, sl.plan.h.tot, nq, sv->cfg.precision, s, nullptr, nullptr)) return rc;
+  if (int rc = launch_batch(dh, dq, sl.plan.h.tot, nq, sv->cfg.precision, s, &sl.fork, nullptr, nullptr)) return rc;
   HIPCHK(hipMemcpyAsync(&((BatchHdr *)sl.pinned)->err, &dh->err, sizeof(int), hipMemcpyDeviceToHost, s));
   if (int rc = copy_back_issue(sl.plan, sl.out_stage.as<double>(), sl.out_host_used, s)) return rc;
   HIPCHK(hipEventRecord(sl.done, s));
@@ -3777,7 +3865,7 @@ int socgpu_serve_start(SocgpuServer **server, const SocgpuServeConfig *cfg) {
   sv->slots.resize(cfg->n_slots);
   for (auto &sl : sv->slots) {
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess || !sl.fork.make()) {
       socgpu_serve_stop(sv);  // releases what the earlier slots hold
       return fail(SOCGPU_EDEVICE, "stream or event creation failed");
     }
@@ -3884,6 +3972,7 @@ int socgpu_serve_stop(SocgpuServer *sv) {
     if (sl.pinned) (void)hipHostFree(sl.pinned);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    sl.fork.drop();
   }
   delete sv;
   return 0;
@@ -4152,6 +4241,7 @@ int socgpu_finalize(void) {
   (void)hipEventDestroy(g_ctx.ev0); (void)hipEventDestroy(g_ctx.ev1);
   (void)hipEventDestroy(g_ctx.ev2); (void)hipEventDestroy(g_ctx.ev3);
   (void)hipStreamDestroy(g_ctx.stream);
+  g_ctx.fork.drop();
   g_ctx.device = -1;
   return 0;
 }
