@@ -55,6 +55,12 @@ extern "C" {
 #define SOCGPU_INHOM_MCICA 2         /* ip_mcica */
 #define SOCGPU_MCICA_SINGLE 1        /* ip_mcica_single_sampling */
 #define SOCGPU_MCICA_OPTIMAL 2       /* ip_mcica_optimal_sampling */
+/* treatment of scattering, runes' i_scatter_method (rad_pcf.F90:539-547) */
+#define SOCGPU_SCATTER_FULL 1        /* ip_scatter_full (default) */
+#define SOCGPU_SCATTER_NONE 2        /* ip_no_scatter_abs: absorption only */
+#define SOCGPU_SCATTER_EXT 3         /* ip_no_scatter_ext: extinction as absorption */
+#define SOCGPU_SCATTER_APPROX 4      /* ip_scatter_approx */
+#define SOCGPU_SCATTER_HYBRID 5      /* ip_scatter_hybrid: per k-term, from the file */
 
 /* Inputs of one radiation call: the subset of runes()'s optional arguments
  * on the two-stream flux path (socrates_runes.F90:47-168). A NULL pointer is
@@ -133,6 +139,15 @@ typedef struct SocgpuRunesArgs {
    * absorption (l_continuum false; runes always includes the file's). */
   int l_ir_source_linear;
   int l_no_continuum;
+  /* Treatment of scattering (socrates_runes.F90:589-591, 0 = full). Other
+   * than full, minor gases with one k-term are added to the grey extinction
+   * (l_grey_single, socrates_set_control.F90:272-277). SCATTER_NONE / EXT
+   * (and hybrid terms the file marks so) solve without scattering
+   * (two_coeff_fast_lw.F90, solver_no_scat.F90): infra-red only, clear sky
+   * and McICA; APPROX is full scattering on those paths. The triple and
+   * mixed columns take full scattering only (their approximate and
+   * non-scattering region solvers are not built). */
+  int i_scatter_method;
 } SocgpuRunesArgs;
 
 #define SOCGPU_GAS_N2O 0

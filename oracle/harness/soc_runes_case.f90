@@ -23,6 +23,7 @@ program soc_runes_case
   character(len=2) :: spec_name
   integer :: iu, n_profile, n_layer, i_source, l_cloud, i_overlap, l_rayleigh
   integer :: magic, version, n_band, i_st_water, i_st_ice, i_cloud_repr
+  integer :: i_scat_method
   integer :: i_inhom, i_mcica_sampling, n_repeat, irep, n_subcol_gen
   integer :: i_solver_kind, i, lfg, lco2
   logical :: l_clear_diag
@@ -134,6 +135,11 @@ program soc_runes_case
     read(iu) lice
     if (lice /= 0) read(iu) ice_mmr, ice_dim, ice_conv_mmr, ice_conv_dim
   end if
+  i_scat_method = 1
+  if (version >= 7) then
+    read(iu) i_scat_method
+    if (i_scat_method < 1) i_scat_method = 1
+  end if
   close(iu)
 
   allocate(flux_up(n_profile, 0:n_layer), flux_down(n_profile, 0:n_layer), &
@@ -188,7 +194,8 @@ program soc_runes_case
         t_ground=t_ground, l_flux_ground=(lfg /= 0), flux_ground=flux_ground, &
         cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
         albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
-        l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity)
+        l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
+        i_scatter_method=i_scat_method)
     else if (i_solver_kind == 0) then
       ! McICA sub-columns over a homogeneous cloud representation
       call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &
@@ -212,7 +219,8 @@ program soc_runes_case
         i_cloud_representation=i_cloud_repr, i_overlap=i_overlap, &
         i_inhom=i_inhom, i_mcica_sampling=i_mcica_sampling, &
         i_st_water=i_st_water, i_st_ice=i_st_ice, &
-        n_subcol_gen=n_subcol_gen, rand_seed=rand_seed)
+        n_subcol_gen=n_subcol_gen, rand_seed=rand_seed, &
+        i_scatter_method=i_scat_method)
     else
       ! Triple-column (clear / stratiform / convective) overlap
       call runes(n_profile, n_layer, diag, spectrum_name=spec_name, &

@@ -32,7 +32,7 @@ class _Case(ctypes.Structure):
                             ("wm_field", _D * 10), ("i_st_ice", ctypes.c_int),
                             ("ice_mmr", _D), ("ice_dim", _D), ("ice_conv_mmr", _D),
                             ("ice_conv_dim", _D), ("l_ir_source_linear", ctypes.c_int),
-                            ("l_no_continuum", ctypes.c_int)]
+                            ("l_no_continuum", ctypes.c_int), ("i_scatter_method", ctypes.c_int)]
 
 
 def build():

@@ -43,6 +43,9 @@ static double jit_exp(double x) {
 
 /* option codes, rad_pcf.F90 */
 enum { IP_SOLAR = 1, IP_INFRA_RED = 2 };
+/* rad_pcf.F90:539-547 */
+enum { IP_SCATTER_FULL = 1, IP_NO_SCATTER_ABS = 2, IP_NO_SCATTER_EXT = 3, IP_SCATTER_APPROX = 4,
+       IP_SCATTER_HYBRID = 5 };
 /* gas identifiers (gas_list_pcf.F90) */
 enum { IP_H2O = 1, IP_CO2 = 2, IP_O3 = 3, IP_N2O = 4, IP_CH4 = 6, IP_O2 = 7, IP_SO2 = 9,
        IP_CFC11 = 14, IP_CFC12 = 15, IP_CFC113 = 16, IP_HCFC22 = 17, IP_HFC134A = 19,
@@ -74,6 +77,7 @@ typedef struct {
   int i_band_k[NB][NG], i_scale_k[NB][NG], i_scale_fnc[NB][NG];
   double p_ref[NB][NG], t_ref[NB][NG];
   double k[NB][NG][NK], w[NB][NG][NK], scale[NB][NG][NK][3];
+  int i_scat[NB][NG][NK]; /* the term's treatment of scattering (hybrid) */
   /* P/T look-up tables of ip_scale_lookup terms (read_spectrum.F90:1404-1507):
    * ln p_lookup, t_lookup(it, ip) as [ip][it], k_lookup per (band, gas) as
    * [term][ip][it] */
@@ -278,6 +282,7 @@ static int read_spectrum(const char *path, spec_t *s) {
             next_line(f, ln, sizeof ln);
             s->k[band][gas][k] = fld(ln, 3, 16);
             s->w[band][gas][k] = fld(ln, 22, 16);
+            s->i_scat[band][gas][k] = (int)fld(ln, 38, 3);
             if (nsv >= 1) s->scale[band][gas][k][0] = fld(ln, 41, 16);
             if (nsv >= 2) s->scale[band][gas][k][1] = fld(ln, 60, 16);
             if (nsv >= 3) {
@@ -575,6 +580,37 @@ void soc_oracle_solver_homogen_direct(int n_layer, const double *trans,
   free(beta);
 }
 
+/* two_coeff_fast_lw.F90:62-113: non-scattering transmission and infra-red
+ * source coefficients (the quadratic one when quad) */
+static void two_coeff_fast_lw_one(double tau, int quad, coeff_t *o) {
+  const double sq_eps_r = sqrt(DBL_EPSILON), tol = sqrt(sq_eps_r);
+  double trans_in = -1.66e+00 * tau;
+  o->trans = exp(trans_in);
+  o->reflect = 0.0;
+  o->trans_0 = 0.0;
+  o->sc1 = (1.0 - o->trans + sq_eps_r) / (1.66e+00 * tau + sq_eps_r);
+  o->sc2 = 0.0;
+  if (quad) {
+    if (tau > tol)
+      o->sc2 = -(1.0 + o->trans - 2.0 * o->sc1) / (1.66e+00 * tau + sq_eps_r);
+    else
+      o->sc2 = -(1.0 + o->trans - 2.0 + 1.66e+00 * tau) / (1.66e+00 * tau + sq_eps_r);
+  }
+}
+
+/* solver_no_scat.F90:60-97 for one column; flux_total as for
+ * soc_oracle_solver_homogen_direct */
+static void solver_no_scat(int n_layer, const double *trans, const double *s_down,
+                           const double *s_up, double diffuse_albedo, double flux_inc_down,
+                           double source_ground, double *flux_total) {
+  flux_total[1] = flux_inc_down;
+  for (int i = 1; i <= n_layer; i++)
+    flux_total[2 * i + 1] = s_down[i - 1] + trans[i - 1] * flux_total[2 * i - 1];
+  flux_total[2 * n_layer] = source_ground + diffuse_albedo * flux_total[2 * n_layer + 1];
+  for (int i = n_layer; i >= 1; i--)
+    flux_total[2 * i - 2] = s_up[i - 1] + trans[i - 1] * flux_total[2 * i];
+}
+
 /* two_stream.F90:160-205 with ir_source.F90:93-120, solar_source.F90:
  * 104-162 and column_solver.F90:160-185 (no spherical geometry, no
  * orography, direct tau unscaled: the default control). */
@@ -774,7 +810,7 @@ typedef struct {
  * (ir_source.F90:93-120). Coefficients are in w; fluxes out in w. */
 static void solve_column(int isolir, work_t *w, const double *adjust, const double *dplanck,
                          const double *dplanck2, double flux_inc_direct, double flux_inc_down,
-                         double alb_diff, double alb_dir, double d_planck_flux_surface) {
+                         double alb_diff, double alb_dir, double d_planck_flux_surface, int nosc) {
   const int L = w->L;
   double source_ground;
   if (isolir == IP_SOLAR) {
@@ -793,8 +829,14 @@ static void solve_column(int isolir, work_t *w, const double *adjust, const doub
     }
     source_ground = (1.0 - alb_diff) * d_planck_flux_surface;
   }
-  soc_oracle_solver_homogen_direct(L, w->trans, w->reflect, w->s_down, w->s_up, alb_diff,
-                                   flux_inc_down, source_ground, w->flux_total);
+  /* column_solver.F90:192-199: the non-scattering solver when scattering is
+   * ignored */
+  if (nosc)
+    solver_no_scat(L, w->trans, w->s_down, w->s_up, alb_diff, flux_inc_down, source_ground,
+                   w->flux_total);
+  else
+    soc_oracle_solver_homogen_direct(L, w->trans, w->reflect, w->s_down, w->s_up, alb_diff,
+                                     flux_inc_down, source_ground, w->flux_total);
 }
 
 /* Padé-2 droplet optics, opt_prop_pade_2_mod.F90:100-143 */
@@ -1236,6 +1278,25 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   if (c->l_cloud && !mcica && !triple) return -10;
   if (triple && c->i_overlap != 0 && c->i_overlap != 1 && c->i_overlap != 3) return -10;
   if (conv_strat && c->i_overlap == 1) return -10;
+  /* treatment of scattering: runes' default full; any other sets
+   * l_grey_single (socrates_set_control.F90:272-277). Ignoring scattering
+   * is an infra-red approximation (two_coeff_region_fast_lw.F90:116-120);
+   * the triple and mixed columns keep full scattering here (their region
+   * coefficients without scattering are not restated), and their
+   * approximate-scattering solvers are not either. */
+  const int scat = c->i_scatter_method > 0 ? c->i_scatter_method : IP_SCATTER_FULL;
+  if (scat > IP_SCATTER_HYBRID) return -13;
+  const int grey_single = scat != IP_SCATTER_FULL;
+  for (int b = 0; b < sp.n_band; b++) {
+    const int g = sp.index_absorb[b][0] - 1;
+    for (int k = 0; k < sp.i_band_k[b][g]; k++) {
+      int sm = scat == IP_SCATTER_HYBRID ? sp.i_scat[b][g][k] : scat;
+      if (sm <= 0) sm = IP_SCATTER_FULL;
+      if ((sm == IP_NO_SCATTER_ABS || sm == IP_NO_SCATTER_EXT) && (isolir == IP_SOLAR || triple))
+        return -13;
+      if (sm == IP_SCATTER_APPROX && triple) return -13;
+    }
+  }
   if (mcica) {
     rc = read_mcica(mcica_file, &mc);
     if (rc) return rc;
@@ -1573,6 +1634,12 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       for (int j = 1; j < ngas; j++) {
         int g = sp.index_absorb[b][j] - 1;
         int nk = sp.i_band_k[b][g];
+        /* a single k-term, with l_grey_single: its absorption is added to
+         * the grey extinction directly (solve_band_k_eqv_scl.F90:941-951) */
+        if (nk == 1 && grey_single) {
+          for (int i = 0; i < L; i++) kgrey[i] = kgrey[i] + KABS(j, 0, i);
+          continue;
+        }
         for (int i = 0; i < L; i++) kmin[i] = huge;
         if (isolir == IP_SOLAR) {
           double sflux = 0.0;
@@ -1714,9 +1781,24 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           flux_inc_down = -planck[0];
           dps = flux_ground - planck[L];
         }
+        /* the term's treatment of scattering (solve_band_k_eqv_scl.F90:
+         * 1053-1058) */
+        int sm = scat == IP_SCATTER_HYBRID ? sp.i_scat[b][gmaj][iex] : scat;
+        if (sm <= 0) sm = IP_SCATTER_FULL;
+        const int nosc = sm == IP_NO_SCATTER_ABS || sm == IP_NO_SCATTER_EXT;
+        const int quad = !c->l_ir_source_linear;
         /* single scattering + delta-rescaling, clear (single_scattering.F90:
-         * 114-128, rescale_tau_omega.F90:73-80 with zero forward scatter) */
+         * 114-128, rescale_tau_omega.F90:73-80 with zero forward scatter);
+         * without scattering the optical depth of absorption (:132-143) or
+         * of extinction (:147-158) and no albedo */
         for (int i = 0; i < L; i++) {
+          if (nosc) {
+            double tau = sm == IP_NO_SCATTER_ABS ? (k_grey_tot[i] + KABS(0, iex, i) - k_ext_scat[i])
+                                                 : (k_grey_tot[i] + KABS(0, iex, i));
+            tau_clr[i] = tau * LAY(c->mass, i, l);
+            om_clr[i] = 0.0;
+            continue;
+          }
           double k_total = k_grey_tot[i] + KABS(0, iex, i);
           double tau = k_total * LAY(c->mass, i, l);
           double om = k_ext_scat[i] / (k_total + tiny);
@@ -1725,10 +1807,12 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           om = om * (1.0 - 0.0) / (1.0 - om * 0.0);
           tau_clr[i] = tau; om_clr[i] = om;
         }
-        /* clear coefficients (two_stream / mcica_column.F90:266-298) */
+        /* clear coefficients (two_stream / mcica_column.F90:266-298; without
+         * scattering two_coeff_fast_lw, :287-298) */
         for (int i = 0; i < L; i++) {
           coeff_t o;
-          two_coeff_one(isolir, tau_clr[i], om_clr[i], 0.0, zen_0, &o);
+          if (nosc) two_coeff_fast_lw_one(tau_clr[i], quad, &o);
+          else two_coeff_one(isolir, tau_clr[i], om_clr[i], 0.0, zen_0, &o);
           w.trans[i] = o.trans; w.reflect[i] = o.reflect; w.trans_0[i] = o.trans_0;
           w.sc1[i] = o.sc1; w.sc2[i] = o.sc2;
         }
@@ -1754,7 +1838,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           FD = tri.flux_direct;
         } else {
           solve_column(isolir, &w, adjust, dpl, dpl2, flux_inc_direct, flux_inc_down, alb_diff,
-                       alb_dir, dps);
+                       alb_dir, dps, nosc);
           F = w.flux_total;
           FD = w.flux_direct;
         }
@@ -1793,14 +1877,21 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
               phf = phf + ki_phf[i] * cs;
               phf = (phf - fwd) / fmax(ksc - fwd, eps);
               fwd = fwd / fmax(ksc, eps);
-              double k_total = kgt + KABS(0, iex, i);
-              double tau = k_total * LAY(c->mass, i, l);
-              double om = ksc / (k_total + tiny);
-              om = fmin(om, 1.0 - 3.2e+01 * eps);
-              tau = tau * (1.0 - om * fwd);
-              om = om * (1.0 - fwd) / (1.0 - om * fwd);
               coeff_t o;
-              two_coeff_one(isolir, tau, om, phf, zen_0, &o);
+              if (nosc) {
+                /* mcica_column.F90:509-522 */
+                double tau = sm == IP_NO_SCATTER_ABS ? (kgt + KABS(0, iex, i) - ksc)
+                                                     : (kgt + KABS(0, iex, i));
+                two_coeff_fast_lw_one(tau * LAY(c->mass, i, l), quad, &o);
+              } else {
+                double k_total = kgt + KABS(0, iex, i);
+                double tau = k_total * LAY(c->mass, i, l);
+                double om = ksc / (k_total + tiny);
+                om = fmin(om, 1.0 - 3.2e+01 * eps);
+                tau = tau * (1.0 - om * fwd);
+                om = om * (1.0 - fwd) / (1.0 - om * fwd);
+                two_coeff_one(isolir, tau, om, phf, zen_0, &o);
+              }
               /* mcica_column.F90:360-520: zero then frac-weighted sum */
               wc.trans[i] = 0.0 + frac1 * o.trans;
               wc.reflect[i] = 0.0 + frac1 * o.reflect;
@@ -1809,7 +1900,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
               if (isolir == IP_SOLAR) wc.trans_0[i] = 0.0 + frac1 * o.trans_0;
             }
             solve_column(isolir, &wc, adjust, dpl, dpl2, flux_inc_direct, flux_inc_down,
-                         alb_diff, alb_dir, dps);
+                         alb_diff, alb_dir, dps, nosc);
             if (m == m0) {
               memcpy(acc_total, wc.flux_total, sizeof(double) * (2 * L + 2));
               memcpy(acc_direct, wc.flux_direct, sizeof(double) * (L + 1));

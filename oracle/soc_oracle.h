@@ -85,6 +85,11 @@ typedef struct {
   /* continuum absorption left out (control%l_continuum false; runes always
    * includes the file's continua) */
   int l_no_continuum;
+  /* treatment of scattering (control%i_scatter_method, rad_pcf.F90:539-547):
+   * 0 or 1 full, 2 scattering ignored (absorption only), 3 extinction taken
+   * as absorption, 4 approximate (as full here), 5 per k-term from the
+   * spectral file (hybrid) */
+  int i_scatter_method;
 } soc_case;
 
 enum { SOC_WM_N2O, SOC_WM_CH4, SOC_WM_O2, SOC_WM_SO2, SOC_WM_OCS, SOC_WM_CFC11, SOC_WM_CFC12,

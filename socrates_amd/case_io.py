@@ -14,12 +14,13 @@ import dataclasses
 import numpy as np
 
 MAGIC = 1397703491
-VERSION = 6     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
+VERSION = 7     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
                 # 3 adds l_flux_ground and flux_ground (n_band, n_profile) after them;
                 # 4 adds a flag and, when it is set, a CO2 field (n_layer, n_profile);
                 # 5 adds a flag per WM_GASES gas and, for each one set, its field;
                 # 6 adds a flag and, when it is set, ice_mmr, ice_dim, ice_conv_mmr,
-                #   ice_conv_dim (n_layer, n_profile)
+                #   ice_conv_dim (n_layer, n_profile);
+                # 7 adds i_scatter_method
 
 # further gases runes takes as well-mixed mixing ratios (the *_mix_ratio
 # arguments, socrates_runes.F90:69-83), in this order
@@ -91,6 +92,7 @@ class Case:
     ice_conv_dim: np.ndarray = None
     l_ir_source_linear: int = 0       # infra-red source linear in tau (not written to files)
     l_no_continuum: int = 0           # continuum absorption left out (not written to files)
+    i_scatter_method: int = 0         # treatment of scattering (0: runes' default, full)
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile
@@ -148,6 +150,7 @@ class Case:
                     v = getattr(self, k)
                     v = np.zeros((L, P)) if v is None else v
                     np.ascontiguousarray(v, dtype=np.float64).tofile(f)
+            np.array([int(self.i_scatter_method)], dtype=np.int32).tofile(f)
 
     @classmethod
     def read(cls, path):
@@ -204,6 +207,9 @@ class Case:
                     setattr(c, k, np.frombuffer(raw, dtype=np.float64, count=L * P,
                                                 offset=off).reshape(L, P).copy())
                     off += 8 * L * P
+        if hdr[1] >= 7:
+            c.i_scatter_method = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
+            off += 4
         return c
 
 

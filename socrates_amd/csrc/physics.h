@@ -367,6 +367,22 @@ SG_HD float one_minus_exp(float x, float e) {
   return x < 0.2f ? p : 1.0f - e;
 }
 
+// two_coeff_fast_lw (two_coeff_fast_lw.F90:62-113): the infra-red layer
+// without scattering (diffusivity 1.66): transmission, no reflection, and the
+// linear and quadratic source coefficients, in the reference's operation
+// order.
+SG_HD void two_coeff_fast_lw(double tau, Coef &o) {
+#pragma clang fp contract(off)
+  const double sq_eps_r = 1.4901161193847656e-08;  // sqrt(epsilon)
+  const double tol = 1.220703125e-04;               // sqrt(sq_eps_r)
+  o.T = exp(-1.66 * tau);
+  o.R = 0.0;
+  o.T0 = 0.0;
+  o.s1 = (1.0 - o.T + sq_eps_r) / (1.66 * tau + sq_eps_r);
+  o.s2 = tau > tol ? -(1.0 + o.T - 2.0 * o.s1) / (1.66 * tau + sq_eps_r)
+                   : -(1.0 + o.T - 2.0 + 1.66 * tau) / (1.66 * tau + sq_eps_r);
+}
+
 // Mixed-precision form of two_coeff (see two_coeff above for the reference
 // routines): the same two-stream solution evaluated in FP32 through forms
 // free of cancellation. With e = exp(-lambda tau), em = 1 - e,

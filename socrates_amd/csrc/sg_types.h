@@ -19,7 +19,7 @@
 #define SG_NDROP 8          // droplet types kept
 #define SG_NICE 16          // ice types kept
 #define SG_NDPARM 16        // droplet fit parameters
-#define SG_MAXL 160         // layers per column
+#define SG_MAXL 192         // layers per column
 #define SG_MAXSUBCOL 64     // generated McICA sub-columns per column
 #define SG_MAXCONT 2        // continua per band
 #define SG_MAXPRE 64        // pressures of the k-term look-up table
@@ -52,6 +52,9 @@ struct SgTerm {             // one k-term (ESFT term)
   int fnc;                  // SG_FNC_*
   int lut;                  // look-up table of the term (SG_FNC_LOOKUP), or
                             // the term's index in the Wenyi tables
+  int iscat;                // the term's treatment of scattering in the file
+                            // (i_scat: 0 or 1 full, 2 none, 3 extinction as
+                            // absorption), used when the call asks for hybrid
 };
 
 struct SgBand {

@@ -181,7 +181,17 @@ struct CallDims {
                      // with the ice beside the droplets in the FP64 form)
   int ir_lin;        // infra-red source linear in optical depth
   int no_cont;       // continuum absorption left out
+  int scat;          // treatment of scattering (SOCGPU_SCATTER_*)
+  int grey1;         // minor gases with one k-term added to the grey extinction
 };
+
+// A major k-term's treatment of scattering for a call with method scat:
+// SOCGPU_SCATTER_NONE or _EXT when the term is solved without scattering,
+// else 0 (solve_band_k_eqv_scl.F90:1053-1058).
+__host__ __device__ inline int term_nosc(int scat, int iscat) {
+  const int m = scat == SOCGPU_SCATTER_HYBRID ? iscat : scat;
+  return (m == SOCGPU_SCATTER_NONE || m == SOCGPU_SCATTER_EXT) ? m : 0;
+}
 
 // Caller arrays -> column records. set_atm (socrates_set_atm.F90:246-300),
 // set_cld (socrates_set_cld.F90:817-852: in-cloud mixing ratio, w_cloud
@@ -612,13 +622,13 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
   const int nch = (L + kWave - 1) / kWave;
   const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
   double lnp[kKeqChunks], lnt[kKeqChunks], mass[kKeqChunks], b0[kKeqChunks], b1[kKeqChunks];
-  double keq[kKeqChunks], adj[kKeqChunks];
+  double keq[kKeqChunks], adj[kKeqChunks], kgrey[kKeqChunks];
   const double *tl = tlev + (size_t)col * (L + 1);
 #pragma unroll
   for (int c = 0; c < kKeqChunks; c++) {
     const int i = c * kWave + lane;
     lnp[c] = 0.0; lnt[c] = 0.0; mass[c] = 0.0; b0[c] = 0.0; b1[c] = 0.0;
-    keq[c] = 0.0; adj[c] = 1.0;
+    keq[c] = 0.0; adj[c] = 1.0; kgrey[c] = 0.0;
     if (c < nch && i < L) {
       const double *r = rec + (size_t)i * SG_NQ;
       lnp[c] = r[SG_Q_LNP]; lnt[c] = r[SG_Q_LNT]; mass[c] = r[SG_Q_MASS];
@@ -674,6 +684,13 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
           kk[c] = 0.0;
         }
         kmin[c] = fmin(kmin[c], kk[c]);
+      }
+      if (d.grey1 && nk == 1) {
+        // a single k-term with l_grey_single: its absorption joins the grey
+        // extinction as it is (solve_band_k_eqv_scl.F90:941-951)
+#pragma unroll
+        for (int c = 0; c < kKeqChunks; c++) kgrey[c] = kgrey[c] + kk[c];
+        continue;
       }
       if (solar) {
         // direct-beam transmission: f = w prod exp(-k m sec0) down to the
@@ -753,6 +770,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
       }
     }
     // this gas's equivalent extinction and (solar) direct-beam adjustment
+    if (!(d.grey1 && nk == 1)) {
 #pragma unroll
     for (int c = 0; c < kKeqChunks; c++) {
       if (c < nch) {
@@ -773,6 +791,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
         if (i < L) keq[c] = keq[c] + kq;
       }
     }
+    }
     t0 += nk;
   }
 #pragma unroll
@@ -780,7 +799,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
     const int i = c * kWave + lane;
     if (c < nch && i < L)
       prep_layer(d, spec, l_rayleigh, colrec, grec, colscal, bandrec, bandscal, cldopt, krange, col, b, i,
-                 keq[c], adj[c], b0[c], b1[c], GEN);
+                 keq[c], kgrey[c], adj[c], b0[c], b1[c], GEN);
   }
 }
 
@@ -798,7 +817,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
                                            double *__restrict__ bandrec,
                                            double *__restrict__ bandscal,
                                            double *__restrict__ cldopt, const int *__restrict__ krange,
-                                           int col, int b, int i, double keq, double adjust, double b0,
+                                           int col, int b, int i, double keq, double kgrey, double adjust, double b0,
                                            double b1, bool gen) {
   const int NB = d.NB, L = d.L;
   const SgBand &B = spec->band[b];
@@ -839,7 +858,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
                                                   grec[((size_t)col * L + i) * SG_NGQ + SG_G_RHO],
                                                   r[SG_Q_MMR + SG_GAS_H2O])
                                     : 0.0;
-  br[0] = (kcont + ray) + keq;
+  br[0] = (kcont + ray) + keq + kgrey;
   if (d.triple) {
     // grey optics of each cloud type (grey_opt_prop.F90:975-1000,
     // 1185-1230: Rayleigh, then the type's droplets and its ice component
@@ -870,7 +889,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
         phf = phf + cp;
         fwd = fwd + cf;
       }
-      kgt = kgt + keq;
+      kgt = kgt + keq + kgrey;
       if (ksc > kTiny) {
         const double inv = 1.0 / ksc;
         fwd = fwd * inv;
@@ -918,6 +937,8 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
 struct LaneCtx {
   bool solar;
   bool ice;                 // FP64 form: ice optics beside the droplets'
+  int nosc;                 // the lane's term without scattering: SOCGPU_SCATTER_NONE
+                            // (absorption only) or _EXT (extinction as absorption); 0
   double sec0, ksc_clear;
 };
 
@@ -1012,6 +1033,13 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
       pf = pf + x.ci[2] * cs;
       fw = fw + x.ci[3] * cs;
     }
+    if (c.nosc) {
+      // single_scattering.F90:132-158, mcica_column.F90:509-522
+      const double kt = kg + kgas;
+      two_coeff_fast_lw((c.nosc == SOCGPU_SCATTER_NONE ? kt - ks : kt) * x.mass, o);
+      o.T *= frac1; o.s1 *= frac1; o.s2 *= frac1;
+      return;
+    }
     pf = (pf - fw) / fmax(ks - fw, kEps);
     fw = fw / fmax(ks, kEps);
     double ktot = kg + kgas;
@@ -1023,6 +1051,10 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
     o.T *= frac1; o.R *= frac1; o.T0 *= frac1; o.s1 *= frac1; o.s2 *= frac1;
   } else {
     double ktot = kgt + kgas;
+    if (c.nosc) {
+      two_coeff_fast_lw((c.nosc == SOCGPU_SCATTER_NONE ? ktot - c.ksc_clear : ktot) * x.mass, o);
+      return;
+    }
     double tau = ktot * x.mass;
     double om = fmin(c.ksc_clear / (ktot + kTiny), 1.0 - 32.0 * kEps);
     two_coeff(c.solar, tau, om, 0.0, c.sec0, o);
@@ -1063,6 +1095,15 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
     fw = SG_FDIV(fw, fmaxf(ks, (float)kEps));
     float ktot = (float)c.ksc_clear + kag + kgas + kext * cs;
     float kab = kag + kgas + kabs * cs;
+    if (c.nosc) {
+      // without scattering: the optical depth of absorption or of
+      // extinction, no albedo (single_scattering.F90:132-158)
+      CoefF f;
+      two_coeff_f32(false, (c.nosc == SOCGPU_SCATTER_NONE ? kab : ktot) * mass, 0.0f, 1.0f, 0.0f, 1.0f, f);
+      coef_from_f32(f, o);
+      o.T *= frac1; o.R *= frac1; o.s1 *= frac1; o.s2 *= frac1;
+      return;
+    }
     tau = ktot * mass;
     omc = fmaxf(SG_FDIV(kab, ktot), omc_min);
     om = 1.0f - omc;
@@ -1073,6 +1114,13 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
     g = pf;
   } else {
     float ktot = (float)c.ksc_clear + kag + kgas;
+    if (c.nosc) {
+      CoefF f;
+      two_coeff_f32(false, (c.nosc == SOCGPU_SCATTER_NONE ? kag + kgas : ktot) * mass, 0.0f, 1.0f, 0.0f, 1.0f,
+                    f);
+      coef_from_f32(f, o);
+      return;
+    }
     tau = ktot * mass;
     omc = ktot > 0.0f ? fmaxf(SG_FDIV(kag + kgas, ktot), omc_min) : 1.0f;
     om = 1.0f - omc;
@@ -1237,6 +1285,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = d.ice && !MIXED;
+  c.nosc = term_nosc(d.scat, t.iscat);
   LaneSrc src;
   src.rec = rec;
   src.br = br;
@@ -1475,6 +1524,7 @@ __device__ __forceinline__ void solve_body_items(const CallDims &d, const SgSpec
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = d.ice && !MIXED;
+  c.nosc = term_nosc(d.scat, t.iscat);
   LaneSrc src;
   src.rec = rec;
   src.br = br;
@@ -1887,6 +1937,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   c.sec0 = cs[SG_S_SEC0];
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = false;
+  c.nosc = 0;
   const double alb_diff = bs[2], alb_dir = bs[3];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down (= flux_inc_direct)
 #define TWS(i, q) ws[((size_t)(i) * kTriQ + (q)) * kWave]
@@ -2987,6 +3038,20 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   }
   d.ir_lin = !solar && a->l_ir_source_linear;
   d.no_cont = a->l_no_continuum;
+  // treatment of scattering (socrates_set_control.F90:247-277, 312-318,
+  // 368-370; solve_band_k_eqv_scl.F90:1053-1058)
+  d.scat = a->i_scatter_method > 0 ? a->i_scatter_method : SOCGPU_SCATTER_FULL;
+  if (d.scat > SOCGPU_SCATTER_HYBRID) return fail(SOCGPU_EINVAL, "i_scatter_method %d unknown", d.scat);
+  d.grey1 = d.scat != SOCGPU_SCATTER_FULL;
+  for (int g = 0; g < S.n_g; g++) {
+    const int m = d.scat == SOCGPU_SCATTER_HYBRID ? (S.g[g].iscat > 0 ? S.g[g].iscat : SOCGPU_SCATTER_FULL) : d.scat;
+    if (term_nosc(d.scat, S.g[g].iscat) && solar)
+      return fail(SOCGPU_EUNSUPPORTED, "scattering can be ignored in the infra-red only (k-term %d)", g + 1);
+    if (term_nosc(d.scat, S.g[g].iscat) && d.triple)
+      return fail(SOCGPU_EUNSUPPORTED, "the triple and mixed columns take full scattering only");
+    if (m == SOCGPU_SCATTER_APPROX && d.triple)
+      return fail(SOCGPU_EUNSUPPORTED, "approximate scattering in the triple and mixed columns is not supported");
+  }
   d.nwpc = (S.n_g + kWave - 1) / kWave;
   solve_layout(d, S, precision == SOCGPU_MIXED && d.gen);
   *entry = &e;

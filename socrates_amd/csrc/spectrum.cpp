@@ -137,7 +137,7 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
   std::vector<std::vector<int>> index_absorb;
   // per (band, gas index): term list
   struct Raw { int nk = 0, scale = 0, fnc = 0; double pref = 0, tref = 0;
-               std::vector<double> k, w, s1, s2, s3; int lut0 = -1; };
+               std::vector<double> k, w, s1, s2, s3; std::vector<int> iscat; int lut0 = -1; };
   std::vector<std::vector<Raw>> raw;
   std::vector<std::vector<int>> order;  // gases per band in block 5's order
   // continua by band: listed types, and the block 9 data stored by type
@@ -258,6 +258,7 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
           for (int k = 0; k < r.nk; k++) {
             const std::string d = L.next();
             r.k.push_back(fnum(d, 3, 16)); r.w.push_back(fnum(d, 22, 16));
+            r.iscat.push_back(inum(d, 38, 3));  // i_scat (read_spectrum.F90:1374-1381)
             r.s1.push_back(nsv >= 1 ? fnum(d, 41, 16) : 0.0);
             r.s2.push_back(nsv >= 2 ? fnum(d, 60, 16) : 0.0);
             r.s3.push_back(nsv >= 3 ? fnum(L.next(), 41, 16) : 0.0);
@@ -487,6 +488,7 @@ int read_spectrum(const char *path, SgSpec *s, SpecTables *tab, std::string *err
         std::memset(&t, 0, sizeof t);
         t.k = r.k[k]; t.w = r.w[k]; t.a = r.s1[k]; t.b = r.s2[k]; t.c = r.s3[k];
         t.fnc = r.fnc;
+        t.iscat = r.iscat[k];
         if (r.fnc != SG_FNC_LOOKUP) {
           t.lnpref = std::log(r.pref); t.lntref = std::log(r.tref);
           t.tinv = 1.0 / r.tref;

@@ -75,6 +75,7 @@ module socrates_gpu_c
     type(c_ptr) :: ice_mmr = c_null_ptr, ice_dim = c_null_ptr
     type(c_ptr) :: ice_conv_mmr = c_null_ptr, ice_conv_dim = c_null_ptr
     integer(c_int) :: l_ir_source_linear = 0, l_no_continuum = 0
+    integer(c_int) :: i_scatter_method = 0
   end type socgpu_runes_args
 
   ! SocgpuDiag
@@ -420,6 +421,7 @@ module socrates_runes
   public :: ip_max_rand, ip_rand, ip_exponential_rand
   public :: ip_homogeneous, ip_mcica, ip_mcica_single_sampling, ip_mcica_optimal_sampling
   public :: ip_precision_fp64, ip_precision_mixed
+  public :: ip_scatter_full, ip_scatter_none, ip_scatter_approx, ip_scatter_hybrid
 
   ! Option codes (rad_pcf.F90 values)
   integer, parameter :: ip_source_illuminate = 1, ip_source_thermal = 2
@@ -429,6 +431,9 @@ module socrates_runes
   integer, parameter :: ip_max_rand = 0, ip_rand = 1, ip_exponential_rand = 3
   integer, parameter :: ip_homogeneous = 0, ip_mcica = 2
   integer, parameter :: ip_mcica_single_sampling = 1, ip_mcica_optimal_sampling = 2
+  ! treatment of scattering (socrates_runes.F90:32-35)
+  integer, parameter :: ip_scatter_full = 1, ip_scatter_none = 2
+  integer, parameter :: ip_scatter_approx = 4, ip_scatter_hybrid = 5
   ! Precision forms of the device solve (no reference counterpart)
   integer, parameter :: ip_precision_fp64 = 0, ip_precision_mixed = 1
 
@@ -509,7 +514,8 @@ contains
       ice_conv_mmr_1d, liq_dim_1d, ice_dim_1d, liq_conv_dim_1d, ice_conv_dim_1d, &
       cloud_vertical_decorr, conv_vertical_decorr, cloud_horizontal_rsd, &
       layer_heat_capacity, layer_heat_capacity_1d, &
-      i_source, i_cloud_representation, i_overlap, i_inhom, i_mcica_sampling, &
+      i_source, i_scatter_method, i_cloud_representation, i_overlap, i_inhom, &
+      i_mcica_sampling, &
       i_st_water, i_cnv_water, i_st_ice, i_cnv_ice, rand_seed, &
       l_rayleigh, l_invert, l_profile_last, precision)
     integer, intent(in) :: n_profile, n_layer
@@ -616,6 +622,7 @@ contains
     real(RealExt), intent(in), optional :: layer_heat_capacity(:, :)
     real(RealExt), intent(in), optional :: layer_heat_capacity_1d(:)
     integer, intent(in), optional :: i_source, i_cloud_representation, i_overlap, i_inhom
+    integer, intent(in), optional :: i_scatter_method
     integer, intent(in), optional :: i_mcica_sampling, i_st_water, i_cnv_water
     integer, intent(in), optional :: i_st_ice, i_cnv_ice
     integer, intent(in), optional :: rand_seed(:)
@@ -680,6 +687,7 @@ contains
     if (present(l_rayleigh)) a%l_rayleigh = merge(1, 0, l_rayleigh)
     if (present(i_cloud_representation)) a%i_cloud_representation = i_cloud_representation
     if (present(i_overlap)) a%i_overlap = i_overlap
+    if (present(i_scatter_method)) a%i_scatter_method = i_scatter_method
     if (present(i_inhom)) a%i_inhom = i_inhom
     if (present(i_mcica_sampling)) a%i_mcica_sampling = i_mcica_sampling
     if (present(i_st_water)) a%i_st_water = i_st_water

@@ -63,7 +63,8 @@ class RunesArgs(ctypes.Structure):
                                              ("i_st_ice", ctypes.c_int), ("i_cnv_ice", ctypes.c_int),
                                              ("ice_mmr", _D), ("ice_dim", _D), ("ice_conv_mmr", _D),
                                              ("ice_conv_dim", _D), ("l_ir_source_linear", ctypes.c_int),
-                                             ("l_no_continuum", ctypes.c_int)]
+                                             ("l_no_continuum", ctypes.c_int),
+                                             ("i_scatter_method", ctypes.c_int)]
 
 
 class Diag(ctypes.Structure):
@@ -162,7 +163,7 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
           rand_seed=None, l_flux_ground=False, flux_ground=None, co2=None,
           l_co2_well_mixed=False, h2o_mix_ratio=0.0, o3_mix_ratio=0.0, i_st_ice=0, i_cnv_ice=0,
           ice_mmr=None, ice_dim=None, ice_conv_mmr=None, ice_conv_dim=None, l_ir_source_linear=False,
-          l_no_continuum=False, l_clear=False, out=None, **gas_amounts):
+          l_no_continuum=False, i_scatter_method=0, l_clear=False, out=None, **gas_amounts):
     """ctypes argument and output structures of one call (and the objects
     that must stay alive while it runs). mix_ratios: the well-mixed mixing
     ratios of further gases, as runes names them (n2o_mix_ratio,
@@ -176,7 +177,9 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     i_cnv_ice, ice_mmr, ice_dim, ice_conv_mmr, ice_conv_dim: the ice
     component (none when i_st_ice is 0); l_ir_source_linear, l_no_continuum:
     the general driver's linear infra-red source and continuum switch (runes
-    has neither); l_clear: also the
+    has neither); i_scatter_method: runes' treatment of scattering
+    (SOCGPU_SCATTER_*: 1 full, 2 none, 3 extinction as absorption, 4
+    approximate, 5 per k-term; 0 the default, full); l_clear: also the
     clear-sky fluxes flux_up_clear, flux_down_clear, flux_direct_clear (an
     out dict holding any of these keys asks for them as well)."""
     init()
@@ -215,6 +218,7 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     a.l_flux_ground, a.l_co2_well_mixed = int(bool(l_flux_ground)), int(bool(l_co2_well_mixed))
     a.i_st_ice, a.i_cnv_ice = int(i_st_ice), int(i_cnv_ice)
     a.l_ir_source_linear, a.l_no_continuum = int(bool(l_ir_source_linear)), int(bool(l_no_continuum))
+    a.i_scatter_method = int(i_scatter_method)
     for k, v in inputs.items():
         if v is not None and _is_torch(v) != on_dev:
             raise SocgpuError("all inputs must be in the same memory space (%s)" % k)
@@ -484,6 +488,8 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         kw["l_ir_source_linear"] = True
     if case.l_no_continuum:
         kw["l_no_continuum"] = True
+    if case.i_scatter_method:
+        kw["i_scatter_method"] = case.i_scatter_method
     if cloudy and case.ice_mmr is not None:
         kw.update(i_st_ice=case.i_st_ice, i_cnv_ice=case.i_st_ice, ice_mmr=g("ice_mmr"),
                   ice_dim=g("ice_dim"))
