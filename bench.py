@@ -33,7 +33,11 @@ Also reported:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
                       [--no-autotune] [--serve-seconds S] [--no-cpu-baseline]
-                      [--no-ragged] [--no-c1]
+                      [--no-ragged] [--no-c1] [--tile R,B,S]
+
+--tile fixes the solve tile of the --precision form (recompute 0/1, waves
+per block, waves per SIMD) instead of the autotuned one; a profiled run uses
+it to count the instructions of a given tile.
 """
 import argparse
 import json
@@ -67,7 +71,7 @@ HBM_PEAK = 8000.0                          # GB/s
 # FETCH_SIZE and WRITE_SIZE runs of this script on the same build, each scaled
 # by what the counter reads for a plain copy of known size; the summary, its
 # commands and the scale factors are in TRAFFIC_SOURCE
-TRAFFIC = {"mixed": 2.775e9, "fp64": 4.483e9}
+TRAFFIC = {"mixed": 2.959e9, "fp64": 3.164e9}
 TRAFFIC_SOURCE = "profiles/c2_solve_traffic.md"
 
 
@@ -241,6 +245,8 @@ def main():
     ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--tile", default=None,
+                    help="recompute,waves_per_block,waves_per_simd for the --precision form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 512 MiB device copy, the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -282,7 +288,10 @@ def main():
         calls = [R.case_call(cases[src][0], src, device_arrays=cases[src][1], out=outs[prec][src])
                  for src in ("sw", "lw")]
         reqs = R.Requests(calls)
-        if not a.no_autotune:
+        if a.tile and prec == a.precision:
+            R.set_tile(prec, *[int(v) for v in a.tile.split(",")])
+            tiles[prec] = R.get_tile(prec)
+        elif not a.no_autotune:
             tiles[prec] = R.autotune(calls, prec)
         else:
             tiles[prec] = R.get_tile(prec)
