@@ -148,6 +148,18 @@ typedef struct SocgpuRunesArgs {
    * mixed columns take full scattering only (their approximate and
    * non-scattering region solvers are not built). */
   int i_scatter_method;
+  /* Modal aerosols (runes' n_aer_mode, aer_mix_ratio, aer_absorption,
+   * aer_scattering, aer_asymmetry, socrates_runes.F90:133-135, 609-621;
+   * set_aer, socrates_set_aer.F90:299-309): per mode a mass mixing ratio
+   * [mode][layer][profile] and, per band of the spectral file and mode,
+   * specific absorption and scattering (m2 kg-1) and asymmetry
+   * [band][mode][layer][profile], added to the clear-air grey optics of
+   * every layer (grey_opt_prop.F90:716-743, opt_prop_ukca_aerosol.F90).
+   * n_aer_mode 0: none (runes' l_aerosol_mode false is passed as 0). Clear
+   * sky and McICA only: the triple and mixed columns with aerosols return
+   * SOCGPU_EUNSUPPORTED. */
+  int n_aer_mode;
+  const double *aer_mix_ratio, *aer_absorption, *aer_scattering, *aer_asymmetry;
 } SocgpuRunesArgs;
 
 #define SOCGPU_GAS_N2O 0

@@ -44,6 +44,12 @@ program soc_runes_case
   real(RealExt), allocatable, dimension(:,:) :: f_n2o, f_ch4, f_o2, f_so2, &
     f_ocs, f_cfc11, f_cfc12, f_cfc113, f_hcfc22, f_hfc134a
   integer :: lwf(10), lice
+  ! modal aerosols (version 8): mixing ratios (n_profile, n_layer, n_aer) and
+  ! optical properties (n_profile, n_layer, n_aer, n_band); unallocated
+  ! (not present) without them
+  integer :: n_aer
+  real(RealExt), allocatable :: aer_mix(:,:,:), aer_abs(:,:,:,:), &
+    aer_sca(:,:,:,:), aer_asy(:,:,:,:)
   integer, allocatable :: rand_seed(:)
   real(RealExt), allocatable, target, dimension(:,:) :: flux_up, flux_down, &
     flux_direct, heating_rate, flux_up_clear, flux_down_clear, &
@@ -140,6 +146,17 @@ program soc_runes_case
     read(iu) i_scat_method
     if (i_scat_method < 1) i_scat_method = 1
   end if
+  n_aer = 0
+  if (version >= 8) then
+    read(iu) n_aer
+    if (n_aer > 0) then
+      allocate(aer_mix(n_profile, n_layer, n_aer), &
+        aer_abs(n_profile, n_layer, n_aer, n_band), &
+        aer_sca(n_profile, n_layer, n_aer, n_band), &
+        aer_asy(n_profile, n_layer, n_aer, n_band))
+      read(iu) aer_mix, aer_abs, aer_sca, aer_asy
+    end if
+  end if
   close(iu)
 
   allocate(flux_up(n_profile, 0:n_layer), flux_down(n_profile, 0:n_layer), &
@@ -195,6 +212,8 @@ program soc_runes_case
         cos_zenith_angle=cos_zen, solar_irrad=solar_irrad, &
         albedo_diff=albedo_diff, albedo_dir=albedo_dir, &
         l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity=layer_heat_capacity, &
+        n_aer_mode=n_aer, aer_mix_ratio=aer_mix, aer_absorption=aer_abs, &
+        aer_scattering=aer_sca, aer_asymmetry=aer_asy, &
         i_scatter_method=i_scat_method)
     else if (i_solver_kind == 0) then
       ! McICA sub-columns over a homogeneous cloud representation
@@ -220,6 +239,8 @@ program soc_runes_case
         i_inhom=i_inhom, i_mcica_sampling=i_mcica_sampling, &
         i_st_water=i_st_water, i_st_ice=i_st_ice, &
         n_subcol_gen=n_subcol_gen, rand_seed=rand_seed, &
+        n_aer_mode=n_aer, aer_mix_ratio=aer_mix, aer_absorption=aer_abs, &
+        aer_scattering=aer_sca, aer_asymmetry=aer_asy, &
         i_scatter_method=i_scat_method)
     else
       ! Triple-column (clear / stratiform / convective) overlap

@@ -32,7 +32,9 @@ class _Case(ctypes.Structure):
                             ("wm_field", _D * 10), ("i_st_ice", ctypes.c_int),
                             ("ice_mmr", _D), ("ice_dim", _D), ("ice_conv_mmr", _D),
                             ("ice_conv_dim", _D), ("l_ir_source_linear", ctypes.c_int),
-                            ("l_no_continuum", ctypes.c_int), ("i_scatter_method", ctypes.c_int)]
+                            ("l_no_continuum", ctypes.c_int), ("i_scatter_method", ctypes.c_int),
+                            ("n_aer_mode", ctypes.c_int), ("aer_mix_ratio", _D),
+                            ("aer_absorption", _D), ("aer_scattering", _D), ("aer_asymmetry", _D)]
 
 
 def build():

@@ -1286,6 +1286,12 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
    * approximate-scattering solvers are not either. */
   const int scat = c->i_scatter_method > 0 ? c->i_scatter_method : IP_SCATTER_FULL;
   if (scat > IP_SCATTER_HYBRID) return -13;
+  /* modal aerosols: restated for the clear and McICA paths; the region
+   * coefficients of the triple and mixed columns with aerosols are not */
+  if (c->n_aer_mode > 0 && triple) return -14;
+  if (c->n_aer_mode > 0 && (!c->aer_mix_ratio || !c->aer_absorption || !c->aer_scattering ||
+                            !c->aer_asymmetry))
+    return -14;
   const int grey_single = scat != IP_SCATTER_FULL;
   for (int b = 0; b < sp.n_band; b++) {
     const int g = sp.index_absorb[b][0] - 1;
@@ -1465,7 +1471,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
 
   /* --- per-column radiative transfer ------------------------------------ */
   work_t w, wc;
-  double *buf = (double *)malloc(sizeof(double) * (size_t)(31 * (L + 2) + 16 * (2 * L + 2)));
+  double *buf = (double *)malloc(sizeof(double) * (size_t)(36 * (L + 2) + 16 * (2 * L + 2)));
   double *q = buf;
 #define TAKE(n) (q += (n), q - (n))
   w.L = wc.L = L;
@@ -1483,6 +1489,10 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
   double *sum_flux = TAKE(2 * L + 3), *sum_k_flux = TAKE(2 * L + 3);
   double *ft = TAKE(2 * L + 2), *tg = TAKE(L), *src = TAKE(L);
   double *kg0 = TAKE(L);  /* clear-air grey extinction before k_eqv */
+  /* clear-air products of scattering and the first phase-function moment
+   * and forward scattering (aerosols), unnormalised; the normalised forward
+   * scattering and delta-rescaled asymmetry of clear air */
+  double *pf_clr = TAKE(L), *fw_clr = TAKE(L), *fw_n = TAKE(L), *g_clr = TAKE(L);
   double *acc_total = TAKE(2 * L + 2), *acc_direct = TAKE(L + 1);
   double *clr_total = TAKE(2 * L + 2), *clr_direct = TAKE(L + 1);
   double *up = TAKE(L + 1), *down = TAKE(L + 1), *dir = TAKE(L + 1);
@@ -1579,7 +1589,25 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       for (int i = 0; i < L; i++) {
         k_ext_scat[i] = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
         k_grey_tot[i] = 0.0;
+        pf_clr[i] = 0.0;
+        fw_clr[i] = 0.0;
       }
+      /* modal aerosols (grey_opt_prop.F90:716-743 over every layer:
+       * opt_prop_ukca_aerosol.F90:135-200, two-stream: one phase-function
+       * moment, forward scattering of order 2), mode by mode */
+      for (int j = 0; j < c->n_aer_mode; j++)
+        for (int i = 0; i < L; i++) {
+          const size_t im = ((size_t)j * L + i) * P + l;
+          const size_t ib = (((size_t)b * c->n_aer_mode + j) * L + i) * P + l;
+          const double mix = c->aer_mix_ratio[im];
+          k_grey_tot[i] = k_grey_tot[i] + mix * c->aer_absorption[ib];
+          const double ksc = mix * c->aer_scattering[ib];
+          k_ext_scat[i] = k_ext_scat[i] + ksc;
+          const double asym = c->aer_asymmetry[ib];
+          pf_clr[i] = pf_clr[i] + ksc * asym;
+          const double ks_phf = ksc * asym;
+          fw_clr[i] = fw_clr[i] + ks_phf * asym;
+        }
       for (int jc = 1; jc <= (c->l_no_continuum ? 0 : sp.n_band_continuum[b]); jc++) {
         const int icont = sp.index_continuum[b][jc - 1];
         const int fnc = sp.i_scale_fnc_cont[b][jc];
@@ -1604,6 +1632,17 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
       for (int i = 0; i < L; i++) {
         k_grey_tot[i] = k_grey_tot[i] + k_ext_scat[i];
         kg0[i] = k_grey_tot[i];
+        /* the clear-air moments divided by the scattering where there is
+         * any (grey_opt_prop.F90:1360-1440, tmp_inv), the phase function
+         * delta-rescaled (rescale_phase_fnc.F90:116-124) */
+        double fw = fw_clr[i], pf = pf_clr[i];
+        if (k_ext_scat[i] > DBL_MIN) {
+          const double inv = 1.0 / k_ext_scat[i];
+          fw = fw * inv;
+          pf = pf * inv;
+        }
+        fw_n[i] = fw;
+        g_clr[i] = (pf - fw) / (1.0 - fw);
       }
       /* Planck terms (diff_planck_source_mod.F90:100-134) */
       double flux_ground = 0.0;
@@ -1803,8 +1842,8 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           double tau = k_total * LAY(c->mass, i, l);
           double om = k_ext_scat[i] / (k_total + tiny);
           om = fmin(om, 1.0 - 3.2e+01 * eps);
-          tau = tau * (1.0 - om * 0.0);
-          om = om * (1.0 - 0.0) / (1.0 - om * 0.0);
+          tau = tau * (1.0 - om * fw_n[i]);
+          om = om * (1.0 - fw_n[i]) / (1.0 - om * fw_n[i]);
           tau_clr[i] = tau; om_clr[i] = om;
         }
         /* clear coefficients (two_stream / mcica_column.F90:266-298; without
@@ -1812,7 +1851,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
         for (int i = 0; i < L; i++) {
           coeff_t o;
           if (nosc) two_coeff_fast_lw_one(tau_clr[i], quad, &o);
-          else two_coeff_one(isolir, tau_clr[i], om_clr[i], 0.0, zen_0, &o);
+          else two_coeff_one(isolir, tau_clr[i], om_clr[i], g_clr[i], zen_0, &o);
           w.trans[i] = o.trans; w.reflect[i] = o.reflect; w.trans_0[i] = o.trans_0;
           w.sc1[i] = o.sc1; w.sc2[i] = o.sc2;
         }
@@ -1871,9 +1910,9 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
               kgt = kgt + ki_tot[i] * cs;
               double ksc = k_ext_scat[i] + kc_scat[i] * cs;
               ksc = ksc + ki_scat[i] * cs;
-              double fwd = 0.0 + kc_fwd[i] * cs;
+              double fwd = fw_clr[i] + kc_fwd[i] * cs;
               fwd = fwd + ki_fwd[i] * cs;
-              double phf = 0.0 + kc_phf[i] * cs;
+              double phf = pf_clr[i] + kc_phf[i] * cs;
               phf = phf + ki_phf[i] * cs;
               phf = (phf - fwd) / fmax(ksc - fwd, eps);
               fwd = fwd / fmax(ksc, eps);

@@ -90,6 +90,12 @@ typedef struct {
    * as absorption, 4 approximate (as full here), 5 per k-term from the
    * spectral file (hybrid) */
   int i_scatter_method;
+  /* modal aerosols (runes' n_aer_mode, aer_mix_ratio, aer_absorption,
+   * aer_scattering, aer_asymmetry; socrates_runes.F90:133-135, 609-621):
+   * mix ratio [mode][layer][profile], optical properties
+   * [band][mode][layer][profile]; n_aer_mode 0: none */
+  int n_aer_mode;
+  const double *aer_mix_ratio, *aer_absorption, *aer_scattering, *aer_asymmetry;
 } soc_case;
 
 enum { SOC_WM_N2O, SOC_WM_CH4, SOC_WM_O2, SOC_WM_SO2, SOC_WM_OCS, SOC_WM_CFC11, SOC_WM_CFC12,

@@ -14,13 +14,16 @@ import dataclasses
 import numpy as np
 
 MAGIC = 1397703491
-VERSION = 7     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
+VERSION = 8     # 2 adds the well-mixed mixing ratios of WM_GASES after the seeds;
                 # 3 adds l_flux_ground and flux_ground (n_band, n_profile) after them;
                 # 4 adds a flag and, when it is set, a CO2 field (n_layer, n_profile);
                 # 5 adds a flag per WM_GASES gas and, for each one set, its field;
                 # 6 adds a flag and, when it is set, ice_mmr, ice_dim, ice_conv_mmr,
                 #   ice_conv_dim (n_layer, n_profile);
-                # 7 adds i_scatter_method
+                # 7 adds i_scatter_method;
+                # 8 adds n_aer_mode and, when it is positive, aer_mix_ratio
+                #   (n_aer_mode, n_layer, n_profile), aer_absorption,
+                #   aer_scattering, aer_asymmetry (n_band, n_aer_mode, n_layer, n_profile)
 
 # further gases runes takes as well-mixed mixing ratios (the *_mix_ratio
 # arguments, socrates_runes.F90:69-83), in this order
@@ -38,6 +41,7 @@ _BAND = ("albedo_diff", "albedo_dir")
 _CLOUD = ("cloud_frac", "liq_mmr", "liq_dim")
 _CONV = ("conv_frac", "liq_conv_mmr", "liq_conv_dim")
 _ICE = ("ice_mmr", "ice_dim", "ice_conv_mmr", "ice_conv_dim")
+_AER = ("aer_mix_ratio", "aer_absorption", "aer_scattering", "aer_asymmetry")
 
 
 @dataclasses.dataclass
@@ -93,6 +97,11 @@ class Case:
     l_ir_source_linear: int = 0       # infra-red source linear in tau (not written to files)
     l_no_continuum: int = 0           # continuum absorption left out (not written to files)
     i_scatter_method: int = 0         # treatment of scattering (0: runes' default, full)
+    n_aer_mode: int = 0               # modal aerosols (runes' n_aer_mode); 0: none
+    aer_mix_ratio: np.ndarray = None  # (n_aer_mode, n_layer, n_profile) kg/kg
+    aer_absorption: np.ndarray = None  # (n_band, n_aer_mode, n_layer, n_profile) m2/kg
+    aer_scattering: np.ndarray = None  # (n_band, n_aer_mode, n_layer, n_profile) m2/kg
+    aer_asymmetry: np.ndarray = None   # (n_band, n_aer_mode, n_layer, n_profile)
 
     def fill_defaults(self):
         L, P = self.n_layer, self.n_profile
@@ -151,6 +160,10 @@ class Case:
                     v = np.zeros((L, P)) if v is None else v
                     np.ascontiguousarray(v, dtype=np.float64).tofile(f)
             np.array([int(self.i_scatter_method)], dtype=np.int32).tofile(f)
+            np.array([int(self.n_aer_mode)], dtype=np.int32).tofile(f)
+            if self.n_aer_mode > 0:
+                for k in _AER:
+                    np.ascontiguousarray(getattr(self, k), dtype=np.float64).tofile(f)
 
     @classmethod
     def read(cls, path):
@@ -210,6 +223,17 @@ class Case:
         if hdr[1] >= 7:
             c.i_scatter_method = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
             off += 4
+        if hdr[1] >= 8:
+            M = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
+            off += 4
+            c.n_aer_mode = M
+            if M > 0:
+                for k in _AER:
+                    shape = (M, L, P) if k == "aer_mix_ratio" else (B, M, L, P)
+                    n = int(np.prod(shape))
+                    setattr(c, k, np.frombuffer(raw, dtype=np.float64, count=n,
+                                                offset=off).reshape(shape).copy())
+                    off += 8 * n
         return c
 
 

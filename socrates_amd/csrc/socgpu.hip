@@ -183,6 +183,18 @@ struct CallDims {
   int no_cont;       // continuum absorption left out
   int scat;          // treatment of scattering (SOCGPU_SCATTER_*)
   int grey1;         // minor gases with one k-term added to the grey extinction
+  int n_aer;         // modal aerosols: modes (0: none; general build only)
+};
+
+// Modal aerosol inputs of a call (device pointers: mixing ratio
+// [mode][layer][profile], absorption, scattering and asymmetry
+// [band][mode][layer][profile]) and the per-(column, band, layer) record the
+// band kernel writes from them: clear-air scattering (Rayleigh plus the
+// modes'), and the products of scattering with the asymmetry and with its
+// square, unnormalised (aerrec[col][band][layer][4]).
+struct AerIn {
+  const double *mix, *abs, *sca, *asy;
+  double *rec;
 };
 
 // A major k-term's treatment of scattering for a call with method scat:
@@ -613,7 +625,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
                                           double *__restrict__ bandrec,
                                           double *__restrict__ bandscal,
                                           double *__restrict__ cldopt, const int *__restrict__ krange,
-                                          int wave, int lane) {
+                                          const AerIn &ae, int wave, int lane) {
   const int L = d.L, NB = d.NB;
   if (wave >= d.P * NB) return;
   const int col = wave / NB, b = wave % NB;
@@ -798,7 +810,7 @@ __device__ __forceinline__ void band_body(const CallDims &d, const SgSpec *__res
   for (int c = 0; c < kKeqChunks; c++) {
     const int i = c * kWave + lane;
     if (c < nch && i < L)
-      prep_layer(d, spec, l_rayleigh, colrec, grec, colscal, bandrec, bandscal, cldopt, krange, col, b, i,
+      prep_layer(d, spec, l_rayleigh, colrec, grec, colscal, bandrec, bandscal, cldopt, krange, ae, col, b, i,
                  keq[c], kgrey[c], adj[c], b0[c], b1[c], GEN);
   }
 }
@@ -817,8 +829,8 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
                                            double *__restrict__ bandrec,
                                            double *__restrict__ bandscal,
                                            double *__restrict__ cldopt, const int *__restrict__ krange,
-                                           int col, int b, int i, double keq, double kgrey, double adjust, double b0,
-                                           double b1, bool gen) {
+                                           const AerIn &ae, int col, int b, int i, double keq, double kgrey,
+                                           double adjust, double b0, double b1, bool gen) {
   const int NB = d.NB, L = d.L;
   const SgBand &B = spec->band[b];
   const double *rec = colrec + (size_t)col * L * SG_NQ;
@@ -858,7 +870,34 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
                                                   grec[((size_t)col * L + i) * SG_NGQ + SG_G_RHO],
                                                   r[SG_Q_MMR + SG_GAS_H2O])
                                     : 0.0;
-  br[0] = (kcont + ray) + keq + kgrey;
+  // modal aerosols (grey_opt_prop.F90:716-743, opt_prop_ukca_aerosol.F90:
+  // 135-200, two-stream: one moment, forward scattering of order 2): their
+  // absorption precedes the continua in the clear-air extinction, their
+  // scattering follows Rayleigh's
+  double kaer = 0.0, ksc = ray;
+  if (gen && d.n_aer > 0) {
+    // the reference's products and sums, uncontracted: in optically thin
+    // infra-red layers its sources amplify a last-bit change of the
+    // scattering a million-fold
+#pragma clang fp contract(off)
+    const int M = d.n_aer;
+    double pf = 0.0, fw = 0.0;
+    for (int m = 0; m < M; m++) {
+      const size_t im = ((size_t)m * L + i) * d.P + col;
+      const size_t ib = (((size_t)b * M + m) * L + i) * d.P + col;
+      const double mix = ae.mix[im];
+      kaer = kaer + mix * ae.abs[ib];
+      const double ks = mix * ae.sca[ib];
+      ksc = ksc + ks;
+      const double g = ae.asy[ib];
+      pf = pf + ks * g;
+      const double ks_phf = ks * g;
+      fw = fw + ks_phf * g;
+    }
+    double *ar = ae.rec + (((size_t)col * NB + b) * L + i) * 4;
+    ar[0] = ksc; ar[1] = pf; ar[2] = fw; ar[3] = 0.0;
+  }
+  br[0] = ((kaer + kcont) + ksc) + keq + kgrey;
   if (d.triple) {
     // grey optics of each cloud type (grey_opt_prop.F90:975-1000,
     // 1185-1230: Rayleigh, then the type's droplets and its ice component
@@ -940,6 +979,7 @@ struct LaneCtx {
   int nosc;                 // the lane's term without scattering: SOCGPU_SCATTER_NONE
                             // (absorption only) or _EXT (extinction as absorption); 0
   double sec0, ksc_clear;
+  bool aer;                 // modal aerosols: clear air scatters with a phase function
 };
 
 // One lane's inputs for one layer, loaded a layer ahead of their use so
@@ -948,6 +988,8 @@ struct LayerIn {
   double lnp, lnt, mmr, mass, frac1;  // column record (wave-uniform but mmr)
   double kx;                          // the term's k_abs, for scalings other than the power law
   double kgt, q1, q2;                 // band record: grey extinction, adjust or Planck terms
+  double ksa, pfa, fwa;               // clear-air scattering and its moment products
+                                      // (Rayleigh only: ksc_clear, 0, 0)
   double cv;                          // the lane's sub-column condensate scaling
   double co[4];                       // cloud optics at unit condensate scaling:
                                       // FP64 form, the droplets' (k_ext, k_scat,
@@ -960,6 +1002,7 @@ struct LaneSrc {
   const double *rec, *br, *csub, *cop;
   const float *csubf, *copf;  // FP32 forms (mixed form)
   double ksc_clear;           // clear-air scattering (Rayleigh)
+  const double *aer;          // the lane's aerosol record (column, band), or null
   int co_stride, ice;         // doubles per layer of cop; ice optics beside the droplets'
   int gas, isub;
   int fnc;                    // the term's scaling function
@@ -988,6 +1031,12 @@ __device__ __forceinline__ void load_layer(const LaneSrc &s, int i, LayerIn &x) 
   x.kgt = q[0];
   x.q1 = q[1];
   x.q2 = q[2];
+  if (GEN && s.aer) {
+    const double *a = s.aer + i * 4;
+    x.ksa = a[0]; x.pfa = a[1]; x.fwa = a[2];
+  } else {
+    x.ksa = s.ksc_clear; x.pfa = 0.0; x.fwa = 0.0;
+  }
   if (CLR) {
     x.frac1 = 0.0;
     x.cv = 0.0;
@@ -1025,8 +1074,8 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
     // the sub-column's cloudy totals (mcica_sample.F90:614-640): each
     // component, droplets then ice, scaled by c_sub and added in turn
     const double cs = csub;
-    double kg = kgt + x.co[0] * cs, ks = c.ksc_clear + x.co[1] * cs;
-    double pf = x.co[2] * cs, fw = x.co[3] * cs;
+    double kg = kgt + x.co[0] * cs, ks = x.ksa + x.co[1] * cs;
+    double pf = x.pfa + x.co[2] * cs, fw = x.fwa + x.co[3] * cs;
     if (c.ice) {
       kg = kg + x.ci[0] * cs;
       ks = ks + x.ci[1] * cs;
@@ -1052,12 +1101,28 @@ __device__ __forceinline__ void layer_coef(const LaneCtx &c, const SgTerm &t, co
   } else {
     double ktot = kgt + kgas;
     if (c.nosc) {
-      two_coeff_fast_lw((c.nosc == SOCGPU_SCATTER_NONE ? ktot - c.ksc_clear : ktot) * x.mass, o);
+      two_coeff_fast_lw((c.nosc == SOCGPU_SCATTER_NONE ? ktot - x.ksa : ktot) * x.mass, o);
       return;
     }
     double tau = ktot * x.mass;
-    double om = fmin(c.ksc_clear / (ktot + kTiny), 1.0 - 32.0 * kEps);
-    two_coeff(c.solar, tau, om, 0.0, c.sec0, o);
+    double om = fmin(x.ksa / (ktot + kTiny), 1.0 - 32.0 * kEps);
+    double pf = 0.0;
+    if (GEN && c.aer) {
+      // clear air with aerosols: moments over the scattering
+      // (grey_opt_prop.F90:1360-1440), the phase function delta-rescaled
+      // (rescale_phase_fnc.F90:116-124), then rescale_tau_omega
+      double fw = x.fwa;
+      pf = x.pfa;
+      if (x.ksa > kTiny) {
+        const double inv = 1.0 / x.ksa;
+        fw = fw * inv;
+        pf = pf * inv;
+      }
+      pf = (pf - fw) / (1.0 - fw);
+      tau = tau * (1.0 - om * fw);
+      om = om * (1.0 - fw) / (1.0 - om * fw);
+    }
+    two_coeff(c.solar, tau, om, pf, c.sec0, o);
   }
 }
 
@@ -1077,7 +1142,7 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
   const double csub = x.cv;
   const double *co = x.co;
   const float frac1 = (float)x.frac1;
-  const float kag = (float)(x.kgt - c.ksc_clear);  // grey absorption (equivalent extinction)
+  const float kag = (float)(x.kgt - x.ksa);  // grey absorption (equivalent extinction)
   const float mass = (float)x.mass;
   const float omc_min = (float)(32.0 * kEps);
   float tau, om, omc, g;
@@ -1088,12 +1153,12 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
     const float kext = (float)co[0], kabs = (float)co[1];
     const float phf = (float)co[2], fwd = (float)co[3];
     const float cs = (float)csub;
-    float ks = (float)c.ksc_clear + (kext - kabs) * cs;
-    float fw = fwd * cs;
-    float pf = phf * cs;
+    float ks = (float)x.ksa + (kext - kabs) * cs;
+    float fw = (float)x.fwa + fwd * cs;
+    float pf = (float)x.pfa + phf * cs;
     pf = SG_FDIV(pf - fw, fmaxf(ks - fw, (float)kEps));
     fw = SG_FDIV(fw, fmaxf(ks, (float)kEps));
-    float ktot = (float)c.ksc_clear + kag + kgas + kext * cs;
+    float ktot = (float)x.ksa + kag + kgas + kext * cs;
     float kab = kag + kgas + kabs * cs;
     if (c.nosc) {
       // without scattering: the optical depth of absorption or of
@@ -1113,7 +1178,7 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
     om = 1.0f - omc;
     g = pf;
   } else {
-    float ktot = (float)c.ksc_clear + kag + kgas;
+    float ktot = (float)x.ksa + kag + kgas;
     if (c.nosc) {
       CoefF f;
       two_coeff_f32(false, (c.nosc == SOCGPU_SCATTER_NONE ? kag + kgas : ktot) * mass, 0.0f, 1.0f, 0.0f, 1.0f,
@@ -1125,6 +1190,22 @@ __device__ __forceinline__ void layer_coef_mixed(const LaneCtx &c, const SgTerm 
     omc = ktot > 0.0f ? fmaxf(SG_FDIV(kag + kgas, ktot), omc_min) : 1.0f;
     om = 1.0f - omc;
     g = 0.0f;
+    if (GEN && c.aer) {
+      // clear air with aerosols (layer_coef): moments over the scattering,
+      // delta-rescaling
+      float fw = (float)x.fwa, pf = (float)x.pfa;
+      const float ksa = (float)x.ksa;
+      if (ksa > 0.0f) {
+        const float inv = SG_FDIV(1.0f, ksa);
+        fw = fw * inv;
+        pf = pf * inv;
+      }
+      g = SG_FDIV(pf - fw, 1.0f - fw);
+      const float sc = 1.0f - om * fw;
+      tau = tau * sc;
+      omc = SG_FDIV(omc, sc);
+      om = 1.0f - omc;
+    }
   }
   CoefF f;
   two_coeff_f32(c.solar, tau, om, omc, g, (float)c.sec0, f);
@@ -1246,6 +1327,7 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
                                            const double *__restrict__ bandscal,
                                            const double *__restrict__ c_sub,
                                            const double *__restrict__ cldopt,
+                                           const double *__restrict__ aerrec,
                                            double *__restrict__ partial, int wave, int lane) {
 #pragma clang fp contract(on)
   static_assert(kSeg == 4, "checkpoints and output blocks share the 4-layer grid");
@@ -1286,7 +1368,9 @@ __device__ __forceinline__ void solve_body(const CallDims &d, const SgSpec *__re
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = d.ice && !MIXED;
   c.nosc = term_nosc(d.scat, t.iscat);
+  c.aer = GEN && aerrec != nullptr;
   LaneSrc src;
+  src.aer = c.aer ? aerrec + ((size_t)col * NB + b) * L * 4 : nullptr;
   src.rec = rec;
   src.br = br;
   src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * d.co_stride : nullptr;
@@ -1478,6 +1562,7 @@ __device__ __forceinline__ void solve_body_items(const CallDims &d, const SgSpec
                                            const double *__restrict__ bandscal,
                                            const double *__restrict__ c_sub,
                                            const double *__restrict__ cldopt,
+                                           const double *__restrict__ aerrec,
                                            double *__restrict__ partial, int wave, int lane) {
 #pragma clang fp contract(on)
   const int L = d.L, NB = d.NB, IM = d.n_items;
@@ -1525,7 +1610,9 @@ __device__ __forceinline__ void solve_body_items(const CallDims &d, const SgSpec
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = d.ice && !MIXED;
   c.nosc = term_nosc(d.scat, t.iscat);
+  c.aer = GEN && aerrec != nullptr;
   LaneSrc src;
+  src.aer = c.aer ? aerrec + ((size_t)col * NB + b) * L * 4 : nullptr;
   src.rec = rec;
   src.br = br;
   src.cop = d.mcica ? cldopt + ((size_t)col * NB + b) * L * d.co_stride : nullptr;
@@ -1938,6 +2025,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = false;
   c.nosc = 0;
+  c.aer = false;
   const double alb_diff = bs[2], alb_dir = bs[3];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down (= flux_inc_direct)
 #define TWS(i, q) ws[((size_t)(i) * kTriQ + (q)) * kWave]
@@ -2180,6 +2268,7 @@ struct ReqDesc {
   double *up, *dn, *dir, *hr;   // device outputs (dir may be null)
   double *colrec, *tlev, *colscal, *bandrec, *bandscal, *csub, *grec, *cldopt, *partial;
   double *ovl, *tws;            // triple column: overlap coefficients, solve workspace
+  double *aerrec;               // modal aerosols: clear-air scattering, moment products
   int *krange;
   long long s[kNWork];          // first work item of this request in each space
 };
@@ -2205,6 +2294,8 @@ __device__ __forceinline__ SocgpuRunesArgs gargs(const SocgpuRunesArgs &a) {
   g.rand_seed = G(a.rand_seed); g.flux_ground = G(a.flux_ground); g.co2 = G(a.co2);
   g.ice_mmr = G(a.ice_mmr); g.ice_dim = G(a.ice_dim); g.ice_conv_mmr = G(a.ice_conv_mmr);
   g.ice_conv_dim = G(a.ice_conv_dim);
+  g.aer_mix_ratio = G(a.aer_mix_ratio); g.aer_absorption = G(a.aer_absorption);
+  g.aer_scattering = G(a.aer_scattering); g.aer_asymmetry = G(a.aer_asymmetry);
   return g;
 }
 
@@ -2390,9 +2481,13 @@ __global__ __launch_bounds__(64 * kMinorWaves, NCH == 2 ? SG_BAND_OCC : 2) void 
     const ReqDesc &q = rq[r];
     const int nch = (q.d.L + kWave - 1) / kWave;
     if ((NCH == 2 && nch <= 2) || (NCH > 2 && nch > 2))
+    {
+      const AerIn ae{G(q.a.aer_mix_ratio), G(q.a.aer_absorption), G(q.a.aer_scattering), G(q.a.aer_asymmetry),
+                     G(q.aerrec)};
       band_body<NCH, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), q.a.gas_mix_ratio, q.a.gas_field,
-                          G(q.tlev), G(q.colscal), G(q.bandrec), G(q.bandscal), G(q.cldopt), G(q.krange),
+                          G(q.tlev), G(q.colscal), G(q.bandrec), G(q.bandscal), G(q.cldopt), G(q.krange), ae,
                           (int)(w - q.s[W_MINOR]), lane);
+    }
   }
 }
 
@@ -2408,11 +2503,13 @@ __global__ __launch_bounds__(256, OCC) void b_solve_kernel(const BatchHdr *h, co
     if ((q.d.gen != 0) != GEN) continue;   // the other build's call
     if (MIXED && GEN)
       solve_body_items<MIXED, RECOMP, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), G(q.colscal),
-                                           G(q.bandrec), G(q.bandscal), G(q.csub), G(q.cldopt), G(q.partial),
+                                           G(q.bandrec), G(q.bandscal), G(q.csub), G(q.cldopt), G(q.aerrec),
+                                           G(q.partial),
                                            (int)(w - q.s[W_SOLVE]), lane);
     else
       solve_body<MIXED, RECOMP, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), G(q.colscal),
-                                     G(q.bandrec), G(q.bandscal), G(q.csub), G(q.cldopt), G(q.partial),
+                                     G(q.bandrec), G(q.bandscal), G(q.csub), G(q.cldopt), G(q.aerrec),
+                                     G(q.partial),
                                      (int)(w - q.s[W_SOLVE]), lane);
   }
 }
@@ -2855,6 +2952,10 @@ static size_t input_fields(const SocgpuRunesArgs *a, SocgpuRunesArgs *dv, int NB
       {&dv->gas_field[8], a->gas_field[8], L * P}, {&dv->gas_field[9], a->gas_field[9], L * P},
       {&dv->ice_mmr, a->ice_mmr, L * P}, {&dv->ice_dim, a->ice_dim, L * P},
       {&dv->ice_conv_mmr, a->ice_conv_mmr, L * P}, {&dv->ice_conv_dim, a->ice_conv_dim, L * P},
+      {&dv->aer_mix_ratio, a->n_aer_mode > 0 ? a->aer_mix_ratio : nullptr, (size_t)a->n_aer_mode * L * P},
+      {&dv->aer_absorption, a->n_aer_mode > 0 ? a->aer_absorption : nullptr, NB * (size_t)a->n_aer_mode * L * P},
+      {&dv->aer_scattering, a->n_aer_mode > 0 ? a->aer_scattering : nullptr, NB * (size_t)a->n_aer_mode * L * P},
+      {&dv->aer_asymmetry, a->n_aer_mode > 0 ? a->aer_asymmetry : nullptr, NB * (size_t)a->n_aer_mode * L * P},
   };
   const size_t n = sizeof g / sizeof g[0];
   static_assert(sizeof g / sizeof g[0] <= kMaxInFields, "input field table too small");
@@ -3052,6 +3153,16 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
     if (m == SOCGPU_SCATTER_APPROX && d.triple)
       return fail(SOCGPU_EUNSUPPORTED, "approximate scattering in the triple and mixed columns is not supported");
   }
+  // modal aerosols (socrates_set_control.F90:378-384: on when modes are
+  // given; socrates_set_aer.F90:299-309): the general build carries them
+  if (a->n_aer_mode > 0) {
+    if (!a->aer_mix_ratio || !a->aer_absorption || !a->aer_scattering || !a->aer_asymmetry)
+      return fail(SOCGPU_EINVAL, "aerosol modes need aer_mix_ratio, aer_absorption, aer_scattering and aer_asymmetry");
+    if (d.triple)
+      return fail(SOCGPU_EUNSUPPORTED, "aerosols with the triple and mixed columns are not supported");
+    d.n_aer = a->n_aer_mode;
+    d.gen = 1;
+  }
   d.nwpc = (S.n_g + kWave - 1) / kWave;
   solve_layout(d, S, precision == SOCGPU_MIXED && d.gen);
   *entry = &e;
@@ -3074,6 +3185,7 @@ static size_t request_ws_doubles(const CallDims &d, const SgSpec &S) {
   if (d.gx) n += P * L * SG_NGQ;
   if (d.mcica) n += P * L * SG_MAXSUBCOL + P * NB * L * d.co_stride;
   if (d.triple) n += P * NB * L * 8 + P * (L + 1) * kOv + P * d.nwpc * L * kTriQ * kWave;
+  if (d.n_aer) n += P * NB * L * 4;
   return n;
 }
 
@@ -3117,10 +3229,11 @@ static void place_request(ReqDesc &q, const SgSpec &S, double *base, size_t *cur
   q.cldopt = d.mcica ? take(P * NB * L * d.co_stride) : (d.triple ? take(P * NB * L * 8) : nullptr);
   q.ovl = d.triple ? take(P * (L + 1) * kOv) : nullptr;
   q.tws = d.triple ? take((size_t)P * d.nwpc * L * kTriQ * kWave) : nullptr;
+  q.aerrec = d.n_aer ? take(P * NB * L * 4) : nullptr;
 }
 
 static size_t padded_ws_doubles(const CallDims &d, const SgSpec &S) {
-  return request_ws_doubles(d, S) + 12 * 4;  // per-array alignment padding
+  return request_ws_doubles(d, S) + 13 * 4;  // per-array alignment padding
 }
 
 static void fill_desc(ReqDesc &q, const CallDims &d, const SpecEntry &e, const SocgpuRunesArgs &dv,

@@ -59,6 +59,12 @@ program runes_lfric_case
   type(StrSpecData) :: sw_spec, lw_spec
   logical :: keep(67)
   character(len=2) :: name
+  ! fields of case-file versions 5-7 this host does not pass on, and the
+  ! modal aerosols of version 8 (stored as the host model stores them)
+  integer :: lwf(10), lice, iscat, n_aer, jm
+  real(RealExt), allocatable :: skip(:, :), aer_mix(:, :, :), aer_abs(:, :, :, :), &
+    aer_sca(:, :, :, :), aer_asy(:, :, :, :)
+  real(RealExt), allocatable, dimension(:) :: aer_mix_1d, aer_abs_1d, aer_sca_1d, aer_asy_1d
 
   call get_command_argument(1, sw_file)
   call get_command_argument(2, lw_file)
@@ -108,6 +114,27 @@ program runes_lfric_case
     read(iu) lco2
     if (lco2 /= 0) read(iu) co2
   end if
+  allocate(skip(n_profile, n_layer))
+  if (version >= 5) then
+    read(iu) lwf
+    do i = 1, 10
+      if (lwf(i) /= 0) read(iu) skip
+    end do
+  end if
+  if (version >= 6) then
+    read(iu) lice
+    if (lice /= 0) read(iu) skip, skip, skip, skip
+  end if
+  if (version >= 7) read(iu) iscat
+  n_aer = 0
+  if (version >= 8) then
+    read(iu) n_aer
+    if (n_aer > 0) then
+      allocate(aer_mix(n_profile, n_layer, n_aer), aer_abs(n_profile, n_layer, n_aer, n_band), &
+        aer_sca(n_profile, n_layer, n_aer, n_band), aer_asy(n_profile, n_layer, n_aer, n_band))
+      read(iu) aer_mix, aer_abs, aer_sca, aer_asy
+    end if
+  end if
   close(iu)
 
   ! host-model storage: n_full columns, the call's column l at pos(l)
@@ -136,6 +163,10 @@ program runes_lfric_case
   liq_conv_mmr_1d = lay1d(liq_conv_mmr); liq_conv_dim_1d = lay1d(liq_conv_dim)
   w = 0.0_RealExt
   ice_mmr_1d = lay1d(w)
+  if (n_aer > 0) then
+    aer_mix_1d = mode1d(aer_mix)
+    aer_abs_1d = prop1d(aer_abs); aer_sca_1d = prop1d(aer_sca); aer_asy_1d = prop1d(aer_asy)
+  end if
   allocate(t_ground_f(n_full), cos_zen_f(n_full), solar_irrad_f(n_full), seed_full(n_full))
   t_ground_f = junk; cos_zen_f = 0.5_RealExt; solar_irrad_f = junk; seed_full = -1
   do l = 1, n_profile
@@ -231,7 +262,10 @@ program runes_lfric_case
       i_cloud_representation=i_cloud_repr, i_overlap=i_overlap, &
       i_inhom=i_inhom, i_mcica_sampling=i_mcica_sampling, &
       i_st_water=i_st_water, i_st_ice=i_st_ice, &
-      n_subcol_gen=n_subcol_gen, rand_seed=seed_full)
+      n_subcol_gen=n_subcol_gen, rand_seed=seed_full, &
+      l_aerosol_mode=(n_aer > 0), n_aer_mode=n_aer, aer_mix_ratio_1d=aer_mix_1d, &
+      aer_absorption_1d=aer_abs_1d, aer_scattering_1d=aer_sca_1d, &
+      aer_asymmetry_1d=aer_asy_1d)
   else
     call runes(n_profile, n_layer, diag, spectrum_name=name, &
       i_source=i_source, profile_list=profile_list, &
@@ -298,7 +332,10 @@ contains
       l_flux_ground=(lfg /= 0), t_ground=t_ground_f, flux_ground_1d=flux_ground_1d, &
       cos_zenith_angle=cos_zen_f, solar_irrad=solar_irrad_f, &
       albedo_diff_1d=albedo_diff_1d, albedo_dir_1d=albedo_dir_1d, &
-      l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity_1d=hc_1d)
+      l_rayleigh=(l_rayleigh /= 0), layer_heat_capacity_1d=hc_1d, &
+      l_aerosol_mode=(n_aer > 0), n_aer_mode=n_aer, aer_mix_ratio_1d=aer_mix_1d, &
+      aer_absorption_1d=aer_abs_1d, aer_scattering_1d=aer_sca_1d, &
+      aer_asymmetry_1d=aer_asy_1d)
   end subroutine clear_call
 
   ! a layer field stored bottom-up per column, n_layer + 1 entries a column
@@ -344,6 +381,47 @@ contains
       end do
     end do
   end function band1d
+
+  ! a modal aerosol mixing ratio stored as the host model stores it
+  ! (socrates_set_aer.F90:466-473 with l_profile_last and l_invert):
+  ! per column, per mode, n_layer + 1 entries bottom-up
+  function mode1d(x) result(y)
+    real(RealExt), intent(in) :: x(:, :, :)
+    real(RealExt), allocatable :: y(:)
+    integer :: ll
+    allocate(y(n_aer * (n_layer + 1) * n_full))
+    y = junk
+    do l = 1, n_profile
+      do jm = 1, n_aer
+        do i = 1, n_layer
+          ll = n_aer * (n_layer + 1) * (profile_list(l) - 1) + (n_layer + 1) * (jm - 1) &
+             + (n_layer + 1 - i)
+          y(ll) = x(l, i, jm)
+        end do
+      end do
+    end do
+  end function mode1d
+
+  ! a modal aerosol optical property (socrates_set_aer.F90:536-545): per
+  ! column, per band, per mode, n_layer + 1 entries bottom-up
+  function prop1d(x) result(y)
+    real(RealExt), intent(in) :: x(:, :, :, :)
+    real(RealExt), allocatable :: y(:)
+    integer :: ll
+    allocate(y(n_band * n_aer * (n_layer + 1) * n_full))
+    y = junk
+    do l = 1, n_profile
+      do ib = 1, n_band
+        do jm = 1, n_aer
+          do i = 1, n_layer
+            ll = n_band * n_aer * (n_layer + 1) * (profile_list(l) - 1) &
+               + n_aer * (n_layer + 1) * (ib - 1) + (n_layer + 1) * (jm - 1) + (n_layer + 1 - i)
+            y(ll) = x(l, i, jm, ib)
+          end do
+        end do
+      end do
+    end do
+  end function prop1d
 
   subroutine put_levels(f)
     real(RealExt), intent(in) :: f(0:, :)

@@ -19,8 +19,10 @@
 ! socrates_set_bound.F90:112-290, socrates_set_cld.F90:861-910,
 ! socrates_set_cld_mcica.F90:159-167) and scatters the outputs as set_diag
 ! does (socrates_set_diag.F90:104-256, 549-623).
-! Arguments of runes outside the path (aerosols, tiles, orography, cloud
-! sub-fractions, photolysis, the clean-air diagnostics, ...) are not
+! Modal aerosols (n_aer_mode, aer_* and their _1d forms, l_aerosol_mode)
+! are taken on the clear-sky and McICA paths. Arguments of runes outside
+! the path (CLASSIC aerosols, tiles, orography, cloud sub-fractions,
+! photolysis, the clean-air diagnostics, ...) are not
 ! declared: a call naming one does not compile, rather than being silently
 ! ignored. Ice takes the spectral file's ice types of Fu's phase-function
 ! fit, Baran's fit and Pade-2; without i_st_ice a cloudy call has no ice
@@ -76,6 +78,11 @@ module socrates_gpu_c
     type(c_ptr) :: ice_conv_mmr = c_null_ptr, ice_conv_dim = c_null_ptr
     integer(c_int) :: l_ir_source_linear = 0, l_no_continuum = 0
     integer(c_int) :: i_scatter_method = 0
+    ! modal aerosols: mixing ratio (profile, layer, mode), optical properties
+    ! (profile, layer, mode, band)
+    integer(c_int) :: n_aer_mode = 0
+    type(c_ptr) :: aer_mix_ratio = c_null_ptr, aer_absorption = c_null_ptr
+    type(c_ptr) :: aer_scattering = c_null_ptr, aer_asymmetry = c_null_ptr
   end type socgpu_runes_args
 
   ! SocgpuDiag
@@ -517,6 +524,9 @@ contains
       i_source, i_scatter_method, i_cloud_representation, i_overlap, i_inhom, &
       i_mcica_sampling, &
       i_st_water, i_cnv_water, i_st_ice, i_cnv_ice, rand_seed, &
+      n_aer_mode, n_aer_layer, l_aerosol_mode, aer_mix_ratio, aer_absorption, &
+      aer_scattering, aer_asymmetry, aer_mix_ratio_1d, aer_absorption_1d, &
+      aer_scattering_1d, aer_asymmetry_1d, &
       l_rayleigh, l_invert, l_profile_last, precision)
     integer, intent(in) :: n_profile, n_layer
     type(StrDiag), intent(inout) :: diag
@@ -626,6 +636,13 @@ contains
     integer, intent(in), optional :: i_mcica_sampling, i_st_water, i_cnv_water
     integer, intent(in), optional :: i_st_ice, i_cnv_ice
     integer, intent(in), optional :: rand_seed(:)
+    integer, intent(in), optional :: n_aer_mode, n_aer_layer
+    logical, intent(in), optional :: l_aerosol_mode
+    real(RealExt), intent(in), optional :: aer_mix_ratio(:, :, :)
+    real(RealExt), intent(in), optional :: aer_absorption(:, :, :, :), &
+      aer_scattering(:, :, :, :), aer_asymmetry(:, :, :, :)
+    real(RealExt), intent(in), optional :: aer_mix_ratio_1d(:), aer_absorption_1d(:), &
+      aer_scattering_1d(:), aer_asymmetry_1d(:)
     logical, intent(in), optional :: l_rayleigh, l_invert, l_profile_last
     integer, intent(in), optional :: precision
 
@@ -634,13 +651,13 @@ contains
     type :: buf
       real(RealExt), allocatable :: v(:, :)
     end type buf
-    integer, parameter :: n_buf = 48
+    integer, parameter :: n_buf = 56
     type(buf), target :: b(n_buf)
     integer :: nb_used
     type(socgpu_runes_args) :: a
     type(socgpu_diag) :: d
     integer :: list(n_profile), l, i, ii, ll, id_ct
-    integer :: layer_offset, level_offset, stride_layer, stride_level
+    integer :: layer_offset, level_offset, stride_layer, stride_level, stride_aer, n_mode
     logical :: last, cloud
     integer(c_int) :: prec, nb, nabs, types(80), status
     integer(c_int), allocatable, target :: seeds(:)
@@ -826,6 +843,24 @@ contains
       a%rand_seed = c_loc(seeds)
     end if
 
+    ! modal aerosols (socrates_set_control.F90:378-384: on when modes are
+    ! given, unless l_aerosol_mode is false; socrates_set_aer.F90:299-309,
+    ! 433-566: an absent field is zero)
+    n_mode = 0
+    if (present(n_aer_mode)) n_mode = max(n_aer_mode, 0)
+    if (present(l_aerosol_mode)) then
+      if (.not. l_aerosol_mode) n_mode = 0
+    end if
+    if (n_mode > 0) then
+      stride_aer = stride_layer
+      if (present(n_aer_layer)) stride_aer = n_aer_layer
+      a%n_aer_mode = n_mode
+      a%aer_mix_ratio = mode_field(aer_mix_ratio, aer_mix_ratio_1d)
+      a%aer_absorption = mode_prop(aer_absorption, aer_absorption_1d)
+      a%aer_scattering = mode_prop(aer_scattering, aer_scattering_1d)
+      a%aer_asymmetry = mode_prop(aer_asymmetry, aer_asymmetry_1d)
+    end if
+
     ! outputs: dense buffers, scattered as set_diag places them
     o_up = 0; o_dn = 0; o_dir = 0; o_hr = 0; o_upc = 0; o_dnc = 0; o_dirc = 0
     if (associated(diag%flux_up) .or. associated(diag%flux_up_surf) .or. &
@@ -966,6 +1001,79 @@ contains
       end do
       p = c_loc(b(j)%v)
     end function band
+
+    ! a modal aerosol mixing ratio (set_mode_field,
+    ! socrates_set_aer.F90:433-491): buffer (profile, layer + n_layer*(mode-1))
+    type(c_ptr) function mode_field(full, oned) result(p)
+      real(RealExt), intent(in), optional :: full(:, :, :), oned(:)
+      integer :: j, jm, c
+      j = new_buf(n_layer * n_mode)
+      b(j)%v = 0.0_RealExt
+      if (present(full) .or. present(oned)) then
+        do jm = 1, n_mode
+          do i = 1, n_layer
+            ii = abs(layer_offset - i)
+            c = i + n_layer * (jm - 1)
+            do l = 1, n_profile
+              if (present(full)) then
+                if (last) then
+                  b(j)%v(l, c) = full(ii, jm, list(l))
+                else
+                  b(j)%v(l, c) = full(list(l), ii, jm)
+                end if
+              else
+                if (last) then
+                  ll = n_mode * stride_aer * (list(l) - 1) + stride_aer * (jm - 1) + ii
+                else
+                  ll = n_profile * stride_aer * (jm - 1) + n_profile * (ii - 1) + list(l)
+                end if
+                b(j)%v(l, c) = oned(ll)
+              end if
+            end do
+          end do
+        end do
+      end if
+      p = c_loc(b(j)%v)
+    end function mode_field
+
+    ! a modal aerosol optical property per band (set_mode_opt_prop,
+    ! socrates_set_aer.F90:494-566): buffer (profile, layer + n_layer*(mode-1)
+    ! + n_layer*n_mode*(band-1))
+    type(c_ptr) function mode_prop(full, oned) result(p)
+      real(RealExt), intent(in), optional :: full(:, :, :, :), oned(:)
+      integer :: j, jm, ib, c
+      j = new_buf(n_layer * n_mode * int(nb))
+      b(j)%v = 0.0_RealExt
+      if (present(full) .or. present(oned)) then
+        do ib = 1, nb
+          do jm = 1, n_mode
+            do i = 1, n_layer
+              ii = abs(layer_offset - i)
+              c = i + n_layer * (jm - 1) + n_layer * n_mode * (ib - 1)
+              do l = 1, n_profile
+                if (present(full)) then
+                  if (last) then
+                    b(j)%v(l, c) = full(ii, jm, ib, list(l))
+                  else
+                    b(j)%v(l, c) = full(list(l), ii, jm, ib)
+                  end if
+                else
+                  if (last) then
+                    ll = nb * n_mode * stride_aer * (list(l) - 1) + n_mode * stride_aer * (ib - 1) &
+                       + stride_aer * (jm - 1) + ii
+                  else
+                    ll = n_mode * stride_aer * n_profile * (ib - 1) + stride_aer * n_profile * (jm - 1) &
+                       + n_profile * (ii - 1) + list(l)
+                  end if
+                  b(j)%v(l, c) = oned(ll)
+                end if
+              end do
+            end do
+          end do
+        end do
+      end if
+      p = c_loc(b(j)%v)
+    end function mode_prop
 
     ! a per-profile field, taken at the listed profiles
     type(c_ptr) function col(x) result(p)

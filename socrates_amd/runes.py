@@ -64,7 +64,10 @@ class RunesArgs(ctypes.Structure):
                                              ("ice_mmr", _D), ("ice_dim", _D), ("ice_conv_mmr", _D),
                                              ("ice_conv_dim", _D), ("l_ir_source_linear", ctypes.c_int),
                                              ("l_no_continuum", ctypes.c_int),
-                                             ("i_scatter_method", ctypes.c_int)]
+                                             ("i_scatter_method", ctypes.c_int),
+                                             ("n_aer_mode", ctypes.c_int), ("aer_mix_ratio", _D),
+                                             ("aer_absorption", _D), ("aer_scattering", _D),
+                                             ("aer_asymmetry", _D)]
 
 
 class Diag(ctypes.Structure):
@@ -163,7 +166,9 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
           rand_seed=None, l_flux_ground=False, flux_ground=None, co2=None,
           l_co2_well_mixed=False, h2o_mix_ratio=0.0, o3_mix_ratio=0.0, i_st_ice=0, i_cnv_ice=0,
           ice_mmr=None, ice_dim=None, ice_conv_mmr=None, ice_conv_dim=None, l_ir_source_linear=False,
-          l_no_continuum=False, i_scatter_method=0, l_clear=False, out=None, **gas_amounts):
+          l_no_continuum=False, i_scatter_method=0, n_aer_mode=0, l_aerosol_mode=True,
+          aer_mix_ratio=None, aer_absorption=None, aer_scattering=None, aer_asymmetry=None,
+          l_clear=False, out=None, **gas_amounts):
     """ctypes argument and output structures of one call (and the objects
     that must stay alive while it runs). mix_ratios: the well-mixed mixing
     ratios of further gases, as runes names them (n2o_mix_ratio,
@@ -179,7 +184,10 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     the general driver's linear infra-red source and continuum switch (runes
     has neither); i_scatter_method: runes' treatment of scattering
     (SOCGPU_SCATTER_*: 1 full, 2 none, 3 extinction as absorption, 4
-    approximate, 5 per k-term; 0 the default, full); l_clear: also the
+    approximate, 5 per k-term; 0 the default, full); n_aer_mode,
+    aer_mix_ratio [mode][layer][profile], aer_absorption, aer_scattering,
+    aer_asymmetry [band][mode][layer][profile]: modal aerosols (off when
+    l_aerosol_mode is False, as in runes); l_clear: also the
     clear-sky fluxes flux_up_clear, flux_down_clear, flux_direct_clear (an
     out dict holding any of these keys asks for them as well)."""
     init()
@@ -194,6 +202,10 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
                   liq_conv_mmr=liq_conv_mmr, liq_conv_dim=liq_conv_dim,
                   flux_ground=flux_ground if l_flux_ground else None, co2=co2, ice_mmr=ice_mmr,
                   ice_dim=ice_dim, ice_conv_mmr=ice_conv_mmr, ice_conv_dim=ice_conv_dim)
+    n_aer = int(n_aer_mode) if l_aerosol_mode else 0
+    if n_aer > 0:
+        inputs.update(aer_mix_ratio=aer_mix_ratio, aer_absorption=aer_absorption,
+                      aer_scattering=aer_scattering, aer_asymmetry=aer_asymmetry)
     on_dev = _is_torch(p_layer)
     keep = []
     a = RunesArgs()
@@ -219,6 +231,7 @@ def _prepare(n_profile, n_layer, spectrum_name, i_source, p_layer, t_layer, mass
     a.i_st_ice, a.i_cnv_ice = int(i_st_ice), int(i_cnv_ice)
     a.l_ir_source_linear, a.l_no_continuum = int(bool(l_ir_source_linear)), int(bool(l_no_continuum))
     a.i_scatter_method = int(i_scatter_method)
+    a.n_aer_mode = max(n_aer, 0)
     for k, v in inputs.items():
         if v is not None and _is_torch(v) != on_dev:
             raise SocgpuError("all inputs must be in the same memory space (%s)" % k)
@@ -490,6 +503,10 @@ def case_call(case, spectrum_name, device_arrays=None, out=None):
         kw["l_no_continuum"] = True
     if case.i_scatter_method:
         kw["i_scatter_method"] = case.i_scatter_method
+    if case.n_aer_mode > 0:
+        kw.update(n_aer_mode=case.n_aer_mode, aer_mix_ratio=g("aer_mix_ratio"),
+                  aer_absorption=g("aer_absorption"), aer_scattering=g("aer_scattering"),
+                  aer_asymmetry=g("aer_asymmetry"))
     if cloudy and case.ice_mmr is not None:
         kw.update(i_st_ice=case.i_st_ice, i_cnv_ice=case.i_st_ice, ice_mmr=g("ice_mmr"),
                   ice_dim=g("ice_dim"))

@@ -164,6 +164,41 @@ def make_case(n_profile, n_layer, source, n_band, seed, cloudy=False,
     return c
 
 
+def add_aerosols(c, n_mode, seed, source=None):
+    """Modal aerosols on a Case (runes' aer_* arguments): per mode a mass
+    mixing ratio decaying from the surface with a 1.5-3 km scale height
+    (surface values log-uniform in 1e-9..2e-8 kg/kg), and per band and mode
+    specific absorption and scattering (m2/kg) and an asymmetry, constant in
+    the horizontal with a few per cent noise per layer: column optical
+    depths of order 0.1 in the solar region, scattering weaker and less
+    forward-peaked in the infra-red."""
+    rng = np.random.default_rng(seed)
+    L, P, B = c.n_layer, c.n_profile, c.n_band
+    src = source or ("sw" if c.i_source == IP_SOLAR else "lw")
+    z = -7.0 * np.log(np.maximum(c.p_layer, 1e-3) / 1.0e5)       # km, approximate
+    mix = np.empty((n_mode, L, P))
+    for m in range(n_mode):
+        h = rng.uniform(1.5, 3.0)
+        q0 = np.exp(rng.uniform(np.log(1e-9), np.log(2e-8), P))
+        mix[m] = q0[None, :] * np.exp(-z / h)
+    shape = (B, n_mode, L, P)
+    noise = lambda: 1.0 + 0.05 * rng.standard_normal(shape)       # noqa: E731
+    if src == "sw":
+        ka = rng.uniform(50.0, 800.0, (B, n_mode))[:, :, None, None] * noise()
+        ks = rng.uniform(1.0e3, 6.0e3, (B, n_mode))[:, :, None, None] * noise()
+        g = rng.uniform(0.55, 0.8, (B, n_mode))[:, :, None, None] * noise()
+    else:
+        ka = rng.uniform(50.0, 600.0, (B, n_mode))[:, :, None, None] * noise()
+        ks = rng.uniform(10.0, 300.0, (B, n_mode))[:, :, None, None] * noise()
+        g = rng.uniform(0.2, 0.6, (B, n_mode))[:, :, None, None] * noise()
+    c.n_aer_mode = n_mode
+    c.aer_mix_ratio = mix
+    c.aer_absorption = np.abs(ka)
+    c.aer_scattering = np.abs(ks)
+    c.aer_asymmetry = np.clip(g, 0.0, 0.95)
+    return c
+
+
 SIGMA_SB = 5.670374419e-8
 
 

@@ -12,7 +12,12 @@ reference compiled from its own sources (oracle/_ref):
   last, gases as strided fields and well-mixed ratios with their switches,
   clear-sky, surface and top-of-atmosphere diagnostics; _drop<ip>: the
   spectra registered without absorber <ip> (set_spectrum's gas flags). The
-  same case through the plain call is stored beside it (ref_*).
+  same case through the plain call is stored beside it (ref_*);
+* lfric_<src>_<kind>_aer.npz: the same with three modal aerosol modes,
+  stored as the host model stores them (aer_*_1d, profiles last,
+  bottom-up).
+
+    python3 tests/golden/make_golden_forms.py --aerosols   (only the _aer files)
 
     python3 tests/golden/make_golden_forms.py     (needs oracle/_ref built)
 """
@@ -61,7 +66,28 @@ def forms_case(src, seed):
     return c
 
 
+def aerosol_fixtures():
+    sw, lw = os.path.join(DATA, "sp_sw_synth"), os.path.join(DATA, "sp_lw_synth")
+    for src, kind, seed in (("sw", "mcica", 1331), ("lw", "clear", 1332), ("lw", "mcica", 1333)):
+        c = synth.make_case(5, 30, src, 6 if src == "sw" else 9, seed=seed, cloudy=kind != "clear",
+                            overlap="mcica")
+        synth.add_aerosols(c, 3, seed + 5)
+        mc = os.path.join(DATA, "mcica_synth") if kind == "mcica" else None
+        raw = run("runes_lfric_case", sw, lw, mc, c)
+        d = {"lfric_raw": raw, "drop": np.array(0)}
+        a = run("soc_runes_case", sw, lw, mc, c, ["1", "clear"])
+        with tempfile.TemporaryDirectory() as t:
+            a.tofile(os.path.join(t, "o"))
+            ref = read_output(os.path.join(t, "o"), c.n_profile, c.n_layer, clear=True)
+        d.update({"ref_" + k: v for k, v in ref.items()})
+        np.savez_compressed(os.path.join(HERE, "lfric_%s_%s_aer.npz" % (src, kind)), **case_to_arrays(c), **d)
+
+
 def main():
+    if "--aerosols" in sys.argv:
+        aerosol_fixtures()
+        return
+    aerosol_fixtures()
     for src, seed in (("sw", 1301), ("lw", 1302)):
         c = forms_case(src, seed)
         a = run("soc_runes_case", os.path.join(GA9, "sp_sw_ga9"), os.path.join(GA9, "sp_lw_ga9"),

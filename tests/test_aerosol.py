@@ -1,0 +1,73 @@
+"""Modal aerosols (runes' n_aer_mode and aer_* arguments) in the oracle,
+against the reference compiled from its own sources
+(tests/golden/aerosol_*.npz, tests/golden/make_golden_aerosol.py): clear sky
+and McICA, synthetic and GA9 spectral files."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from socrates_amd import synth
+from socrates_amd.case_io import Case, case_from_arrays
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DATA = os.path.join(os.path.dirname(GOLD), "..", "socrates_amd", "data")
+FILES = sorted(glob.glob(os.path.join(GOLD, "aerosol_*.npz")))
+
+
+def spectral_files(path):
+    if "_ga9_" in os.path.basename(path):
+        d = os.path.join(DATA, "ga9")
+        return {"sw": os.path.join(d, "sp_sw_ga9"), "lw": os.path.join(d, "sp_lw_ga9"),
+                "mcica": os.path.join(d, "mcica_data")}
+    return {"sw": os.path.join(DATA, "sp_sw_synth"), "lw": os.path.join(DATA, "sp_lw_synth"),
+            "mcica": os.path.join(DATA, "mcica_synth")}
+
+
+def run_oracle(c, path):
+    f = spectral_files(path)
+    src = "sw" if c.i_source == 1 else "lw"
+    return oracle.runes(c, f[src], f["mcica"] if c.l_cloud else None)
+
+
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_oracle_matches_reference_aerosol_golden(path):
+    d = dict(np.load(path))
+    out = run_oracle(case_from_arrays(d), path)
+    for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+        r = d["ref_" + k]
+        err = np.abs(out[k] - r)
+        assert (err <= 1e-13 * np.maximum(np.abs(r), 1.0)).all(), (k, err.max())
+
+
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_goldens_carry_the_aerosol_effect(path):
+    """The aerosols move the fluxes by far more than the tolerance, and so do
+    their scattering and asymmetry alone: a restatement that dropped the
+    forward scattering or the phase function would fail the goldens."""
+    d = dict(np.load(path))
+    c = case_from_arrays(d)
+    none = case_from_arrays(d)
+    none.n_aer_mode = 0
+    assert np.abs(run_oracle(none, path)["flux_up"] - d["ref_flux_up"]).max() > 0.05
+    c.aer_asymmetry = np.zeros_like(c.aer_asymmetry)
+    assert np.abs(run_oracle(c, path)["flux_down"] - d["ref_flux_down"]).max() > 1e-3
+
+
+def test_aerosols_refused_with_triple_column():
+    c = synth.make_case(3, 20, "lw", 9, seed=6, cloudy=True, overlap="triple")
+    synth.add_aerosols(c, 2, 7)
+    with pytest.raises(RuntimeError):
+        run_oracle(c, "synth")
+
+
+def test_case_file_keeps_the_aerosols(tmp_path):
+    c = synth.add_aerosols(synth.make_case(2, 10, "sw", 6, seed=5), 2, 9)
+    p = str(tmp_path / "case")
+    c.write(p)
+    r = Case.read(p)
+    assert r.n_aer_mode == 2
+    for k in ("aer_mix_ratio", "aer_absorption", "aer_scattering", "aer_asymmetry"):
+        assert np.array_equal(getattr(r, k), getattr(c, k)), k

@@ -1,0 +1,133 @@
+"""Modal aerosols on the GPU, through the C ABI: clear sky and McICA on the
+synthetic and GA9 spectral files, the FP64 form against the reference's
+golden vectors (tests/golden/aerosol_*.npz) and the mixed form within its
+bounds; the same call alone and inside a batch with aerosol-free calls;
+inputs in HBM; a host-model-sized call against the oracle; the triple and
+mixed columns with aerosols refused."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from socrates_amd import runes as R
+from socrates_amd import synth
+from socrates_amd.case_io import case_from_arrays
+from test_gpu_parity import check_fp64, check_mixed
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DATA = os.path.join(os.path.dirname(GOLD), "..", "socrates_amd", "data")
+GA9 = os.path.join(DATA, "ga9")
+FILES = sorted(glob.glob(os.path.join(GOLD, "aerosol_*.npz")))
+OUTS = ("flux_up", "flux_down", "flux_direct", "heating_rate")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def spectra():
+    R.init(0)
+    R.register_default_spectra()
+    for src in ("sw", "lw"):
+        R.set_spectrum(src + "_ga9", os.path.join(GA9, "sp_%s_ga9" % src))
+    R.set_mcica(os.path.join(GA9, "mcica_data"), sw_spectrum_name="sw_ga9", lw_spectrum_name="lw_ga9")
+
+
+def name_of(path, src):
+    return src + "_ga9" if "_ga9_" in os.path.basename(path) else src
+
+
+def files_of(path):
+    if "_ga9_" in os.path.basename(path):
+        return {"sw": os.path.join(GA9, "sp_sw_ga9"), "lw": os.path.join(GA9, "sp_lw_ga9"),
+                "mcica": os.path.join(GA9, "mcica_data")}
+    return {"sw": os.path.join(DATA, "sp_sw_synth"), "lw": os.path.join(DATA, "sp_lw_synth"),
+            "mcica": os.path.join(DATA, "mcica_synth")}
+
+
+def exp_spread(c, f, src, runs=4):
+    """How far the reference's own fluxes move when every exp is perturbed by
+    one ulp (the oracle's jittered build; the largest change over `runs`
+    runs): aerosol scattering in optically thin infra-red layers makes the
+    reference's sources ill-conditioned, by up to ~5e-7 W m-2 on these
+    columns, far above the 1e-9 W m-2 floor. The FP64 checks add twice this
+    spread, as one run's change at an element varies by a factor ~10."""
+    mc = f["mcica"] if c.l_cloud else None
+    ref = oracle.runes(c, f[src], mc)
+    sp = {k: np.zeros_like(ref[k]) for k in ("flux_up", "flux_down", "flux_direct")}
+    for _ in range(runs):
+        j = oracle.runes(c, f[src], mc, variant="jitter")
+        for k in sp:
+            sp[k] = np.maximum(sp[k], np.abs(j[k] - ref[k]))
+    return sp
+
+
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_aerosols_match_reference(path):
+    d = dict(np.load(path))
+    c = case_from_arrays(d)
+    src = "sw" if c.i_source == 1 else "lw"
+    print(os.path.basename(path), flush=True)
+    ref = {k: d["ref_" + k] for k in OUTS}
+    gpu = R.runes_case(c, name_of(path, src), precision="fp64")
+    check_fp64(gpu, ref, os.path.basename(path), src, c.layer_heat_capacity,
+               spread=exp_spread(c, files_of(path), src), factor=2.0)
+    mixed = R.runes_case(c, name_of(path, src), precision="mixed")
+    check_mixed(mixed, ref, os.path.basename(path) + " mixed")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_aerosol_call_alone_and_in_a_batch(precision):
+    """An aerosol call packed with aerosol-free calls (the general build for
+    the whole batch) gives the bits it gives alone, and so do the others."""
+    a = synth.add_aerosols(synth.make_case(37, 60, "sw", 6, seed=81, cloudy=True), 3, 82)
+    b = synth.make_case(21, 45, "lw", 9, seed=83, cloudy=True)
+    c = synth.add_aerosols(synth.make_case(9, 72, "lw", 9, seed=84), 2, 85)
+    calls = [R.case_call(x, s) for x, s in ((a, "sw"), (b, "lw"), (c, "lw"))]
+    alone = [R.runes_case(x, s, precision=precision) for x, s in ((a, "sw"), (b, "lw"), (c, "lw"))]
+    batch = R.runes_batch(calls, precision=precision)
+    for one, got in zip(alone, batch):
+        for k in ("flux_up", "flux_down", "heating_rate"):
+            assert np.array_equal(one[k], got[k]), k
+
+
+def test_aerosols_inputs_in_hbm():
+    import torch
+    c = synth.add_aerosols(synth.make_case(64, 60, "sw", 6, seed=91, cloudy=True), 3, 92)
+    host = R.runes_case(c, "sw", precision="fp64")
+    names = ("p_layer", "t_layer", "mass", "density", "layer_heat_capacity", "h2o", "o3", "p_level",
+             "t_level", "t_ground", "solar_irrad", "albedo_diff", "albedo_dir", "cloud_frac", "liq_mmr",
+             "liq_dim", "rand_seed", "aer_mix_ratio", "aer_absorption", "aer_scattering", "aer_asymmetry")
+    arrs = {k: torch.from_numpy(np.ascontiguousarray(getattr(c, k))).cuda() for k in names}
+    arrs["cos_zenith_angle"] = torch.from_numpy(np.ascontiguousarray(c.cos_zen)).cuda()
+    dev = R.runes_case(c, "sw", precision="fp64", device_arrays=arrs)
+    for k in ("flux_up", "flux_down", "flux_direct", "heating_rate"):
+        assert np.array_equal(host[k], dev[k].cpu().numpy()), k
+
+
+@pytest.mark.parametrize("src", ["sw", "lw"])
+def test_aerosols_at_size(src):
+    """A host model's segment (1024 columns x 85 layers, McICA, three
+    modes), FP64 against the oracle."""
+    nb = 6 if src == "sw" else 9
+    c = synth.add_aerosols(synth.make_case(1024, 85, src, nb, seed=95, cloudy=True), 3, 96)
+    f = files_of("synth")
+    ref = oracle.runes(c, f[src], f["mcica"])
+    gpu = R.runes_case(c, src, precision="fp64")
+    check_fp64(gpu, ref, "aerosols 1024x85 " + src, src, c.layer_heat_capacity,
+               spread=exp_spread(c, f, src), factor=2.0)
+    mixed = R.runes_case(c, src, precision="mixed")
+    check_mixed(mixed, ref, "aerosols 1024x85 %s mixed" % src)
+
+
+@pytest.mark.parametrize("kind", ["triple", "mixcol"])
+def test_aerosols_refused_with_region_solvers(kind):
+    c = synth.make_case(4, 20, "lw", 9, seed=7, cloudy=True, overlap="triple")
+    if kind == "mixcol":
+        c.conv_frac[:] = 0.0
+        c.liq_conv_mmr[:] = 0.0
+        c.i_cloud_repr, c.i_inhom = 1, 0
+    synth.add_aerosols(c, 2, 8)
+    with pytest.raises(R.SocgpuError):
+        R.runes_case(c, "lw", precision="fp64")

@@ -140,6 +140,7 @@ int main(void) {
   F(SocgpuRunesArgs, co2_mix_ratio) F(SocgpuRunesArgs, p_layer) F(SocgpuRunesArgs, rand_seed)
   F(SocgpuRunesArgs, gas_mix_ratio) F(SocgpuRunesArgs, l_flux_ground) F(SocgpuRunesArgs, flux_ground)
   F(SocgpuRunesArgs, co2) F(SocgpuRunesArgs, h2o_mix_ratio) F(SocgpuRunesArgs, gas_field)
+  F(SocgpuRunesArgs, i_scatter_method) F(SocgpuRunesArgs, n_aer_mode) F(SocgpuRunesArgs, aer_asymmetry)
   F(SocgpuDiag, l_outputs_on_device) F(SocgpuDiag, flux_up_clear) F(SocgpuDiag, flux_direct_clear)
   return 0;
 }

@@ -25,7 +25,7 @@ from socrates_amd import runes as R
 from socrates_amd import synth
 from socrates_amd.case_io import case_from_arrays
 from test_forms import CLEAR, OUTS, lfric_parse
-from test_gpu_parity import MCICA, NBAND, SPEC, check_fp64, check_mixed
+from test_gpu_parity import MCICA, NBAND, SPEC, check_fp64, check_mixed, oracle_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -170,9 +170,18 @@ def test_fortran_host_model_form_matches_reference(path):
         r = subprocess.run(args, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         got = lfric_parse(np.fromfile(of), P, L)
-    check_fp64(got, {k: want[k] for k in OUTS}, os.path.basename(path), src, case.layer_heat_capacity)
+    # with modal aerosols in optically thin infra-red layers the reference's
+    # own fluxes move by up to ~2e-7 W m-2 under a one-ulp change of exp:
+    # that measured movement (four jittered oracle runs, twice) is added
+    spread = (oracle_spread(case, src, case.l_cloud and case.i_cloud_repr == 1, runs=4)
+              if case.n_aer_mode > 0 else None)
+    check_fp64(got, {k: want[k] for k in OUTS}, os.path.basename(path), src, case.layer_heat_capacity,
+               spread=spread, factor=2.0)
     for k in CLEAR + ("up_surf", "down_surf", "direct_surf", "up_toa", "down_toa", "direct_toa"):
         r = want[k]
-        assert (np.abs(got[k] - r) <= np.maximum(1e-10 * np.abs(r), 1e-9)).all(), k
+        tol = np.maximum(1e-10 * np.abs(r), 1e-9)
+        if spread is not None:
+            tol = tol + 2.0 * spread["flux_up"].max()
+        assert (np.abs(got[k] - r) <= tol).all(), k
     for k in ("hr_above", "hr_below", "unused_columns"):
         assert np.array_equal(got[k], want[k]), k
