@@ -155,9 +155,10 @@ typedef struct SocgpuRunesArgs {
    * specific absorption and scattering (m2 kg-1) and asymmetry
    * [band][mode][layer][profile], added to the clear-air grey optics of
    * every layer (grey_opt_prop.F90:716-743, opt_prop_ukca_aerosol.F90).
-   * n_aer_mode 0: none (runes' l_aerosol_mode false is passed as 0). Clear
-   * sky and McICA only: the triple and mixed columns with aerosols return
-   * SOCGPU_EUNSUPPORTED. */
+   * n_aer_mode 0: none (runes' l_aerosol_mode false is passed as 0). Every
+   * cloud representation of the path takes them (clear sky, McICA, the
+   * triple and mixed columns, whose cloudy regions start from the clear-air
+   * totals). */
   int n_aer_mode;
   const double *aer_mix_ratio, *aer_absorption, *aer_scattering, *aer_asymmetry;
 } SocgpuRunesArgs;

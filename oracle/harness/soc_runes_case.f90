@@ -266,7 +266,9 @@ program soc_runes_case
         conv_vertical_decorr=conv_vertical_decorr, &
         i_cloud_representation=i_cloud_repr, i_overlap=i_overlap, &
         i_st_water=i_st_water, i_st_ice=i_st_ice, &
-        i_cnv_water=i_st_water, i_cnv_ice=i_st_ice)
+        i_cnv_water=i_st_water, i_cnv_ice=i_st_ice, &
+        n_aer_mode=n_aer, aer_mix_ratio=aer_mix, aer_absorption=aer_abs, &
+        aer_scattering=aer_sca, aer_asymmetry=aer_asy)
     end if
   end do
   call system_clock(t1)

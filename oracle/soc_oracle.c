@@ -1001,7 +1001,7 @@ typedef struct {
  * (solver_triple_hogan.F90:207-411). Layer arrays are 0-based (layer i at
  * [i-1]); region 0 clear, 1 stratiform, 2 convective. */
 static void solve_triple(int isolir, tri_t *t, const double *tau_c, const double *om_c,
-                         const double *tau_k[2], const double *om_k[2], const double *g_k[2],
+                         const double *g_c, const double *tau_k[2], const double *om_k[2], const double *g_k[2],
                          const double *frac_k[2], const double *w_cloud, const double *adjust,
                          const double *dpl, const double *dpl2, double flux_inc_direct,
                          double flux_inc_down, double alb_diff, double alb_dir, double dps,
@@ -1012,7 +1012,7 @@ static void solve_triple(int isolir, tri_t *t, const double *tau_c, const double
   /* two_coeff_region: clear region, all layers */
   for (int i = 1; i <= L; i++) {
     coeff_t o;
-    two_coeff_one(isolir, tau_c[i - 1], om_c[i - 1], 0.0, zen_0, &o);
+    two_coeff_one(isolir, tau_c[i - 1], om_c[i - 1], g_c[i - 1], zen_0, &o);
     t->tr[0][i - 1] = o.trans; t->rf[0][i - 1] = o.reflect; t->t0[0][i - 1] = o.trans_0;
     t->sc1[0][i - 1] = o.sc1; t->sc2[0][i - 1] = o.sc2;
   }
@@ -1286,9 +1286,6 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
    * approximate-scattering solvers are not either. */
   const int scat = c->i_scatter_method > 0 ? c->i_scatter_method : IP_SCATTER_FULL;
   if (scat > IP_SCATTER_HYBRID) return -13;
-  /* modal aerosols: restated for the clear and McICA paths; the region
-   * coefficients of the triple and mixed columns with aerosols are not */
-  if (c->n_aer_mode > 0 && triple) return -14;
   if (c->n_aer_mode > 0 && (!c->aer_mix_ratio || !c->aer_absorption || !c->aer_scattering ||
                             !c->aer_asymmetry))
     return -14;
@@ -1773,10 +1770,10 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
           const double *mm = k == 0 ? tms : tmc, *rr = k == 0 ? trs : trc;
           const double *im = k == 0 ? tis : tic, *id = k == 0 ? tds : tdc;
           for (int i = 0; i < L; i++) {
-            const double ray = (isolir == IP_SOLAR && c->l_rayleigh) ? sp.rayleigh_coeff[b] : 0.0;
             /* every type starts from the clear-air totals (grey_opt_prop.F90:
-             * 983-990) */
-            double kgt = kg0[i], ksc = ray, phf = 0.0, fwd = 0.0;
+             * 983-990): Rayleigh and the aerosols' scattering, and the
+             * aerosols' unnormalised moment products */
+            double kgt = kg0[i], ksc = k_ext_scat[i], phf = pf_clr[i], fwd = fw_clr[i];
             if (i + 1 >= nct && LAY(tw, i, l) > 0.0) {
               double ct, cs, cp, cf;
               drop_optics(&sp, it, b, LAY(mm, i, l), LAY(rr, i, l), &ct, &cs, &cp, &cf);
@@ -1871,7 +1868,7 @@ int soc_oracle_runes(const char *spectral_file, const char *mcica_file, const so
             }
           const double *tk[2] = {tau_k[0], tau_k[1]}, *ok[2] = {om_k[0], om_k[1]};
           const double *gk[2] = {g_k[0], g_k[1]}, *fk[2] = {frk[0], frk[1]};
-          solve_triple(isolir, &tri, tau_clr, om_clr, tk, ok, gk, fk, wcol, adjust, dpl, dpl2,
+          solve_triple(isolir, &tri, tau_clr, om_clr, g_clr, tk, ok, gk, fk, wcol, adjust, dpl, dpl2,
                        flux_inc_direct, flux_inc_down, alb_diff, alb_dir, dps, ovl, zen_0);
           F = tri.flux_total;
           FD = tri.flux_direct;

@@ -874,7 +874,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
   // 135-200, two-stream: one moment, forward scattering of order 2): their
   // absorption precedes the continua in the clear-air extinction, their
   // scattering follows Rayleigh's
-  double kaer = 0.0, ksc = ray;
+  double kaer = 0.0, ksc = ray, pfa = 0.0, fwa = 0.0;
   if (gen && d.n_aer > 0) {
     // the reference's products and sums, uncontracted: in optically thin
     // infra-red layers its sources amplify a last-bit change of the
@@ -896,8 +896,11 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
     }
     double *ar = ae.rec + (((size_t)col * NB + b) * L + i) * 4;
     ar[0] = ksc; ar[1] = pf; ar[2] = fw; ar[3] = 0.0;
+    pfa = pf;
+    fwa = fw;
   }
   br[0] = ((kaer + kcont) + ksc) + keq + kgrey;
+  const double ksc_clr = ksc;
   if (d.triple) {
     // grey optics of each cloud type (grey_opt_prop.F90:975-1000,
     // 1185-1230: Rayleigh, then the type's droplets and its ice component
@@ -908,7 +911,8 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
     const int nct = krange[0] < L ? krange[0] : L;
     double *co = cldopt + (((size_t)col * NB + b) * L + i) * 8;
     for (int k = 0; k < 2; k++) {
-      double kgt = kcont + ray, ksc = ray, phf = 0.0, fwd = 0.0;
+      // every type starts from the clear-air totals (aerosols included)
+      double kgt = (kaer + kcont) + ksc_clr, ksc = ksc_clr, phf = pfa, fwd = fwa;
       if (i + 1 >= nct && r[SG_Q_WCLD] > 0.0) {
         const SgDrop &dr = spec->drop[k == 0 ? d.drop : d.drop2];
         const double mmr = r[k == 0 ? SG_Q_LWC : SG_Q_LWC2], re = r[k == 0 ? SG_Q_RE : SG_Q_RE2];
@@ -1961,10 +1965,27 @@ __device__ __forceinline__ double lane_kgas(const SgTerm &t, const SgSpec *spec,
 // cloud regions only from the cloud top (1-based nct) down.
 template <bool MIXED>
 __device__ __forceinline__ void tri_layer_coef(const LaneCtx &c, double kgas, const double *r,
-                                               const double *bq, const double *co, bool cloud_rows,
-                                               Coef (&o)[3]) {
+                                               const double *bq, const double *co, const double *ar,
+                                               bool cloud_rows, Coef (&o)[3]) {
   const double mass = r[SG_Q_MASS];
-  region_coef<MIXED>(c, kgas, mass, bq[0], c.ksc_clear, 0.0, 0.0, o[0]);
+  if (c.aer) {
+    // clear region with aerosols: moments over the scattering, the phase
+    // function delta-rescaled (grey_opt_prop.F90:1360-1440,
+    // rescale_phase_fnc.F90:116-124)
+    double ks = ar[0], pf = ar[1], fw = ar[2];
+    {
+#pragma clang fp contract(off)
+      if (ks > kTiny) {
+        const double inv = 1.0 / ks;
+        fw = fw * inv;
+        pf = pf * inv;
+      }
+      pf = (pf - fw) / (1.0 - fw);
+    }
+    region_coef<MIXED>(c, kgas, mass, bq[0], ks, pf, fw, o[0]);
+  } else {
+    region_coef<MIXED>(c, kgas, mass, bq[0], c.ksc_clear, 0.0, 0.0, o[0]);
+  }
   if (cloud_rows) {
     const double f1 = r[SG_Q_FRAC1], f2 = r[SG_Q_FRAC2];
     if (f1 > kTiny) region_coef<MIXED>(c, kgas, mass, co[0], co[1], co[2], co[3], o[1]);
@@ -1999,6 +2020,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
                                                const double *__restrict__ cldopt,
                                                const double *__restrict__ ovl,
                                                const int *__restrict__ krange, double *__restrict__ tws,
+                                               const double *__restrict__ aerrec,
                                                double *__restrict__ partial, int wave, int lane) {
   const int L = d.L, NB = d.NB;
   const int col = wave / d.nwpc, chunk = wave % d.nwpc;
@@ -2025,7 +2047,8 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
   c.ksc_clear = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0;
   c.ice = false;
   c.nosc = 0;
-  c.aer = false;
+  c.aer = GEN && aerrec != nullptr;
+  const double *ab = c.aer ? aerrec + ((size_t)col * NB + b) * L * 4 : nullptr;
   const double alb_diff = bs[2], alb_dir = bs[3];
   const double finc = solar ? bs[0] / c.sec0 : -bs[0];  // flux_inc_down (= flux_inc_direct)
 #define TWS(i, q) ws[((size_t)(i) * kTriQ + (q)) * kWave]
@@ -2054,7 +2077,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
       const double kgas = lane_kgas<MIXED, GEN>(t, spec, r, grec ? grec + ((size_t)col * L + i) * SG_NGQ : nullptr);
       Coef o[3];
       const bool cloud_rows = i + 1 >= nct;
-      tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, cloud_rows, o);
+      tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, ab ? ab + (size_t)i * 4 : nullptr, cloud_rows, o);
       double top0, top1 = 0.0, top2 = 0.0;
       if (!cloud_rows) {
         top0 = base0;
@@ -2106,7 +2129,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
     const double kgas = lane_kgas<MIXED, GEN>(t, spec, r, grec ? grec + ((size_t)col * L + i) * SG_NGQ : nullptr);
     const bool cloud_rows = i + 1 >= nct;
     Coef o[3];
-    tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, cloud_rows, o);
+    tri_layer_coef<MIXED>(c, kgas, r, bq, cop + (size_t)i * 8, ab ? ab + (size_t)i * 4 : nullptr, cloud_rows, o);
     double su[3], sd[3];
     const int nreg = cloud_rows ? 3 : 1;
     if (solar) {
@@ -2573,8 +2596,8 @@ __global__ __launch_bounds__(256) void b_tsolve_kernel(const BatchHdr *h, const 
     const int r = __builtin_amdgcn_readfirstlane(find_req(h, rq, W_TSOLVE, w));
     const ReqDesc &q = rq[r];
     tri_solve_body<MIXED, GEN>(q.d, G(q.spec), q.l_rayleigh, G(q.colrec), G(q.grec), G(q.colscal), G(q.bandrec),
-                               G(q.bandscal), G(q.cldopt), G(q.ovl), G(q.krange), G(q.tws), G(q.partial),
-                               (int)(w - q.s[W_TSOLVE]), lane);
+                               G(q.bandscal), G(q.cldopt), G(q.ovl), G(q.krange), G(q.tws), G(q.aerrec),
+                               G(q.partial), (int)(w - q.s[W_TSOLVE]), lane);
   }
 }
 
@@ -3158,8 +3181,6 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
   if (a->n_aer_mode > 0) {
     if (!a->aer_mix_ratio || !a->aer_absorption || !a->aer_scattering || !a->aer_asymmetry)
       return fail(SOCGPU_EINVAL, "aerosol modes need aer_mix_ratio, aer_absorption, aer_scattering and aer_asymmetry");
-    if (d.triple)
-      return fail(SOCGPU_EUNSUPPORTED, "aerosols with the triple and mixed columns are not supported");
     d.n_aer = a->n_aer_mode;
     d.gen = 1;
   }

@@ -291,7 +291,10 @@ program runes_lfric_case
       conv_vertical_decorr=conv_vertical_decorr, &
       i_cloud_representation=i_cloud_repr, i_overlap=i_overlap, &
       i_st_water=i_st_water, i_st_ice=i_st_ice, &
-      i_cnv_water=i_st_water, i_cnv_ice=i_st_ice)
+      i_cnv_water=i_st_water, i_cnv_ice=i_st_ice, &
+      l_aerosol_mode=(n_aer > 0), n_aer_mode=n_aer, aer_mix_ratio_1d=aer_mix_1d, &
+      aer_absorption_1d=aer_abs_1d, aer_scattering_1d=aer_sca_1d, &
+      aer_asymmetry_1d=aer_asy_1d)
   end if
 
   ! back to the case file's order: top-down, the call's columns

@@ -20,7 +20,7 @@
 ! socrates_set_cld_mcica.F90:159-167) and scatters the outputs as set_diag
 ! does (socrates_set_diag.F90:104-256, 549-623).
 ! Modal aerosols (n_aer_mode, aer_* and their _1d forms, l_aerosol_mode)
-! are taken on the clear-sky and McICA paths. Arguments of runes outside
+! are taken with every cloud representation. Arguments of runes outside
 ! the path (CLASSIC aerosols, tiles, orography, cloud sub-fractions,
 ! photolysis, the clean-air diagnostics, ...) are not
 ! declared: a call naming one does not compile, rather than being silently

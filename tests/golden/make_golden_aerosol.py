@@ -4,7 +4,8 @@ aer_absorption, aer_scattering, aer_asymmetry; socrates_runes.F90:133-135,
 609-621; socrates_set_aer.F90:299-309; grey_opt_prop.F90:716-743 with
 opt_prop_ukca_aerosol.F90): synthetic columns with three modes through the
 reference compiled from its own sources (oracle/_ref/soc_runes_case), clear
-sky and McICA, on the synthetic spectral files and on the GA9 files (the
+sky, McICA, the triple column (stratiform and convective cloud) and the
+mixed column, on the synthetic spectral files and on the GA9 files (the
 infra-red GA9 case with the per-k-term scattering of the file, as LFRic
 configures it). Writes tests/golden/aerosol_<files>_<src>_<kind>.npz.
 
@@ -32,7 +33,9 @@ WM = np.array([4.7255e-7, 9.637e-7, 0.23139, 2.0e-10, 8.0e-10, 1.1e-9, 2.4e-9, 3
 CASES = [("synth", "sw", "clear", 9, 60, 1301, 0), ("synth", "lw", "clear", 9, 60, 1302, 0),
          ("synth", "sw", "mcica", 8, 60, 1303, 0), ("synth", "lw", "mcica", 8, 60, 1304, 0),
          ("ga9", "sw", "mcica", 6, 50, 1305, 0), ("ga9", "lw", "mcica", 6, 50, 1306, 5),
-         ("ga9", "lw", "clear", 6, 50, 1307, 0)]
+         ("ga9", "lw", "clear", 6, 50, 1307, 0),
+         ("synth", "sw", "triple", 7, 60, 1308, 0), ("synth", "lw", "triple", 7, 60, 1309, 0),
+         ("synth", "sw", "mixcol", 7, 60, 1310, 0), ("ga9", "lw", "triple", 6, 50, 1311, 0)]
 
 
 def files(kind_files):
@@ -45,7 +48,13 @@ def files(kind_files):
 
 def aerosol_case(fs, src, kind, P, L, seed, method):
     nb = {"synth": {"sw": 6, "lw": 9}, "ga9": {"sw": 6, "lw": 9}}[fs][src]
-    c = synth.make_case(P, L, src, nb, seed=seed, cloudy=kind != "clear", overlap="mcica")
+    c = synth.make_case(P, L, src, nb, seed=seed, cloudy=kind != "clear",
+                        overlap="mcica" if kind in ("clear", "mcica") else "triple")
+    if kind == "mixcol":
+        # the mixed column: homogeneous cloud without McICA (maximum-random)
+        c.conv_frac[:] = 0.0
+        c.liq_conv_mmr[:] = 0.0
+        c.i_cloud_repr, c.i_inhom = 1, 0
     if fs == "ga9":
         c.wm_mix_ratio = WM.copy()
     c.i_scatter_method = method

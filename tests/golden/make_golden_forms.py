@@ -68,9 +68,10 @@ def forms_case(src, seed):
 
 def aerosol_fixtures():
     sw, lw = os.path.join(DATA, "sp_sw_synth"), os.path.join(DATA, "sp_lw_synth")
-    for src, kind, seed in (("sw", "mcica", 1331), ("lw", "clear", 1332), ("lw", "mcica", 1333)):
+    for src, kind, seed in (("sw", "mcica", 1331), ("lw", "clear", 1332), ("lw", "mcica", 1333),
+                            ("sw", "triple", 1334)):
         c = synth.make_case(5, 30, src, 6 if src == "sw" else 9, seed=seed, cloudy=kind != "clear",
-                            overlap="mcica")
+                            overlap="triple" if kind == "triple" else "mcica")
         synth.add_aerosols(c, 3, seed + 5)
         mc = os.path.join(DATA, "mcica_synth") if kind == "mcica" else None
         raw = run("runes_lfric_case", sw, lw, mc, c)

@@ -1,7 +1,7 @@
 """Modal aerosols (runes' n_aer_mode and aer_* arguments) in the oracle,
 against the reference compiled from its own sources
-(tests/golden/aerosol_*.npz, tests/golden/make_golden_aerosol.py): clear sky
-and McICA, synthetic and GA9 spectral files."""
+(tests/golden/aerosol_*.npz, tests/golden/make_golden_aerosol.py): clear sky,
+McICA, the triple and the mixed column, synthetic and GA9 spectral files."""
 import glob
 import os
 
@@ -54,13 +54,6 @@ def test_goldens_carry_the_aerosol_effect(path):
     assert np.abs(run_oracle(none, path)["flux_up"] - d["ref_flux_up"]).max() > 0.05
     c.aer_asymmetry = np.zeros_like(c.aer_asymmetry)
     assert np.abs(run_oracle(c, path)["flux_down"] - d["ref_flux_down"]).max() > 1e-3
-
-
-def test_aerosols_refused_with_triple_column():
-    c = synth.make_case(3, 20, "lw", 9, seed=6, cloudy=True, overlap="triple")
-    synth.add_aerosols(c, 2, 7)
-    with pytest.raises(RuntimeError):
-        run_oracle(c, "synth")
 
 
 def test_case_file_keeps_the_aerosols(tmp_path):
