@@ -263,9 +263,54 @@ def ga9(steps=10, warmup=2, P=4096, L=90, seed=20261211):
     return line
 
 
+def aerosol(steps=10, warmup=2, P=4096, L=90, n_mode=3, seed=20261212):
+    """The C2 workload (synthetic files, 4096 x 90 all-sky McICA, SW and LW
+    in one graph-captured batch per step) with n_mode modal aerosols on
+    every column, FP64 and mixed; and the same batch without them, so the
+    cost of the aerosols (their sums in the band kernel, clear air with a
+    phase function in the solve, the general build) is the difference."""
+    import torch
+    dev = torch.device("cuda:0")
+    R.init(0)
+    R.register_default_spectra()
+    n_g = {"sw": 40, "lw": 50}
+    res = {}
+    for with_aer in (False, True):
+        calls, keep = [], []
+        for src in ("sw", "lw"):
+            case = synth.make_case(P, L, src, 6 if src == "sw" else 9, seed=seed + (src == "sw"), cloudy=True)
+            if with_aer:
+                synth.add_aerosols(case, n_mode, seed + 10 + (src == "sw"))
+            arrs, out = _device_case(torch, case, dev)
+            if with_aer:
+                for k in ("aer_mix_ratio", "aer_absorption", "aer_scattering", "aer_asymmetry"):
+                    arrs[k] = torch.from_numpy(np.ascontiguousarray(getattr(case, k))).to(dev)
+            keep.append((arrs, out))
+            calls.append(R.case_call(case, src, device_arrays=arrs, out=out))
+        reqs = R.Requests(calls)
+        solves = P * (n_g["sw"] + n_g["lw"])
+        for prec in ("fp64", "mixed"):
+            b = R.Batch(prec)
+            for _ in range(warmup):
+                b.run(reqs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.run(reqs)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+            b.close()
+            res[("aerosols" if with_aer else "none") + "_" + prec] = dict(ms_per_step=dt * 1e3,
+                                                                         solves_per_s=solves / dt)
+    line = {"benchmark": "C2 workload with %d modal aerosol modes vs none, one graph-captured batch of the "
+                         "SW and LW calls per step" % n_mode, "columns": P, "layers": L, **res}
+    print(json.dumps(line), flush=True)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["ragged", "serve", "triple", "ga9"])
+    ap.add_argument("what", choices=["ragged", "serve", "triple", "ga9", "aerosol"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--rates", default="50,200,1000,2000")
     ap.add_argument("--seconds", type=float, default=10.0)
@@ -278,6 +323,8 @@ def main():
         triple(steps=a.steps)
     elif a.what == "ga9":
         ga9(steps=a.steps)
+    elif a.what == "aerosol":
+        aerosol(steps=a.steps)
     else:
         serve(rates=[int(x) for x in a.rates.split(",")], seconds=a.seconds, max_requests=a.max_requests,
               inputs=a.inputs)

@@ -189,9 +189,10 @@ struct CallDims {
 // Modal aerosol inputs of a call (device pointers: mixing ratio
 // [mode][layer][profile], absorption, scattering and asymmetry
 // [band][mode][layer][profile]) and the per-(column, band, layer) record the
-// band kernel writes from them: clear-air scattering (Rayleigh plus the
-// modes'), and the products of scattering with the asymmetry and with its
-// square, unnormalised (aerrec[col][band][layer][4]).
+// pack kernel writes from them: clear-air scattering (Rayleigh plus the
+// modes'), the products of scattering with the asymmetry and with its
+// square, unnormalised, and the modes' absorption
+// (aerrec[col][band][layer][4]).
 struct AerIn {
   const double *mix, *abs, *sca, *asy;
   double *rec;
@@ -870,34 +871,13 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
                                                   grec[((size_t)col * L + i) * SG_NGQ + SG_G_RHO],
                                                   r[SG_Q_MMR + SG_GAS_H2O])
                                     : 0.0;
-  // modal aerosols (grey_opt_prop.F90:716-743, opt_prop_ukca_aerosol.F90:
-  // 135-200, two-stream: one moment, forward scattering of order 2): their
-  // absorption precedes the continua in the clear-air extinction, their
-  // scattering follows Rayleigh's
+  // modal aerosols: their absorption precedes the continua in the clear-air
+  // extinction, their scattering follows Rayleigh's (the sums over the
+  // modes are formed by the pack kernel, aer_sums)
   double kaer = 0.0, ksc = ray, pfa = 0.0, fwa = 0.0;
   if (gen && d.n_aer > 0) {
-    // the reference's products and sums, uncontracted: in optically thin
-    // infra-red layers its sources amplify a last-bit change of the
-    // scattering a million-fold
-#pragma clang fp contract(off)
-    const int M = d.n_aer;
-    double pf = 0.0, fw = 0.0;
-    for (int m = 0; m < M; m++) {
-      const size_t im = ((size_t)m * L + i) * d.P + col;
-      const size_t ib = (((size_t)b * M + m) * L + i) * d.P + col;
-      const double mix = ae.mix[im];
-      kaer = kaer + mix * ae.abs[ib];
-      const double ks = mix * ae.sca[ib];
-      ksc = ksc + ks;
-      const double g = ae.asy[ib];
-      pf = pf + ks * g;
-      const double ks_phf = ks * g;
-      fw = fw + ks_phf * g;
-    }
-    double *ar = ae.rec + (((size_t)col * NB + b) * L + i) * 4;
-    ar[0] = ksc; ar[1] = pf; ar[2] = fw; ar[3] = 0.0;
-    pfa = pf;
-    fwa = fw;
+    const double *ar = ae.rec + (((size_t)col * NB + b) * L + i) * 4;
+    ksc = ar[0]; pfa = ar[1]; fwa = ar[2]; kaer = ar[3];
   }
   br[0] = ((kaer + kcont) + ksc) + keq + kgrey;
   const double ksc_clr = ksc;
@@ -2345,6 +2325,37 @@ __global__ void b_init_kernel(const BatchHdr *h, const ReqDesc *rq) {
 // kPackL records per column.
 constexpr int kPackC = 64, kPackL = 4;
 
+// Modal aerosols of one (column, layer), every band (grey_opt_prop.F90:
+// 716-743, opt_prop_ukca_aerosol.F90:135-200, two-stream: one moment,
+// forward scattering of order 2): clear-air scattering (Rayleigh, then the
+// modes'), the products of the modes' scattering with the asymmetry and
+// with its square, and their absorption, into aerrec[col][band][layer][4].
+// The threads of a wave take consecutive profiles, so the caller's
+// [band][mode][layer][profile] fields are read along the profiles.
+__device__ __forceinline__ void aer_sums(const CallDims &d, const SgSpec *spec, int l_rayleigh, const AerIn &ae,
+                                         int col, int i) {
+#pragma clang fp contract(off)
+  const int L = d.L, M = d.n_aer;
+  const bool solar = d.isolir == SOCGPU_SOURCE_SOLAR;
+  for (int b = 0; b < d.NB; b++) {
+    double ksc = (solar && l_rayleigh) ? spec->band[b].rayleigh : 0.0, kaer = 0.0, pf = 0.0, fw = 0.0;
+    for (int m = 0; m < M; m++) {
+      const size_t im = ((size_t)m * L + i) * d.P + col;
+      const size_t ib = (((size_t)b * M + m) * L + i) * d.P + col;
+      const double mix = ae.mix[im];
+      kaer = kaer + mix * ae.abs[ib];
+      const double ks = mix * ae.sca[ib];
+      ksc = ksc + ks;
+      const double g = ae.asy[ib];
+      pf = pf + ks * g;
+      const double ks_phf = ks * g;
+      fw = fw + ks_phf * g;
+    }
+    double *ar = ae.rec + (((size_t)col * d.NB + b) * L + i) * 4;
+    ar[0] = ksc; ar[1] = pf; ar[2] = fw; ar[3] = kaer;
+  }
+}
+
 __host__ __device__ inline int pack_tiles(const CallDims &d) {
   return ((d.P + kPackC - 1) / kPackC) * ((d.L + 1 + kPackL - 1) / kPackL);
 }
@@ -2368,6 +2379,10 @@ __global__ __launch_bounds__(kPackC * kPackL) void b_pack_kernel(const BatchHdr 
     if (col < P && i < L) {
       if (pack_layer(q.d, spec, a, i * P + col, q.dp_cf, q.dp_cw, sh + (tx * kPackL + ty) * SG_NQ))
         atomicMax(&((BatchHdr *)h)->err, r + 1);
+      if (q.d.n_aer > 0) {
+        const AerIn ae{a.aer_mix_ratio, a.aer_absorption, a.aer_scattering, a.aer_asymmetry, G(q.aerrec)};
+        aer_sums(q.d, spec, q.l_rayleigh, ae, col, i);
+      }
       if (q.d.gx) {
         // gas record: look-up factors and the density (continua)
         double *g = grec + ((size_t)col * L + i) * SG_NGQ;
