@@ -183,7 +183,8 @@ struct CallDims {
   int no_cont;       // continuum absorption left out
   int scat;          // treatment of scattering (SOCGPU_SCATTER_*)
   int grey1;         // minor gases with one k-term added to the grey extinction
-  int n_aer;         // modal aerosols: modes (0: none; general build only)
+  int n_aer;         // modal aerosols: modes (0: none; the general solve build)
+  int ggen;          // the spectrum needs the general gas optics (band kernel build)
 };
 
 // Modal aerosol inputs of a call (device pointers: mixing ratio
@@ -875,7 +876,7 @@ __device__ __forceinline__ void prep_layer(const CallDims &d, const SgSpec *__re
   // extinction, their scattering follows Rayleigh's (the sums over the
   // modes are formed by the pack kernel, aer_sums)
   double kaer = 0.0, ksc = ray, pfa = 0.0, fwa = 0.0;
-  if (gen && d.n_aer > 0) {
+  if (d.n_aer > 0) {
     const double *ar = ae.rec + (((size_t)col * NB + b) * L + i) * 4;
     ksc = ar[0]; pfa = ar[1]; fwa = ar[2]; kaer = ar[3];
   }
@@ -2259,7 +2260,7 @@ __device__ __forceinline__ void tri_solve_body(const CallDims &d, const SgSpec *
 // W_PLAIN counts the calls that do not: the solve kernel of each build runs
 // only the calls of its kind, so a call's solve is the one it gets alone.
 enum { W_LEV, W_LAY, W_CG, W_MINOR, W_PREP, W_SOLVE, W_TOVL, W_TSOLVE, W_PTILE, W_CCOL, W_FTILE, W_LONG, W_GEN,
-       W_PLAIN, kNWork };
+       W_PLAIN, W_GBAND, kNWork };
 
 struct ReqDesc {
   CallDims d;
@@ -3192,7 +3193,9 @@ static int resolve_request(const char *spectrum_name, const SocgpuRunesArgs *a, 
       return fail(SOCGPU_EUNSUPPORTED, "approximate scattering in the triple and mixed columns is not supported");
   }
   // modal aerosols (socrates_set_control.F90:378-384: on when modes are
-  // given; socrates_set_aer.F90:299-309): the general build carries them
+  // given; socrates_set_aer.F90:299-309): the general solve build carries
+  // them; the band kernel's build follows the gas optics alone
+  d.ggen = d.gen;
   if (a->n_aer_mode > 0) {
     if (!a->aer_mix_ratio || !a->aer_absorption || !a->aer_scattering || !a->aer_asymmetry)
       return fail(SOCGPU_EINVAL, "aerosol modes need aer_mix_ratio, aer_absorption, aer_scattering and aer_asymmetry");
@@ -3241,6 +3244,7 @@ static void request_work(const CallDims &d, const SgSpec &S, long long w[kNWork]
   w[W_FTILE] = finish_tiles(d);
   w[W_GEN] = d.gen ? 1 : 0;
   w[W_PLAIN] = d.gen ? 0 : 1;
+  w[W_GBAND] = d.ggen ? 1 : 0;
 }
 
 // Lays out the workspace of request r at *cursor (doubles, 32-byte aligned)
@@ -3327,10 +3331,11 @@ static int launch_batch(const BatchHdr *hdr, const ReqDesc *rq, const long long 
   {
     // calls of up to 128 layers, then the longer ones (each kernel skips the
     // other's waves)
-    auto *bk = gen ? b_band_kernel<2, true> : b_band_kernel<2, false>;
+    const bool gband = tot[W_GBAND] > 0;
+    auto *bk = gband ? b_band_kernel<2, true> : b_band_kernel<2, false>;
     bk<<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
     if (tot[W_LONG] > 0) {
-      auto *bl = gen ? b_band_kernel<kKeqMaxChunks, true> : b_band_kernel<kKeqMaxChunks, false>;
+      auto *bl = gband ? b_band_kernel<kKeqMaxChunks, true> : b_band_kernel<kKeqMaxChunks, false>;
       bl<<<blocks(tot[W_MINOR], kMinorWaves), 64 * kMinorWaves, 0, s>>>(hdr, rq);
     }
   }
